@@ -1,0 +1,168 @@
+/*
+ * libgpw_amd.h -- C ABI of the MI355X collocate/integrate library.
+ *
+ * Plain C types only (no STL, no torch). Every entry point returns a
+ * gpw_status; GPW_OK is 0. Device pointers are HIP device memory on the
+ * current device; `stream` is a hipStream_t passed as void* (NULL = default
+ * stream). Grids are row-major (z, y, x), x fastest, FP64, and are always
+ * ACCUMULATED into (+=), as in the reference.
+ *
+ * Reference interfaces replaced (paths relative to the reference root):
+ *   gpw_collocate_pgf_product_rspace  <- grid_collocate_pgf_product_rspace,
+ *                                         src/lib/grid_base_ref_c.h:40-67
+ *   gpw_integrate_pgf_product_rspace  <- no counterpart in the reference: the
+ *                                         adjoint of the above with the same
+ *                                         argument list (integrate wiring
+ *                                         sketched at src/lib/collocate.cc:963-1113)
+ *   gpw_tasklist_*                    <- batched form of the above; batching
+ *                                         drafts at src/lib/gaussian.hpp:1-114 and
+ *                                         src/lib/collocate_gpu.cc:1-124
+ *   gpw_collocate_core / gpw_integrate_core
+ *                                     <- collocate_core<T> / integrate_core<T>,
+ *                                         src/lib/collocate.cc:543 / :438
+ */
+#ifndef LIBGPW_AMD_H
+#define LIBGPW_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int gpw_status;
+
+#define GPW_OK 0
+#define GPW_ERR_ARG 1        /* bad angular momentum range, sizes or pointers   */
+#define GPW_ERR_LMAX 2       /* lmax below the local angular momentum           */
+#define GPW_ERR_BOUNDS 3     /* non-periodic cube outside the local grid window  */
+#define GPW_ERR_SPHERE 4     /* sphere_bounds inconsistent with lb/ub_cube       */
+#define GPW_ERR_HIP 5        /* a HIP runtime call failed                        */
+#define GPW_ERR_ALLOC 6      /* device or host allocation failed                 */
+#define GPW_ERR_UNSUPPORTED 7
+
+/* largest angular momentum per centre after tau / force shifts */
+#define GPW_LMAX 8
+
+/* Grid description shared by every pair of a task list. */
+typedef struct gpw_grid_desc {
+    int npts[3];        /* global points per axis (x, y, z)                     */
+    int ngrid[3];       /* local points per axis; == npts on periodic axes       */
+    int lb_grid[3];     /* lower bounds of the local window                      */
+    int periodic[3];    /* 1 = periodic axis                                     */
+    double dh[3][3];    /* dh[i] = grid step vector of axis i (rows)             */
+    double dh_inv[3][3];
+} gpw_grid_desc;
+
+/* One primitive Gaussian pair. */
+typedef struct gpw_pair_desc {
+    int la_max, la_min, lb_max, lb_min;
+    double zeta, zetb, rscale;
+    double ra[3], rab[3];
+    double radius;            /* cube radius; the cube and sphere bounds follow it  */
+    long long block_offset;   /* offset (doubles) of the pair's pab / hab block;
+                                 the block is ncoset(lb_max) rows (b functions) x
+                                 ncoset(la_max) columns (a functions), row-major   */
+} gpw_pair_desc;
+
+typedef struct gpw_tasklist gpw_tasklist;
+
+/* Statistics of a task list (algorithmic work for roofline accounting). */
+typedef struct gpw_tasklist_stats {
+    long long npairs;
+    long long nsub;           /* sub-boxes after splitting cubes larger than the period */
+    long long ntiles;
+    long long nentries;       /* (sub-box, tile) work items                             */
+    long long cube_points;    /* grid points inside all pairs' spheres (folded count)   */
+    long long tile_points;    /* points swept by the tile kernels                       */
+    int lp_max;
+    int tile[3];              /* tile shape (x, y, z)                                   */
+    int splits;               /* work-list splits per tile                              */
+} gpw_tasklist_stats;
+
+/* ---- single task, host memory (drop-in for the reference's collocate) ---- */
+
+/* grid_collocate_pgf_product_rspace (src/lib/grid_base_ref_c.h:40-67).
+ * dh, dh_inv: 3x3 row-major; pab: maxco x maxco row-major host array, the
+ * sub-block starts at pab[o2][o1]; grid: ngrid[2] x ngrid[1] x ngrid[0] host
+ * array, accumulated. lb_cube/ub_cube/sphere_bounds are only read when
+ * use_ortho != 0. Runs on the GPU (one pair, plain kernels). */
+gpw_status gpw_collocate_pgf_product_rspace(
+    int compute_tau, int use_ortho, int la_max, int la_min, int lb_max, int lb_min,
+    double zeta, double zetb, double rscale, const double *dh, const double *dh_inv,
+    const double *ra, const double *rab, const int *npts, const int *ngrid,
+    const int *lb_grid, const int *periodic, int lmax, double radius, const int *lb_cube,
+    const int *ub_cube, int nspheres, const int *sphere_bounds, int maxco, int o1, int o2,
+    const double *pab, double *grid);
+
+/* Adjoint of the above (no counterpart in the reference):
+ *   hab[o2+jco][o1+ico] += d <grid, collocate(pab)> / d pab[o2+jco][o1+ico]
+ * for every (ico, jco) the collocation reads. If forces != NULL (6 doubles),
+ * forces[0..2] += dE/dRa and forces[3..5] += dE/dRb with E = <grid,
+ * collocate(pab)> at a fixed grid index set; pab is then read as well. */
+gpw_status gpw_integrate_pgf_product_rspace(
+    int compute_tau, int use_ortho, int la_max, int la_min, int lb_max, int lb_min,
+    double zeta, double zetb, double rscale, const double *dh, const double *dh_inv,
+    const double *ra, const double *rab, const int *npts, const int *ngrid,
+    const int *lb_grid, const int *periodic, int lmax, double radius, const int *lb_cube,
+    const int *ub_cube, int nspheres, const int *sphere_bounds, int maxco, int o1, int o2,
+    const double *grid, const double *pab, double *hab, double *forces);
+
+/* ---- batched task lists (device memory, stream ordered) ------------------ */
+
+/* Builds the per-pair geometry (cube centre, cube and sphere bounds from each
+ * pair's radius, splitting of cubes larger than the period) and the tile
+ * work lists on the host, and uploads them. compute_tau applies to all pairs.
+ * tile: NULL for the default shape, else {tx, ty, tz} (tx*... see DESIGN). */
+gpw_status gpw_tasklist_create(const gpw_grid_desc *grid, int npairs,
+                               const gpw_pair_desc *pairs, int compute_tau,
+                               const int *tile, int splits, gpw_tasklist **out);
+gpw_status gpw_tasklist_destroy(gpw_tasklist *tl);
+gpw_status gpw_tasklist_stats_get(const gpw_tasklist *tl, gpw_tasklist_stats *out);
+
+/* Host-only planning (no device needed): the statistics gpw_tasklist_create
+ * would report for these pairs. */
+gpw_status gpw_tasklist_plan_stats(const gpw_grid_desc *grid, int npairs,
+                                   const gpw_pair_desc *pairs, int compute_tau, int splits,
+                                   gpw_tasklist_stats *out);
+/* Host-only: the geometry the task list derives for one pair -- cube centre
+ * (FLOOR(dh_inv rp), src/lib/grid_base_ref_c.cc:534-542), the offset roffset
+ * (:544-547), lb_cube / ub_cube and sphere_bounds in the layout consumed at
+ * src/lib/grid_base_ref_c.cc:413-470 (what CP2K's cube_utils supplies). */
+gpw_status gpw_pair_geometry(const gpw_grid_desc *grid, const gpw_pair_desc *pair,
+                             int compute_tau, int cube_centre[3], double roffset[3],
+                             int lb_cube[3], int ub_cube[3], int *sphere_bounds, int cap,
+                             int *nspheres);
+
+/* grid_dev += collocation of every pair (pab_dev: packed blocks). */
+gpw_status gpw_tasklist_collocate(gpw_tasklist *tl, const double *pab_dev, double *grid_dev,
+                                  void *stream);
+/* hab_dev (packed blocks) += integrate; forces_dev (npairs x 6) += dE/dRa,
+ * dE/dRb when non-NULL (pab_dev then read). */
+gpw_status gpw_tasklist_integrate(gpw_tasklist *tl, const double *grid_dev,
+                                  const double *pab_dev, double *hab_dev, double *forces_dev,
+                                  void *stream);
+/* The same two operations, one kernel launch sequence per pair (plain form). */
+gpw_status gpw_tasklist_collocate_plain(gpw_tasklist *tl, const double *pab_dev,
+                                        double *grid_dev, void *stream);
+gpw_status gpw_tasklist_integrate_plain(gpw_tasklist *tl, const double *grid_dev,
+                                        const double *pab_dev, double *hab_dev,
+                                        double *forces_dev, void *stream);
+
+/* ---- libgpw tensor-path contraction (device memory) ---------------------- */
+/* collocate_core (src/lib/collocate.cc:543):
+ *   V[z][y][x] = sum co[a][g][b] * P[0][g][z] * P[1][b][y] * P[2][a][x]
+ * co: L x L x L, P: 3 x L x ldp, V: length[0] x length[1] x length[2]
+ * (overwritten, as the reference's Vtmp). */
+gpw_status gpw_collocate_core(const int *length, int L, const double *co_dev,
+                              const double *P_dev, int ldp, double *V_dev, void *stream);
+/* integrate_core (src/lib/collocate.cc:438):
+ *   Velems[g][b][a] = sum V[z][y][x] * P[0][g][z] * P[1][b][y] * P[2][a][x] */
+gpw_status gpw_integrate_core(const int *length, int L, const double *P_dev, int ldp,
+                              const double *V_dev, double *Velems_dev, void *stream);
+
+/* last HIP error string recorded by the library (thread-local) */
+const char *gpw_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

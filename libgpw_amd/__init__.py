@@ -1,0 +1,292 @@
+"""libgpw_amd: MI355X-native collocate / integrate of primitive Gaussian pairs.
+
+The compute path is the HIP library ``_lib/libgpw_amd.so`` (kernels in
+``csrc/``), reached through its C ABI (``include/libgpw_amd.h``). This module
+is the Python host side: ctypes bindings, a task-list wrapper over torch
+device tensors, the single-task drop-in, and the autograd Function. There is
+no CPU fallback: every compute entry point raises if the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libgpw_amd.so")
+
+GPW_LMAX = 8
+STATUS = {0: "GPW_OK", 1: "GPW_ERR_ARG", 2: "GPW_ERR_LMAX", 3: "GPW_ERR_BOUNDS",
+          4: "GPW_ERR_SPHERE", 5: "GPW_ERR_HIP", 6: "GPW_ERR_ALLOC", 7: "GPW_ERR_UNSUPPORTED"}
+
+# every symbol declared in include/libgpw_amd.h
+EXPORTS = (
+    "gpw_collocate_pgf_product_rspace", "gpw_integrate_pgf_product_rspace",
+    "gpw_tasklist_create", "gpw_tasklist_destroy", "gpw_tasklist_stats_get",
+    "gpw_tasklist_plan_stats", "gpw_pair_geometry",
+    "gpw_tasklist_collocate", "gpw_tasklist_integrate",
+    "gpw_tasklist_collocate_plain", "gpw_tasklist_integrate_plain",
+    "gpw_collocate_core", "gpw_integrate_core", "gpw_last_error",
+)
+
+
+class GpwError(RuntimeError):
+    def __init__(self, code, msg=""):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("npts", C.c_int * 3), ("ngrid", C.c_int * 3), ("lb_grid", C.c_int * 3),
+                ("periodic", C.c_int * 3), ("dh", (C.c_double * 3) * 3),
+                ("dh_inv", (C.c_double * 3) * 3)]
+
+
+class PairDesc(C.Structure):
+    _fields_ = [("la_max", C.c_int), ("la_min", C.c_int), ("lb_max", C.c_int), ("lb_min", C.c_int),
+                ("zeta", C.c_double), ("zetb", C.c_double), ("rscale", C.c_double),
+                ("ra", C.c_double * 3), ("rab", C.c_double * 3), ("radius", C.c_double),
+                ("block_offset", C.c_longlong)]
+
+
+class TaskListStats(C.Structure):
+    _fields_ = [("npairs", C.c_longlong), ("nsub", C.c_longlong), ("ntiles", C.c_longlong),
+                ("nentries", C.c_longlong), ("cube_points", C.c_longlong),
+                ("tile_points", C.c_longlong), ("lp_max", C.c_int), ("tile", C.c_int * 3),
+                ("splits", C.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"libgpw_amd: HIP library not built ({LIB_PATH}); run __graft_entry__.build() "
+                "or python libgpw_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        dp, ip, vp = C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_void_p
+        single = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                  C.c_double, dp, dp, dp, dp, ip, ip, ip, ip, C.c_int, C.c_double, ip, ip, C.c_int,
+                  ip, C.c_int, C.c_int, C.c_int]
+        L.gpw_collocate_pgf_product_rspace.argtypes = single + [dp, dp]
+        L.gpw_integrate_pgf_product_rspace.argtypes = single + [dp, dp, dp, dp]
+        L.gpw_tasklist_create.argtypes = [C.POINTER(GridDesc), C.c_int, C.POINTER(PairDesc),
+                                          C.c_int, ip, C.c_int, C.POINTER(vp)]
+        L.gpw_tasklist_destroy.argtypes = [vp]
+        L.gpw_tasklist_stats_get.argtypes = [vp, C.POINTER(TaskListStats)]
+        L.gpw_tasklist_plan_stats.argtypes = [C.POINTER(GridDesc), C.c_int, C.POINTER(PairDesc),
+                                              C.c_int, C.c_int, C.POINTER(TaskListStats)]
+        L.gpw_pair_geometry.argtypes = [C.POINTER(GridDesc), C.POINTER(PairDesc), C.c_int, ip, dp,
+                                        ip, ip, ip, C.c_int, ip]
+        L.gpw_tasklist_collocate.argtypes = [vp, vp, vp, vp]
+        L.gpw_tasklist_integrate.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.gpw_tasklist_collocate_plain.argtypes = [vp, vp, vp, vp]
+        L.gpw_tasklist_integrate_plain.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.gpw_collocate_core.argtypes = [ip, C.c_int, vp, vp, C.c_int, vp, vp]
+        L.gpw_integrate_core.argtypes = [ip, C.c_int, vp, C.c_int, vp, vp, vp]
+        L.gpw_last_error.argtypes = []
+        L.gpw_last_error.restype = C.c_char_p
+        for n in EXPORTS:
+            if n != "gpw_last_error":
+                getattr(L, n).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _check(code):
+    if code != 0:
+        msg = lib().gpw_last_error()
+        raise GpwError(code, msg.decode() if msg else "")
+
+
+def ncoset(l: int) -> int:
+    return 0 if l < 0 else (l + 1) * (l + 2) * (l + 3) // 6
+
+
+def grid_desc(npts, dh, ngrid=None, lb_grid=(0, 0, 0), periodic=(1, 1, 1)) -> GridDesc:
+    g = GridDesc()
+    dh = np.asarray(dh, dtype=np.float64)
+    inv = np.linalg.inv(dh)
+    ngrid = npts if ngrid is None else ngrid
+    for i in range(3):
+        g.npts[i] = int(npts[i]); g.ngrid[i] = int(ngrid[i])
+        g.lb_grid[i] = int(lb_grid[i]); g.periodic[i] = int(bool(periodic[i]))
+        for j in range(3):
+            g.dh[i][j] = float(dh[i, j]); g.dh_inv[i][j] = float(inv[i, j])
+    return g
+
+
+def pair_array(p: dict):
+    n = len(p["la_max"])
+    arr = (PairDesc * n)()
+    la_max, la_min = p["la_max"], p["la_min"]
+    lb_max, lb_min = p["lb_max"], p["lb_min"]
+    for i in range(n):
+        q = arr[i]
+        q.la_max = int(la_max[i]); q.la_min = int(la_min[i])
+        q.lb_max = int(lb_max[i]); q.lb_min = int(lb_min[i])
+        q.zeta = float(p["zeta"][i]); q.zetb = float(p["zetb"][i])
+        q.rscale = float(p["rscale"][i]); q.radius = float(p["radius"][i])
+        ra, rab = p["ra"][i], p["rab"][i]
+        q.ra[0], q.ra[1], q.ra[2] = float(ra[0]), float(ra[1]), float(ra[2])
+        q.rab[0], q.rab[1], q.rab[2] = float(rab[0]), float(rab[1]), float(rab[2])
+        q.block_offset = int(p["block_offset"][i])
+    return arr
+
+
+def _stats_dict(s: TaskListStats) -> dict:
+    return {k: (list(getattr(s, k)) if k == "tile" else getattr(s, k)) for k, _ in s._fields_}
+
+
+def plan_stats(grid: GridDesc, pairs: dict, compute_tau: bool = False, splits: int = 1) -> dict:
+    """Host-only planning statistics (no device needed)."""
+    arr = pair_array(pairs)
+    s = TaskListStats()
+    _check(lib().gpw_tasklist_plan_stats(C.byref(grid), len(arr), arr, int(compute_tau),
+                                         int(splits), C.byref(s)))
+    return _stats_dict(s)
+
+
+def pair_geometry(grid: GridDesc, pairs: dict, i: int, compute_tau: bool = False) -> dict:
+    """Host-only geometry of pair i: cube centre, roffset, cube and sphere bounds."""
+    arr = pair_array({k: np.asarray(v)[i:i + 1] for k, v in pairs.items()})
+    cc, lb, ub = (C.c_int * 3)(), (C.c_int * 3)(), (C.c_int * 3)()
+    roff = (C.c_double * 3)()
+    cap = 1 << 18
+    sb = np.zeros(cap, dtype=np.int32)
+    n = C.c_int(0)
+    _check(lib().gpw_pair_geometry(C.byref(grid), arr, int(compute_tau), cc, roff, lb, ub,
+                                   sb.ctypes.data_as(C.POINTER(C.c_int)), cap, C.byref(n)))
+    return {"cube_centre": list(cc), "roffset": list(roff), "lb_cube": list(lb),
+            "ub_cube": list(ub), "sphere_bounds": sb[:n.value].copy()}
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    import torch
+    assert t.is_cuda and t.dtype == torch.float64 and t.is_contiguous(), "need contiguous fp64 device tensor"
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    return C.c_void_p(int(stream))
+
+
+class TaskList:
+    """Batched collocate / integrate of many pairs on one grid (device resident).
+
+    Mirrors a CP2K-style task list: geometry (cube centres, sphere bounds, tile
+    work lists) is built once here; collocate/integrate run stream-ordered on
+    torch device tensors and allocate nothing.
+    """
+
+    def __init__(self, grid: GridDesc, pairs: dict, compute_tau: bool = False, splits: int = 1):
+        self.grid = grid
+        self.npairs = len(pairs["la_max"])
+        self._pairs = pair_array(pairs)
+        h = C.c_void_p()
+        _check(lib().gpw_tasklist_create(C.byref(grid), self.npairs, self._pairs,
+                                         int(compute_tau), None, int(splits), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().gpw_tasklist_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def stats(self) -> dict:
+        s = TaskListStats()
+        _check(lib().gpw_tasklist_stats_get(self._h, C.byref(s)))
+        return _stats_dict(s)
+
+    def collocate(self, pab, grid, stream=None, plain=False):
+        f = lib().gpw_tasklist_collocate_plain if plain else lib().gpw_tasklist_collocate
+        _check(f(self._h, _ptr(pab), _ptr(grid), _stream(stream)))
+        return grid
+
+    def integrate(self, grid, hab, pab=None, forces=None, stream=None, plain=False):
+        f = lib().gpw_tasklist_integrate_plain if plain else lib().gpw_tasklist_integrate
+        _check(f(self._h, _ptr(grid), _ptr(pab), _ptr(hab), _ptr(forces), _stream(stream)))
+        return hab
+
+
+def _ci(v):
+    return (C.c_int * len(v))(*[int(x) for x in v])
+
+
+def _cd(v):
+    v = np.ascontiguousarray(v, dtype=np.float64).ravel()
+    return (C.c_double * len(v))(*v)
+
+
+def _single_args(t: dict):
+    sb = np.asarray(t.get("sphere_bounds", []), dtype=np.int32)
+    if sb.size == 0:
+        sb = np.zeros(1, dtype=np.int32)
+    return [int(t["compute_tau"]), int(t["use_ortho"]), int(t["la_max"]), int(t["la_min"]),
+            int(t["lb_max"]), int(t["lb_min"]), float(t["zeta"]), float(t["zetb"]),
+            float(t["rscale"]), _cd(t["dh"]), _cd(t["dh_inv"]), _cd(t["ra"]), _cd(t["rab"]),
+            _ci(t["npts"]), _ci(t["ngrid"]), _ci(t["lb_grid"]), _ci(t["periodic"]),
+            int(t["lmax"]), float(t["radius"]), _ci(t.get("lb_cube", (0, 0, 0))),
+            _ci(t.get("ub_cube", (0, 0, 0))), int(t.get("nspheres", len(sb))), _ci(sb),
+            int(t["maxco"]), int(t["o1"]), int(t["o2"])]
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def collocate_pgf_product_rspace(t: dict, pab: np.ndarray, grid: np.ndarray) -> np.ndarray:
+    """Drop-in for grid_collocate_pgf_product_rspace on host arrays (grid += ...)."""
+    pab = np.ascontiguousarray(pab, dtype=np.float64)
+    assert grid.dtype == np.float64 and grid.flags["C_CONTIGUOUS"]
+    _check(lib().gpw_collocate_pgf_product_rspace(*_single_args(t), _dptr(pab), _dptr(grid)))
+    return grid
+
+
+def integrate_pgf_product_rspace(t: dict, grid: np.ndarray, pab: np.ndarray | None = None,
+                                 hab: np.ndarray | None = None, forces: bool = False):
+    """Adjoint of collocate_pgf_product_rspace; returns (hab, forces or None)."""
+    grid = np.ascontiguousarray(grid, dtype=np.float64)
+    if hab is None:
+        hab = np.zeros((t["maxco"], t["maxco"]))
+    f = np.zeros(6) if forces else None
+    pab_c = None if pab is None else np.ascontiguousarray(pab, dtype=np.float64)
+    _check(lib().gpw_integrate_pgf_product_rspace(
+        *_single_args(t), _dptr(grid), None if pab_c is None else _dptr(pab_c), _dptr(hab),
+        None if f is None else _dptr(f)))
+    return hab, (None if f is None else f.reshape(2, 3))
+
+
+def collocate_core(length, co, P, stream=None):
+    """libgpw collocate_core on device tensors: returns V (length[0..2])."""
+    import torch
+    L = co.shape[0]
+    V = torch.empty(tuple(int(x) for x in length), dtype=torch.float64, device=co.device)
+    _check(lib().gpw_collocate_core(_ci(length), L, _ptr(co), _ptr(P), int(P.shape[2]), _ptr(V),
+                                    _stream(stream)))
+    return V
+
+
+def integrate_core(length, P, V, stream=None):
+    import torch
+    L = P.shape[1]
+    out = torch.empty((L, L, L), dtype=torch.float64, device=V.device)
+    _check(lib().gpw_integrate_core(_ci(length), L, _ptr(P), int(P.shape[2]), _ptr(V), _ptr(out),
+                                    _stream(stream)))
+    return out

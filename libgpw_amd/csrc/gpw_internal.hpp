@@ -1,0 +1,79 @@
+// gpw_internal.hpp -- structures shared by the host planner and the HIP kernels.
+#pragma once
+
+#include <cstdint>
+
+#include "libgpw_amd.h"
+
+namespace gpw {
+
+constexpr int LMAX = GPW_LMAX;            // per centre, after tau / force shifts
+constexpr int LPMAX = 2 * LMAX;           // product angular momentum
+constexpr int NCOSET_LMAX = (LMAX + 1) * (LMAX + 2) * (LMAX + 3) / 6;
+constexpr int BIG = 0x3fffffff;           // "outside" marker for offsets
+
+// Tile of the grid owned by one workgroup in the tile kernels: TX points
+// along x per row, TY x TZ rows, one row per thread.
+constexpr int TX = 32, TY = 16, TZ = 16;
+constexpr int NROW = TY * TZ;             // 256 threads per workgroup
+
+__host__ __device__ constexpr int ncoset(int l) {
+    return l < 0 ? 0 : ((l + 1) * (l + 2) * (l + 3)) / 6;
+}
+__host__ __device__ constexpr int ncoef2(int l) { return ((l + 1) * (l + 2)) / 2; }
+
+// CP2K Cartesian order (reference src/lib/grid_base_ref_c.cc:39-46)
+__host__ __device__ constexpr int coset(int lx, int ly, int lz) {
+    const int l = lx + ly + lz;
+    return l == 0 ? 0 : ncoset(l - 1) + ((l - lx) * (l - lx + 1)) / 2 + lz;
+}
+
+// packed (lzp, lyp, lxp), lzp outermost, lxp fastest, lxp+lyp+lzp <= lp
+// (reference src/lib/grid_base_ref_c.cc:250-257)
+__host__ __device__ constexpr int cidx(int lp, int lzp, int lyp, int lxp) {
+    // sum_{z<lzp} C2(lp-z) = C3(lp) - C3(lp-lzp)
+    const int m = lp - lzp;
+    return ncoset(lp) - ncoset(m) + lyp * (m + 1) - (lyp * (lyp - 1)) / 2 + lxp;
+}
+// packed (lyp, lxp) triangle over lp: lyp outer
+__host__ __device__ constexpr int c2idx(int lp, int lyp, int lxp) {
+    return lyp * (lp + 1) - (lyp * (lyp - 1)) / 2 + lxp;
+}
+
+// Per-pair parameters on the device (one per primitive pair).
+struct PairDev {
+    double zeta, zetb, prefactor, zetp;
+    double ra[3], rb[3], rp[3];
+    int la_max, la_min, lb_max, lb_min;   // as given (before tau shift)
+    int lp;                               // product l of the collocated block
+    int coef_off;                         // offset of coef (doubles)
+    long long block_off;                  // offset of pab/hab block (doubles)
+    int ld;                               // ncoset(la_max): block leading dimension
+    int pad;
+};
+
+// One sub-box of a pair's cube that does not overlap itself under the
+// periodic map. Offsets i along axis d lie in [lb[d], ub[d]]; grid index g
+// maps back to i = lb + mod(g - base, period).
+struct SubDev {
+    double roff[3];   // ortho: displacement X = i*dr - roff
+    double zetp;
+    int base[3];
+    int period[3];
+    int lb[3], ub[3];
+    int lp;
+    int pair;
+    int coef_off;
+    int prof_off;     // offset of the sphere profile (ints), -1: full box
+    int prof_stride;  // row stride of the profile (n_y + 1)
+    int pad;
+};
+
+// Tile-kernel launch description.
+struct TileGrid {
+    int nx, ny, nz;          // local grid dims
+    int ntx, nty, ntz;       // tiles per axis
+    double dr[3];
+};
+
+}  // namespace gpw

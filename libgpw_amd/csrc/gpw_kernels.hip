@@ -1,0 +1,1041 @@
+// gpw_kernels.hip -- HIP kernels for gfx950 (MI355X): collocate / integrate of
+// primitive Gaussian pairs on an orthorhombic periodic (or windowed) grid.
+//
+// Math follows the reference's CP2K path (src/lib/grid_base_ref_c.cc):
+//   value at cube offset (i, j, k), displacement X = i*dx - roff_x etc.:
+//     sum_{lzp,lyp,lxp} coef[lzp][lyp][lxp] Z^lzp Y^lyp X^lxp exp(-zetp (X^2+Y^2+Z^2))
+//   contracted in the order z, y, x (grid_base_ref_c.cc:431-494), restricted to
+//   the sphere encoded by sphere_bounds (:413-470), mapped periodically (:318-351).
+// Integrate is the exact transpose of that contraction.
+//
+// Work decomposition (MI355X-first, not a translation of the reference loops):
+//   * tile kernels: one workgroup owns a TX x TY x TZ tile of the grid, one
+//     grid row (TX points along x) per thread, held in registers; it sweeps
+//     every (sub-box, tile) work item of its tile, staging per-pair 1D tables
+//     and the z-stage coefficients in LDS, chunk of Q pairs per barrier. Each
+//     grid point is written by exactly one workgroup: no atomics, results are
+//     deterministic.
+//   * plain kernels: one launch per sub-box of one pair (the baseline form).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "gpw_internal.hpp"
+
+namespace gpw {
+
+__device__ __forceinline__ int imod(int a, int m) {
+    int r = a % m;
+    return r < 0 ? r + m : r;
+}
+
+// mirror index of an offset: the reference pairs offsets ig <= 0 with 1 - ig
+// (grid_base_ref_c.cc:418-474); ii' = -ig for both members of the pair.
+__device__ __forceinline__ int fold_off(int i) { return i <= 0 ? -i : i - 1; }
+
+// row threshold of the sphere: offsets along x with fold_off(i) <= T are inside
+__device__ __forceinline__ int row_threshold(const int *__restrict__ prof, int prof_off,
+                                             int stride, int kk, int jj) {
+    if (kk < 0 || jj < 0) return -1;
+    if (prof_off < 0) return BIG;
+    return prof[prof_off + kk * stride + jj];
+}
+
+// ---------------------------------------------------------------------------
+// coefficient transform: pab block -> coef_xyz (cab -> cxyz)
+// (grid_base_ref_c.cc:65-315 as the direct triple product)
+
+struct AlphaView {
+    double *a;  // [3][lbm+1][lam+1][lpm+1]
+    int lam, lbm, lpm;
+    __device__ double &at(int d, int lb, int la, int k) const {
+        return a[((d * (lbm + 1) + lb) * (lam + 1) + la) * (lpm + 1) + k];
+    }
+};
+
+__device__ void fill_alpha(const PairDev &p, AlphaView A, int tid, int nthr) {
+    const int n = 3 * (A.lbm + 1) * (A.lam + 1);
+    for (int w = tid; w < n; w += nthr) {
+        const int la = w % (A.lam + 1);
+        const int lb = (w / (A.lam + 1)) % (A.lbm + 1);
+        const int d = w / ((A.lam + 1) * (A.lbm + 1));
+        for (int k = 0; k <= A.lpm; k++) A.at(d, lb, la, k) = 0.0;
+        const double pa = p.rp[d] - p.ra[d];
+        const double pb = p.rp[d] - p.rb[d];
+        double ca = 1.0, powa = 1.0;
+        for (int i = 0; i <= la; i++) {
+            double cb = 1.0, powb = 1.0;
+            for (int j = 0; j <= lb; j++) {
+                A.at(d, lb, la, (la - i) + (lb - j)) += ca * cb * powa * powb;
+                cb *= (double)(lb - j) / (double)(j + 1);
+                powb *= pb;
+            }
+            ca *= (double)(la - i) / (double)(i + 1);
+            powa *= pa;
+        }
+    }
+}
+
+__device__ __forceinline__ void decode_coset(int l, int c, int &lx, int &ly, int &lz) {
+    // inverse of coset() within shell l: c = ((l-lx)(l-lx+1))/2 + lz
+    int m = 0;
+    while (((m + 1) * (m + 2)) / 2 <= c) m++;
+    lx = l - m;
+    lz = c - (m * (m + 1)) / 2;
+    ly = l - lx - lz;
+}
+
+__device__ __forceinline__ void decode_c3(int lp, int c, int &lzp, int &lyp, int &lxp) {
+    lzp = 0;
+    while (c >= ncoef2(lp - lzp)) {
+        c -= ncoef2(lp - lzp);
+        lzp++;
+    }
+    const int m = lp - lzp;
+    lyp = 0;
+    while (c > m - lyp) {
+        c -= m - lyp + 1;
+        lyp++;
+    }
+    lxp = c;
+}
+
+__device__ __forceinline__ void decode_c2(int lp, int c, int &lyp, int &lxp) {
+    lyp = 0;
+    while (c > lp - lyp) {
+        c -= lp - lyp + 1;
+        lyp++;
+    }
+    lxp = c;
+}
+
+// effective block of the local (tau-shifted) ranges, eff[jco*ldE + ico]
+__device__ void build_eff(const PairDev &p, int tau, const double *__restrict__ pab, double *eff,
+                          int ldE, int nlb, int nla, int tid, int nthr) {
+    for (int w = tid; w < nla * nlb; w += nthr) eff[(w / nla) * ldE + (w % nla)] = 0.0;
+    __syncthreads();
+    // one thread per original (a, b) entry; tau scatters to neighbours, which
+    // can collide, so the tau branch is serialised over b inside one thread.
+    const int na0 = ncoset(p.la_max), na_lo = ncoset(p.la_min - 1);
+    const int nb0 = ncoset(p.lb_max), nb_lo = ncoset(p.lb_min - 1);
+    if (!tau) {
+        for (int w = tid; w < (na0 - na_lo) * (nb0 - nb_lo); w += nthr) {
+            const int ico = na_lo + w % (na0 - na_lo);
+            const int jco = nb_lo + w / (na0 - na_lo);
+            eff[jco * ldE + ico] = pab[p.block_off + (long long)jco * p.ld + ico];
+        }
+        __syncthreads();
+        return;
+    }
+    if (tid == 0) {
+        const double za = p.zeta, zb = p.zetb;
+        for (int jco = nb_lo; jco < nb0; jco++) {
+            int lb = 0;
+            while (ncoset(lb) <= jco) lb++;
+            int bx, by, bz;
+            decode_coset(lb, jco - ncoset(lb - 1), bx, by, bz);
+            const int b[3] = {bx, by, bz};
+            for (int ico = na_lo; ico < na0; ico++) {
+                int la = 0;
+                while (ncoset(la) <= ico) la++;
+                int ax, ay, az;
+                decode_coset(la, ico - ncoset(la - 1), ax, ay, az);
+                const int a[3] = {ax, ay, az};
+                const double v = pab[p.block_off + (long long)jco * p.ld + ico];
+                for (int d = 0; d < 3; d++) {
+                    int am[3] = {a[0], a[1], a[2]}, ap[3] = {a[0], a[1], a[2]};
+                    int bm[3] = {b[0], b[1], b[2]}, bp[3] = {b[0], b[1], b[2]};
+                    am[d] = a[d] > 0 ? a[d] - 1 : 0;
+                    bm[d] = b[d] > 0 ? b[d] - 1 : 0;
+                    ap[d] = a[d] + 1;
+                    bp[d] = b[d] + 1;
+                    const int iam = coset(am[0], am[1], am[2]), iap = coset(ap[0], ap[1], ap[2]);
+                    const int jbm = coset(bm[0], bm[1], bm[2]), jbp = coset(bp[0], bp[1], bp[2]);
+                    eff[jbm * ldE + iam] += 0.5 * a[d] * b[d] * v;
+                    eff[jbp * ldE + iam] += 0.5 * (-2.0 * a[d] * zb) * v;
+                    eff[jbm * ldE + iap] += 0.5 * (-2.0 * za * b[d]) * v;
+                    eff[jbp * ldE + iap] += 0.5 * (4.0 * za * zb) * v;
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// one workgroup per pair; dynamic LDS holds alpha and the effective block
+__global__ __launch_bounds__(64) void k_coef(const PairDev *__restrict__ pairs, int pair0,
+                                             int tau, const double *__restrict__ pab,
+                                             double *__restrict__ coef) {
+    extern __shared__ double sm_coef[];
+    const PairDev p = pairs[pair0 + blockIdx.x];
+    const int lam = p.la_max + tau, lbm = p.lb_max + tau;
+    const int lam_lo = max(p.la_min - tau, 0), lbm_lo = max(p.lb_min - tau, 0);
+    const int lp = lam + lbm;
+    AlphaView A{sm_coef, lam, lbm, lp};
+    const int nalpha = 3 * (lbm + 1) * (lam + 1) * (lp + 1);
+    double *eff = sm_coef + nalpha;
+    const int nla = ncoset(lam), nlb = ncoset(lbm);
+    fill_alpha(p, A, threadIdx.x, blockDim.x);
+    build_eff(p, tau, pab, eff, nla, nlb, nla, threadIdx.x, blockDim.x);
+    const int na_lo = ncoset(lam_lo - 1), nb_lo = ncoset(lbm_lo - 1);
+    for (int c = threadIdx.x; c < ncoset(lp); c += blockDim.x) {
+        int lzp, lyp, lxp;
+        decode_c3(lp, c, lzp, lyp, lxp);
+        double acc = 0.0;
+        for (int jco = nb_lo; jco < nlb; jco++) {
+            int lb = 0;
+            while (ncoset(lb) <= jco) lb++;
+            int bx, by, bz;
+            decode_coset(lb, jco - ncoset(lb - 1), bx, by, bz);
+            for (int ico = na_lo; ico < nla; ico++) {
+                const double e = eff[jco * nla + ico];
+                if (e == 0.0) continue;
+                int la = 0;
+                while (ncoset(la) <= ico) la++;
+                int ax, ay, az;
+                decode_coset(la, ico - ncoset(la - 1), ax, ay, az);
+                if (lxp > ax + bx || lyp > ay + by || lzp > az + bz) continue;
+                acc += e * A.at(0, bx, ax, lxp) * A.at(1, by, ay, lyp) * A.at(2, bz, az, lzp);
+            }
+        }
+        coef[p.coef_off + c] = p.prefactor * acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// tile kernels
+
+constexpr int Q = 4;   // pairs per LDS chunk (collocate)
+constexpr int QI = 2;  // pairs per LDS chunk (integrate)
+static_assert(TX + TY + TZ == 64, "table phase maps one wave per pair");
+static_assert(Q * 64 == NROW, "table phase uses the whole workgroup");
+
+struct TileMeta {
+    int lp, xa, xb, prof_off, prof_stride, valid;
+};
+
+// Phase 1: 1D tables of one pair over the tile. Wave w fills pair slot w:
+// lanes 0..TX-1 the x table, then TY y entries, then TZ z entries.
+template <int LM>
+__device__ __forceinline__ void tile_tables(const SubDev &s, int lane, int x0, int y0, int z0,
+                                            const TileGrid &tg, double (*X)[LM],
+                                            double (*Y)[LM], double (*Z)[LM], int *xi, int *yj,
+                                            int *zk, TileMeta &meta) {
+    // per-axis values picked with selects: indexing the struct by a
+    // lane-dependent axis would force it into scratch memory
+    int d, pos, g, nmax, lb, ub, base, period;
+    double dr, roff;
+    double(*T)[LM];
+    int *code;
+    if (lane < TX) {
+        d = 0; pos = lane; g = x0 + lane; nmax = tg.nx; T = X; code = xi;
+        lb = s.lb[0]; ub = s.ub[0]; base = s.base[0]; period = s.period[0];
+        dr = tg.dr[0]; roff = s.roff[0];
+    } else if (lane < TX + TY) {
+        d = 1; pos = lane - TX; g = y0 + pos; nmax = tg.ny; T = Y; code = yj;
+        lb = s.lb[1]; ub = s.ub[1]; base = s.base[1]; period = s.period[1];
+        dr = tg.dr[1]; roff = s.roff[1];
+    } else {
+        d = 2; pos = lane - TX - TY; g = z0 + pos; nmax = tg.nz; T = Z; code = zk;
+        lb = s.lb[2]; ub = s.ub[2]; base = s.base[2]; period = s.period[2];
+        dr = tg.dr[2]; roff = s.roff[2];
+    }
+    int i = BIG;
+    bool ok = g < nmax;
+    if (ok) {
+        i = lb + imod(g - base, period);
+        ok = i <= ub;
+    }
+    const int lp = s.lp;
+    if (ok) {
+        const double r = i * dr - roff;
+        double v = exp(-s.zetp * r * r);
+        for (int k = 0; k <= lp; k++) {
+            T[pos][k] = v;
+            v *= r;
+        }
+        code[pos] = fold_off(i);
+    } else {
+        for (int k = 0; k <= lp; k++) T[pos][k] = 0.0;
+        code[pos] = d == 0 ? BIG : -1;
+    }
+    // x-range of the pair inside the tile (wave-uniform)
+    const unsigned long long m = __ballot(ok && d == 0);
+    if (lane == 0) {
+        meta.lp = lp;
+        meta.prof_off = s.prof_off;
+        meta.prof_stride = s.prof_stride;
+        meta.valid = m != 0ull;
+        meta.xa = m ? __builtin_ctzll(m) : 1;
+        meta.xb = m ? 63 - __builtin_clzll(m) : 0;
+    }
+}
+
+template <int LM>
+struct CollocSmem {
+    double X[Q][TX][LM];
+    double Y[Q][TY][LM];
+    double Z[Q][TZ][LM];
+    double cxy[Q][TZ][LM * (LM + 1) / 2];
+    int xi[Q][TX];
+    int yj[Q][TY];
+    int zk[Q][TZ];
+    TileMeta meta[Q];
+};
+
+template <int L, int LM>
+__device__ __forceinline__ void colloc_row(const CollocSmem<LM> &sm, int q, int ry, int rz, int T,
+                                           double (&acc)[TX]) {
+    constexpr int lp = L - 1;
+    double cx[L];
+#pragma unroll
+    for (int l = 0; l < L; l++) cx[l] = 0.0;
+    const double *cxy = sm.cxy[q][rz];
+    const double *Yr = sm.Y[q][ry];
+#pragma unroll
+    for (int lyp = 0; lyp <= lp; lyp++) {
+        const double y = Yr[lyp];
+#pragma unroll
+        for (int lxp = 0; lxp <= lp - lyp; lxp++) cx[lxp] += cxy[c2idx(lp, lyp, lxp)] * y;
+    }
+    const int xa = __builtin_amdgcn_readfirstlane(sm.meta[q].xa);
+    const int xb = __builtin_amdgcn_readfirstlane(sm.meta[q].xb);
+#pragma unroll
+    for (int x = 0; x < TX; x++) {
+        if (x >= xa && x <= xb) {
+            const double *Xr = sm.X[q][x];
+            double v = 0.0;
+#pragma unroll
+            for (int l = 0; l < L; l++) v += cx[l] * Xr[l];
+            acc[x] += sm.xi[q][x] <= T ? v : 0.0;
+        }
+    }
+}
+
+template <int LM>
+__device__ __forceinline__ void colloc_row_dispatch(const CollocSmem<LM> &sm, int q, int lp,
+                                                    int ry, int rz, int T, double (&acc)[TX]) {
+    switch (lp) {
+    case 0: colloc_row<1, LM>(sm, q, ry, rz, T, acc); break;
+    case 1: colloc_row<2, LM>(sm, q, ry, rz, T, acc); break;
+    case 2: colloc_row<3, LM>(sm, q, ry, rz, T, acc); break;
+    case 3: colloc_row<4, LM>(sm, q, ry, rz, T, acc); break;
+    case 4: colloc_row<5, LM>(sm, q, ry, rz, T, acc); break;
+    case 5: colloc_row<6, LM>(sm, q, ry, rz, T, acc); break;
+    case 6: colloc_row<7, LM>(sm, q, ry, rz, T, acc); break;
+    case 7: colloc_row<8, LM>(sm, q, ry, rz, T, acc); break;
+    default:
+        if constexpr (LM > 8) {
+            switch (lp) {
+            case 8: colloc_row<9, LM>(sm, q, ry, rz, T, acc); break;
+            case 9: colloc_row<10, LM>(sm, q, ry, rz, T, acc); break;
+            case 10: colloc_row<11, LM>(sm, q, ry, rz, T, acc); break;
+            case 11: colloc_row<12, LM>(sm, q, ry, rz, T, acc); break;
+            case 12: colloc_row<13, LM>(sm, q, ry, rz, T, acc); break;
+            case 13: colloc_row<14, LM>(sm, q, ry, rz, T, acc); break;
+            case 14: colloc_row<15, LM>(sm, q, ry, rz, T, acc); break;
+            case 15: colloc_row<16, LM>(sm, q, ry, rz, T, acc); break;
+            default: colloc_row<17, LM>(sm, q, ry, rz, T, acc); break;
+            }
+        }
+        break;
+    }
+}
+
+// out_mode 0: grid[g] += acc (one workgroup per tile); 1: partial[split][g] = acc
+template <int LM>
+__global__ __launch_bounds__(NROW) void k_collocate_tiles(
+    TileGrid tg, const SubDev *__restrict__ subs, const int *__restrict__ tile_start,
+    const int *__restrict__ tile_ent, const double *__restrict__ coef,
+    const int *__restrict__ prof, int splits, double *__restrict__ out, int out_mode) {
+    __shared__ CollocSmem<LM> sm;
+    const int tile = blockIdx.x / splits, split = blockIdx.x % splits;
+    const int tix = tile % tg.ntx, tiy = (tile / tg.ntx) % tg.nty, tiz = tile / (tg.ntx * tg.nty);
+    const int x0 = tix * TX, y0 = tiy * TY, z0 = tiz * TZ;
+    const int t = threadIdx.x, ry = t % TY, rz = t / TY;
+    const int beg = tile_start[tile], end = tile_start[tile + 1];
+    const int per = (end - beg + splits - 1) / splits;
+    const int mybeg = min(end, beg + split * per), myend = min(end, mybeg + per);
+
+    if (out_mode == 0 && mybeg >= myend) return;
+    double acc[TX];
+#pragma unroll
+    for (int x = 0; x < TX; x++) acc[x] = 0.0;
+
+    for (int c = mybeg; c < myend; c += Q) {
+        const int nq = min(Q, myend - c);
+        {
+            const int q = t / 64, lane = t % 64;
+            if (q < nq) {
+                const SubDev s = subs[tile_ent[c + q]];
+                tile_tables<LM>(s, lane, x0, y0, z0, tg, sm.X[q], sm.Y[q], sm.Z[q], sm.xi[q],
+                                sm.yj[q], sm.zk[q], sm.meta[q]);
+            }
+        }
+        __syncthreads();
+        // z-stage: cxy[q][z][(lyp,lxp)] = sum_lzp coef * Z^lzp
+        for (int q = 0; q < nq; q++) {
+            const int lp = sm.meta[q].lp;
+            const int n2 = ncoef2(lp);
+            const int coff = subs[tile_ent[c + q]].coef_off;
+            for (int w = t; w < TZ * n2; w += NROW) {
+                const int z = w / n2, c2 = w % n2;
+                int lyp, lxp;
+                decode_c2(lp, c2, lyp, lxp);
+                double a = 0.0;
+                if (sm.zk[q][z] >= 0) {
+                    for (int lzp = 0; lzp <= lp - lyp - lxp; lzp++)
+                        a += coef[coff + cidx(lp, lzp, lyp, lxp)] * sm.Z[q][z][lzp];
+                }
+                sm.cxy[q][z][c2] = a;
+            }
+        }
+        __syncthreads();
+        for (int q = 0; q < nq; q++) {
+            if (!sm.meta[q].valid) continue;
+            const int T = row_threshold(prof, sm.meta[q].prof_off, sm.meta[q].prof_stride,
+                                        sm.zk[q][rz], sm.yj[q][ry]);
+            if (T >= 0) colloc_row_dispatch<LM>(sm, q, sm.meta[q].lp, ry, rz, T, acc);
+        }
+        __syncthreads();
+    }
+    const int gy = y0 + ry, gz = z0 + rz;
+    if (gy >= tg.ny || gz >= tg.nz) return;
+    const long long row = ((long long)gz * tg.ny + gy) * tg.nx;
+    if (out_mode == 0) {
+#pragma unroll
+        for (int x = 0; x < TX; x++)
+            if (x0 + x < tg.nx) out[row + x0 + x] += acc[x];
+    } else {
+        const long long ng = (long long)tg.nx * tg.ny * tg.nz;
+#pragma unroll
+        for (int x = 0; x < TX; x++)
+            if (x0 + x < tg.nx) out[split * ng + row + x0 + x] = acc[x];
+    }
+}
+
+__global__ void k_reduce_partials(const double *__restrict__ partial, int splits, long long ng,
+                                  double *__restrict__ grid) {
+    for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < ng;
+         g += (long long)gridDim.x * blockDim.x) {
+        double a = 0.0;
+        for (int s = 0; s < splits; s++) a += partial[s * ng + g];
+        grid[g] += a;
+    }
+}
+
+template <int LM>
+struct IntegSmem {
+    double X[QI][TX][LM];
+    double Y[QI][TY][LM];
+    double Z[QI][TZ][LM];
+    double tx[QI][NROW][LM];
+    double txy[QI][TZ][LM * (LM + 1) / 2];
+    int xi[QI][TX];
+    int yj[QI][TY];
+    int zk[QI][TZ];
+    TileMeta meta[QI];
+};
+
+template <int L, int LM>
+__device__ __forceinline__ void integ_row(IntegSmem<LM> &sm, int q, int T, const double (&v)[TX],
+                                          int t) {
+    double a[L];
+#pragma unroll
+    for (int l = 0; l < L; l++) a[l] = 0.0;
+    const int xa = __builtin_amdgcn_readfirstlane(sm.meta[q].xa);
+    const int xb = __builtin_amdgcn_readfirstlane(sm.meta[q].xb);
+#pragma unroll
+    for (int x = 0; x < TX; x++) {
+        if (x >= xa && x <= xb) {
+            const double w = sm.xi[q][x] <= T ? v[x] : 0.0;
+            const double *Xr = sm.X[q][x];
+#pragma unroll
+            for (int l = 0; l < L; l++) a[l] += w * Xr[l];
+        }
+    }
+#pragma unroll
+    for (int l = 0; l < L; l++) sm.tx[q][t][l] = a[l];
+}
+
+template <int LM>
+__device__ __forceinline__ void integ_row_dispatch(IntegSmem<LM> &sm, int q, int lp, int T,
+                                                   const double (&v)[TX], int t) {
+    switch (lp) {
+    case 0: integ_row<1, LM>(sm, q, T, v, t); break;
+    case 1: integ_row<2, LM>(sm, q, T, v, t); break;
+    case 2: integ_row<3, LM>(sm, q, T, v, t); break;
+    case 3: integ_row<4, LM>(sm, q, T, v, t); break;
+    case 4: integ_row<5, LM>(sm, q, T, v, t); break;
+    case 5: integ_row<6, LM>(sm, q, T, v, t); break;
+    case 6: integ_row<7, LM>(sm, q, T, v, t); break;
+    case 7: integ_row<8, LM>(sm, q, T, v, t); break;
+    default:
+        if constexpr (LM > 8) {
+            switch (lp) {
+            case 8: integ_row<9, LM>(sm, q, T, v, t); break;
+            case 9: integ_row<10, LM>(sm, q, T, v, t); break;
+            case 10: integ_row<11, LM>(sm, q, T, v, t); break;
+            case 11: integ_row<12, LM>(sm, q, T, v, t); break;
+            case 12: integ_row<13, LM>(sm, q, T, v, t); break;
+            case 13: integ_row<14, LM>(sm, q, T, v, t); break;
+            case 14: integ_row<15, LM>(sm, q, T, v, t); break;
+            case 15: integ_row<16, LM>(sm, q, T, v, t); break;
+            default: integ_row<17, LM>(sm, q, T, v, t); break;
+            }
+        }
+        break;
+    }
+}
+
+// partial moments of every (sub-box, tile) item: partial[ent_off[e] + c3]
+template <int LM>
+__global__ __launch_bounds__(NROW) void k_integrate_tiles(
+    TileGrid tg, const SubDev *__restrict__ subs, const int *__restrict__ tile_start,
+    const int *__restrict__ tile_ent, const int *__restrict__ prof,
+    const double *__restrict__ grid, int lp_add, const long long *__restrict__ ent_off,
+    int splits, double *__restrict__ partial) {
+    __shared__ IntegSmem<LM> sm;
+    const int tile = blockIdx.x / splits, split = blockIdx.x % splits;
+    const int tix = tile % tg.ntx, tiy = (tile / tg.ntx) % tg.nty, tiz = tile / (tg.ntx * tg.nty);
+    const int x0 = tix * TX, y0 = tiy * TY, z0 = tiz * TZ;
+    const int t = threadIdx.x, ry = t % TY, rz = t / TY;
+    const int beg = tile_start[tile], end = tile_start[tile + 1];
+    const int per = (end - beg + splits - 1) / splits;
+    const int mybeg = min(end, beg + split * per), myend = min(end, mybeg + per);
+    if (mybeg >= myend) return;
+
+    double v[TX];
+    {
+        const int gy = y0 + ry, gz = z0 + rz;
+        const bool rowok = gy < tg.ny && gz < tg.nz;
+        const long long row = ((long long)gz * tg.ny + gy) * tg.nx;
+#pragma unroll
+        for (int x = 0; x < TX; x++) v[x] = (rowok && x0 + x < tg.nx) ? grid[row + x0 + x] : 0.0;
+    }
+
+    for (int c = mybeg; c < myend; c += QI) {
+        const int nq = min(QI, myend - c);
+        {
+            const int q = t / 64, lane = t % 64;
+            if (q < nq) {
+                SubDev s = subs[tile_ent[c + q]];
+                s.lp += lp_add;
+                tile_tables<LM>(s, lane, x0, y0, z0, tg, sm.X[q], sm.Y[q], sm.Z[q], sm.xi[q],
+                                sm.yj[q], sm.zk[q], sm.meta[q]);
+            }
+        }
+        __syncthreads();
+        for (int q = 0; q < nq; q++) {
+            const int lp = sm.meta[q].lp;
+            const int T = sm.meta[q].valid
+                              ? row_threshold(prof, sm.meta[q].prof_off, sm.meta[q].prof_stride,
+                                              sm.zk[q][rz], sm.yj[q][ry])
+                              : -1;
+            if (T >= 0) {
+                integ_row_dispatch<LM>(sm, q, lp, T, v, t);
+            } else {
+                for (int l = 0; l <= lp; l++) sm.tx[q][t][l] = 0.0;
+            }
+        }
+        __syncthreads();
+        // y-stage: txy[q][z][(lyp,lxp)] = sum_y tx[q][(z,y)][lxp] Y[y][lyp]
+        for (int q = 0; q < nq; q++) {
+            const int lp = sm.meta[q].lp, n2 = ncoef2(lp);
+            for (int w = t; w < TZ * n2; w += NROW) {
+                const int z = w / n2, c2 = w % n2;
+                int lyp, lxp;
+                decode_c2(lp, c2, lyp, lxp);
+                double a = 0.0;
+#pragma unroll 4
+                for (int y = 0; y < TY; y++) a += sm.tx[q][z * TY + y][lxp] * sm.Y[q][y][lyp];
+                sm.txy[q][z][c2] = a;
+            }
+        }
+        __syncthreads();
+        // z-stage: M[(lzp,lyp,lxp)] = sum_z txy[q][z][(lyp,lxp)] Z[z][lzp]
+        for (int q = 0; q < nq; q++) {
+            const int lp = sm.meta[q].lp, n3 = ncoset(lp);
+            const long long off = ent_off[c + q];
+            for (int w = t; w < n3; w += NROW) {
+                int lzp, lyp, lxp;
+                decode_c3(lp, w, lzp, lyp, lxp);
+                const int c2 = c2idx(lp, lyp, lxp);
+                double a = 0.0;
+#pragma unroll 4
+                for (int z = 0; z < TZ; z++) a += sm.txy[q][z][c2] * sm.Z[q][z][lzp];
+                partial[off + w] = a;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// moments per pair from the partial buffer, then hab / forces
+
+__global__ void k_fill(double *__restrict__ p, long long n, double v) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Mbuf[pair*mstride + c] = sum over the pair's entries of partial[ent_off[e] + c]
+__global__ void k_sum_entries(const PairDev *__restrict__ pairs, int tau, int lp_add,
+                              const int *__restrict__ pe_start, const int *__restrict__ pe_list,
+                              const long long *__restrict__ ent_off,
+                              const double *__restrict__ partial, int mstride,
+                              double *__restrict__ Mbuf) {
+    const int p = blockIdx.x;
+    const PairDev P = pairs[p];
+    const int lp = P.lp + lp_add;
+    const int n3 = ncoset(lp);
+    const int b = pe_start[p], e = pe_start[p + 1];
+    for (int c = threadIdx.x; c < n3; c += blockDim.x) {
+        double a = 0.0;
+        for (int k = b; k < e; k++) a += partial[ent_off[pe_list[k]] + c];
+        Mbuf[(long long)p * mstride + c] = a;
+    }
+}
+
+// one workgroup per pair: heff = prefactor * alpha^T M over the extended block,
+// then hab (rho copy / tau adjoint) and forces (d phi/dA = 2 zeta phi_{a+1} - a phi_{a-1})
+__global__ __launch_bounds__(64) void k_hab(const PairDev *__restrict__ pairs, int pair0, int tau,
+                                            int with_forces, const double *__restrict__ Mbuf,
+                                            int mstride, const double *__restrict__ pab,
+                                            double *__restrict__ hab, double *__restrict__ forces) {
+    extern __shared__ double sm_hab[];
+    const int pi = pair0 + blockIdx.x;
+    const PairDev p = pairs[pi];
+    const int fx = with_forces ? 1 : 0;
+    const int lam = p.la_max + tau, lbm = p.lb_max + tau;          // local
+    const int lamx = lam + fx, lbmx = lbm + fx;                    // extended
+    const int lamx_lo = max(max(p.la_min - tau, 0) - fx, 0);
+    const int lbmx_lo = max(max(p.lb_min - tau, 0) - fx, 0);
+    const int lpx = lamx + lbmx;
+    AlphaView A{sm_hab, lamx, lbmx, lpx};
+    const int nalpha = 3 * (lbmx + 1) * (lamx + 1) * (lpx + 1);
+    const int nlax = ncoset(lamx), nlbx = ncoset(lbmx);
+    double *heff = sm_hab + nalpha;             // nlbx x nlax
+    double *M = heff + nlax * nlbx;             // ncoset(lpx)
+    double *eff = M + ncoset(lpx);              // ncoset(lbm) x ncoset(lam)
+    double *red = eff + ncoset(lam) * ncoset(lbm);  // 6 x 64
+    fill_alpha(p, A, threadIdx.x, blockDim.x);
+    for (int c = threadIdx.x; c < ncoset(lpx); c += blockDim.x)
+        M[c] = Mbuf[(long long)pi * mstride + c];
+    __syncthreads();
+    const int na_lo = ncoset(lamx_lo - 1), nb_lo = ncoset(lbmx_lo - 1);
+    for (int w = threadIdx.x; w < nlax * nlbx; w += blockDim.x) {
+        const int ico = w % nlax, jco = w / nlax;
+        double acc = 0.0;
+        if (ico >= na_lo && jco >= nb_lo) {
+            int la = 0, lb = 0;
+            while (ncoset(la) <= ico) la++;
+            while (ncoset(lb) <= jco) lb++;
+            int ax, ay, az, bx, by, bz;
+            decode_coset(la, ico - ncoset(la - 1), ax, ay, az);
+            decode_coset(lb, jco - ncoset(lb - 1), bx, by, bz);
+            for (int lzp = 0; lzp <= az + bz; lzp++) {
+                const double cz = A.at(2, bz, az, lzp);
+                for (int lyp = 0; lyp <= ay + by; lyp++) {
+                    const double cyz = cz * A.at(1, by, ay, lyp);
+                    for (int lxp = 0; lxp <= ax + bx; lxp++)
+                        acc += M[cidx(lpx, lzp, lyp, lxp)] * cyz * A.at(0, bx, ax, lxp);
+                }
+            }
+        }
+        heff[jco * nlax + ico] = p.prefactor * acc;
+    }
+    __syncthreads();
+    // hab: loop over the original block entries
+    const int na0 = ncoset(p.la_max), na_lo0 = ncoset(p.la_min - 1);
+    const int nb0 = ncoset(p.lb_max), nb_lo0 = ncoset(p.lb_min - 1);
+    const int nA = na0 - na_lo0, nB = nb0 - nb_lo0;
+    for (int w = threadIdx.x; w < nA * nB; w += blockDim.x) {
+        const int ico = na_lo0 + w % nA, jco = nb_lo0 + w / nA;
+        double acc;
+        if (!tau) {
+            acc = heff[jco * nlax + ico];
+        } else {
+            int la = 0, lb = 0;
+            while (ncoset(la) <= ico) la++;
+            while (ncoset(lb) <= jco) lb++;
+            int a[3], b[3];
+            decode_coset(la, ico - ncoset(la - 1), a[0], a[1], a[2]);
+            decode_coset(lb, jco - ncoset(lb - 1), b[0], b[1], b[2]);
+            acc = 0.0;
+            for (int d = 0; d < 3; d++) {
+                int am[3] = {a[0], a[1], a[2]}, ap[3] = {a[0], a[1], a[2]};
+                int bm[3] = {b[0], b[1], b[2]}, bp[3] = {b[0], b[1], b[2]};
+                am[d] = a[d] > 0 ? a[d] - 1 : 0;
+                bm[d] = b[d] > 0 ? b[d] - 1 : 0;
+                ap[d] = a[d] + 1;
+                bp[d] = b[d] + 1;
+                const int iam = coset(am[0], am[1], am[2]), iap = coset(ap[0], ap[1], ap[2]);
+                const int jbm = coset(bm[0], bm[1], bm[2]), jbp = coset(bp[0], bp[1], bp[2]);
+                acc += 0.5 * a[d] * b[d] * heff[jbm * nlax + iam];
+                acc += 0.5 * (-2.0 * a[d] * p.zetb) * heff[jbp * nlax + iam];
+                acc += 0.5 * (-2.0 * p.zeta * b[d]) * heff[jbm * nlax + iap];
+                acc += 0.5 * (4.0 * p.zeta * p.zetb) * heff[jbp * nlax + iap];
+            }
+        }
+        hab[p.block_off + (long long)jco * p.ld + ico] += acc;
+    }
+    if (!with_forces) return;
+    const int nla = ncoset(lam), nlb = ncoset(lbm);
+    build_eff(p, tau, pab, eff, nla, nlb, nla, threadIdx.x, blockDim.x);
+    double f[6] = {0, 0, 0, 0, 0, 0};
+    const int la_lo = ncoset(max(p.la_min - tau, 0) - 1), lb_lo = ncoset(max(p.lb_min - tau, 0) - 1);
+    for (int w = threadIdx.x; w < nla * nlb; w += blockDim.x) {
+        const int ico = w % nla, jco = w / nla;
+        if (ico < la_lo || jco < lb_lo) continue;
+        const double e = eff[jco * nla + ico];
+        if (e == 0.0) continue;
+        int la = 0, lb = 0;
+        while (ncoset(la) <= ico) la++;
+        while (ncoset(lb) <= jco) lb++;
+        int a[3], b[3];
+        decode_coset(la, ico - ncoset(la - 1), a[0], a[1], a[2]);
+        decode_coset(lb, jco - ncoset(lb - 1), b[0], b[1], b[2]);
+        for (int d = 0; d < 3; d++) {
+            int ap[3] = {a[0], a[1], a[2]}, am[3] = {a[0], a[1], a[2]};
+            int bp[3] = {b[0], b[1], b[2]}, bm[3] = {b[0], b[1], b[2]};
+            ap[d]++;
+            bp[d]++;
+            double ga = 2.0 * p.zeta * heff[jco * nlax + coset(ap[0], ap[1], ap[2])];
+            double gb = 2.0 * p.zetb * heff[coset(bp[0], bp[1], bp[2]) * nlax + ico];
+            if (a[d] > 0) {
+                am[d]--;
+                ga -= a[d] * heff[jco * nlax + coset(am[0], am[1], am[2])];
+            }
+            if (b[d] > 0) {
+                bm[d]--;
+                gb -= b[d] * heff[coset(bm[0], bm[1], bm[2]) * nlax + ico];
+            }
+            f[d] += e * ga;
+            f[3 + d] += e * gb;
+        }
+    }
+    for (int k = 0; k < 6; k++) red[k * 64 + threadIdx.x] = f[k];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        double s = 0.0;
+        for (int i = 0; i < 64; i++) s += red[threadIdx.x * 64 + i];
+        forces[(long long)pi * 6 + threadIdx.x] += s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// plain kernels: one launch per sub-box. Workgroup = 8 rows x 32 lanes over
+// one z-plane of the sub-box; blockIdx.x -> z offset, blockIdx.y -> row chunk.
+
+constexpr int PR = 8, PL = 32;
+
+__global__ __launch_bounds__(PR *PL) void k_collocate_plain(TileGrid tg, const SubDev *__restrict__ subs,
+                                                            int si, const double *__restrict__ coef,
+                                                            const int *__restrict__ prof,
+                                                            double *__restrict__ grid) {
+    __shared__ double cxy[ncoef2(LPMAX)];
+    __shared__ double zt[LPMAX + 1];
+    const SubDev s = subs[si];
+    const int lp = s.lp;
+    const int k = s.lb[2] + blockIdx.x;
+    const int t = threadIdx.x, r = t / PL, lane = t % PL;
+    if (t == 0) {
+        const double rz = k * tg.dr[2] - s.roff[2];
+        double v = exp(-s.zetp * rz * rz);
+        for (int l = 0; l <= lp; l++) {
+            zt[l] = v;
+            v *= rz;
+        }
+    }
+    __syncthreads();
+    for (int c2 = t; c2 < ncoef2(lp); c2 += blockDim.x) {
+        int lyp, lxp;
+        decode_c2(lp, c2, lyp, lxp);
+        double a = 0.0;
+        for (int lzp = 0; lzp <= lp - lyp - lxp; lzp++) a += coef[s.coef_off + cidx(lp, lzp, lyp, lxp)] * zt[lzp];
+        cxy[c2] = a;
+    }
+    __syncthreads();
+    const int j = s.lb[1] + blockIdx.y * PR + r;
+    if (j > s.ub[1]) return;
+    const int T = row_threshold(prof, s.prof_off, s.prof_stride, fold_off(k), fold_off(j));
+    if (T < 0) return;
+    double cx[LPMAX + 1];
+    const double ry = j * tg.dr[1] - s.roff[1];
+    const double ey = exp(-s.zetp * ry * ry);
+    for (int l = 0; l <= lp; l++) cx[l] = 0.0;
+    double yp = ey;
+    for (int lyp = 0; lyp <= lp; lyp++) {
+        for (int lxp = 0; lxp <= lp - lyp; lxp++) cx[lxp] += cxy[c2idx(lp, lyp, lxp)] * yp;
+        yp *= ry;
+    }
+    const int gz = imod(s.base[2] + (k - s.lb[2]), s.period[2]);
+    const int gy = imod(s.base[1] + (j - s.lb[1]), s.period[1]);
+    double *row = grid + ((long long)gz * tg.ny + gy) * tg.nx;
+    for (int i = s.lb[0] + lane; i <= s.ub[0]; i += PL) {
+        if (fold_off(i) > T) continue;
+        const double rx = i * tg.dr[0] - s.roff[0];
+        double xp = exp(-s.zetp * rx * rx), v = 0.0;
+        for (int l = 0; l <= lp; l++) {
+            v += cx[l] * xp;
+            xp *= rx;
+        }
+        row[imod(s.base[0] + (i - s.lb[0]), s.period[0])] += v;
+    }
+}
+
+// moments of one sub-box, atomically accumulated into Mbuf[pair]
+__global__ __launch_bounds__(PR *PL) void k_moments_plain(TileGrid tg, const SubDev *__restrict__ subs,
+                                                          int si, int lp_add,
+                                                          const int *__restrict__ prof,
+                                                          const double *__restrict__ grid,
+                                                          int mstride, double *__restrict__ Mbuf) {
+    __shared__ double tyl[PR][ncoef2(LPMAX)];
+    const SubDev s = subs[si];
+    const int lp = s.lp + lp_add;
+    const int n2 = ncoef2(lp);
+    const int k = s.lb[2] + blockIdx.x;
+    const int t = threadIdx.x, r = t / PL, lane = t % PL;
+    const int j = s.lb[1] + blockIdx.y * PR + r;
+    double tx[LPMAX + 1];
+    for (int l = 0; l <= lp; l++) tx[l] = 0.0;
+    double ry = 0.0;
+    const int T = j <= s.ub[1] ? row_threshold(prof, s.prof_off, s.prof_stride, fold_off(k), fold_off(j))
+                               : -1;
+    if (T >= 0) {
+        ry = j * tg.dr[1] - s.roff[1];
+        const int gz = imod(s.base[2] + (k - s.lb[2]), s.period[2]);
+        const int gy = imod(s.base[1] + (j - s.lb[1]), s.period[1]);
+        const double *row = grid + ((long long)gz * tg.ny + gy) * tg.nx;
+        for (int i = s.lb[0] + lane; i <= s.ub[0]; i += PL) {
+            if (fold_off(i) > T) continue;
+            const double rx = i * tg.dr[0] - s.roff[0];
+            double xp = exp(-s.zetp * rx * rx) * row[imod(s.base[0] + (i - s.lb[0]), s.period[0])];
+            for (int l = 0; l <= lp; l++) {
+                tx[l] += xp;
+                xp *= rx;
+            }
+        }
+    }
+    // reduce tx over the 32 lanes of the row
+    for (int l = 0; l <= lp; l++) {
+        double v = tx[l];
+        for (int o = PL / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, PL);
+        tx[l] = v;
+    }
+    if (lane == 0) {
+        const double ey = T >= 0 ? exp(-s.zetp * ry * ry) : 0.0;
+        double yp = ey;
+        for (int lyp = 0; lyp <= lp; lyp++) {
+            for (int lxp = 0; lxp <= lp - lyp; lxp++) tyl[r][c2idx(lp, lyp, lxp)] = tx[lxp] * yp;
+            yp *= ry;
+        }
+    }
+    __syncthreads();
+    const double rz = k * tg.dr[2] - s.roff[2];
+    const double ez = exp(-s.zetp * rz * rz);
+    for (int c = t; c < ncoset(lp); c += blockDim.x) {
+        int lzp, lyp, lxp;
+        decode_c3(lp, c, lzp, lyp, lxp);
+        double zp = ez;
+        for (int q = 0; q < lzp; q++) zp *= rz;
+        const int c2 = c2idx(lp, lyp, lxp);
+        double a = 0.0;
+        for (int rr = 0; rr < PR; rr++) a += tyl[rr][c2];
+        atomicAdd(&Mbuf[(long long)s.pair * mstride + c], a * zp);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// libgpw tensor-path contractions (reference src/lib/collocate.cc:438-623)
+
+// one thread per (z, y) row: xyz[a] = sum_{g,b} co[a][g][b] Pz[g][z] Py[b][y];
+// V[z][y][x] = sum_a xyz[a] Px[a][x]
+__global__ void k_collocate_core(int n0, int n1, int n2, int L, const double *__restrict__ co,
+                                 const double *__restrict__ P, int ldp, double *__restrict__ V) {
+    const long long rowid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (rowid >= (long long)n0 * n1) return;
+    const int z = rowid / n1, y = rowid % n1;
+    const double *Pz = P, *Py = P + (long long)L * ldp, *Px = P + 2LL * L * ldp;
+    double xyz[LPMAX + 1];
+    for (int a = 0; a < L; a++) {
+        double s = 0.0;
+        for (int g = 0; g < L; g++) {
+            const double pz = Pz[(long long)g * ldp + z];
+            for (int b = 0; b < L; b++) s += co[((long long)a * L + g) * L + b] * pz * Py[(long long)b * ldp + y];
+        }
+        xyz[a] = s;
+    }
+    double *row = V + rowid * n2;
+    for (int x = 0; x < n2; x++) {
+        double v = 0.0;
+        for (int a = 0; a < L; a++) v += xyz[a] * Px[(long long)a * ldp + x];
+        row[x] = v;
+    }
+}
+
+// one workgroup per z plane: Velems[g][b][a] += sum_y sum_x ...
+__global__ __launch_bounds__(256) void k_integrate_core(int n0, int n1, int n2, int L,
+                                                        const double *__restrict__ P, int ldp,
+                                                        const double *__restrict__ V,
+                                                        double *__restrict__ Velems) {
+    extern __shared__ double sm_ic[];  // [L][L] partial (b, a)
+    const int z = blockIdx.x;
+    const double *Pz = P, *Py = P + (long long)L * ldp, *Px = P + 2LL * L * ldp;
+    for (int w = threadIdx.x; w < L * L; w += blockDim.x) sm_ic[w] = 0.0;
+    __syncthreads();
+    for (int y = threadIdx.x; y < n1; y += blockDim.x) {
+        const double *row = V + ((long long)z * n1 + y) * n2;
+        double t[LPMAX + 1];
+        for (int a = 0; a < L; a++) t[a] = 0.0;
+        for (int x = 0; x < n2; x++) {
+            const double v = row[x];
+            for (int a = 0; a < L; a++) t[a] += v * Px[(long long)a * ldp + x];
+        }
+        for (int b = 0; b < L; b++) {
+            const double py = Py[(long long)b * ldp + y];
+            for (int a = 0; a < L; a++) atomicAdd(&sm_ic[b * L + a], t[a] * py);
+        }
+    }
+    __syncthreads();
+    for (int w = threadIdx.x; w < L * L * L; w += blockDim.x) {
+        const int g = w / (L * L), ba = w % (L * L);
+        atomicAdd(&Velems[w], sm_ic[ba] * Pz[(long long)g * ldp + z]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers (host side)
+
+template <int LM>
+static hipError_t launch_collocate_tiles_lm(const TileGrid &tg, const SubDev *subs,
+                                            const int *tile_start, const int *tile_ent,
+                                            const double *coef, const int *prof, int ntiles,
+                                            int splits, double *out, int out_mode,
+                                            hipStream_t st) {
+    hipLaunchKernelGGL((k_collocate_tiles<LM>), dim3(ntiles * splits), dim3(NROW), 0, st, tg,
+                       subs, tile_start, tile_ent, coef, prof, splits, out, out_mode);
+    return hipGetLastError();
+}
+
+hipError_t launch_collocate_tiles(int lp_max, const TileGrid &tg, const SubDev *subs,
+                                  const int *tile_start, const int *tile_ent, const double *coef,
+                                  const int *prof, int ntiles, int splits, double *out,
+                                  int out_mode, hipStream_t st) {
+    if (lp_max < 8)
+        return launch_collocate_tiles_lm<8>(tg, subs, tile_start, tile_ent, coef, prof, ntiles,
+                                            splits, out, out_mode, st);
+    return launch_collocate_tiles_lm<17>(tg, subs, tile_start, tile_ent, coef, prof, ntiles, splits,
+                                         out, out_mode, st);
+}
+
+hipError_t launch_reduce_partials(const double *partial, int splits, long long ng, double *grid,
+                                  hipStream_t st) {
+    const int blocks = (int)std::min<long long>((ng + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(blocks), dim3(256), 0, st, partial, splits, ng, grid);
+    return hipGetLastError();
+}
+
+template <int LM>
+static hipError_t launch_integrate_tiles_lm(const TileGrid &tg, const SubDev *subs,
+                                            const int *tile_start, const int *tile_ent,
+                                            const int *prof, const double *grid, int lp_add,
+                                            const long long *ent_off, int ntiles, int splits,
+                                            double *partial, hipStream_t st) {
+    hipLaunchKernelGGL((k_integrate_tiles<LM>), dim3(ntiles * splits), dim3(NROW), 0, st, tg,
+                       subs, tile_start, tile_ent, prof, grid, lp_add, ent_off, splits, partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_integrate_tiles(int lp_max, const TileGrid &tg, const SubDev *subs,
+                                  const int *tile_start, const int *tile_ent, const int *prof,
+                                  const double *grid, int lp_add, const long long *ent_off,
+                                  int ntiles, int splits, double *partial, hipStream_t st) {
+    if (lp_max + lp_add < 8)
+        return launch_integrate_tiles_lm<8>(tg, subs, tile_start, tile_ent, prof, grid, lp_add,
+                                            ent_off, ntiles, splits, partial, st);
+    return launch_integrate_tiles_lm<17>(tg, subs, tile_start, tile_ent, prof, grid, lp_add, ent_off,
+                                         ntiles, splits, partial, st);
+}
+
+size_t coef_smem_bytes(int lam, int lbm) {
+    const int lp = lam + lbm;
+    return sizeof(double) * (size_t)(3 * (lbm + 1) * (lam + 1) * (lp + 1) + ncoset(lam) * ncoset(lbm));
+}
+
+hipError_t launch_coef(const PairDev *pairs, int pair0, int npairs, int tau, int lam_max,
+                       int lbm_max, const double *pab, double *coef, hipStream_t st) {
+    const size_t smem = coef_smem_bytes(lam_max + tau, lbm_max + tau);
+    hipLaunchKernelGGL(k_coef, dim3(npairs), dim3(64), smem, st, pairs, pair0, tau, pab, coef);
+    return hipGetLastError();
+}
+
+size_t hab_smem_bytes(int lamx, int lbmx, int lam, int lbm) {
+    const int lpx = lamx + lbmx;
+    return sizeof(double) * (size_t)(3 * (lbmx + 1) * (lamx + 1) * (lpx + 1) + ncoset(lamx) * ncoset(lbmx) +
+                                     ncoset(lpx) + ncoset(lam) * ncoset(lbm) + 6 * 64);
+}
+
+hipError_t launch_hab(const PairDev *pairs, int pair0, int npairs, int tau, int with_forces,
+                      int la_max, int lb_max, const double *Mbuf, int mstride, const double *pab,
+                      double *hab, double *forces, hipStream_t st) {
+    const int fx = with_forces ? 1 : 0;
+    const size_t smem = hab_smem_bytes(la_max + tau + fx, lb_max + tau + fx, la_max + tau, lb_max + tau);
+    hipLaunchKernelGGL(k_hab, dim3(npairs), dim3(64), smem, st, pairs, pair0, tau, with_forces,
+                       Mbuf, mstride, pab, hab, forces);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_entries(const PairDev *pairs, int npairs, int tau, int lp_add,
+                              const int *pe_start, const int *pe_list, const long long *ent_off,
+                              const double *partial, int mstride, double *Mbuf, hipStream_t st) {
+    hipLaunchKernelGGL(k_sum_entries, dim3(npairs), dim3(64), 0, st, pairs, tau, lp_add, pe_start,
+                       pe_list, ent_off, partial, mstride, Mbuf);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(double *p, long long n, double v, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const int blocks = (int)std::min<long long>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill, dim3(blocks), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_collocate_plain(const TileGrid &tg, const SubDev *subs, int si, int nz_box,
+                                  int ny_box, const double *coef, const int *prof, double *grid,
+                                  hipStream_t st) {
+    hipLaunchKernelGGL(k_collocate_plain, dim3(nz_box, (ny_box + PR - 1) / PR), dim3(PR * PL), 0, st,
+                       tg, subs, si, coef, prof, grid);
+    return hipGetLastError();
+}
+
+hipError_t launch_moments_plain(const TileGrid &tg, const SubDev *subs, int si, int nz_box,
+                                int ny_box, int lp_add, const int *prof, const double *grid,
+                                int mstride, double *Mbuf, hipStream_t st) {
+    hipLaunchKernelGGL(k_moments_plain, dim3(nz_box, (ny_box + PR - 1) / PR), dim3(PR * PL), 0, st,
+                       tg, subs, si, lp_add, prof, grid, mstride, Mbuf);
+    return hipGetLastError();
+}
+
+hipError_t launch_collocate_core(const int *length, int L, const double *co, const double *P,
+                                 int ldp, double *V, hipStream_t st) {
+    const long long rows = (long long)length[0] * length[1];
+    const int blocks = (int)((rows + 127) / 128);
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_collocate_core, dim3(blocks), dim3(128), 0, st, length[0], length[1],
+                       length[2], L, co, P, ldp, V);
+    return hipGetLastError();
+}
+
+hipError_t launch_integrate_core(const int *length, int L, const double *P, int ldp,
+                                 const double *V, double *Velems, hipStream_t st) {
+    hipError_t e = launch_fill(Velems, (long long)L * L * L, 0.0, st);
+    if (e != hipSuccess || length[0] == 0) return e;
+    hipLaunchKernelGGL(k_integrate_core, dim3(length[0]), dim3(256), sizeof(double) * L * L, st,
+                       length[0], length[1], length[2], L, P, ldp, V, Velems);
+    return hipGetLastError();
+}
+
+}  // namespace gpw

@@ -1,0 +1,916 @@
+// gpw_plan.cpp -- host side of the library: per-pair geometry, sub-box split,
+// tile work lists, device buffers, and the C ABI of include/libgpw_amd.h.
+//
+// Geometry is computed here in FP64 with contraction disabled (-ffp-contract=off)
+// so that cube centres and sphere bounds are bit-identical to a straightforward
+// CPU evaluation of the reference formulas:
+//   pair set-up ....... src/lib/grid_base_ref_c.cc:878-886
+//   cube centre ....... src/lib/grid_base_ref_c.cc:534-547
+//   periodic map ...... src/lib/grid_base_ref_c.cc:318-351
+//   sphere bounds ..... consumed as at src/lib/grid_base_ref_c.cc:413-470; for
+//                       task lists they are generated from the pair radius by
+//                       the rule of CP2K's cube_utils (radius rounded up to a
+//                       multiple of the smallest spacing), which reproduces the
+//                       two orthorhombic fixtures of the reference exactly.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "gpw_internal.hpp"
+
+namespace gpw {
+hipError_t launch_collocate_tiles(int lp_max, const TileGrid &tg, const SubDev *subs,
+                                  const int *tile_start, const int *tile_ent, const double *coef,
+                                  const int *prof, int ntiles, int splits, double *out,
+                                  int out_mode, hipStream_t st);
+hipError_t launch_reduce_partials(const double *partial, int splits, long long ng, double *grid,
+                                  hipStream_t st);
+hipError_t launch_integrate_tiles(int lp_max, const TileGrid &tg, const SubDev *subs,
+                                  const int *tile_start, const int *tile_ent, const int *prof,
+                                  const double *grid, int lp_add, const long long *ent_off,
+                                  int ntiles, int splits, double *partial, hipStream_t st);
+hipError_t launch_coef(const PairDev *pairs, int pair0, int npairs, int tau, int lam_max,
+                       int lbm_max, const double *pab, double *coef, hipStream_t st);
+hipError_t launch_hab(const PairDev *pairs, int pair0, int npairs, int tau, int with_forces,
+                      int la_max, int lb_max, const double *Mbuf, int mstride, const double *pab,
+                      double *hab, double *forces, hipStream_t st);
+hipError_t launch_sum_entries(const PairDev *pairs, int npairs, int tau, int lp_add,
+                              const int *pe_start, const int *pe_list, const long long *ent_off,
+                              const double *partial, int mstride, double *Mbuf, hipStream_t st);
+hipError_t launch_fill(double *p, long long n, double v, hipStream_t st);
+hipError_t launch_collocate_plain(const TileGrid &tg, const SubDev *subs, int si, int nz_box,
+                                  int ny_box, const double *coef, const int *prof, double *grid,
+                                  hipStream_t st);
+hipError_t launch_moments_plain(const TileGrid &tg, const SubDev *subs, int si, int nz_box,
+                                int ny_box, int lp_add, const int *prof, const double *grid,
+                                int mstride, double *Mbuf, hipStream_t st);
+hipError_t launch_collocate_core(const int *length, int L, const double *co, const double *P,
+                                 int ldp, double *V, hipStream_t st);
+hipError_t launch_integrate_core(const int *length, int L, const double *P, int ldp,
+                                 const double *V, double *Velems, hipStream_t st);
+}  // namespace gpw
+
+using namespace gpw;
+
+static thread_local std::string g_last_error;
+
+static gpw_status fail(gpw_status code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t _e = (expr);                                                            \
+        if (_e != hipSuccess)                                                              \
+            return fail(GPW_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));   \
+    } while (0)
+
+static int imod_h(int a, int m) { return (a % m + m) % m; }
+
+// ---------------------------------------------------------------------------
+// sphere profiles: T[kk][jj] = -igmin (>= 0) or -1 for an empty row
+
+struct Profile {
+    int n[3];              // table covers fold offsets 0..n[d]
+    std::vector<int> T;    // (n[2]+1) x (n[1]+1)
+};
+
+static int sphere_extent(double base, double h, double R2) {
+    int n = 0;
+    for (;;) {
+        const double x = (double)(n + 1) * h;
+        if (base + x * x <= R2)
+            n++;
+        else
+            return n;
+    }
+}
+
+// Profile for a radius on an orthorhombic grid with spacings dr (the rule
+// that regenerates the reference's fixture sphere_bounds).
+static void profile_from_radius(double radius, const double dr[3], Profile &P, int lb[3],
+                                int ub[3]) {
+    double drmin = std::min(dr[0], std::min(dr[1], dr[2]));
+    int imr = (int)std::ceil(radius / drmin);
+    if (imr < 1) imr = 1;
+    const double R = imr * drmin;
+    const double R2 = R * R;
+    for (int d = 0; d < 3; d++) {
+        const int n = sphere_extent(0.0, dr[d], R2);
+        P.n[d] = n;
+        lb[d] = -n;
+        ub[d] = 1 + n;
+    }
+    P.T.assign((size_t)(P.n[2] + 1) * (P.n[1] + 1), -1);
+    for (int kk = 0; kk <= P.n[2]; kk++) {
+        const double zk = -kk * dr[2];
+        const double base = zk * zk;
+        const int jn = sphere_extent(base, dr[1], R2);
+        for (int jj = 0; jj <= jn; jj++) {
+            const double yj = -jj * dr[1];
+            const double b2 = base + yj * yj;
+            P.T[(size_t)kk * (P.n[1] + 1) + jj] = sphere_extent(b2, dr[0], R2);
+        }
+    }
+}
+
+// Profile from a caller's sphere_bounds (grid_base_ref_c.cc:413-470 walk).
+static gpw_status profile_from_bounds(const int lb[3], const int ub[3], int nspheres,
+                                      const int *sb, Profile &P) {
+    if (nspheres < 1 || !sb) return fail(GPW_ERR_SPHERE, "sphere_bounds empty");
+    for (int d = 0; d < 3; d++) P.n[d] = std::max(std::max(-lb[d], ub[d] - 1), 0);
+    P.T.assign((size_t)(P.n[2] + 1) * (P.n[1] + 1), -1);
+    int sci = 0;
+    const int kgmin = sb[sci++];
+    if (kgmin > 0 || kgmin < lb[2] || 1 - kgmin > ub[2])
+        return fail(GPW_ERR_SPHERE, "sphere_bounds: kgmin outside the cube");
+    for (int kg = kgmin; kg <= 0; kg++) {
+        if (sci >= nspheres) return fail(GPW_ERR_SPHERE, "sphere_bounds too short");
+        const int jgmin = sb[sci++];
+        if (jgmin > 0 || jgmin < lb[1] || 1 - jgmin > ub[1])
+            return fail(GPW_ERR_SPHERE, "sphere_bounds: jgmin outside the cube");
+        for (int jg = jgmin; jg <= 0; jg++) {
+            if (sci >= nspheres) return fail(GPW_ERR_SPHERE, "sphere_bounds too short");
+            const int igmin = sb[sci++];
+            if (igmin > 0 || igmin < lb[0] || 1 - igmin > ub[0])
+                return fail(GPW_ERR_SPHERE, "sphere_bounds: igmin outside the cube");
+            P.T[(size_t)(-kg) * (P.n[1] + 1) + (-jg)] = -igmin;
+        }
+    }
+    return GPW_OK;
+}
+
+// ---------------------------------------------------------------------------
+
+struct PairGeom {
+    int cc[3];
+    double roff[3];
+    int lb[3], ub[3];
+    int prof;  // index into profile list, -1 none
+};
+
+struct gpw_tasklist {
+    gpw_grid_desc grid{};
+    int compute_tau = 0;
+    int npairs = 0;
+    int lp_max = 0;
+    int la_max = 0, lb_max = 0;  // largest given la_max / lb_max
+    TileGrid tg{};
+    int ntiles = 0, splits = 1;
+    long long nentries = 0;
+    long long cube_points = 0, tile_points = 0;
+
+    std::vector<PairDev> pairs_h;
+    std::vector<SubDev> subs_h;
+    std::vector<int> pair_sub_start;  // CSR pair -> subs (subs are contiguous per pair)
+
+    PairDev *d_pairs = nullptr;
+    SubDev *d_subs = nullptr;
+    int *d_prof = nullptr;
+    int *d_tile_start = nullptr, *d_tile_ent = nullptr;
+    int *d_pe_start = nullptr, *d_pe_list = nullptr;
+    long long *d_ent_off0 = nullptr, *d_ent_off2 = nullptr;
+    double *d_coef = nullptr;
+    double *d_partial = nullptr;   // integrate partial moments
+    double *d_cpartial = nullptr;  // collocate partial grids (splits > 1)
+    double *d_Mbuf = nullptr;
+    int mstride = 0;
+
+    ~gpw_tasklist() {
+        void *ptrs[] = {d_pairs, d_subs, d_prof, d_tile_start, d_tile_ent, d_pe_start, d_pe_list,
+                        d_ent_off0, d_ent_off2, d_coef, d_partial, d_cpartial, d_Mbuf};
+        for (void *p : ptrs)
+            if (p) (void)hipFree(p);
+    }
+};
+
+template <class T>
+static gpw_status upload(T **dst, const std::vector<T> &src) {
+    const size_t n = std::max<size_t>(src.size(), 1);
+    HIP_TRY(hipMalloc((void **)dst, n * sizeof(T)));
+    if (!src.empty()) HIP_TRY(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    return GPW_OK;
+}
+
+static bool is_ortho(const gpw_grid_desc &g) {
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            if (i != j && g.dh[i][j] != 0.0) return false;
+    return true;
+}
+
+static gpw_status check_pair(const gpw_pair_desc &p, int tau) {
+    if (p.la_min < 0 || p.lb_min < 0 || p.la_max < p.la_min || p.lb_max < p.lb_min)
+        return fail(GPW_ERR_ARG, "bad angular momentum range");
+    // +1 headroom for the force extension
+    if (p.la_max + tau + 1 > LMAX || p.lb_max + tau + 1 > LMAX)
+        return fail(GPW_ERR_ARG, "angular momentum above GPW_LMAX");
+    if (!(p.zeta > 0.0) || !(p.zetb > 0.0)) return fail(GPW_ERR_ARG, "exponents must be > 0");
+    return GPW_OK;
+}
+
+// FP64 pair set-up exactly as grid_collocate_internal (grid_base_ref_c.cc:878-886)
+static PairDev make_pair_dev(const gpw_pair_desc &p, int tau) {
+    PairDev d{};
+    d.zeta = p.zeta;
+    d.zetb = p.zetb;
+    d.zetp = p.zeta + p.zetb;
+    const double f = p.zetb / d.zetp;
+    const double rab2 = p.rab[0] * p.rab[0] + p.rab[1] * p.rab[1] + p.rab[2] * p.rab[2];
+    d.prefactor = p.rscale * std::exp(-p.zeta * f * rab2);
+    for (int i = 0; i < 3; i++) {
+        d.ra[i] = p.ra[i];
+        d.rp[i] = p.ra[i] + f * p.rab[i];
+        d.rb[i] = p.ra[i] + p.rab[i];
+    }
+    d.la_max = p.la_max;
+    d.la_min = p.la_min;
+    d.lb_max = p.lb_max;
+    d.lb_min = p.lb_min;
+    d.lp = p.la_max + p.lb_max + 2 * tau;
+    d.block_off = p.block_offset;
+    d.ld = ncoset(p.la_max);
+    return d;
+}
+
+// cube centre and offset (grid_base_ref_c.cc:534-547)
+static void cube_centre(const gpw_grid_desc &g, const double rp[3], int cc[3], double roff[3]) {
+    for (int i = 0; i < 3; i++) {
+        double v = 0.0;
+        for (int j = 0; j < 3; j++) v += g.dh_inv[j][i] * rp[j];
+        cc[i] = (int)std::floor(v);
+        roff[i] = rp[i] - ((double)cc[i]) * g.dh[i][i];
+    }
+}
+
+// Split the pair's cube into sub-boxes that fit the period, append SubDevs.
+static gpw_status make_subs(const gpw_tasklist &tl, const PairDev &pd, int pair_index,
+                            const PairGeom &geo, int prof_off, int prof_stride,
+                            std::vector<SubDev> &out) {
+    const gpw_grid_desc &g = tl.grid;
+    std::vector<std::pair<int, int>> ranges[3];
+    int base_of[3][64];
+    for (int d = 0; d < 3; d++) {
+        const int lb = geo.lb[d], ub = geo.ub[d];
+        if (g.periodic[d]) {
+            const int N = g.npts[d];
+            int k = 0;
+            for (int s = lb; s <= ub; s += N) {
+                if (k >= 64) return fail(GPW_ERR_ARG, "cube spans more than 64 periods");
+                ranges[d].push_back({s, std::min(ub, s + N - 1)});
+                base_of[d][k++] = imod_h(geo.cc[d] + s, N);
+            }
+        } else {
+            const int gshift = imod_h(geo.cc[d] + lb + g.lb_grid[d], g.npts[d]) - lb;
+            if (lb + gshift < 0 || ub + gshift > g.ngrid[d] - 1)
+                return fail(GPW_ERR_BOUNDS, "non-periodic cube outside the local grid");
+            ranges[d].push_back({lb, ub});
+            base_of[d][0] = lb + gshift;
+        }
+    }
+    for (size_t iz = 0; iz < ranges[2].size(); iz++)
+        for (size_t iy = 0; iy < ranges[1].size(); iy++)
+            for (size_t ix = 0; ix < ranges[0].size(); ix++) {
+                SubDev s{};
+                const size_t idx[3] = {ix, iy, iz};
+                for (int d = 0; d < 3; d++) {
+                    s.roff[d] = geo.roff[d];
+                    s.lb[d] = ranges[d][idx[d]].first;
+                    s.ub[d] = ranges[d][idx[d]].second;
+                    s.base[d] = base_of[d][idx[d]];
+                    s.period[d] = g.periodic[d] ? g.npts[d] : BIG;
+                }
+                s.zetp = pd.zetp;
+                s.lp = pd.lp;
+                s.pair = pair_index;
+                s.coef_off = pd.coef_off;
+                s.prof_off = prof_off;
+                s.prof_stride = prof_stride;
+                out.push_back(s);
+            }
+    return GPW_OK;
+}
+
+// tiles along one axis touched by the grid indices of a sub-box
+static void axis_tiles(const SubDev &s, int d, int n, int T, int ntile, std::vector<int> &out) {
+    out.clear();
+    const int len = s.ub[d] - s.lb[d] + 1;
+    std::vector<char> hit(ntile, 0);
+    if (s.period[d] != BIG) {
+        const int start = s.base[d];
+        const int stop = start + len - 1;  // may exceed n: wraps
+        if (stop < n) {
+            for (int t = start / T; t <= stop / T; t++) hit[t] = 1;
+        } else {
+            for (int t = start / T; t < ntile; t++) hit[t] = 1;
+            for (int t = 0; t <= (stop - n) / T; t++) hit[t] = 1;
+        }
+    } else {
+        for (int t = s.base[d] / T; t <= (s.base[d] + len - 1) / T; t++) hit[t] = 1;
+    }
+    for (int t = 0; t < ntile; t++)
+        if (hit[t]) out.push_back(t);
+}
+
+// number of sphere points in a sub-box (for roofline accounting)
+static long long sub_points(const SubDev &s, const std::vector<int> &prof) {
+    auto fo = [](int i) { return i <= 0 ? -i : i - 1; };
+    long long n = 0;
+    for (int k = s.lb[2]; k <= s.ub[2]; k++)
+        for (int j = s.lb[1]; j <= s.ub[1]; j++) {
+            int T;
+            if (s.prof_off < 0)
+                T = BIG;
+            else
+                T = prof[s.prof_off + fo(k) * s.prof_stride + fo(j)];
+            if (T < 0) continue;
+            // offsets i in [lb, ub] with fo(i) <= T  <=>  i in [-T, T+1]
+            const int lo = std::max(s.lb[0], -T), hi = std::min(s.ub[0], T + 1);
+            if (hi >= lo) n += hi - lo + 1;
+        }
+    return n;
+}
+
+// Host part of a task list: profiles, pairs, sub-boxes, tile work lists.
+struct HostPlan {
+    std::vector<int> prof_flat;
+    std::vector<int> tile_start, tile_ent, pe_start, pe_list;
+    std::vector<long long> off0, off2;
+    long long coef_total = 0;
+};
+
+static gpw_status plan_tasklist(gpw_tasklist *tl, const std::vector<gpw_pair_desc> &pairs,
+                                const std::vector<PairGeom> &geos,
+                                const std::vector<Profile> &profiles, int splits, HostPlan &hp) {
+    const gpw_grid_desc &g = tl->grid;
+    std::vector<int> &prof_flat = hp.prof_flat;
+    std::vector<int> prof_off(profiles.size()), prof_stride(profiles.size());
+    for (size_t i = 0; i < profiles.size(); i++) {
+        prof_off[i] = (int)prof_flat.size();
+        prof_stride[i] = profiles[i].n[1] + 1;
+        prof_flat.insert(prof_flat.end(), profiles[i].T.begin(), profiles[i].T.end());
+    }
+    // pairs + coefficient offsets
+    tl->pairs_h.resize(pairs.size());
+    long long coef_total = 0;
+    for (size_t i = 0; i < pairs.size(); i++) {
+        PairDev pd = make_pair_dev(pairs[i], tl->compute_tau);
+        pd.coef_off = (int)coef_total;
+        coef_total += ncoset(pd.lp);
+        if (coef_total > 0x7fffffffLL) return fail(GPW_ERR_ARG, "too many coefficients");
+        tl->pairs_h[i] = pd;
+        tl->lp_max = std::max(tl->lp_max, pd.lp);
+        tl->la_max = std::max(tl->la_max, pairs[i].la_max);
+        tl->lb_max = std::max(tl->lb_max, pairs[i].lb_max);
+    }
+    hp.coef_total = coef_total;
+    // sub-boxes
+    tl->subs_h.clear();
+    tl->pair_sub_start.assign(pairs.size() + 1, 0);
+    for (size_t i = 0; i < pairs.size(); i++) {
+        tl->pair_sub_start[i] = (int)tl->subs_h.size();
+        const int pi = geos[i].prof;
+        gpw_status st = make_subs(*tl, tl->pairs_h[i], (int)i, geos[i], pi >= 0 ? prof_off[pi] : -1,
+                                  pi >= 0 ? prof_stride[pi] : 0, tl->subs_h);
+        if (st) return st;
+    }
+    tl->pair_sub_start[pairs.size()] = (int)tl->subs_h.size();
+    // tiles
+    TileGrid &tg = tl->tg;
+    tg.nx = g.ngrid[0];
+    tg.ny = g.ngrid[1];
+    tg.nz = g.ngrid[2];
+    tg.ntx = (tg.nx + TX - 1) / TX;
+    tg.nty = (tg.ny + TY - 1) / TY;
+    tg.ntz = (tg.nz + TZ - 1) / TZ;
+    for (int d = 0; d < 3; d++) tg.dr[d] = g.dh[d][d];
+    tl->ntiles = tg.ntx * tg.nty * tg.ntz;
+    tl->splits = std::max(1, splits);
+    // entries (tile, sub)
+    std::vector<int> count(tl->ntiles + 1, 0);
+    std::vector<int> tx, ty, tz;
+    std::vector<std::pair<int, int>> ents;
+    tl->cube_points = 0;
+    for (size_t si = 0; si < tl->subs_h.size(); si++) {
+        const SubDev &s = tl->subs_h[si];
+        axis_tiles(s, 0, tg.nx, TX, tg.ntx, tx);
+        axis_tiles(s, 1, tg.ny, TY, tg.nty, ty);
+        axis_tiles(s, 2, tg.nz, TZ, tg.ntz, tz);
+        for (int a : tz)
+            for (int b : ty)
+                for (int c : tx) {
+                    const int tile = (a * tg.nty + b) * tg.ntx + c;
+                    ents.push_back({tile, (int)si});
+                    count[tile]++;
+                }
+        tl->cube_points += sub_points(s, prof_flat);
+    }
+    tl->nentries = (long long)ents.size();
+    tl->tile_points = tl->nentries * (long long)(TX * TY * TZ);
+    hp.tile_start.assign(tl->ntiles + 1, 0);
+    for (int t = 0; t < tl->ntiles; t++) hp.tile_start[t + 1] = hp.tile_start[t] + count[t];
+    hp.tile_ent.assign(ents.size(), 0);
+    {
+        std::vector<int> fillp(hp.tile_start.begin(), hp.tile_start.end() - 1);
+        for (auto &e : ents) hp.tile_ent[fillp[e.first]++] = e.second;
+    }
+    // partial-moment offsets per entry (entry id = position in tile_ent)
+    hp.off0.assign(ents.size(), 0);
+    hp.off2.assign(ents.size(), 0);
+    long long acc0 = 0, acc2 = 0;
+    for (size_t e = 0; e < hp.tile_ent.size(); e++) {
+        const int lp = tl->subs_h[hp.tile_ent[e]].lp;
+        hp.off0[e] = acc0;
+        hp.off2[e] = acc2;
+        acc0 += ncoset(lp);
+        acc2 += ncoset(lp + 2);
+    }
+    // pair -> entries
+    hp.pe_start.assign(pairs.size() + 1, 0);
+    hp.pe_list.assign(hp.tile_ent.size(), 0);
+    for (size_t e = 0; e < hp.tile_ent.size(); e++) hp.pe_start[tl->subs_h[hp.tile_ent[e]].pair + 1]++;
+    for (size_t p = 0; p < pairs.size(); p++) hp.pe_start[p + 1] += hp.pe_start[p];
+    {
+        std::vector<int> fillp(hp.pe_start.begin(), hp.pe_start.end() - 1);
+        for (size_t e = 0; e < hp.tile_ent.size(); e++)
+            hp.pe_list[fillp[tl->subs_h[hp.tile_ent[e]].pair]++] = (int)e;
+    }
+    tl->mstride = ncoset(tl->lp_max + 2);
+    return GPW_OK;
+}
+
+static gpw_status upload_tasklist(gpw_tasklist *tl, const HostPlan &hp) {
+    gpw_status st;
+    if ((st = upload(&tl->d_pairs, tl->pairs_h))) return st;
+    if ((st = upload(&tl->d_subs, tl->subs_h))) return st;
+    if ((st = upload(&tl->d_prof, hp.prof_flat))) return st;
+    if ((st = upload(&tl->d_tile_start, hp.tile_start))) return st;
+    if ((st = upload(&tl->d_tile_ent, hp.tile_ent))) return st;
+    if ((st = upload(&tl->d_pe_start, hp.pe_start))) return st;
+    if ((st = upload(&tl->d_pe_list, hp.pe_list))) return st;
+    if ((st = upload(&tl->d_ent_off0, hp.off0))) return st;
+    if ((st = upload(&tl->d_ent_off2, hp.off2))) return st;
+    long long acc2 = 0;
+    for (size_t e = 0; e < hp.tile_ent.size(); e++) acc2 += ncoset(tl->subs_h[hp.tile_ent[e]].lp + 2);
+    HIP_TRY(hipMalloc((void **)&tl->d_coef, sizeof(double) * std::max<long long>(hp.coef_total, 1)));
+    HIP_TRY(hipMalloc((void **)&tl->d_partial, sizeof(double) * std::max<long long>(acc2, 1)));
+    HIP_TRY(hipMalloc((void **)&tl->d_Mbuf,
+                      sizeof(double) * std::max<long long>((long long)tl->npairs * tl->mstride, 1)));
+    if (tl->splits > 1) {
+        const long long ng = (long long)tl->tg.nx * tl->tg.ny * tl->tg.nz;
+        HIP_TRY(hipMalloc((void **)&tl->d_cpartial, sizeof(double) * ng * tl->splits));
+    }
+    return GPW_OK;
+}
+
+static gpw_status finish_tasklist(gpw_tasklist *tl, const std::vector<gpw_pair_desc> &pairs,
+                                  const std::vector<PairGeom> &geos,
+                                  const std::vector<Profile> &profiles, int splits) {
+    HostPlan hp;
+    gpw_status st = plan_tasklist(tl, pairs, geos, profiles, splits, hp);
+    if (st) return st;
+    return upload_tasklist(tl, hp);
+}
+
+static gpw_status check_grid(const gpw_grid_desc *g) {
+    if (!g) return fail(GPW_ERR_ARG, "grid descriptor is NULL");
+    for (int d = 0; d < 3; d++) {
+        if (g->npts[d] <= 0 || g->ngrid[d] <= 0) return fail(GPW_ERR_ARG, "grid sizes must be > 0");
+        if (g->periodic[d] && g->ngrid[d] != g->npts[d])
+            return fail(GPW_ERR_ARG, "periodic axes need ngrid == npts");
+    }
+    if (!is_ortho(*g)) return fail(GPW_ERR_UNSUPPORTED, "non-orthorhombic grids are not supported yet");
+    return GPW_OK;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+
+extern "C" {
+
+const char *gpw_last_error(void) { return g_last_error.c_str(); }
+
+// geometry of every pair from its radius (cube rule of CP2K's cube_utils)
+static gpw_status radius_geometry(const gpw_grid_desc *grid, int npairs, const gpw_pair_desc *pairs,
+                                  int tau, std::vector<PairGeom> &geos,
+                                  std::vector<Profile> &profiles) {
+    geos.assign(npairs, PairGeom{});
+    std::map<long long, int> prof_cache;  // imr -> profile index
+    const double dr[3] = {grid->dh[0][0], grid->dh[1][1], grid->dh[2][2]};
+    const double drmin = std::min(dr[0], std::min(dr[1], dr[2]));
+    gpw_status st;
+    for (int i = 0; i < npairs; i++) {
+        if ((st = check_pair(pairs[i], tau))) return st;
+        if (!(pairs[i].radius > 0.0)) return fail(GPW_ERR_ARG, "radius must be > 0");
+        const PairDev pd = make_pair_dev(pairs[i], tau);
+        PairGeom &geo = geos[i];
+        cube_centre(*grid, pd.rp, geo.cc, geo.roff);
+        long long imr = (long long)std::ceil(pairs[i].radius / drmin);
+        if (imr < 1) imr = 1;
+        auto it = prof_cache.find(imr);
+        if (it == prof_cache.end()) {
+            Profile P;
+            int lb[3], ub[3];
+            profile_from_radius(pairs[i].radius, dr, P, lb, ub);
+            profiles.push_back(std::move(P));
+            it = prof_cache.emplace(imr, (int)profiles.size() - 1).first;
+        }
+        geo.prof = it->second;
+        for (int d = 0; d < 3; d++) {
+            geo.lb[d] = -profiles[geo.prof].n[d];
+            geo.ub[d] = 1 + profiles[geo.prof].n[d];
+        }
+    }
+    return GPW_OK;
+}
+
+gpw_status gpw_tasklist_create(const gpw_grid_desc *grid, int npairs, const gpw_pair_desc *pairs,
+                               int compute_tau, const int *tile, int splits, gpw_tasklist **out) {
+    if (!out || npairs < 0 || (npairs > 0 && !pairs)) return fail(GPW_ERR_ARG, "bad arguments");
+    *out = nullptr;
+    gpw_status st = check_grid(grid);
+    if (st) return st;
+    if (tile && (tile[0] != TX || tile[1] != TY || tile[2] != TZ))
+        return fail(GPW_ERR_UNSUPPORTED, "only the built-in tile shape is compiled");
+    auto *tl = new (std::nothrow) gpw_tasklist();
+    if (!tl) return fail(GPW_ERR_ALLOC, "host allocation failed");
+    tl->grid = *grid;
+    tl->compute_tau = compute_tau ? 1 : 0;
+    tl->npairs = npairs;
+    std::vector<gpw_pair_desc> pv(pairs, pairs + npairs);
+    std::vector<PairGeom> geos;
+    std::vector<Profile> profiles;
+    if ((st = radius_geometry(grid, npairs, pairs, tl->compute_tau, geos, profiles)) ||
+        (st = finish_tasklist(tl, pv, geos, profiles, splits))) {
+        delete tl;
+        return st;
+    }
+    *out = tl;
+    return GPW_OK;
+}
+
+gpw_status gpw_tasklist_plan_stats(const gpw_grid_desc *grid, int npairs, const gpw_pair_desc *pairs,
+                                   int compute_tau, int splits, gpw_tasklist_stats *out) {
+    if (!out || npairs < 0 || (npairs > 0 && !pairs)) return fail(GPW_ERR_ARG, "bad arguments");
+    gpw_status st = check_grid(grid);
+    if (st) return st;
+    gpw_tasklist tl;
+    tl.grid = *grid;
+    tl.compute_tau = compute_tau ? 1 : 0;
+    tl.npairs = npairs;
+    std::vector<gpw_pair_desc> pv(pairs, pairs + npairs);
+    std::vector<PairGeom> geos;
+    std::vector<Profile> profiles;
+    HostPlan hp;
+    if ((st = radius_geometry(grid, npairs, pairs, tl.compute_tau, geos, profiles)) ||
+        (st = plan_tasklist(&tl, pv, geos, profiles, splits, hp)))
+        return st;
+    return gpw_tasklist_stats_get(&tl, out);
+}
+
+gpw_status gpw_pair_geometry(const gpw_grid_desc *grid, const gpw_pair_desc *pair, int compute_tau,
+                             int cube_centre_out[3], double roffset_out[3], int lb_cube_out[3],
+                             int ub_cube_out[3], int *sphere_bounds, int cap, int *nspheres) {
+    if (!grid || !pair || !cube_centre_out || !roffset_out || !lb_cube_out || !ub_cube_out || !nspheres)
+        return fail(GPW_ERR_ARG, "NULL argument");
+    gpw_status st = check_grid(grid);
+    if (st) return st;
+    std::vector<PairGeom> geos;
+    std::vector<Profile> profiles;
+    if ((st = radius_geometry(grid, 1, pair, compute_tau ? 1 : 0, geos, profiles))) return st;
+    const PairGeom &g = geos[0];
+    const Profile &P = profiles[g.prof];
+    for (int d = 0; d < 3; d++) {
+        cube_centre_out[d] = g.cc[d];
+        roffset_out[d] = g.roff[d];
+        lb_cube_out[d] = g.lb[d];
+        ub_cube_out[d] = g.ub[d];
+    }
+    // re-encode the profile in the reference's sphere_bounds layout
+    int n = 0;
+    auto put = [&](int v) {
+        if (sphere_bounds && n < cap) sphere_bounds[n] = v;
+        n++;
+    };
+    const int kgmin = g.lb[2];
+    put(kgmin);
+    for (int kg = kgmin; kg <= 0; kg++) {
+        int jn = -1;
+        for (int jj = 0; jj <= P.n[1]; jj++)
+            if (P.T[(size_t)(-kg) * (P.n[1] + 1) + jj] >= 0) jn = jj;
+        put(-jn);
+        for (int jg = -jn; jg <= 0; jg++) put(-P.T[(size_t)(-kg) * (P.n[1] + 1) + (-jg)]);
+    }
+    *nspheres = n;
+    return n <= cap ? GPW_OK : fail(GPW_ERR_ARG, "sphere_bounds buffer too small");
+}
+
+gpw_status gpw_tasklist_destroy(gpw_tasklist *tl) {
+    delete tl;
+    return GPW_OK;
+}
+
+gpw_status gpw_tasklist_stats_get(const gpw_tasklist *tl, gpw_tasklist_stats *o) {
+    if (!tl || !o) return fail(GPW_ERR_ARG, "NULL argument");
+    o->npairs = tl->npairs;
+    o->nsub = (long long)tl->subs_h.size();
+    o->ntiles = tl->ntiles;
+    o->nentries = tl->nentries;
+    o->cube_points = tl->cube_points;
+    o->tile_points = tl->tile_points;
+    o->lp_max = tl->lp_max;
+    o->tile[0] = TX;
+    o->tile[1] = TY;
+    o->tile[2] = TZ;
+    o->splits = tl->splits;
+    return GPW_OK;
+}
+
+static gpw_status run_coef(gpw_tasklist *tl, const double *pab, int pair0, int n, hipStream_t st) {
+    if (n <= 0) return GPW_OK;
+    HIP_TRY(launch_coef(tl->d_pairs, pair0, n, tl->compute_tau, tl->la_max, tl->lb_max, pab,
+                        tl->d_coef, st));
+    return GPW_OK;
+}
+
+gpw_status gpw_tasklist_collocate(gpw_tasklist *tl, const double *pab, double *grid, void *stream) {
+    if (!tl || !pab || !grid) return fail(GPW_ERR_ARG, "NULL argument");
+    if (tl->npairs == 0) return GPW_OK;
+    hipStream_t st = (hipStream_t)stream;
+    gpw_status s = run_coef(tl, pab, 0, tl->npairs, st);
+    if (s) return s;
+    if (tl->splits == 1) {
+        HIP_TRY(launch_collocate_tiles(tl->lp_max, tl->tg, tl->d_subs, tl->d_tile_start, tl->d_tile_ent,
+                                       tl->d_coef, tl->d_prof, tl->ntiles, 1, grid, 0, st));
+    } else {
+        const long long ng = (long long)tl->tg.nx * tl->tg.ny * tl->tg.nz;
+        HIP_TRY(launch_collocate_tiles(tl->lp_max, tl->tg, tl->d_subs, tl->d_tile_start, tl->d_tile_ent,
+                                       tl->d_coef, tl->d_prof, tl->ntiles, tl->splits, tl->d_cpartial,
+                                       1, st));
+        HIP_TRY(launch_reduce_partials(tl->d_cpartial, tl->splits, ng, grid, st));
+    }
+    return GPW_OK;
+}
+
+gpw_status gpw_tasklist_integrate(gpw_tasklist *tl, const double *grid, const double *pab,
+                                  double *hab, double *forces, void *stream) {
+    if (!tl || !grid || !hab) return fail(GPW_ERR_ARG, "NULL argument");
+    if (forces && !pab) return fail(GPW_ERR_ARG, "forces need pab");
+    if (tl->npairs == 0) return GPW_OK;
+    hipStream_t st = (hipStream_t)stream;
+    const int wf = forces ? 1 : 0;
+    const int lp_add = 2 * wf;
+    HIP_TRY(launch_integrate_tiles(tl->lp_max, tl->tg, tl->d_subs, tl->d_tile_start, tl->d_tile_ent,
+                                   tl->d_prof, grid, lp_add, wf ? tl->d_ent_off2 : tl->d_ent_off0,
+                                   tl->ntiles, tl->splits, tl->d_partial, st));
+    HIP_TRY(launch_sum_entries(tl->d_pairs, tl->npairs, tl->compute_tau, lp_add, tl->d_pe_start,
+                               tl->d_pe_list, wf ? tl->d_ent_off2 : tl->d_ent_off0, tl->d_partial,
+                               tl->mstride, tl->d_Mbuf, st));
+    HIP_TRY(launch_hab(tl->d_pairs, 0, tl->npairs, tl->compute_tau, wf, tl->la_max, tl->lb_max,
+                       tl->d_Mbuf, tl->mstride, pab, hab, forces, st));
+    return GPW_OK;
+}
+
+gpw_status gpw_tasklist_collocate_plain(gpw_tasklist *tl, const double *pab, double *grid,
+                                        void *stream) {
+    if (!tl || !pab || !grid) return fail(GPW_ERR_ARG, "NULL argument");
+    hipStream_t st = (hipStream_t)stream;
+    for (int p = 0; p < tl->npairs; p++) {
+        gpw_status s = run_coef(tl, pab, p, 1, st);
+        if (s) return s;
+        for (int si = tl->pair_sub_start[p]; si < tl->pair_sub_start[p + 1]; si++) {
+            const SubDev &sd = tl->subs_h[si];
+            HIP_TRY(launch_collocate_plain(tl->tg, tl->d_subs, si, sd.ub[2] - sd.lb[2] + 1,
+                                           sd.ub[1] - sd.lb[1] + 1, tl->d_coef, tl->d_prof, grid, st));
+        }
+    }
+    return GPW_OK;
+}
+
+gpw_status gpw_tasklist_integrate_plain(gpw_tasklist *tl, const double *grid, const double *pab,
+                                        double *hab, double *forces, void *stream) {
+    if (!tl || !grid || !hab) return fail(GPW_ERR_ARG, "NULL argument");
+    if (forces && !pab) return fail(GPW_ERR_ARG, "forces need pab");
+    hipStream_t st = (hipStream_t)stream;
+    const int wf = forces ? 1 : 0;
+    for (int p = 0; p < tl->npairs; p++) {
+        HIP_TRY(launch_fill(tl->d_Mbuf + (long long)p * tl->mstride, tl->mstride, 0.0, st));
+        for (int si = tl->pair_sub_start[p]; si < tl->pair_sub_start[p + 1]; si++) {
+            const SubDev &sd = tl->subs_h[si];
+            HIP_TRY(launch_moments_plain(tl->tg, tl->d_subs, si, sd.ub[2] - sd.lb[2] + 1,
+                                         sd.ub[1] - sd.lb[1] + 1, 2 * wf, tl->d_prof, grid,
+                                         tl->mstride, tl->d_Mbuf, st));
+        }
+        HIP_TRY(launch_hab(tl->d_pairs, p, 1, tl->compute_tau, wf, tl->pairs_h[p].la_max,
+                           tl->pairs_h[p].lb_max, tl->d_Mbuf, tl->mstride, pab, hab, forces, st));
+    }
+    return GPW_OK;
+}
+
+gpw_status gpw_collocate_core(const int *length, int L, const double *co, const double *P, int ldp,
+                              double *V, void *stream) {
+    if (!length || !co || !P || !V || L < 1 || L > LPMAX + 1 || ldp < std::max(length[0], std::max(length[1], length[2])))
+        return fail(GPW_ERR_ARG, "bad arguments");
+    HIP_TRY(launch_collocate_core(length, L, co, P, ldp, V, (hipStream_t)stream));
+    return GPW_OK;
+}
+
+gpw_status gpw_integrate_core(const int *length, int L, const double *P, int ldp, const double *V,
+                              double *Velems, void *stream) {
+    if (!length || !P || !V || !Velems || L < 1 || L > LPMAX + 1 ||
+        ldp < std::max(length[0], std::max(length[1], length[2])))
+        return fail(GPW_ERR_ARG, "bad arguments");
+    HIP_TRY(launch_integrate_core(length, L, P, ldp, V, Velems, (hipStream_t)stream));
+    return GPW_OK;
+}
+
+// ---- single task (drop-in) ------------------------------------------------
+
+struct SingleTask {
+    gpw_grid_desc g;
+    gpw_pair_desc p;
+    int tau;
+};
+
+// Validates the reference arguments and builds a one-pair task list whose
+// geometry is the caller's lb_cube / ub_cube / sphere_bounds.
+static gpw_status single_setup(int compute_tau, int use_ortho, int la_max, int la_min, int lb_max,
+                               int lb_min, double zeta, double zetb, double rscale, const double *dh,
+                               const double *dh_inv, const double *ra, const double *rab,
+                               const int *npts, const int *ngrid, const int *lb_grid,
+                               const int *periodic, int lmax, double radius, const int *lb_cube,
+                               const int *ub_cube, int nspheres, const int *sphere_bounds,
+                               int maxco, int o1, int o2, gpw_tasklist **out) {
+    *out = nullptr;
+    if (!dh || !dh_inv || !ra || !rab || !npts || !ngrid || !lb_grid || !periodic)
+        return fail(GPW_ERR_ARG, "NULL argument");
+    const int tau = compute_tau ? 1 : 0;
+    if (la_min < 0 || lb_min < 0 || la_max < la_min || lb_max < lb_min)
+        return fail(GPW_ERR_ARG, "bad angular momentum range");
+    // reference preconditions (alpha sized by lmax, grid_base_ref_c.cc:946;
+    // coef_xyt sized by 2*lmax, :246-248)
+    if (lmax < la_max + tau || lmax < lb_max + tau || 2 * lmax < la_max + lb_max + 2 * tau)
+        return fail(GPW_ERR_LMAX, "lmax below the local angular momentum");
+    if (o1 < 0 || o2 < 0 || o1 + ncoset(la_max) > maxco || o2 + ncoset(lb_max) > maxco)
+        return fail(GPW_ERR_ARG, "pab block outside maxco");
+    if (!use_ortho) return fail(GPW_ERR_UNSUPPORTED, "general (non-orthorhombic) path not built yet");
+    if (!lb_cube || !ub_cube) return fail(GPW_ERR_ARG, "ortho path needs lb_cube/ub_cube");
+    gpw_grid_desc g{};
+    for (int i = 0; i < 3; i++) {
+        g.npts[i] = npts[i];
+        g.ngrid[i] = ngrid[i];
+        g.lb_grid[i] = lb_grid[i];
+        g.periodic[i] = periodic[i] ? 1 : 0;
+        for (int j = 0; j < 3; j++) {
+            g.dh[i][j] = dh[3 * i + j];
+            g.dh_inv[i][j] = dh_inv[3 * i + j];
+        }
+    }
+    for (int d = 0; d < 3; d++) {
+        if (g.npts[d] <= 0 || g.ngrid[d] <= 0) return fail(GPW_ERR_ARG, "grid sizes must be > 0");
+        if (g.periodic[d] && g.ngrid[d] != g.npts[d])
+            return fail(GPW_ERR_ARG, "periodic axes need ngrid == npts");
+    }
+    gpw_pair_desc p{};
+    p.la_max = la_max;
+    p.la_min = la_min;
+    p.lb_max = lb_max;
+    p.lb_min = lb_min;
+    p.zeta = zeta;
+    p.zetb = zetb;
+    p.rscale = rscale;
+    for (int i = 0; i < 3; i++) {
+        p.ra[i] = ra[i];
+        p.rab[i] = rab[i];
+    }
+    p.radius = radius;
+    // the device block is re-packed: ncoset(lb_max) x ncoset(la_max)
+    p.block_offset = 0;
+    gpw_status st = check_pair(p, tau);
+    if (st) return st;
+    auto *tl = new (std::nothrow) gpw_tasklist();
+    if (!tl) return fail(GPW_ERR_ALLOC, "host allocation failed");
+    tl->grid = g;
+    tl->compute_tau = tau;
+    tl->npairs = 1;
+    std::vector<Profile> profiles(1);
+    if ((st = profile_from_bounds(lb_cube, ub_cube, nspheres, sphere_bounds, profiles[0]))) {
+        delete tl;
+        return st;
+    }
+    std::vector<PairGeom> geos(1);
+    const PairDev pd = make_pair_dev(p, tau);
+    cube_centre(g, pd.rp, geos[0].cc, geos[0].roff);
+    for (int d = 0; d < 3; d++) {
+        geos[0].lb[d] = lb_cube[d];
+        geos[0].ub[d] = ub_cube[d];
+    }
+    geos[0].prof = 0;
+    std::vector<gpw_pair_desc> pv(1, p);
+    if ((st = finish_tasklist(tl, pv, geos, profiles, 1))) {
+        delete tl;
+        return st;
+    }
+    *out = tl;
+    return GPW_OK;
+}
+
+gpw_status gpw_collocate_pgf_product_rspace(
+    int compute_tau, int use_ortho, int la_max, int la_min, int lb_max, int lb_min, double zeta,
+    double zetb, double rscale, const double *dh, const double *dh_inv, const double *ra,
+    const double *rab, const int *npts, const int *ngrid, const int *lb_grid, const int *periodic,
+    int lmax, double radius, const int *lb_cube, const int *ub_cube, int nspheres,
+    const int *sphere_bounds, int maxco, int o1, int o2, const double *pab, double *grid) {
+    if (!pab || !grid) return fail(GPW_ERR_ARG, "NULL argument");
+    gpw_tasklist *tl = nullptr;
+    gpw_status st = single_setup(compute_tau, use_ortho, la_max, la_min, lb_max, lb_min, zeta, zetb,
+                                 rscale, dh, dh_inv, ra, rab, npts, ngrid, lb_grid, periodic, lmax,
+                                 radius, lb_cube, ub_cube, nspheres, sphere_bounds, maxco, o1, o2, &tl);
+    if (st) return st;
+    const int na = ncoset(la_max), nb = ncoset(lb_max);
+    std::vector<double> blk((size_t)na * nb);
+    for (int j = 0; j < nb; j++)
+        for (int i = 0; i < na; i++) blk[(size_t)j * na + i] = pab[(size_t)(o2 + j) * maxco + o1 + i];
+    const size_t ng = (size_t)ngrid[0] * ngrid[1] * ngrid[2];
+    double *d_pab = nullptr, *d_grid = nullptr;
+    gpw_status rc = GPW_OK;
+    hipError_t e;
+    if ((e = hipMalloc((void **)&d_pab, blk.size() * sizeof(double))) != hipSuccess ||
+        (e = hipMalloc((void **)&d_grid, ng * sizeof(double))) != hipSuccess) {
+        rc = fail(GPW_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    } else if ((e = hipMemcpy(d_pab, blk.data(), blk.size() * sizeof(double), hipMemcpyHostToDevice)) !=
+                   hipSuccess ||
+               (e = hipMemcpy(d_grid, grid, ng * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess) {
+        rc = fail(GPW_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+    } else if ((rc = gpw_tasklist_collocate_plain(tl, d_pab, d_grid, nullptr)) == GPW_OK) {
+        if ((e = hipMemcpy(grid, d_grid, ng * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess)
+            rc = fail(GPW_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+    }
+    if (d_pab) (void)hipFree(d_pab);
+    if (d_grid) (void)hipFree(d_grid);
+    delete tl;
+    return rc;
+}
+
+gpw_status gpw_integrate_pgf_product_rspace(
+    int compute_tau, int use_ortho, int la_max, int la_min, int lb_max, int lb_min, double zeta,
+    double zetb, double rscale, const double *dh, const double *dh_inv, const double *ra,
+    const double *rab, const int *npts, const int *ngrid, const int *lb_grid, const int *periodic,
+    int lmax, double radius, const int *lb_cube, const int *ub_cube, int nspheres,
+    const int *sphere_bounds, int maxco, int o1, int o2, const double *grid, const double *pab,
+    double *hab, double *forces) {
+    if (!grid || !hab || (forces && !pab)) return fail(GPW_ERR_ARG, "NULL argument");
+    gpw_tasklist *tl = nullptr;
+    gpw_status st = single_setup(compute_tau, use_ortho, la_max, la_min, lb_max, lb_min, zeta, zetb,
+                                 rscale, dh, dh_inv, ra, rab, npts, ngrid, lb_grid, periodic, lmax,
+                                 radius, lb_cube, ub_cube, nspheres, sphere_bounds, maxco, o1, o2, &tl);
+    if (st) return st;
+    const int na = ncoset(la_max), nb = ncoset(lb_max);
+    std::vector<double> blk((size_t)na * nb, 0.0);
+    if (pab)
+        for (int j = 0; j < nb; j++)
+            for (int i = 0; i < na; i++) blk[(size_t)j * na + i] = pab[(size_t)(o2 + j) * maxco + o1 + i];
+    const size_t ng = (size_t)ngrid[0] * ngrid[1] * ngrid[2];
+    double *d_pab = nullptr, *d_grid = nullptr, *d_hab = nullptr, *d_f = nullptr;
+    gpw_status rc = GPW_OK;
+    hipError_t e = hipSuccess;
+    std::vector<double> hblk(blk.size(), 0.0), f(6, 0.0);
+    if ((e = hipMalloc((void **)&d_pab, blk.size() * sizeof(double))) != hipSuccess ||
+        (e = hipMalloc((void **)&d_hab, blk.size() * sizeof(double))) != hipSuccess ||
+        (e = hipMalloc((void **)&d_f, 6 * sizeof(double))) != hipSuccess ||
+        (e = hipMalloc((void **)&d_grid, ng * sizeof(double))) != hipSuccess) {
+        rc = fail(GPW_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    } else if ((e = hipMemcpy(d_pab, blk.data(), blk.size() * sizeof(double), hipMemcpyHostToDevice)) !=
+                   hipSuccess ||
+               (e = hipMemcpy(d_hab, hblk.data(), hblk.size() * sizeof(double), hipMemcpyHostToDevice)) !=
+                   hipSuccess ||
+               (e = hipMemcpy(d_f, f.data(), 6 * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess ||
+               (e = hipMemcpy(d_grid, grid, ng * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess) {
+        rc = fail(GPW_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+    } else if ((rc = gpw_tasklist_integrate_plain(tl, d_grid, d_pab, d_hab, forces ? d_f : nullptr,
+                                                  nullptr)) == GPW_OK) {
+        if ((e = hipMemcpy(hblk.data(), d_hab, hblk.size() * sizeof(double), hipMemcpyDeviceToHost)) !=
+                hipSuccess ||
+            (e = hipMemcpy(f.data(), d_f, 6 * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess)
+            rc = fail(GPW_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+    }
+    if (rc == GPW_OK) {
+        for (int j = 0; j < nb; j++)
+            for (int i = 0; i < na; i++) hab[(size_t)(o2 + j) * maxco + o1 + i] += hblk[(size_t)j * na + i];
+        if (forces)
+            for (int k = 0; k < 6; k++) forces[k] += f[k];
+    }
+    for (double *p : {d_pab, d_grid, d_hab, d_f})
+        if (p) (void)hipFree(p);
+    delete tl;
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,269 @@
+"""ctypes binding of the CPU oracle (oracle/gpw_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()``
+and by the CPU-baseline leg of ``bench.py``, always as the checker or the
+CPU reference, never as the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libgpw_oracle.so")
+
+ORC_LMAX = 8
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+class OrcTask(C.Structure):
+    _fields_ = [
+        ("compute_tau", C.c_int), ("use_ortho", C.c_int),
+        ("la_max", C.c_int), ("la_min", C.c_int), ("lb_max", C.c_int), ("lb_min", C.c_int),
+        ("zeta", C.c_double), ("zetb", C.c_double), ("rscale", C.c_double),
+        ("dh", (C.c_double * 3) * 3), ("dh_inv", (C.c_double * 3) * 3),
+        ("ra", C.c_double * 3), ("rab", C.c_double * 3),
+        ("npts", C.c_int * 3), ("ngrid", C.c_int * 3), ("lb_grid", C.c_int * 3),
+        ("periodic", C.c_int * 3),
+        ("lmax", C.c_int),
+        ("radius", C.c_double),
+        ("lb_cube", C.c_int * 3), ("ub_cube", C.c_int * 3),
+        ("nspheres", C.c_int),
+        ("sphere_bounds", C.POINTER(C.c_int)),
+        ("maxco", C.c_int), ("o1", C.c_int), ("o2", C.c_int),
+        ("pab_ld", C.c_int),
+    ]
+
+
+class OrcPair(C.Structure):
+    _fields_ = [
+        ("la_max", C.c_int), ("la_min", C.c_int), ("lb_max", C.c_int), ("lb_min", C.c_int),
+        ("zeta", C.c_double), ("zetb", C.c_double), ("rscale", C.c_double),
+        ("ra", C.c_double * 3), ("rab", C.c_double * 3),
+        ("radius", C.c_double),
+        ("block_offset", C.c_longlong),
+    ]
+
+
+class OrcGrid(C.Structure):
+    _fields_ = [
+        ("npts", C.c_int * 3),
+        ("dh", (C.c_double * 3) * 3), ("dh_inv", (C.c_double * 3) * 3),
+        ("orthorhombic", C.c_int),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        dp = C.POINTER(C.c_double)
+        ip = C.POINTER(C.c_int)
+        L.orc_collocate.argtypes = [C.POINTER(OrcTask), dp, dp]
+        L.orc_integrate.argtypes = [C.POINTER(OrcTask), dp, dp, dp, dp]
+        L.orc_collocate_brute.argtypes = [C.POINTER(OrcTask), dp, dp]
+        L.orc_coef_xyz.argtypes = [C.POINTER(OrcTask), dp, dp, ip]
+        L.orc_sphere_bounds.argtypes = [C.c_double, dp, ip, ip, ip, C.c_int]
+        L.orc_collocate_core.argtypes = [ip, C.c_int, dp, dp, C.c_int, dp]
+        L.orc_integrate_core.argtypes = [ip, C.c_int, dp, C.c_int, dp, dp]
+        L.orc_batch_collocate.argtypes = [C.POINTER(OrcGrid), C.c_int, C.POINTER(OrcPair),
+                                          C.c_int, dp, dp, C.c_int]
+        L.orc_batch_integrate.argtypes = [C.POINTER(OrcGrid), C.c_int, C.POINTER(OrcPair),
+                                          C.c_int, dp, dp, dp, dp, C.c_int]
+        for f in ("orc_collocate", "orc_integrate", "orc_collocate_brute", "orc_coef_xyz",
+                  "orc_sphere_bounds", "orc_batch_collocate", "orc_batch_integrate",
+                  "orc_ncoset", "orc_coset"):
+            getattr(L, f).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def ncoset(l: int) -> int:
+    return 0 if l < 0 else (l + 1) * (l + 2) * (l + 3) // 6
+
+
+def coset(lx: int, ly: int, lz: int) -> int:
+    return lib().orc_coset(lx, ly, lz)
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def make_task(t: dict):
+    """Build an OrcTask from a dict with the keys of the CP2K task format."""
+    ot = OrcTask()
+    ot.compute_tau = int(t["compute_tau"])
+    ot.use_ortho = int(t["use_ortho"])
+    for k in ("la_max", "la_min", "lb_max", "lb_min", "lmax", "maxco", "o1", "o2"):
+        setattr(ot, k, int(t[k]))
+    for k in ("zeta", "zetb", "rscale", "radius"):
+        setattr(ot, k, float(t[k]))
+    for i in range(3):
+        for j in range(3):
+            ot.dh[i][j] = float(t["dh"][i][j])
+            ot.dh_inv[i][j] = float(t["dh_inv"][i][j])
+        ot.ra[i] = float(t["ra"][i])
+        ot.rab[i] = float(t["rab"][i])
+        ot.npts[i] = int(t["npts"][i])
+        ot.ngrid[i] = int(t["ngrid"][i])
+        ot.lb_grid[i] = int(t["lb_grid"][i])
+        ot.periodic[i] = int(t["periodic"][i])
+        ot.lb_cube[i] = int(t.get("lb_cube", (0, 0, 0))[i])
+        ot.ub_cube[i] = int(t.get("ub_cube", (0, 0, 0))[i])
+    sb = np.ascontiguousarray(np.asarray(t.get("sphere_bounds", []), dtype=np.int32))
+    if sb.size == 0:
+        sb = np.zeros(1, dtype=np.int32)
+    ot.nspheres = int(t.get("nspheres", 0))
+    ot.sphere_bounds = sb.ctypes.data_as(C.POINTER(C.c_int))
+    ot.pab_ld = int(t.get("pab_ld", 0))
+    return ot, sb  # keep sb alive alongside the struct
+
+
+def _check(code):
+    if code != 0:
+        raise OracleError(f"oracle error code {code}")
+
+
+def collocate(t: dict, pab: np.ndarray, grid: np.ndarray | None = None) -> np.ndarray:
+    ot, _keep = make_task(t)
+    if grid is None:
+        grid = np.zeros((t["ngrid"][2], t["ngrid"][1], t["ngrid"][0]))
+    _check(lib().orc_collocate(C.byref(ot), _dp(np.ascontiguousarray(pab, dtype=np.float64)),
+                               _dp(grid)))
+    return grid
+
+
+def collocate_brute(t: dict, pab: np.ndarray, grid: np.ndarray | None = None) -> np.ndarray:
+    ot, _keep = make_task(t)
+    if grid is None:
+        grid = np.zeros((t["ngrid"][2], t["ngrid"][1], t["ngrid"][0]))
+    _check(lib().orc_collocate_brute(C.byref(ot),
+                                     _dp(np.ascontiguousarray(pab, dtype=np.float64)), _dp(grid)))
+    return grid
+
+
+def integrate(t: dict, grid: np.ndarray, pab: np.ndarray | None = None, forces: bool = False):
+    """Return (hab, forces[2,3] or None); hab has pab's shape."""
+    ot, _keep = make_task(t)
+    ld = t.get("pab_ld", 0) or t["maxco"]
+    rows = t["maxco"] if not t.get("pab_ld") else t["pab_rows"]
+    hab = np.zeros((rows, ld))
+    f = np.zeros(6) if forces else None
+    pab_c = None if pab is None else np.ascontiguousarray(pab, dtype=np.float64)
+    _check(lib().orc_integrate(C.byref(ot), _dp(np.ascontiguousarray(grid)), _dp(pab_c),
+                               _dp(hab), _dp(f)))
+    return hab, (f.reshape(2, 3) if forces else None)
+
+
+def coef_xyz(t: dict, pab: np.ndarray):
+    ot, _keep = make_task(t)
+    coef = np.zeros(1024)
+    lp = C.c_int(0)
+    _check(lib().orc_coef_xyz(C.byref(ot), _dp(np.ascontiguousarray(pab, dtype=np.float64)),
+                              _dp(coef), C.byref(lp)))
+    return coef[: ncoset(lp.value)].copy(), lp.value
+
+
+def sphere_bounds(radius: float, dr):
+    dr_a = np.ascontiguousarray(dr, dtype=np.float64)
+    lb = (C.c_int * 3)()
+    ub = (C.c_int * 3)()
+    cap = 1 << 16
+    buf = np.zeros(cap, dtype=np.int32)
+    n = lib().orc_sphere_bounds(float(radius), _dp(dr_a), lb, ub,
+                                buf.ctypes.data_as(C.POINTER(C.c_int)), cap)
+    if n < 0:
+        raise OracleError("sphere bounds buffer too small")
+    return list(lb), list(ub), buf[:n].copy()
+
+
+def collocate_core(length, co: np.ndarray, P: np.ndarray) -> np.ndarray:
+    L = co.shape[0]
+    V = np.zeros(tuple(length))
+    lng = (C.c_int * 3)(*length)
+    Pc = np.ascontiguousarray(P, dtype=np.float64)
+    lib().orc_collocate_core(lng, L, _dp(np.ascontiguousarray(co, dtype=np.float64)), _dp(Pc),
+                             Pc.shape[2], _dp(V))
+    return V
+
+
+def integrate_core(length, P: np.ndarray, V: np.ndarray) -> np.ndarray:
+    L = P.shape[1]
+    out = np.zeros((L, L, L))
+    lng = (C.c_int * 3)(*length)
+    Pc = np.ascontiguousarray(P, dtype=np.float64)
+    lib().orc_integrate_core(lng, L, _dp(Pc), Pc.shape[2], _dp(np.ascontiguousarray(V)), _dp(out))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# batches
+
+def make_grid(npts, dh) -> OrcGrid:
+    g = OrcGrid()
+    dh = np.asarray(dh, dtype=np.float64)
+    dh_inv = np.linalg.inv(dh)
+    for i in range(3):
+        g.npts[i] = int(npts[i])
+        for j in range(3):
+            g.dh[i][j] = dh[i, j]
+            g.dh_inv[i][j] = dh_inv[i, j]
+    g.orthorhombic = int(np.count_nonzero(dh - np.diag(np.diag(dh))) == 0)
+    return g
+
+
+def make_pairs(p: dict):
+    """p: dict of numpy arrays (see libgpw_amd.workloads)."""
+    n = len(p["la_max"])
+    arr = (OrcPair * n)()
+    for i in range(n):
+        q = arr[i]
+        q.la_max = int(p["la_max"][i]); q.la_min = int(p["la_min"][i])
+        q.lb_max = int(p["lb_max"][i]); q.lb_min = int(p["lb_min"][i])
+        q.zeta = float(p["zeta"][i]); q.zetb = float(p["zetb"][i])
+        q.rscale = float(p["rscale"][i]); q.radius = float(p["radius"][i])
+        for d in range(3):
+            q.ra[d] = float(p["ra"][i][d]); q.rab[d] = float(p["rab"][i][d])
+        q.block_offset = int(p["block_offset"][i])
+    return arr
+
+
+def batch_collocate(grid_desc: OrcGrid, pairs, pab: np.ndarray, compute_tau=False,
+                    nthreads=0, grid: np.ndarray | None = None) -> np.ndarray:
+    n = [grid_desc.npts[i] for i in range(3)]
+    if grid is None:
+        grid = np.zeros((n[2], n[1], n[0]))
+    _check(lib().orc_batch_collocate(C.byref(grid_desc), len(pairs), pairs, int(compute_tau),
+                                     _dp(np.ascontiguousarray(pab)), _dp(grid), int(nthreads)))
+    return grid
+
+
+def batch_integrate(grid_desc: OrcGrid, pairs, grid: np.ndarray, nblock: int,
+                    pab: np.ndarray | None = None, forces=False, compute_tau=False, nthreads=0):
+    hab = np.zeros(nblock)
+    f = np.zeros((len(pairs), 6)) if forces else None
+    _check(lib().orc_batch_integrate(C.byref(grid_desc), len(pairs), pairs, int(compute_tau),
+                                     _dp(np.ascontiguousarray(grid)),
+                                     _dp(None if pab is None else np.ascontiguousarray(pab)),
+                                     _dp(hab), _dp(f), int(nthreads)))
+    return hab, (None if f is None else f.reshape(len(pairs), 2, 3))

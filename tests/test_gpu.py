@@ -1,0 +1,216 @@
+"""GPU parity tests: the HIP product, called through the C ABI, against the
+CPU oracle on the same seeded inputs (FP64; tolerance 1e-10 relative,
+max|x - ref| / max|ref|, unless stated). Index work (which grid point a
+cube point lands on) must be exact: counts of non-zero grid points match."""
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import FIXTURES, fixture, fixture_plane, random_task, rel_err
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+
+
+# ---------------------------------------------------------------- drop-in
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_dropin_fixtures(gpw, oracle, torch_cuda, name):
+    say("dropin fixture", name)
+    t = fixture(name)
+    grid = np.zeros((t["ngrid"][2], t["ngrid"][1], t["ngrid"][0]))
+    if not t["use_ortho"]:
+        with pytest.raises(gpw.GpwError) as e:
+            gpw.collocate_pgf_product_rspace(t, t["pab"], grid)
+        assert e.value.code == 7  # general path not built in the product yet
+        return
+    gpw.collocate_pgf_product_rspace(t, t["pab"], grid)
+    ref = oracle.collocate(t, t["pab"])
+    assert np.count_nonzero(grid) == np.count_nonzero(ref)
+    assert rel_err(grid, ref) < 1e-13
+    plane = fixture_plane(t)
+    if plane.any():
+        assert rel_err(grid[:, :, 0], plane) < 1e-13
+    else:
+        assert not grid[:, :, 0].any()
+
+
+@pytest.mark.parametrize("tau", [False, True])
+@pytest.mark.parametrize("window", [False, True])
+@pytest.mark.parametrize("la,lb", [((0, 0), (0, 0)), ((0, 2), (1, 2)), ((3, 3), (2, 3))])
+def test_dropin_random(gpw, oracle, torch_cuda, tau, window, la, lb):
+    say("dropin random", tau, window, la, lb)
+    rng = np.random.default_rng(100 + 10 * tau + 20 * window + la[1] + 4 * lb[1])
+    kw = {}
+    if window:
+        kw = dict(npts=(30, 20, 32), cell=(9.0, 6.0, 9.5), periodic=(0, 1, 0),
+                  window=(24, 20, 24), zeta=2.0, zetb=2.0)
+    t, pab = random_task(rng, oracle, la=la, lb=lb, tau=tau, **kw)
+    if window:
+        t["ra"] = np.array([4.5, 3.0, 4.7]); t["rab"] = np.array([0.2, -0.3, 0.1])
+        t["lb_grid"] = [-2, 0, -5]
+    ref = oracle.collocate(t, pab)
+    grid = np.zeros_like(ref)
+    gpw.collocate_pgf_product_rspace(t, pab, grid)
+    assert np.count_nonzero(grid) == np.count_nonzero(ref)
+    assert rel_err(grid, ref) < TOL
+    W = rng.uniform(-1, 1, ref.shape)
+    h_ref, f_ref = oracle.integrate(t, W, pab=pab, forces=True)
+    h, f = gpw.integrate_pgf_product_rspace(t, W, pab=pab, forces=True)
+    assert rel_err(h, h_ref) < TOL
+    assert rel_err(f, f_ref) < TOL
+
+
+# ---------------------------------------------------------------- task lists
+
+
+def run_tasklist(gpw, torch, cfg, plain=False, splits=1, forces=False, tau=False, W=None):
+    g = gpw.grid_desc(cfg["npts"], cfg["dh"])
+    tl = gpw.TaskList(g, cfg["pairs"], compute_tau=tau, splits=splits)
+    n = cfg["npts"]
+    grid = torch.zeros((n[2], n[1], n[0]), dtype=torch.float64, device="cuda")
+    pab = dev(torch, cfg["pab"])
+    tl.collocate(pab, grid, plain=plain)
+    out = {"grid": grid.cpu().numpy(), "stats": tl.stats()}
+    if W is not None:
+        hab = torch.zeros(cfg["nblock"], dtype=torch.float64, device="cuda")
+        fr = torch.zeros((len(cfg["pairs"]["la_max"]), 6), dtype=torch.float64, device="cuda") if forces else None
+        tl.integrate(dev(torch, W), hab, pab=pab, forces=fr, plain=plain)
+        out["hab"] = hab.cpu().numpy()
+        out["forces"] = None if fr is None else fr.cpu().numpy().reshape(-1, 2, 3)
+    torch.cuda.synchronize()
+    tl.close()
+    return out
+
+
+def oracle_batch(oracle, cfg, W=None, forces=False, tau=False):
+    g = oracle.make_grid(cfg["npts"], cfg["dh"])
+    pairs = oracle.make_pairs(cfg["pairs"])
+    grid = oracle.batch_collocate(g, pairs, cfg["pab"], compute_tau=tau)
+    out = {"grid": grid}
+    if W is not None:
+        hab, f = oracle.batch_integrate(g, pairs, W, cfg["nblock"], pab=cfg["pab"], forces=forces,
+                                        compute_tau=tau)
+        out["hab"] = hab
+        out["forces"] = f
+    return out
+
+
+@pytest.mark.parametrize("plain", [False, True])
+def test_config1_single_pair(gpw, oracle, torch_cuda, plain):
+    from libgpw_amd import workloads as Wk
+    say("config1", "plain" if plain else "batched")
+    cfg = Wk.config(1)
+    W = Wk.potential(cfg)
+    got = run_tasklist(gpw, torch_cuda, cfg, plain=plain, forces=True, W=W)
+    ref = oracle_batch(oracle, cfg, W=W, forces=True)
+    assert np.count_nonzero(got["grid"]) == np.count_nonzero(ref["grid"])
+    assert rel_err(got["grid"], ref["grid"]) < TOL
+    assert rel_err(got["hab"], ref["hab"]) < TOL
+    assert rel_err(got["forces"], ref["forces"]) < TOL
+
+
+@pytest.mark.parametrize("splits", [1, 3])
+def test_config2_subset_batched_and_plain(gpw, oracle, torch_cuda, splits):
+    from libgpw_amd import workloads as Wk
+    say("config2 subset 600 pairs splits", splits)
+    cfg = Wk.subset(Wk.config(2), np.arange(600))
+    W = Wk.potential(cfg)
+    ref = oracle_batch(oracle, cfg, W=W)
+    b = run_tasklist(gpw, torch_cuda, cfg, splits=splits, W=W)
+    assert rel_err(b["grid"], ref["grid"]) < TOL
+    assert rel_err(b["hab"], ref["hab"]) < TOL
+    if splits == 1:
+        p = run_tasklist(gpw, torch_cuda, cfg, plain=True, W=W)
+        assert rel_err(p["grid"], ref["grid"]) < TOL
+        assert rel_err(p["hab"], ref["hab"]) < TOL
+
+
+def test_config2_full_size(gpw, oracle, torch_cuda):
+    """All 20,000 pairs on 128^3: oracle parity plus the adjoint identity."""
+    from libgpw_amd import workloads as Wk
+    say("config2 full 20000 pairs")
+    cfg = Wk.config(2)
+    W = Wk.potential(cfg)
+    got = run_tasklist(gpw, torch_cuda, cfg, W=W)
+    ref = oracle_batch(oracle, cfg, W=W)
+    assert rel_err(got["grid"], ref["grid"]) < TOL
+    assert rel_err(got["hab"], ref["hab"]) < TOL
+    lhs = float((got["grid"] * W).sum())
+    rhs = float((cfg["pab"] * got["hab"]).sum())
+    assert abs(lhs - rhs) < 1e-11 * np.abs(got["grid"]).sum()
+
+
+def test_config3_forces(gpw, oracle, torch_cuda):
+    from libgpw_amd import workloads as Wk
+    say("config3 forces 512 pairs")
+    cfg = Wk.subset(Wk.config(3), np.arange(512))
+    W = Wk.potential(cfg)
+    got = run_tasklist(gpw, torch_cuda, cfg, W=W, forces=True)
+    ref = oracle_batch(oracle, cfg, W=W, forces=True)
+    assert rel_err(got["grid"], ref["grid"]) < TOL
+    assert rel_err(got["hab"], ref["hab"]) < TOL
+    assert rel_err(got["forces"], ref["forces"]) < TOL
+
+
+def test_tau_batched(gpw, oracle, torch_cuda):
+    from libgpw_amd import workloads as Wk
+    say("tau batched 300 pairs")
+    cfg = Wk.subset(Wk.config(3), np.arange(300))
+    W = Wk.potential(cfg)
+    got = run_tasklist(gpw, torch_cuda, cfg, W=W, tau=True, forces=True)
+    ref = oracle_batch(oracle, cfg, W=W, tau=True, forces=True)
+    assert rel_err(got["grid"], ref["grid"]) < TOL
+    assert rel_err(got["hab"], ref["hab"]) < TOL
+    assert rel_err(got["forces"], ref["forces"]) < TOL
+
+
+def test_batched_is_deterministic(gpw, torch_cuda):
+    from libgpw_amd import workloads as Wk
+    say("determinism 2000 pairs")
+    cfg = Wk.subset(Wk.config(2), np.arange(2000))
+    W = Wk.potential(cfg)
+    a = run_tasklist(gpw, torch_cuda, cfg, W=W)
+    b = run_tasklist(gpw, torch_cuda, cfg, W=W)
+    assert np.array_equal(a["grid"], b["grid"])
+    assert np.array_equal(a["hab"], b["hab"])
+
+
+def test_collocate_accumulates(gpw, oracle, torch_cuda):
+    from libgpw_amd import workloads as Wk
+    say("accumulate")
+    cfg = Wk.subset(Wk.config(2), np.arange(100))
+    g = gpw.grid_desc(cfg["npts"], cfg["dh"])
+    tl = gpw.TaskList(g, cfg["pairs"])
+    n = cfg["npts"]
+    base = np.random.default_rng(1).uniform(-1, 1, (n[2], n[1], n[0]))
+    grid = dev(torch_cuda, base)
+    tl.collocate(dev(torch_cuda, cfg["pab"]), grid)
+    ref = oracle_batch(oracle, cfg)["grid"] + base
+    assert rel_err(grid.cpu().numpy(), ref) < 1e-14
+
+
+@pytest.mark.parametrize("shape", [(14, 23, 17, 3), (14, 23, 17, 1), (1, 12, 29, 5), (13, 45, 29, 7)])
+def test_libgpw_core(gpw, oracle, torch_cuda, shape):
+    say("libgpw core", shape)
+    i, j, k, L = shape
+    rng = np.random.default_rng(sum(shape))
+    P = rng.uniform(-1, 1, (3, L, max(i, j, k)))
+    co = rng.uniform(-1, 1, (L, L, L))
+    V = gpw.collocate_core((i, j, k), dev(torch_cuda, co), dev(torch_cuda, P)).cpu().numpy()
+    assert np.abs(V - oracle.collocate_core((i, j, k), co, P)).max() < 1e-14 * max(1, np.abs(V).max())
+    Wg = rng.uniform(-1, 1, (i, j, k))
+    out = gpw.integrate_core((i, j, k), dev(torch_cuda, P), dev(torch_cuda, Wg)).cpu().numpy()
+    ref = oracle.integrate_core((i, j, k), P, Wg)
+    assert np.abs(out - ref).max() < 1e-13 * max(1, np.abs(ref).max())

@@ -1,0 +1,161 @@
+"""CPU-only checks of the product library: the C ABI loads and exports every
+declared symbol, the host planner's index sets equal the oracle's, the
+work accounting is right, and argument errors come back as status codes
+before any device work."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import FIXTURES, fixture, random_task
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    txt = open(os.path.join(ROOT, "include", "libgpw_amd.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w[\w\s\*]*?\b(gpw_\w+)\s*\(", txt, re.M)))
+
+
+def test_library_exports_header_symbols(gpw):
+    syms = header_symbols()
+    assert len(syms) >= 12
+    out = subprocess.run(["nm", "-D", "--defined-only", gpw.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r"\sT\s(\w+)$", out, re.M))
+    missing = [s for s in syms if s not in exported]
+    assert not missing, missing
+    assert set(gpw.EXPORTS) == set(syms)
+
+
+def test_planner_index_sets_match_oracle(gpw, oracle):
+    from libgpw_amd import workloads as W
+    for idx in (2, 3):
+        cfg = W.config(idx, npairs=200)
+        g = gpw.grid_desc(cfg["npts"], cfg["dh"])
+        dh_inv = np.linalg.inv(cfg["dh"])
+        p = cfg["pairs"]
+        for i in range(0, 200, 7):
+            geo = gpw.pair_geometry(g, p, i)
+            lb, ub, sb = oracle.sphere_bounds(p["radius"][i], np.diag(cfg["dh"]))
+            assert geo["lb_cube"] == lb and geo["ub_cube"] == ub
+            assert np.array_equal(geo["sphere_bounds"], sb)
+            f = p["zetb"][i] / (p["zeta"][i] + p["zetb"][i])
+            rp = p["ra"][i] + f * p["rab"][i]
+            assert geo["cube_centre"] == [int(np.floor(rp[d] * dh_inv[d, d])) for d in range(3)]
+
+
+@pytest.mark.parametrize("name", ["collocate_ortho_density", "collocate_ortho_tau"])
+def test_planner_reproduces_fixture_bounds(gpw, name):
+    t = fixture(name)
+    g = gpw.grid_desc(t["npts"], t["dh"])
+    pairs = {"la_max": [t["la_max"]], "la_min": [t["la_min"]], "lb_max": [t["lb_max"]],
+             "lb_min": [t["lb_min"]], "zeta": [t["zeta"]], "zetb": [t["zetb"]],
+             "rscale": [t["rscale"]], "ra": [t["ra"]], "rab": [t["rab"]], "radius": [t["radius"]],
+             "block_offset": [0]}
+    geo = gpw.pair_geometry(g, pairs, 0, compute_tau=t["compute_tau"])
+    assert geo["lb_cube"] == t["lb_cube"] and geo["ub_cube"] == t["ub_cube"]
+    n = len(geo["sphere_bounds"])
+    assert np.array_equal(geo["sphere_bounds"], t["sphere_bounds"][:n])
+    assert not np.asarray(t["sphere_bounds"][n:]).any()
+
+
+def sphere_count(sb, lb, ub):
+    """points of the sphere_bounds walk (grid_base_ref_c.cc:413-474), 8 per block"""
+    n, sci = 0, 0
+    kgmin = sb[sci]; sci += 1
+    for kg in range(kgmin, 1):
+        jgmin = sb[sci]; sci += 1
+        for jg in range(jgmin, 1):
+            igmin = sb[sci]; sci += 1
+            n += 8 * (1 - igmin)
+    return n
+
+
+def test_plan_stats_counts_sphere_points(gpw, oracle):
+    from libgpw_amd import workloads as W
+    cfg = W.config(2, npairs=300)
+    g = gpw.grid_desc(cfg["npts"], cfg["dh"])
+    st = gpw.plan_stats(g, cfg["pairs"])
+    p = cfg["pairs"]
+    total = 0
+    for i in range(300):
+        lb, ub, sb = oracle.sphere_bounds(p["radius"][i], np.diag(cfg["dh"]))
+        total += sphere_count(sb, lb, ub)
+    assert st["cube_points"] == total
+    assert st["npairs"] == 300 and st["nsub"] >= 300 and st["nentries"] >= st["nsub"]
+    assert st["tile_points"] == st["nentries"] * 32 * 16 * 16
+
+
+def test_plan_splits_wide_cubes(gpw):
+    # a cube wider than the period is cut into sub-boxes that each fit it
+    from libgpw_amd import workloads as W
+    cfg = W.config(2, npairs=50)
+    p = cfg["pairs"]
+    p["zeta"][:] = 0.05
+    p["zetb"][:] = 0.05
+    p["radius"][:] = W.radius_from_eps(0.1)  # ~15 bohr: cube ~ 2 periods of 15 bohr
+    g = gpw.grid_desc(cfg["npts"], cfg["dh"])
+    st = gpw.plan_stats(g, p)
+    assert st["nsub"] >= 50 * 8
+    # every grid tile is touched by every pair
+    assert st["nentries"] >= 50 * st["ntiles"]
+
+
+def _single(t, pab, gpw):
+    grid = np.zeros((t["ngrid"][2], t["ngrid"][1], t["ngrid"][0]))
+    gpw.collocate_pgf_product_rspace(t, pab, grid)
+
+
+def test_dropin_argument_errors(gpw, oracle):
+    rng = np.random.default_rng(0)
+    t, pab = random_task(rng, oracle, la=(0, 2), lb=(0, 2))
+    cases = [
+        (dict(t, lmax=1), 2),                                   # lmax below la_max
+        (dict(t, la_min=3), 1),                               # la_min > la_max
+        (dict(t, ngrid=[t["ngrid"][0] - 1] + t["ngrid"][1:]), 1),  # periodic, ngrid != npts
+        (dict(t, o1=t["maxco"]), 1),                          # block outside pab
+    ]
+    bad_sb = np.array(t["sphere_bounds"]).copy()
+    bad_sb[0] = t["lb_cube"][2] - 1
+    cases.append((dict(t, sphere_bounds=bad_sb), 4))
+    cases.append((dict(t, sphere_bounds=np.array(t["sphere_bounds"][:3])), 4))
+    for tt, code in cases:
+        with pytest.raises(gpw.GpwError) as e:
+            _single(tt, pab, gpw)
+        assert e.value.code == code, (e.value, code)
+
+
+def test_tasklist_argument_errors(gpw):
+    from libgpw_amd import workloads as W
+    cfg = W.config(2, npairs=4)
+    g = gpw.grid_desc(cfg["npts"], cfg["dh"])
+    p = {k: np.array(v, copy=True) for k, v in cfg["pairs"].items()}
+    p["la_max"][0] = 9
+    with pytest.raises(gpw.GpwError) as e:
+        gpw.plan_stats(g, p)
+    assert e.value.code == 1
+    p = {k: np.array(v, copy=True) for k, v in cfg["pairs"].items()}
+    p["radius"][1] = 0.0
+    with pytest.raises(gpw.GpwError):
+        gpw.plan_stats(g, p)
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_taskfile_roundtrip(tmp_path, name):
+    from libgpw_amd.taskfile import read_task, write_task
+    t = fixture(name)
+    grid = np.zeros((t["ngrid"][2], t["ngrid"][1], t["ngrid"][0]))
+    for z, y, x, v in t["grid_entries"]:
+        grid[z, y, x] = v
+    path = tmp_path / "t.task"
+    write_task(str(path), t, grid)
+    u = read_task(str(path))
+    for k in ("la_max", "la_min", "lb_max", "lb_min", "maxco", "o1", "o2", "lmax", "npts", "ngrid"):
+        assert u[k] == t[k]
+    for k in ("zeta", "zetb", "rscale", "radius"):
+        assert u[k] == t[k]
+    assert np.array_equal(u["pab"], t["pab"]) and np.array_equal(u["dh"], t["dh"])
+    assert u["grid_entries"] == t["grid_entries"]
