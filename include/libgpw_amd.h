@@ -147,6 +147,22 @@ gpw_status gpw_tasklist_integrate_plain(gpw_tasklist *tl, const double *grid_dev
                                         const double *pab_dev, double *hab_dev,
                                         double *forces_dev, void *stream);
 
+/* Per-phase device timing (hipEvents recorded on the launch stream).
+ * gpw_tasklist_set_timing(tl, 1) creates the events (do it outside timed or
+ * captured regions); gpw_tasklist_timings waits for the recorded work and
+ * returns the summed milliseconds per phase since the previous query in
+ * ms[0..GPW_NPHASES-1] (and the number of timed calls per phase in calls,
+ * which may be NULL), then resets. Phases: */
+#define GPW_PHASE_COEF 0          /* pab -> product-centre coefficients  */
+#define GPW_PHASE_COLLOCATE 1     /* collocate tile kernels              */
+#define GPW_PHASE_REDUCE 2        /* sum of collocate split buffers      */
+#define GPW_PHASE_INTEGRATE 3     /* integrate tile kernels              */
+#define GPW_PHASE_MOMENTS 4       /* per-pair sums of partial moments    */
+#define GPW_PHASE_HAB 5           /* back-transform to hab and forces    */
+#define GPW_NPHASES 6
+gpw_status gpw_tasklist_set_timing(gpw_tasklist *tl, int enable);
+gpw_status gpw_tasklist_timings(gpw_tasklist *tl, double *ms, int n, int *calls);
+
 /* ---- libgpw tensor-path contraction (device memory) ---------------------- */
 /* collocate_core (src/lib/collocate.cc:543):
  *   V[z][y][x] = sum co[a][g][b] * P[0][g][z] * P[1][b][y] * P[2][a][x]

@@ -27,8 +27,10 @@ EXPORTS = (
     "gpw_tasklist_plan_stats", "gpw_pair_geometry",
     "gpw_tasklist_collocate", "gpw_tasklist_integrate",
     "gpw_tasklist_collocate_plain", "gpw_tasklist_integrate_plain",
+    "gpw_tasklist_set_timing", "gpw_tasklist_timings",
     "gpw_collocate_core", "gpw_integrate_core", "gpw_last_error",
 )
+PHASES = ("coef", "collocate", "reduce", "integrate", "moments", "hab")
 
 
 class GpwError(RuntimeError):
@@ -68,6 +70,10 @@ def lib():
             raise RuntimeError(
                 f"libgpw_amd: HIP library not built ({LIB_PATH}); run __graft_entry__.build() "
                 "or python libgpw_amd/build.py")
+        # torch links the HIP runtime by its unversioned name and would load a
+        # second copy next to ours (SONAME libamdhip64.so.7) if ours came
+        # first; loading torch first makes both use the one runtime.
+        import torch  # noqa: F401
         L = C.CDLL(LIB_PATH)
         dp, ip, vp = C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_void_p
         single = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
@@ -87,6 +93,8 @@ def lib():
         L.gpw_tasklist_integrate.argtypes = [vp, vp, vp, vp, vp, vp]
         L.gpw_tasklist_collocate_plain.argtypes = [vp, vp, vp, vp]
         L.gpw_tasklist_integrate_plain.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.gpw_tasklist_set_timing.argtypes = [vp, C.c_int]
+        L.gpw_tasklist_timings.argtypes = [vp, dp, C.c_int, ip]
         L.gpw_collocate_core.argtypes = [ip, C.c_int, vp, vp, C.c_int, vp, vp]
         L.gpw_integrate_core.argtypes = [ip, C.c_int, vp, C.c_int, vp, vp, vp]
         L.gpw_last_error.argtypes = []
@@ -213,6 +221,16 @@ class TaskList:
         s = TaskListStats()
         _check(lib().gpw_tasklist_stats_get(self._h, C.byref(s)))
         return _stats_dict(s)
+
+    def set_timing(self, enable: bool = True):
+        _check(lib().gpw_tasklist_set_timing(self._h, int(enable)))
+
+    def timings(self) -> dict:
+        """{phase: (total ms, calls)} since the previous query (waits for the GPU)."""
+        ms = (C.c_double * len(PHASES))()
+        calls = (C.c_int * len(PHASES))()
+        _check(lib().gpw_tasklist_timings(self._h, ms, len(PHASES), calls))
+        return {p: (ms[i], calls[i]) for i, p in enumerate(PHASES)}
 
     def collocate(self, pab, grid, stream=None, plain=False):
         f = lib().gpw_tasklist_collocate_plain if plain else lib().gpw_tasklist_collocate

@@ -9,12 +9,13 @@
 // Integrate is the exact transpose of that contraction.
 //
 // Work decomposition (MI355X-first, not a translation of the reference loops):
-//   * tile kernels: one workgroup owns a TX x TY x TZ tile of the grid, one
-//     grid row (TX points along x) per thread, held in registers; it sweeps
-//     every (sub-box, tile) work item of its tile, staging per-pair 1D tables
-//     and the z-stage coefficients in LDS, chunk of Q pairs per barrier. Each
-//     grid point is written by exactly one workgroup: no atomics, results are
-//     deterministic.
+//   * tile kernels: one workgroup owns a TX x TY x TZ tile of the grid, each
+//     thread a 4-row x 8-point register block of it; the workgroup sweeps every
+//     (sub-box, tile) work item of its tile, staging per-pair 1D tables and
+//     z-stage coefficients in LDS, Q pairs per barrier. Work items are
+//     bucketed by polynomial length into classes, one kernel instantiation
+//     each. Each grid point is written by exactly one workgroup (or one split
+//     buffer): no atomics, results are deterministic.
 //   * plain kernels: one launch per sub-box of one pair (the baseline form).
 #include <hip/hip_runtime.h>
 
@@ -204,213 +205,284 @@ __global__ __launch_bounds__(64) void k_coef(const PairDev *__restrict__ pairs, 
 
 // ---------------------------------------------------------------------------
 // tile kernels
+//
+// Workgroup = one TX x TY x TZ tile (32 x 16 x 16 points), 256 threads. Thread
+// t owns a register block of RB = 4 consecutive rows (same z) x XB = 8 points:
+//   xg = t & 3 (x block), yb = (t >> 2) & 3 (row block), z = t >> 4.
+// The 16 threads of one z plane are one DPP row (16 consecutive lanes).
+// Every LDS table value read in the x stage feeds RB FMAs.
 
 constexpr int Q = 4;   // pairs per LDS chunk (collocate)
 constexpr int QI = 2;  // pairs per LDS chunk (integrate)
+constexpr int XB = 8, RB = 4;
+constexpr int NXG = TX / XB;  // x blocks per tile
 static_assert(TX + TY + TZ == 64, "table phase maps one wave per pair");
 static_assert(Q * 64 == NROW, "table phase uses the whole workgroup");
+static_assert(NXG * (TY / RB) * TZ == NROW, "one register block per thread");
+static_assert(NXG * (TY / RB) == 16, "one z plane per DPP row");
+
+template <int LM>
+struct XLayout {
+    static constexpr int GS = XB * LM + 4;  // doubles per x block (+32 B to spread banks)
+    static constexpr int SIZE = NXG * GS;
+    __device__ static int at(int x) { return (x / XB) * GS + (x % XB) * LM; }
+};
 
 struct TileMeta {
-    int lp, xa, xb, prof_off, prof_stride, valid;
+    int lp, prof_off, prof_stride, valid;
 };
 
 // Phase 1: 1D tables of one pair over the tile. Wave w fills pair slot w:
-// lanes 0..TX-1 the x table, then TY y entries, then TZ z entries.
+// lanes 0..TX-1 the x table, then TY y entries, then TZ z entries. The
+// per-axis fields are read from global memory with the lane's axis as index
+// (a private copy of the struct indexed that way would live in scratch).
 template <int LM>
-__device__ __forceinline__ void tile_tables(const SubDev &s, int lane, int x0, int y0, int z0,
-                                            const TileGrid &tg, double (*X)[LM],
+__device__ __forceinline__ void tile_tables(const SubDev *__restrict__ sp, int lp_add, int lane,
+                                            int x0, int y0, int z0, const TileGrid &tg, double *X,
                                             double (*Y)[LM], double (*Z)[LM], int *xi, int *yj,
                                             int *zk, TileMeta &meta) {
-    // per-axis values picked with selects: indexing the struct by a
-    // lane-dependent axis would force it into scratch memory
-    int d, pos, g, nmax, lb, ub, base, period;
-    double dr, roff;
-    double(*T)[LM];
-    int *code;
-    if (lane < TX) {
-        d = 0; pos = lane; g = x0 + lane; nmax = tg.nx; T = X; code = xi;
-        lb = s.lb[0]; ub = s.ub[0]; base = s.base[0]; period = s.period[0];
-        dr = tg.dr[0]; roff = s.roff[0];
-    } else if (lane < TX + TY) {
-        d = 1; pos = lane - TX; g = y0 + pos; nmax = tg.ny; T = Y; code = yj;
-        lb = s.lb[1]; ub = s.ub[1]; base = s.base[1]; period = s.period[1];
-        dr = tg.dr[1]; roff = s.roff[1];
-    } else {
-        d = 2; pos = lane - TX - TY; g = z0 + pos; nmax = tg.nz; T = Z; code = zk;
-        lb = s.lb[2]; ub = s.ub[2]; base = s.base[2]; period = s.period[2];
-        dr = tg.dr[2]; roff = s.roff[2];
-    }
+    const int d = lane < TX ? 0 : lane < TX + TY ? 1 : 2;
+    const int pos = d == 0 ? lane : d == 1 ? lane - TX : lane - TX - TY;
+    const int g = (d == 0 ? x0 : d == 1 ? y0 : z0) + pos;
+    const int nmax = d == 0 ? tg.nx : d == 1 ? tg.ny : tg.nz;
+    const double dr = d == 0 ? tg.dr[0] : d == 1 ? tg.dr[1] : tg.dr[2];
+    double *T = d == 0 ? X + XLayout<LM>::at(lane) : d == 1 ? Y[pos] : Z[pos];
+    int *code = d == 0 ? xi : d == 1 ? yj : zk;
+    const int lb = sp->lb[d], ub = sp->ub[d], base = sp->base[d], period = sp->period[d];
+    const double roff = sp->roff[d];
+    const double zetp = sp->zetp;
+    const int lp = sp->lp + lp_add;
     int i = BIG;
     bool ok = g < nmax;
     if (ok) {
         i = lb + imod(g - base, period);
         ok = i <= ub;
     }
-    const int lp = s.lp;
     if (ok) {
         const double r = i * dr - roff;
-        double v = exp(-s.zetp * r * r);
+        double v = exp(-zetp * r * r);
         for (int k = 0; k <= lp; k++) {
-            T[pos][k] = v;
+            T[k] = v;
             v *= r;
         }
         code[pos] = fold_off(i);
     } else {
-        for (int k = 0; k <= lp; k++) T[pos][k] = 0.0;
+        for (int k = 0; k <= lp; k++) T[k] = 0.0;
         code[pos] = d == 0 ? BIG : -1;
     }
-    // x-range of the pair inside the tile (wave-uniform)
     const unsigned long long m = __ballot(ok && d == 0);
     if (lane == 0) {
         meta.lp = lp;
-        meta.prof_off = s.prof_off;
-        meta.prof_stride = s.prof_stride;
+        meta.prof_off = sp->prof_off;
+        meta.prof_stride = sp->prof_stride;
         meta.valid = m != 0ull;
-        meta.xa = m ? __builtin_ctzll(m) : 1;
-        meta.xb = m ? 63 - __builtin_clzll(m) : 0;
+    }
+}
+
+// Phase 2 helpers: row thresholds and x-block classes of pair slot q.
+// T[row] (row = z*TY + y): offsets with fold_off(i) <= T[row] are inside.
+// bmax/bmin over a block: max fold (BIG if any x invalid) / min over valid x.
+__device__ __forceinline__ void tile_rows(const int *__restrict__ prof, const TileMeta &meta,
+                                          const int *yj, const int *zk, const int *xi, int *T,
+                                          int *bmax, int *bmin, int t) {
+    const int z = t / TY, y = t % TY;
+    T[t] = meta.valid ? row_threshold(prof, meta.prof_off, meta.prof_stride, zk[z], yj[y]) : -1;
+    if (t < NXG) {
+        int mx = 0, mn = BIG;
+        for (int x = 0; x < XB; x++) {
+            const int v = xi[t * XB + x];
+            mx = max(mx, v);
+            mn = min(mn, v);
+        }
+        bmax[t] = mx;
+        bmin[t] = mn;
     }
 }
 
 template <int LM>
 struct CollocSmem {
-    double X[Q][TX][LM];
+    double X[Q][XLayout<LM>::SIZE];
     double Y[Q][TY][LM];
     double Z[Q][TZ][LM];
     double cxy[Q][TZ][LM * (LM + 1) / 2];
     int xi[Q][TX];
     int yj[Q][TY];
     int zk[Q][TZ];
+    int T[Q][NROW];
+    int bmax[Q][NXG], bmin[Q][NXG];
     TileMeta meta[Q];
 };
 
+// register block of one thread for one pair: rows y = 4*yb + r of plane z,
+// points x = 8*xg + j
 template <int L, int LM>
-__device__ __forceinline__ void colloc_row(const CollocSmem<LM> &sm, int q, int ry, int rz, int T,
-                                           double (&acc)[TX]) {
+__device__ __forceinline__ void colloc_block(const CollocSmem<LM> &sm, int q, int xg, int yb, int z,
+                                             const int (&Tr)[RB], bool full,
+                                             double (&acc)[RB][XB]) {
     constexpr int lp = L - 1;
-    double cx[L];
+    double cx[RB][L];
+    const double *cxy = sm.cxy[q][z];
 #pragma unroll
-    for (int l = 0; l < L; l++) cx[l] = 0.0;
-    const double *cxy = sm.cxy[q][rz];
-    const double *Yr = sm.Y[q][ry];
+    for (int r = 0; r < RB; r++) {
+        const double *Yr = sm.Y[q][yb * RB + r];
 #pragma unroll
-    for (int lyp = 0; lyp <= lp; lyp++) {
-        const double y = Yr[lyp];
+        for (int l = 0; l < L; l++) cx[r][l] = 0.0;
 #pragma unroll
-        for (int lxp = 0; lxp <= lp - lyp; lxp++) cx[lxp] += cxy[c2idx(lp, lyp, lxp)] * y;
-    }
-    const int xa = __builtin_amdgcn_readfirstlane(sm.meta[q].xa);
-    const int xb = __builtin_amdgcn_readfirstlane(sm.meta[q].xb);
+        for (int lyp = 0; lyp <= lp; lyp++) {
+            const double y = Yr[lyp];
 #pragma unroll
-    for (int x = 0; x < TX; x++) {
-        if (x >= xa && x <= xb) {
-            const double *Xr = sm.X[q][x];
-            double v = 0.0;
-#pragma unroll
-            for (int l = 0; l < L; l++) v += cx[l] * Xr[l];
-            acc[x] += sm.xi[q][x] <= T ? v : 0.0;
+            for (int lxp = 0; lxp <= lp - lyp; lxp++) cx[r][lxp] += cxy[c2idx(lp, lyp, lxp)] * y;
         }
     }
-}
-
-template <int LM>
-__device__ __forceinline__ void colloc_row_dispatch(const CollocSmem<LM> &sm, int q, int lp,
-                                                    int ry, int rz, int T, double (&acc)[TX]) {
-    switch (lp) {
-    case 0: colloc_row<1, LM>(sm, q, ry, rz, T, acc); break;
-    case 1: colloc_row<2, LM>(sm, q, ry, rz, T, acc); break;
-    case 2: colloc_row<3, LM>(sm, q, ry, rz, T, acc); break;
-    case 3: colloc_row<4, LM>(sm, q, ry, rz, T, acc); break;
-    case 4: colloc_row<5, LM>(sm, q, ry, rz, T, acc); break;
-    case 5: colloc_row<6, LM>(sm, q, ry, rz, T, acc); break;
-    case 6: colloc_row<7, LM>(sm, q, ry, rz, T, acc); break;
-    case 7: colloc_row<8, LM>(sm, q, ry, rz, T, acc); break;
-    default:
-        if constexpr (LM > 8) {
-            switch (lp) {
-            case 8: colloc_row<9, LM>(sm, q, ry, rz, T, acc); break;
-            case 9: colloc_row<10, LM>(sm, q, ry, rz, T, acc); break;
-            case 10: colloc_row<11, LM>(sm, q, ry, rz, T, acc); break;
-            case 11: colloc_row<12, LM>(sm, q, ry, rz, T, acc); break;
-            case 12: colloc_row<13, LM>(sm, q, ry, rz, T, acc); break;
-            case 13: colloc_row<14, LM>(sm, q, ry, rz, T, acc); break;
-            case 14: colloc_row<15, LM>(sm, q, ry, rz, T, acc); break;
-            case 15: colloc_row<16, LM>(sm, q, ry, rz, T, acc); break;
-            default: colloc_row<17, LM>(sm, q, ry, rz, T, acc); break;
+    const double *Xg = sm.X[q] + xg * XLayout<LM>::GS;
+    if (full) {
+#pragma unroll
+        for (int j = 0; j < XB; j++) {
+            double xv[L];
+#pragma unroll
+            for (int l = 0; l < L; l++) xv[l] = Xg[j * LM + l];
+#pragma unroll
+            for (int r = 0; r < RB; r++) {
+                double v = 0.0;
+#pragma unroll
+                for (int l = 0; l < L; l++) v += cx[r][l] * xv[l];
+                acc[r][j] += v;
             }
         }
-        break;
+    } else {
+#pragma unroll
+        for (int j = 0; j < XB; j++) {
+            double xv[L];
+#pragma unroll
+            for (int l = 0; l < L; l++) xv[l] = Xg[j * LM + l];
+            const int code = sm.xi[q][xg * XB + j];
+#pragma unroll
+            for (int r = 0; r < RB; r++) {
+                double v = 0.0;
+#pragma unroll
+                for (int l = 0; l < L; l++) v += cx[r][l] * xv[l];
+                acc[r][j] += code <= Tr[r] ? v : 0.0;
+            }
+        }
     }
 }
 
-// out_mode 0: grid[g] += acc (one workgroup per tile); 1: partial[split][g] = acc
+#define GPW_LP_DISPATCH(LM, CALL)                                                   \
+    switch (lp) {                                                                   \
+    case 0: CALL(1); break;                                                         \
+    case 1: CALL(2); break;                                                         \
+    case 2: CALL(3); break;                                                         \
+    case 3: CALL(4); break;                                                         \
+    case 4: CALL(5); break;                                                         \
+    case 5: CALL(6); break;                                                         \
+    case 6: CALL(7); break;                                                         \
+    case 7: CALL(8); break;                                                         \
+    default:                                                                        \
+        if constexpr (LM > 8) {                                                     \
+            switch (lp) {                                                           \
+            case 8: CALL(9); break;                                                 \
+            case 9: CALL(10); break;                                                \
+            case 10: CALL(11); break;                                               \
+            case 11: CALL(12); break;                                               \
+            case 12: CALL(13); break;                                               \
+            case 13: CALL(14); break;                                               \
+            case 14: CALL(15); break;                                               \
+            case 15: CALL(16); break;                                               \
+            default: CALL(17); break;                                               \
+            }                                                                       \
+        }                                                                           \
+        break;                                                                      \
+    }
+
+// out_mode 0: grid += acc (splits == 1); 1: partial[split] = acc; 2: partial[split] += acc
 template <int LM>
-__global__ __launch_bounds__(NROW) void k_collocate_tiles(
+__global__ __launch_bounds__(NROW, 2) void k_collocate_tiles(
     TileGrid tg, const SubDev *__restrict__ subs, const int *__restrict__ tile_start,
-    const int *__restrict__ tile_ent, const double *__restrict__ coef,
-    const int *__restrict__ prof, int splits, double *__restrict__ out, int out_mode) {
+    const int *__restrict__ tile_list, const int *__restrict__ ent_sub,
+    const double *__restrict__ coef, const int *__restrict__ prof, int splits,
+    double *__restrict__ out, int out_mode) {
     __shared__ CollocSmem<LM> sm;
     const int tile = blockIdx.x / splits, split = blockIdx.x % splits;
     const int tix = tile % tg.ntx, tiy = (tile / tg.ntx) % tg.nty, tiz = tile / (tg.ntx * tg.nty);
     const int x0 = tix * TX, y0 = tiy * TY, z0 = tiz * TZ;
-    const int t = threadIdx.x, ry = t % TY, rz = t / TY;
+    const int t = threadIdx.x, xg = t & 3, yb = (t >> 2) & 3, z = t >> 4;
     const int beg = tile_start[tile], end = tile_start[tile + 1];
     const int per = (end - beg + splits - 1) / splits;
     const int mybeg = min(end, beg + split * per), myend = min(end, mybeg + per);
+    if (out_mode != 1 && mybeg >= myend) return;
 
-    if (out_mode == 0 && mybeg >= myend) return;
-    double acc[TX];
+    double acc[RB][XB];
 #pragma unroll
-    for (int x = 0; x < TX; x++) acc[x] = 0.0;
+    for (int r = 0; r < RB; r++)
+#pragma unroll
+        for (int j = 0; j < XB; j++) acc[r][j] = 0.0;
 
     for (int c = mybeg; c < myend; c += Q) {
         const int nq = min(Q, myend - c);
         {
             const int q = t / 64, lane = t % 64;
-            if (q < nq) {
-                const SubDev s = subs[tile_ent[c + q]];
-                tile_tables<LM>(s, lane, x0, y0, z0, tg, sm.X[q], sm.Y[q], sm.Z[q], sm.xi[q],
-                                sm.yj[q], sm.zk[q], sm.meta[q]);
-            }
+            if (q < nq)
+                tile_tables<LM>(subs + ent_sub[tile_list[c + q]], 0, lane, x0, y0, z0, tg, sm.X[q],
+                                sm.Y[q], sm.Z[q], sm.xi[q], sm.yj[q], sm.zk[q], sm.meta[q]);
         }
         __syncthreads();
-        // z-stage: cxy[q][z][(lyp,lxp)] = sum_lzp coef * Z^lzp
+#pragma unroll 1
         for (int q = 0; q < nq; q++) {
+            // z-stage: cxy[q][z][(lyp,lxp)] = sum_lzp coef * Z^lzp
             const int lp = sm.meta[q].lp;
             const int n2 = ncoef2(lp);
-            const int coff = subs[tile_ent[c + q]].coef_off;
+            const int coff = subs[ent_sub[tile_list[c + q]]].coef_off;
             for (int w = t; w < TZ * n2; w += NROW) {
-                const int z = w / n2, c2 = w % n2;
+                const int zz = w / n2, c2 = w % n2;
                 int lyp, lxp;
                 decode_c2(lp, c2, lyp, lxp);
                 double a = 0.0;
-                if (sm.zk[q][z] >= 0) {
+                if (sm.zk[q][zz] >= 0) {
                     for (int lzp = 0; lzp <= lp - lyp - lxp; lzp++)
-                        a += coef[coff + cidx(lp, lzp, lyp, lxp)] * sm.Z[q][z][lzp];
+                        a += coef[coff + cidx(lp, lzp, lyp, lxp)] * sm.Z[q][zz][lzp];
                 }
-                sm.cxy[q][z][c2] = a;
+                sm.cxy[q][zz][c2] = a;
             }
+            tile_rows(prof, sm.meta[q], sm.yj[q], sm.zk[q], sm.xi[q], sm.T[q], sm.bmax[q],
+                      sm.bmin[q], t);
         }
         __syncthreads();
+#pragma unroll 1
         for (int q = 0; q < nq; q++) {
-            if (!sm.meta[q].valid) continue;
-            const int T = row_threshold(prof, sm.meta[q].prof_off, sm.meta[q].prof_stride,
-                                        sm.zk[q][rz], sm.yj[q][ry]);
-            if (T >= 0) colloc_row_dispatch<LM>(sm, q, sm.meta[q].lp, ry, rz, T, acc);
+            int Tr[RB], tmin = BIG, tmax = -1;
+#pragma unroll
+            for (int r = 0; r < RB; r++) {
+                Tr[r] = sm.T[q][z * TY + yb * RB + r];
+                tmin = min(tmin, Tr[r]);
+                tmax = max(tmax, Tr[r]);
+            }
+            if (tmax < 0 || sm.bmin[q][xg] > tmax) continue;
+            const bool full = sm.bmax[q][xg] <= tmin;
+            const int lp = sm.meta[q].lp;
+#define GPW_COLLOC_CALL(L) colloc_block<L, LM>(sm, q, xg, yb, z, Tr, full, acc)
+            GPW_LP_DISPATCH(LM, GPW_COLLOC_CALL)
+#undef GPW_COLLOC_CALL
         }
         __syncthreads();
     }
-    const int gy = y0 + ry, gz = z0 + rz;
-    if (gy >= tg.ny || gz >= tg.nz) return;
-    const long long row = ((long long)gz * tg.ny + gy) * tg.nx;
-    if (out_mode == 0) {
+    const int gz = z0 + z;
+    if (gz >= tg.nz) return;
+    const long long ng = (long long)tg.nx * tg.ny * tg.nz;
+    double *dst = out_mode == 0 ? out : out + split * ng;
 #pragma unroll
-        for (int x = 0; x < TX; x++)
-            if (x0 + x < tg.nx) out[row + x0 + x] += acc[x];
-    } else {
-        const long long ng = (long long)tg.nx * tg.ny * tg.nz;
+    for (int r = 0; r < RB; r++) {
+        const int gy = y0 + yb * RB + r;
+        if (gy >= tg.ny) continue;
+        double *row = dst + ((long long)gz * tg.ny + gy) * tg.nx + x0 + xg * XB;
 #pragma unroll
-        for (int x = 0; x < TX; x++)
-            if (x0 + x < tg.nx) out[split * ng + row + x0 + x] = acc[x];
+        for (int j = 0; j < XB; j++) {
+            if (x0 + xg * XB + j < tg.nx) {
+                if (out_mode == 1)
+                    row[j] = acc[r][j];
+                else
+                    row[j] += acc[r][j];
+            }
+        }
     }
 }
 
@@ -424,146 +496,170 @@ __global__ void k_reduce_partials(const double *__restrict__ partial, int splits
     }
 }
 
-template <int LM>
-struct IntegSmem {
-    double X[QI][TX][LM];
-    double Y[QI][TY][LM];
-    double Z[QI][TZ][LM];
-    double tx[QI][NROW][LM];
-    double txy[QI][TZ][LM * (LM + 1) / 2];
-    int xi[QI][TX];
-    int yj[QI][TY];
-    int zk[QI][TZ];
-    TileMeta meta[QI];
-};
-
-template <int L, int LM>
-__device__ __forceinline__ void integ_row(IntegSmem<LM> &sm, int q, int T, const double (&v)[TX],
-                                          int t) {
-    double a[L];
-#pragma unroll
-    for (int l = 0; l < L; l++) a[l] = 0.0;
-    const int xa = __builtin_amdgcn_readfirstlane(sm.meta[q].xa);
-    const int xb = __builtin_amdgcn_readfirstlane(sm.meta[q].xb);
-#pragma unroll
-    for (int x = 0; x < TX; x++) {
-        if (x >= xa && x <= xb) {
-            const double w = sm.xi[q][x] <= T ? v[x] : 0.0;
-            const double *Xr = sm.X[q][x];
-#pragma unroll
-            for (int l = 0; l < L; l++) a[l] += w * Xr[l];
+// sum of a double over the 16 lanes of a DPP row; every lane gets the sum
+__device__ __forceinline__ double row_sum16(double v) {
+    auto step = [](double a, int ctrl) -> double {
+        const int lo = __double2loint(a), hi = __double2hiint(a);
+        int plo, phi;
+        switch (ctrl) {
+        case 0: plo = __builtin_amdgcn_update_dpp(0, lo, 0xB1, 0xF, 0xF, false);
+                phi = __builtin_amdgcn_update_dpp(0, hi, 0xB1, 0xF, 0xF, false); break;
+        case 1: plo = __builtin_amdgcn_update_dpp(0, lo, 0x4E, 0xF, 0xF, false);
+                phi = __builtin_amdgcn_update_dpp(0, hi, 0x4E, 0xF, 0xF, false); break;
+        case 2: plo = __builtin_amdgcn_update_dpp(0, lo, 0x124, 0xF, 0xF, false);
+                phi = __builtin_amdgcn_update_dpp(0, hi, 0x124, 0xF, 0xF, false); break;
+        default: plo = __builtin_amdgcn_update_dpp(0, lo, 0x128, 0xF, 0xF, false);
+                 phi = __builtin_amdgcn_update_dpp(0, hi, 0x128, 0xF, 0xF, false); break;
         }
-    }
-#pragma unroll
-    for (int l = 0; l < L; l++) sm.tx[q][t][l] = a[l];
+        return a + __hiloint2double(phi, plo);
+    };
+    v = step(v, 0);  // quad_perm [1,0,3,2]
+    v = step(v, 1);  // quad_perm [2,3,0,1]
+    v = step(v, 2);  // row_ror:4
+    v = step(v, 3);  // row_ror:8
+    return v;
 }
 
 template <int LM>
-__device__ __forceinline__ void integ_row_dispatch(IntegSmem<LM> &sm, int q, int lp, int T,
-                                                   const double (&v)[TX], int t) {
-    switch (lp) {
-    case 0: integ_row<1, LM>(sm, q, T, v, t); break;
-    case 1: integ_row<2, LM>(sm, q, T, v, t); break;
-    case 2: integ_row<3, LM>(sm, q, T, v, t); break;
-    case 3: integ_row<4, LM>(sm, q, T, v, t); break;
-    case 4: integ_row<5, LM>(sm, q, T, v, t); break;
-    case 5: integ_row<6, LM>(sm, q, T, v, t); break;
-    case 6: integ_row<7, LM>(sm, q, T, v, t); break;
-    case 7: integ_row<8, LM>(sm, q, T, v, t); break;
-    default:
-        if constexpr (LM > 8) {
-            switch (lp) {
-            case 8: integ_row<9, LM>(sm, q, T, v, t); break;
-            case 9: integ_row<10, LM>(sm, q, T, v, t); break;
-            case 10: integ_row<11, LM>(sm, q, T, v, t); break;
-            case 11: integ_row<12, LM>(sm, q, T, v, t); break;
-            case 12: integ_row<13, LM>(sm, q, T, v, t); break;
-            case 13: integ_row<14, LM>(sm, q, T, v, t); break;
-            case 14: integ_row<15, LM>(sm, q, T, v, t); break;
-            case 15: integ_row<16, LM>(sm, q, T, v, t); break;
-            default: integ_row<17, LM>(sm, q, T, v, t); break;
+struct IntegSmem {
+    double X[QI][XLayout<LM>::SIZE];
+    double Y[QI][TY][LM];
+    double Z[QI][TZ][LM];
+    double uz[QI][TZ][LM * (LM + 1) / 2];  // per-plane (lyp, lxp) moments
+    int xi[QI][TX];
+    int yj[QI][TY];
+    int zk[QI][TZ];
+    int T[QI][NROW];
+    int bmax[QI][NXG], bmin[QI][NXG];
+    TileMeta meta[QI];
+};
+
+// x stage on the register block, local y contraction, DPP sum over the z plane
+template <int L, int LM>
+__device__ __forceinline__ void integ_block(IntegSmem<LM> &sm, int q, int xg, int yb, int z,
+                                            const int (&Tr)[RB], int mode,
+                                            const double (&v)[RB][XB], int t) {
+    constexpr int lp = L - 1;
+    constexpr int N2 = L * (L + 1) / 2;
+    double u[N2];
+#pragma unroll
+    for (int c = 0; c < N2; c++) u[c] = 0.0;
+    if (mode != 0) {  // 0: nothing inside, 1: masked, 2: whole block inside
+        double tx[RB][L];
+#pragma unroll
+        for (int r = 0; r < RB; r++)
+#pragma unroll
+            for (int l = 0; l < L; l++) tx[r][l] = 0.0;
+        const double *Xg = sm.X[q] + xg * XLayout<LM>::GS;
+#pragma unroll
+        for (int j = 0; j < XB; j++) {
+            double xv[L];
+#pragma unroll
+            for (int l = 0; l < L; l++) xv[l] = Xg[j * LM + l];
+            const int code = sm.xi[q][xg * XB + j];
+#pragma unroll
+            for (int r = 0; r < RB; r++) {
+                const double w = (mode == 2 || code <= Tr[r]) ? v[r][j] : 0.0;
+#pragma unroll
+                for (int l = 0; l < L; l++) tx[r][l] += w * xv[l];
             }
         }
-        break;
+#pragma unroll
+        for (int r = 0; r < RB; r++) {
+            const double *Yr = sm.Y[q][yb * RB + r];
+#pragma unroll
+            for (int lyp = 0; lyp <= lp; lyp++) {
+                const double y = Yr[lyp];
+#pragma unroll
+                for (int lxp = 0; lxp <= lp - lyp; lxp++) u[c2idx(lp, lyp, lxp)] += tx[r][lxp] * y;
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < N2; c++) {
+        const double s = row_sum16(u[c]);
+        if ((t & 15) == 0) sm.uz[q][z][c] = s;
     }
 }
 
 // partial moments of every (sub-box, tile) item: partial[ent_off[e] + c3]
 template <int LM>
-__global__ __launch_bounds__(NROW) void k_integrate_tiles(
+__global__ __launch_bounds__(NROW, 2) void k_integrate_tiles(
     TileGrid tg, const SubDev *__restrict__ subs, const int *__restrict__ tile_start,
-    const int *__restrict__ tile_ent, const int *__restrict__ prof,
-    const double *__restrict__ grid, int lp_add, const long long *__restrict__ ent_off,
-    int splits, double *__restrict__ partial) {
+    const int *__restrict__ tile_list, const int *__restrict__ ent_sub,
+    const int *__restrict__ prof, const double *__restrict__ grid, int lp_add,
+    const long long *__restrict__ ent_off, int splits, double *__restrict__ partial) {
     __shared__ IntegSmem<LM> sm;
     const int tile = blockIdx.x / splits, split = blockIdx.x % splits;
     const int tix = tile % tg.ntx, tiy = (tile / tg.ntx) % tg.nty, tiz = tile / (tg.ntx * tg.nty);
     const int x0 = tix * TX, y0 = tiy * TY, z0 = tiz * TZ;
-    const int t = threadIdx.x, ry = t % TY, rz = t / TY;
+    const int t = threadIdx.x, xg = t & 3, yb = (t >> 2) & 3, z = t >> 4;
     const int beg = tile_start[tile], end = tile_start[tile + 1];
     const int per = (end - beg + splits - 1) / splits;
     const int mybeg = min(end, beg + split * per), myend = min(end, mybeg + per);
     if (mybeg >= myend) return;
 
-    double v[TX];
+    double v[RB][XB];
     {
-        const int gy = y0 + ry, gz = z0 + rz;
-        const bool rowok = gy < tg.ny && gz < tg.nz;
-        const long long row = ((long long)gz * tg.ny + gy) * tg.nx;
+        const int gz = z0 + z;
 #pragma unroll
-        for (int x = 0; x < TX; x++) v[x] = (rowok && x0 + x < tg.nx) ? grid[row + x0 + x] : 0.0;
+        for (int r = 0; r < RB; r++) {
+            const int gy = y0 + yb * RB + r;
+            const bool rowok = gy < tg.ny && gz < tg.nz;
+            const long long row = ((long long)gz * tg.ny + gy) * tg.nx + x0 + xg * XB;
+#pragma unroll
+            for (int j = 0; j < XB; j++)
+                v[r][j] = (rowok && x0 + xg * XB + j < tg.nx) ? grid[row + j] : 0.0;
+        }
     }
 
     for (int c = mybeg; c < myend; c += QI) {
         const int nq = min(QI, myend - c);
         {
             const int q = t / 64, lane = t % 64;
-            if (q < nq) {
-                SubDev s = subs[tile_ent[c + q]];
-                s.lp += lp_add;
-                tile_tables<LM>(s, lane, x0, y0, z0, tg, sm.X[q], sm.Y[q], sm.Z[q], sm.xi[q],
-                                sm.yj[q], sm.zk[q], sm.meta[q]);
-            }
+            if (q < nq)
+                tile_tables<LM>(subs + ent_sub[tile_list[c + q]], lp_add, lane, x0, y0, z0, tg,
+                                sm.X[q], sm.Y[q], sm.Z[q], sm.xi[q], sm.yj[q], sm.zk[q],
+                                sm.meta[q]);
         }
         __syncthreads();
+#pragma unroll 1
+        for (int q = 0; q < nq; q++)
+            tile_rows(prof, sm.meta[q], sm.yj[q], sm.zk[q], sm.xi[q], sm.T[q], sm.bmax[q],
+                      sm.bmin[q], t);
+        __syncthreads();
+#pragma unroll 1
         for (int q = 0; q < nq; q++) {
+            int Tr[RB], tmin = BIG, tmax = -1;
+#pragma unroll
+            for (int r = 0; r < RB; r++) {
+                Tr[r] = sm.T[q][z * TY + yb * RB + r];
+                tmin = min(tmin, Tr[r]);
+                tmax = max(tmax, Tr[r]);
+            }
+            int mode = 1;
+            if (tmax < 0 || sm.bmin[q][xg] > tmax)
+                mode = 0;
+            else if (sm.bmax[q][xg] <= tmin)
+                mode = 2;
             const int lp = sm.meta[q].lp;
-            const int T = sm.meta[q].valid
-                              ? row_threshold(prof, sm.meta[q].prof_off, sm.meta[q].prof_stride,
-                                              sm.zk[q][rz], sm.yj[q][ry])
-                              : -1;
-            if (T >= 0) {
-                integ_row_dispatch<LM>(sm, q, lp, T, v, t);
-            } else {
-                for (int l = 0; l <= lp; l++) sm.tx[q][t][l] = 0.0;
-            }
+#define GPW_INTEG_CALL(L) integ_block<L, LM>(sm, q, xg, yb, z, Tr, mode, v, t)
+            GPW_LP_DISPATCH(LM, GPW_INTEG_CALL)
+#undef GPW_INTEG_CALL
         }
         __syncthreads();
-        // y-stage: txy[q][z][(lyp,lxp)] = sum_y tx[q][(z,y)][lxp] Y[y][lyp]
-        for (int q = 0; q < nq; q++) {
-            const int lp = sm.meta[q].lp, n2 = ncoef2(lp);
-            for (int w = t; w < TZ * n2; w += NROW) {
-                const int z = w / n2, c2 = w % n2;
-                int lyp, lxp;
-                decode_c2(lp, c2, lyp, lxp);
-                double a = 0.0;
-#pragma unroll 4
-                for (int y = 0; y < TY; y++) a += sm.tx[q][z * TY + y][lxp] * sm.Y[q][y][lyp];
-                sm.txy[q][z][c2] = a;
-            }
-        }
-        __syncthreads();
-        // z-stage: M[(lzp,lyp,lxp)] = sum_z txy[q][z][(lyp,lxp)] Z[z][lzp]
+        // z-stage: M[(lzp,lyp,lxp)] = sum_z uz[q][z][(lyp,lxp)] Z[z][lzp]
+#pragma unroll 1
         for (int q = 0; q < nq; q++) {
             const int lp = sm.meta[q].lp, n3 = ncoset(lp);
-            const long long off = ent_off[c + q];
+            const long long off = ent_off[tile_list[c + q]];
             for (int w = t; w < n3; w += NROW) {
                 int lzp, lyp, lxp;
                 decode_c3(lp, w, lzp, lyp, lxp);
                 const int c2 = c2idx(lp, lyp, lxp);
                 double a = 0.0;
 #pragma unroll 4
-                for (int z = 0; z < TZ; z++) a += sm.txy[q][z][c2] * sm.Z[q][z][lzp];
+                for (int zz = 0; zz < TZ; zz++) a += sm.uz[q][zz][c2] * sm.Z[q][zz][lzp];
                 partial[off + w] = a;
             }
         }
@@ -580,10 +676,10 @@ __global__ void k_fill(double *__restrict__ p, long long n, double v) {
         p[i] = v;
 }
 
-// Mbuf[pair*mstride + c] = sum over the pair's entries of partial[ent_off[e] + c]
+// Mbuf[pair*mstride + c] = sum over the pair's entries of partial[ent_off[e] + c];
+// entries are numbered pair-major, so the pair's partials are one contiguous run
 __global__ void k_sum_entries(const PairDev *__restrict__ pairs, int tau, int lp_add,
-                              const int *__restrict__ pe_start, const int *__restrict__ pe_list,
-                              const long long *__restrict__ ent_off,
+                              const int *__restrict__ pe_start, const long long *__restrict__ ent_off,
                               const double *__restrict__ partial, int mstride,
                               double *__restrict__ Mbuf) {
     const int p = blockIdx.x;
@@ -591,9 +687,10 @@ __global__ void k_sum_entries(const PairDev *__restrict__ pairs, int tau, int lp
     const int lp = P.lp + lp_add;
     const int n3 = ncoset(lp);
     const int b = pe_start[p], e = pe_start[p + 1];
+    const double *src = b < e ? partial + ent_off[b] : partial;
     for (int c = threadIdx.x; c < n3; c += blockDim.x) {
         double a = 0.0;
-        for (int k = b; k < e; k++) a += partial[ent_off[pe_list[k]] + c];
+        for (int k = 0; k < e - b; k++) a += src[(long long)k * n3 + c];
         Mbuf[(long long)p * mstride + c] = a;
     }
 }
@@ -911,24 +1008,26 @@ __global__ __launch_bounds__(256) void k_integrate_core(int n0, int n1, int n2, 
 
 template <int LM>
 static hipError_t launch_collocate_tiles_lm(const TileGrid &tg, const SubDev *subs,
-                                            const int *tile_start, const int *tile_ent,
-                                            const double *coef, const int *prof, int ntiles,
-                                            int splits, double *out, int out_mode,
-                                            hipStream_t st) {
+                                            const int *tile_start, const int *tile_list,
+                                            const int *ent_sub, const double *coef,
+                                            const int *prof, int ntiles, int splits, double *out,
+                                            int out_mode, hipStream_t st) {
     hipLaunchKernelGGL((k_collocate_tiles<LM>), dim3(ntiles * splits), dim3(NROW), 0, st, tg,
-                       subs, tile_start, tile_ent, coef, prof, splits, out, out_mode);
+                       subs, tile_start, tile_list, ent_sub, coef, prof, splits, out, out_mode);
     return hipGetLastError();
 }
 
-hipError_t launch_collocate_tiles(int lp_max, const TileGrid &tg, const SubDev *subs,
-                                  const int *tile_start, const int *tile_ent, const double *coef,
-                                  const int *prof, int ntiles, int splits, double *out,
-                                  int out_mode, hipStream_t st) {
-    if (lp_max < 8)
-        return launch_collocate_tiles_lm<8>(tg, subs, tile_start, tile_ent, coef, prof, ntiles,
-                                            splits, out, out_mode, st);
-    return launch_collocate_tiles_lm<17>(tg, subs, tile_start, tile_ent, coef, prof, ntiles, splits,
-                                         out, out_mode, st);
+// class c handles L = lp + 1 in (4c, 4c + 4], the last class up to 17
+hipError_t launch_collocate_tiles(int cls, const TileGrid &tg, const SubDev *subs,
+                                  const int *tile_start, const int *tile_list, const int *ent_sub,
+                                  const double *coef, const int *prof, int ntiles, int splits,
+                                  double *out, int out_mode, hipStream_t st) {
+    switch (cls) {
+    case 0: return launch_collocate_tiles_lm<4>(tg, subs, tile_start, tile_list, ent_sub, coef, prof, ntiles, splits, out, out_mode, st);
+    case 1: return launch_collocate_tiles_lm<8>(tg, subs, tile_start, tile_list, ent_sub, coef, prof, ntiles, splits, out, out_mode, st);
+    case 2: return launch_collocate_tiles_lm<12>(tg, subs, tile_start, tile_list, ent_sub, coef, prof, ntiles, splits, out, out_mode, st);
+    default: return launch_collocate_tiles_lm<17>(tg, subs, tile_start, tile_list, ent_sub, coef, prof, ntiles, splits, out, out_mode, st);
+    }
 }
 
 hipError_t launch_reduce_partials(const double *partial, int splits, long long ng, double *grid,
@@ -940,24 +1039,28 @@ hipError_t launch_reduce_partials(const double *partial, int splits, long long n
 
 template <int LM>
 static hipError_t launch_integrate_tiles_lm(const TileGrid &tg, const SubDev *subs,
-                                            const int *tile_start, const int *tile_ent,
-                                            const int *prof, const double *grid, int lp_add,
+                                            const int *tile_start, const int *tile_list,
+                                            const int *ent_sub, const int *prof,
+                                            const double *grid, int lp_add,
                                             const long long *ent_off, int ntiles, int splits,
                                             double *partial, hipStream_t st) {
     hipLaunchKernelGGL((k_integrate_tiles<LM>), dim3(ntiles * splits), dim3(NROW), 0, st, tg,
-                       subs, tile_start, tile_ent, prof, grid, lp_add, ent_off, splits, partial);
+                       subs, tile_start, tile_list, ent_sub, prof, grid, lp_add, ent_off, splits,
+                       partial);
     return hipGetLastError();
 }
 
-hipError_t launch_integrate_tiles(int lp_max, const TileGrid &tg, const SubDev *subs,
-                                  const int *tile_start, const int *tile_ent, const int *prof,
-                                  const double *grid, int lp_add, const long long *ent_off,
-                                  int ntiles, int splits, double *partial, hipStream_t st) {
-    if (lp_max + lp_add < 8)
-        return launch_integrate_tiles_lm<8>(tg, subs, tile_start, tile_ent, prof, grid, lp_add,
-                                            ent_off, ntiles, splits, partial, st);
-    return launch_integrate_tiles_lm<17>(tg, subs, tile_start, tile_ent, prof, grid, lp_add, ent_off,
-                                         ntiles, splits, partial, st);
+hipError_t launch_integrate_tiles(int cls, const TileGrid &tg, const SubDev *subs,
+                                  const int *tile_start, const int *tile_list, const int *ent_sub,
+                                  const int *prof, const double *grid, int lp_add,
+                                  const long long *ent_off, int ntiles, int splits,
+                                  double *partial, hipStream_t st) {
+    switch (cls) {
+    case 0: return launch_integrate_tiles_lm<4>(tg, subs, tile_start, tile_list, ent_sub, prof, grid, lp_add, ent_off, ntiles, splits, partial, st);
+    case 1: return launch_integrate_tiles_lm<8>(tg, subs, tile_start, tile_list, ent_sub, prof, grid, lp_add, ent_off, ntiles, splits, partial, st);
+    case 2: return launch_integrate_tiles_lm<12>(tg, subs, tile_start, tile_list, ent_sub, prof, grid, lp_add, ent_off, ntiles, splits, partial, st);
+    default: return launch_integrate_tiles_lm<17>(tg, subs, tile_start, tile_list, ent_sub, prof, grid, lp_add, ent_off, ntiles, splits, partial, st);
+    }
 }
 
 size_t coef_smem_bytes(int lam, int lbm) {
@@ -989,10 +1092,10 @@ hipError_t launch_hab(const PairDev *pairs, int pair0, int npairs, int tau, int 
 }
 
 hipError_t launch_sum_entries(const PairDev *pairs, int npairs, int tau, int lp_add,
-                              const int *pe_start, const int *pe_list, const long long *ent_off,
+                              const int *pe_start, const long long *ent_off,
                               const double *partial, int mstride, double *Mbuf, hipStream_t st) {
     hipLaunchKernelGGL(k_sum_entries, dim3(npairs), dim3(64), 0, st, pairs, tau, lp_add, pe_start,
-                       pe_list, ent_off, partial, mstride, Mbuf);
+                       ent_off, partial, mstride, Mbuf);
     return hipGetLastError();
 }
 

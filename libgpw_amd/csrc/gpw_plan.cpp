@@ -25,23 +25,24 @@
 #include "gpw_internal.hpp"
 
 namespace gpw {
-hipError_t launch_collocate_tiles(int lp_max, const TileGrid &tg, const SubDev *subs,
-                                  const int *tile_start, const int *tile_ent, const double *coef,
-                                  const int *prof, int ntiles, int splits, double *out,
-                                  int out_mode, hipStream_t st);
+hipError_t launch_collocate_tiles(int cls, const TileGrid &tg, const SubDev *subs,
+                                  const int *tile_start, const int *tile_list, const int *ent_sub,
+                                  const double *coef, const int *prof, int ntiles, int splits,
+                                  double *out, int out_mode, hipStream_t st);
 hipError_t launch_reduce_partials(const double *partial, int splits, long long ng, double *grid,
                                   hipStream_t st);
-hipError_t launch_integrate_tiles(int lp_max, const TileGrid &tg, const SubDev *subs,
-                                  const int *tile_start, const int *tile_ent, const int *prof,
-                                  const double *grid, int lp_add, const long long *ent_off,
-                                  int ntiles, int splits, double *partial, hipStream_t st);
+hipError_t launch_integrate_tiles(int cls, const TileGrid &tg, const SubDev *subs,
+                                  const int *tile_start, const int *tile_list, const int *ent_sub,
+                                  const int *prof, const double *grid, int lp_add,
+                                  const long long *ent_off, int ntiles, int splits,
+                                  double *partial, hipStream_t st);
 hipError_t launch_coef(const PairDev *pairs, int pair0, int npairs, int tau, int lam_max,
                        int lbm_max, const double *pab, double *coef, hipStream_t st);
 hipError_t launch_hab(const PairDev *pairs, int pair0, int npairs, int tau, int with_forces,
                       int la_max, int lb_max, const double *Mbuf, int mstride, const double *pab,
                       double *hab, double *forces, hipStream_t st);
 hipError_t launch_sum_entries(const PairDev *pairs, int npairs, int tau, int lp_add,
-                              const int *pe_start, const int *pe_list, const long long *ent_off,
+                              const int *pe_start, const long long *ent_off,
                               const double *partial, int mstride, double *Mbuf, hipStream_t st);
 hipError_t launch_fill(double *p, long long n, double v, hipStream_t st);
 hipError_t launch_collocate_plain(const TileGrid &tg, const SubDev *subs, int si, int nz_box,
@@ -156,6 +157,17 @@ struct PairGeom {
     int prof;  // index into profile list, -1 none
 };
 
+// work-item classes by polynomial length L = lp + 1 (one kernel instantiation
+// each, so registers are sized for the class, not for the largest L)
+constexpr int NCLS = 4;
+static int class_of(int lp) {
+    const int L = lp + 1;
+    return L <= 4 ? 0 : L <= 8 ? 1 : L <= 12 ? 2 : 3;
+}
+
+enum { PH_COEF, PH_CTILES, PH_CREDUCE, PH_ITILES, PH_ISUM, PH_HAB, NPH };
+constexpr int NEVENT = 2048;
+
 struct gpw_tasklist {
     gpw_grid_desc grid{};
     int compute_tau = 0;
@@ -166,6 +178,8 @@ struct gpw_tasklist {
     int ntiles = 0, splits = 1;
     long long nentries = 0;
     long long cube_points = 0, tile_points = 0;
+    long long class_entries[2][NCLS] = {};  // [lp_add/2][class]
+    long long list_off[2][NCLS] = {};       // segment of the class in the tile lists
 
     std::vector<PairDev> pairs_h;
     std::vector<SubDev> subs_h;
@@ -174,8 +188,10 @@ struct gpw_tasklist {
     PairDev *d_pairs = nullptr;
     SubDev *d_subs = nullptr;
     int *d_prof = nullptr;
-    int *d_tile_start = nullptr, *d_tile_ent = nullptr;
-    int *d_pe_start = nullptr, *d_pe_list = nullptr;
+    int *d_tile_start = nullptr;  // [2][NCLS][ntiles + 1]
+    int *d_tile_list = nullptr;   // entry ids, segments per (lp_add, class)
+    int *d_ent_sub = nullptr;
+    int *d_pe_start = nullptr;
     long long *d_ent_off0 = nullptr, *d_ent_off2 = nullptr;
     double *d_coef = nullptr;
     double *d_partial = nullptr;   // integrate partial moments
@@ -183,11 +199,35 @@ struct gpw_tasklist {
     double *d_Mbuf = nullptr;
     int mstride = 0;
 
+    // per-phase device timing (events recorded on the launch stream)
+    bool timing = false;
+    std::vector<hipEvent_t> events;
+    std::vector<int> event_phase;
+    int events_used = 0;
+
     ~gpw_tasklist() {
-        void *ptrs[] = {d_pairs, d_subs, d_prof, d_tile_start, d_tile_ent, d_pe_start, d_pe_list,
+        void *ptrs[] = {d_pairs, d_subs, d_prof, d_tile_start, d_tile_list, d_ent_sub, d_pe_start,
                         d_ent_off0, d_ent_off2, d_coef, d_partial, d_cpartial, d_Mbuf};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
+    }
+};
+
+// brackets one phase with events when timing is on
+struct PhaseTimer {
+    gpw_tasklist *tl;
+    hipStream_t st;
+    int slot = -1;
+    PhaseTimer(gpw_tasklist *t, hipStream_t s, int phase) : tl(t), st(s) {
+        if (!tl->timing || tl->events_used + 2 > (int)tl->events.size()) return;
+        slot = tl->events_used;
+        tl->events_used += 2;
+        tl->event_phase[slot / 2] = phase;
+        (void)hipEventRecord(tl->events[slot], st);
+    }
+    ~PhaseTimer() {
+        if (slot >= 0) (void)hipEventRecord(tl->events[slot + 1], st);
     }
 };
 
@@ -339,11 +379,16 @@ static long long sub_points(const SubDev &s, const std::vector<int> &prof) {
 }
 
 // Host part of a task list: profiles, pairs, sub-boxes, tile work lists.
+// Work items ("entries") are (sub-box, tile) pairs numbered pair-major, so a
+// pair's partial moments are contiguous; per (lp_add, class) a CSR over tiles
+// lists the entry ids the tile kernels sweep.
 struct HostPlan {
     std::vector<int> prof_flat;
-    std::vector<int> tile_start, tile_ent, pe_start, pe_list;
+    std::vector<int> ent_sub, pe_start;
     std::vector<long long> off0, off2;
-    long long coef_total = 0;
+    std::vector<int> tile_start;  // [2][NCLS][ntiles + 1]
+    std::vector<int> tile_list;
+    long long coef_total = 0, partial_total = 0;
 };
 
 static gpw_status plan_tasklist(gpw_tasklist *tl, const std::vector<gpw_pair_desc> &pairs,
@@ -371,7 +416,7 @@ static gpw_status plan_tasklist(gpw_tasklist *tl, const std::vector<gpw_pair_des
         tl->lb_max = std::max(tl->lb_max, pairs[i].lb_max);
     }
     hp.coef_total = coef_total;
-    // sub-boxes
+    // sub-boxes (contiguous per pair)
     tl->subs_h.clear();
     tl->pair_sub_start.assign(pairs.size() + 1, 0);
     for (size_t i = 0; i < pairs.size(); i++) {
@@ -393,10 +438,10 @@ static gpw_status plan_tasklist(gpw_tasklist *tl, const std::vector<gpw_pair_des
     for (int d = 0; d < 3; d++) tg.dr[d] = g.dh[d][d];
     tl->ntiles = tg.ntx * tg.nty * tg.ntz;
     tl->splits = std::max(1, splits);
-    // entries (tile, sub)
-    std::vector<int> count(tl->ntiles + 1, 0);
+    // entries, pair-major
+    std::vector<int> ent_tile;
     std::vector<int> tx, ty, tz;
-    std::vector<std::pair<int, int>> ents;
+    hp.pe_start.assign(pairs.size() + 1, 0);
     tl->cube_points = 0;
     for (size_t si = 0; si < tl->subs_h.size(); si++) {
         const SubDev &s = tl->subs_h[si];
@@ -406,41 +451,50 @@ static gpw_status plan_tasklist(gpw_tasklist *tl, const std::vector<gpw_pair_des
         for (int a : tz)
             for (int b : ty)
                 for (int c : tx) {
-                    const int tile = (a * tg.nty + b) * tg.ntx + c;
-                    ents.push_back({tile, (int)si});
-                    count[tile]++;
+                    ent_tile.push_back((a * tg.nty + b) * tg.ntx + c);
+                    hp.ent_sub.push_back((int)si);
+                    hp.pe_start[s.pair + 1]++;
                 }
         tl->cube_points += sub_points(s, prof_flat);
     }
-    tl->nentries = (long long)ents.size();
+    for (size_t p = 0; p < pairs.size(); p++) hp.pe_start[p + 1] += hp.pe_start[p];
+    const size_t ne = ent_tile.size();
+    if (ne > 0x7fffffffULL) return fail(GPW_ERR_ARG, "too many work items");
+    tl->nentries = (long long)ne;
     tl->tile_points = tl->nentries * (long long)(TX * TY * TZ);
-    hp.tile_start.assign(tl->ntiles + 1, 0);
-    for (int t = 0; t < tl->ntiles; t++) hp.tile_start[t + 1] = hp.tile_start[t] + count[t];
-    hp.tile_ent.assign(ents.size(), 0);
-    {
-        std::vector<int> fillp(hp.tile_start.begin(), hp.tile_start.end() - 1);
-        for (auto &e : ents) hp.tile_ent[fillp[e.first]++] = e.second;
-    }
-    // partial-moment offsets per entry (entry id = position in tile_ent)
-    hp.off0.assign(ents.size(), 0);
-    hp.off2.assign(ents.size(), 0);
+    // partial-moment offsets, pair-major
+    hp.off0.assign(ne, 0);
+    hp.off2.assign(ne, 0);
     long long acc0 = 0, acc2 = 0;
-    for (size_t e = 0; e < hp.tile_ent.size(); e++) {
-        const int lp = tl->subs_h[hp.tile_ent[e]].lp;
+    for (size_t e = 0; e < ne; e++) {
+        const int lp = tl->subs_h[hp.ent_sub[e]].lp;
         hp.off0[e] = acc0;
         hp.off2[e] = acc2;
         acc0 += ncoset(lp);
         acc2 += ncoset(lp + 2);
     }
-    // pair -> entries
-    hp.pe_start.assign(pairs.size() + 1, 0);
-    hp.pe_list.assign(hp.tile_ent.size(), 0);
-    for (size_t e = 0; e < hp.tile_ent.size(); e++) hp.pe_start[tl->subs_h[hp.tile_ent[e]].pair + 1]++;
-    for (size_t p = 0; p < pairs.size(); p++) hp.pe_start[p + 1] += hp.pe_start[p];
-    {
-        std::vector<int> fillp(hp.pe_start.begin(), hp.pe_start.end() - 1);
-        for (size_t e = 0; e < hp.tile_ent.size(); e++)
-            hp.pe_list[fillp[tl->subs_h[hp.tile_ent[e]].pair]++] = (int)e;
+    hp.partial_total = acc2;
+    // per (lp_add, class): CSR of entry ids over tiles
+    const int nt1 = tl->ntiles + 1;
+    hp.tile_start.assign((size_t)2 * NCLS * nt1, 0);
+    hp.tile_list.clear();
+    hp.tile_list.reserve(2 * ne);
+    for (int a = 0; a < 2; a++) {
+        std::vector<int> cls(ne);
+        for (size_t e = 0; e < ne; e++) cls[e] = class_of(tl->subs_h[hp.ent_sub[e]].lp + 2 * a);
+        for (int c = 0; c < NCLS; c++) {
+            int *start = hp.tile_start.data() + (size_t)(a * NCLS + c) * nt1;
+            for (size_t e = 0; e < ne; e++)
+                if (cls[e] == c) start[ent_tile[e] + 1]++;
+            for (int t = 0; t < tl->ntiles; t++) start[t + 1] += start[t];
+            tl->list_off[a][c] = (long long)hp.tile_list.size();
+            tl->class_entries[a][c] = start[tl->ntiles];
+            hp.tile_list.resize(hp.tile_list.size() + start[tl->ntiles]);
+            int *list = hp.tile_list.data() + tl->list_off[a][c];
+            std::vector<int> fillp(start, start + tl->ntiles);
+            for (size_t e = 0; e < ne; e++)
+                if (cls[e] == c) list[fillp[ent_tile[e]]++] = (int)e;
+        }
     }
     tl->mstride = ncoset(tl->lp_max + 2);
     return GPW_OK;
@@ -452,15 +506,13 @@ static gpw_status upload_tasklist(gpw_tasklist *tl, const HostPlan &hp) {
     if ((st = upload(&tl->d_subs, tl->subs_h))) return st;
     if ((st = upload(&tl->d_prof, hp.prof_flat))) return st;
     if ((st = upload(&tl->d_tile_start, hp.tile_start))) return st;
-    if ((st = upload(&tl->d_tile_ent, hp.tile_ent))) return st;
+    if ((st = upload(&tl->d_tile_list, hp.tile_list))) return st;
+    if ((st = upload(&tl->d_ent_sub, hp.ent_sub))) return st;
     if ((st = upload(&tl->d_pe_start, hp.pe_start))) return st;
-    if ((st = upload(&tl->d_pe_list, hp.pe_list))) return st;
     if ((st = upload(&tl->d_ent_off0, hp.off0))) return st;
     if ((st = upload(&tl->d_ent_off2, hp.off2))) return st;
-    long long acc2 = 0;
-    for (size_t e = 0; e < hp.tile_ent.size(); e++) acc2 += ncoset(tl->subs_h[hp.tile_ent[e]].lp + 2);
     HIP_TRY(hipMalloc((void **)&tl->d_coef, sizeof(double) * std::max<long long>(hp.coef_total, 1)));
-    HIP_TRY(hipMalloc((void **)&tl->d_partial, sizeof(double) * std::max<long long>(acc2, 1)));
+    HIP_TRY(hipMalloc((void **)&tl->d_partial, sizeof(double) * std::max<long long>(hp.partial_total, 1)));
     HIP_TRY(hipMalloc((void **)&tl->d_Mbuf,
                       sizeof(double) * std::max<long long>((long long)tl->npairs * tl->mstride, 1)));
     if (tl->splits > 1) {
@@ -644,16 +696,33 @@ gpw_status gpw_tasklist_collocate(gpw_tasklist *tl, const double *pab, double *g
     if (!tl || !pab || !grid) return fail(GPW_ERR_ARG, "NULL argument");
     if (tl->npairs == 0) return GPW_OK;
     hipStream_t st = (hipStream_t)stream;
-    gpw_status s = run_coef(tl, pab, 0, tl->npairs, st);
-    if (s) return s;
-    if (tl->splits == 1) {
-        HIP_TRY(launch_collocate_tiles(tl->lp_max, tl->tg, tl->d_subs, tl->d_tile_start, tl->d_tile_ent,
-                                       tl->d_coef, tl->d_prof, tl->ntiles, 1, grid, 0, st));
-    } else {
+    {
+        PhaseTimer ph(tl, st, PH_COEF);
+        gpw_status s = run_coef(tl, pab, 0, tl->npairs, st);
+        if (s) return s;
+    }
+    const int nt1 = tl->ntiles + 1;
+    {
+        PhaseTimer ph(tl, st, PH_CTILES);
+        bool first = true;
+        for (int c = 0; c < NCLS; c++) {
+            if (tl->class_entries[0][c] == 0) continue;
+            const int *start = tl->d_tile_start + (size_t)c * nt1;
+            const int *list = tl->d_tile_list + tl->list_off[0][c];
+            if (tl->splits == 1) {
+                HIP_TRY(launch_collocate_tiles(c, tl->tg, tl->d_subs, start, list, tl->d_ent_sub,
+                                               tl->d_coef, tl->d_prof, tl->ntiles, 1, grid, 0, st));
+            } else {
+                HIP_TRY(launch_collocate_tiles(c, tl->tg, tl->d_subs, start, list, tl->d_ent_sub,
+                                               tl->d_coef, tl->d_prof, tl->ntiles, tl->splits,
+                                               tl->d_cpartial, first ? 1 : 2, st));
+                first = false;
+            }
+        }
+    }
+    if (tl->splits > 1) {
+        PhaseTimer ph(tl, st, PH_CREDUCE);
         const long long ng = (long long)tl->tg.nx * tl->tg.ny * tl->tg.nz;
-        HIP_TRY(launch_collocate_tiles(tl->lp_max, tl->tg, tl->d_subs, tl->d_tile_start, tl->d_tile_ent,
-                                       tl->d_coef, tl->d_prof, tl->ntiles, tl->splits, tl->d_cpartial,
-                                       1, st));
         HIP_TRY(launch_reduce_partials(tl->d_cpartial, tl->splits, ng, grid, st));
     }
     return GPW_OK;
@@ -667,14 +736,57 @@ gpw_status gpw_tasklist_integrate(gpw_tasklist *tl, const double *grid, const do
     hipStream_t st = (hipStream_t)stream;
     const int wf = forces ? 1 : 0;
     const int lp_add = 2 * wf;
-    HIP_TRY(launch_integrate_tiles(tl->lp_max, tl->tg, tl->d_subs, tl->d_tile_start, tl->d_tile_ent,
-                                   tl->d_prof, grid, lp_add, wf ? tl->d_ent_off2 : tl->d_ent_off0,
-                                   tl->ntiles, tl->splits, tl->d_partial, st));
-    HIP_TRY(launch_sum_entries(tl->d_pairs, tl->npairs, tl->compute_tau, lp_add, tl->d_pe_start,
-                               tl->d_pe_list, wf ? tl->d_ent_off2 : tl->d_ent_off0, tl->d_partial,
-                               tl->mstride, tl->d_Mbuf, st));
-    HIP_TRY(launch_hab(tl->d_pairs, 0, tl->npairs, tl->compute_tau, wf, tl->la_max, tl->lb_max,
-                       tl->d_Mbuf, tl->mstride, pab, hab, forces, st));
+    const long long *off = wf ? tl->d_ent_off2 : tl->d_ent_off0;
+    const int nt1 = tl->ntiles + 1;
+    {
+        PhaseTimer ph(tl, st, PH_ITILES);
+        for (int c = 0; c < NCLS; c++) {
+            if (tl->class_entries[wf][c] == 0) continue;
+            const int *start = tl->d_tile_start + (size_t)(wf * NCLS + c) * nt1;
+            const int *list = tl->d_tile_list + tl->list_off[wf][c];
+            HIP_TRY(launch_integrate_tiles(c, tl->tg, tl->d_subs, start, list, tl->d_ent_sub,
+                                           tl->d_prof, grid, lp_add, off, tl->ntiles, tl->splits,
+                                           tl->d_partial, st));
+        }
+    }
+    {
+        PhaseTimer ph(tl, st, PH_ISUM);
+        HIP_TRY(launch_sum_entries(tl->d_pairs, tl->npairs, tl->compute_tau, lp_add, tl->d_pe_start,
+                                   off, tl->d_partial, tl->mstride, tl->d_Mbuf, st));
+    }
+    {
+        PhaseTimer ph(tl, st, PH_HAB);
+        HIP_TRY(launch_hab(tl->d_pairs, 0, tl->npairs, tl->compute_tau, wf, tl->la_max, tl->lb_max,
+                           tl->d_Mbuf, tl->mstride, pab, hab, forces, st));
+    }
+    return GPW_OK;
+}
+
+gpw_status gpw_tasklist_set_timing(gpw_tasklist *tl, int enable) {
+    if (!tl) return fail(GPW_ERR_ARG, "NULL argument");
+    if (enable && tl->events.empty()) {
+        tl->events.resize(NEVENT);
+        tl->event_phase.assign(NEVENT / 2, 0);
+        for (int i = 0; i < NEVENT; i++) HIP_TRY(hipEventCreate(&tl->events[i]));
+    }
+    tl->timing = enable != 0;
+    tl->events_used = 0;
+    return GPW_OK;
+}
+
+gpw_status gpw_tasklist_timings(gpw_tasklist *tl, double *ms, int n, int *calls) {
+    if (!tl || !ms || n < NPH) return fail(GPW_ERR_ARG, "need room for GPW_NPHASES values");
+    for (int i = 0; i < n; i++) ms[i] = 0.0;
+    if (calls)
+        for (int i = 0; i < NPH; i++) calls[i] = 0;
+    if (tl->events_used > 0) HIP_TRY(hipEventSynchronize(tl->events[tl->events_used - 1]));
+    for (int k = 0; k < tl->events_used / 2; k++) {
+        float t = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&t, tl->events[2 * k], tl->events[2 * k + 1]));
+        ms[tl->event_phase[k]] += t;
+        if (calls) calls[tl->event_phase[k]]++;
+    }
+    tl->events_used = 0;
     return GPW_OK;
 }
 
