@@ -69,10 +69,26 @@ struct SubDev {
     int pad;
 };
 
+// One (sub-box, tile) work item as the tile kernels read it: stored in the
+// order of the per-tile work lists so a workgroup streams its records.
+struct EntryRec {
+    double roff[3];      // displacement of offset i along d: i*dr[d] - roff[d]
+    double zetp;
+    double R2;           // squared sphere radius (imr * min spacing)^2
+    long long ent_off;   // offset of the item's partial moments (integrate)
+    int base[3];         // grid index g -> offset i = lb + mod(g - base, period)
+    int lb[3], ub[3];
+    int lp;              // product l (without the force extension)
+    int coef_off;
+    int pad;
+};
+static_assert(sizeof(EntryRec) == 96, "EntryRec layout");
+
 // Tile-kernel launch description.
 struct TileGrid {
     int nx, ny, nz;          // local grid dims
     int ntx, nty, ntz;       // tiles per axis
+    int period[3];           // npts on periodic axes, BIG otherwise
     double dr[3];
 };
 

@@ -211,106 +211,201 @@ __global__ __launch_bounds__(64) void k_coef(const PairDev *__restrict__ pairs, 
 //   xg = t & 3 (x block), yb = (t >> 2) & 3 (row block), z = t >> 4.
 // The 16 threads of one z plane are one DPP row (16 consecutive lanes).
 // Every LDS table value read in the x stage feeds RB FMAs.
+//
+// A workgroup walks its work items in chunks of Q pairs. Table phase: wave q
+// builds everything of pair q of the chunk (1D tables, x codes and block
+// classes, the z-stage, the sphere threshold of every tile row) using only
+// LDS and in-wave shuffles. The item records of the chunk after next and the
+// coefficients of the next chunk are loaded into registers at the start of a
+// chunk and stored to LDS rings at its end, so global latency stays off the
+// critical path.
 
-constexpr int Q = 4;   // pairs per LDS chunk (collocate)
-constexpr int QI = 2;  // pairs per LDS chunk (integrate)
 constexpr int XB = 8, RB = 4;
 constexpr int NXG = TX / XB;  // x blocks per tile
+constexpr int RECW = (int)(sizeof(EntryRec) / 4);
 static_assert(TX + TY + TZ == 64, "table phase maps one wave per pair");
-static_assert(Q * 64 == NROW, "table phase uses the whole workgroup");
 static_assert(NXG * (TY / RB) * TZ == NROW, "one register block per thread");
 static_assert(NXG * (TY / RB) == 16, "one z plane per DPP row");
+static_assert(TX == 4 * XB && TY == 16 && TZ == 16, "lane layout of the table phase");
 
 template <int LM>
-struct XLayout {
-    static constexpr int GS = XB * LM + 4;  // doubles per x block (+32 B to spread banks)
-    static constexpr int SIZE = NXG * GS;
-    __device__ static int at(int x) { return (x / XB) * GS + (x % XB) * LM; }
+struct TileCfg {
+    static constexpr int Q = LM <= 8 ? 4 : 2;           // pairs per chunk
+    static constexpr int C2M = LM * (LM + 1) / 2;       // (lyp, lxp) terms
+    static constexpr int C3M = LM * (LM + 1) * (LM + 2) / 6;
+    static constexpr int GS = XB * LM + 4;              // doubles per x block (+32 B spreads banks)
+    static constexpr int XS = NXG * GS;
+    static constexpr int NPF = (Q * C3M + NROW - 1) / NROW;  // coef prefetch registers
+    __device__ static int xat(int x) { return (x / XB) * GS + (x % XB) * LM; }
 };
 
-struct TileMeta {
-    int lp, prof_off, prof_stride, valid;
-};
+// Largest n >= 0 with b + (n h)^2 <= R2, evaluated with the same IEEE
+// operations (no contraction) as the host rule that generates the cube.
+__device__ __forceinline__ int sphere_extent_dev(double b, double h, double R2) {
+    const double rem = R2 - b;
+    int n = rem > 0.0 ? (int)floor(sqrt(rem) / h) : 0;
+    while (n > 0) {
+        const double x = __dmul_rn((double)n, h);
+        if (__dadd_rn(b, __dmul_rn(x, x)) <= R2) break;
+        n--;
+    }
+    for (;;) {
+        const double x = __dmul_rn((double)(n + 1), h);
+        if (__dadd_rn(b, __dmul_rn(x, x)) <= R2)
+            n++;
+        else
+            return n;
+    }
+}
 
-// Phase 1: 1D tables of one pair over the tile. Wave w fills pair slot w:
-// lanes 0..TX-1 the x table, then TY y entries, then TZ z entries. The
-// per-axis fields are read from global memory with the lane's axis as index
-// (a private copy of the struct indexed that way would live in scratch).
-template <int LM>
-__device__ __forceinline__ void tile_tables(const SubDev *__restrict__ sp, int lp_add, int lane,
-                                            int x0, int y0, int z0, const TileGrid &tg, double *X,
-                                            double (*Y)[LM], double (*Z)[LM], int *xi, int *yj,
-                                            int *zk, TileMeta &meta) {
+// Sphere threshold of row (kk, jj) (fold offsets): x offsets with
+// fold <= T are inside; -1 if the row is empty. Matches the profile the
+// host planner builds for the reference's sphere_bounds layout.
+__device__ __forceinline__ int row_T(int kk, int jj, const TileGrid &tg, double R2) {
+    if (kk < 0 || jj < 0) return -1;
+    const double zk = __dmul_rn((double)kk, tg.dr[2]);
+    const double base = __dmul_rn(zk, zk);
+    if (kk > 0 && base > R2) return -1;
+    const double yj = __dmul_rn((double)jj, tg.dr[1]);
+    const double b2 = __dadd_rn(base, __dmul_rn(yj, yj));
+    if (jj > 0 && b2 > R2) return -1;
+    return sphere_extent_dev(b2, tg.dr[0], R2);
+}
+
+// Table phase of pair slot q, run by one whole wave (lane = t % 64).
+// Lanes 0..31: x entries, 32..47: y entries, 48..63: z entries.
+// WITH_CXY: z lanes also contract coef with their Z powers (collocate).
+template <int L, int LM, bool WITH_CXY>
+__device__ __forceinline__ void tile_phase1(const EntryRec &R, int lp_add, int lane, int x0,
+                                            int y0, int z0, const TileGrid &tg,
+                                            const double *__restrict__ coef, double *X,
+                                            double (*Y)[LM], double (*Z)[LM],
+                                            double (*cxy)[TileCfg<LM>::C2M], int *xi, int *T,
+                                            int *bmax, int *bmin) {
+    constexpr int lp = L - 1;
     const int d = lane < TX ? 0 : lane < TX + TY ? 1 : 2;
     const int pos = d == 0 ? lane : d == 1 ? lane - TX : lane - TX - TY;
     const int g = (d == 0 ? x0 : d == 1 ? y0 : z0) + pos;
     const int nmax = d == 0 ? tg.nx : d == 1 ? tg.ny : tg.nz;
+    const int period = d == 0 ? tg.period[0] : d == 1 ? tg.period[1] : tg.period[2];
     const double dr = d == 0 ? tg.dr[0] : d == 1 ? tg.dr[1] : tg.dr[2];
-    double *T = d == 0 ? X + XLayout<LM>::at(lane) : d == 1 ? Y[pos] : Z[pos];
-    int *code = d == 0 ? xi : d == 1 ? yj : zk;
-    const int lb = sp->lb[d], ub = sp->ub[d], base = sp->base[d], period = sp->period[d];
-    const double roff = sp->roff[d];
-    const double zetp = sp->zetp;
-    const int lp = sp->lp + lp_add;
     int i = BIG;
     bool ok = g < nmax;
     if (ok) {
-        i = lb + imod(g - base, period);
-        ok = i <= ub;
+        i = R.lb[d] + imod(g - R.base[d], period);
+        ok = i <= R.ub[d];
     }
-    if (ok) {
-        const double r = i * dr - roff;
-        double v = exp(-zetp * r * r);
-        for (int k = 0; k <= lp; k++) {
-            T[k] = v;
+    double tv[L];
+    {
+        const double r = i * dr - R.roff[d];
+        double v = ok ? exp(-R.zetp * r * r) : 0.0;
+#pragma unroll
+        for (int k = 0; k < L; k++) {
+            tv[k] = v;
             v *= r;
         }
-        code[pos] = fold_off(i);
+    }
+    const int code = ok ? fold_off(i) : (d == 0 ? BIG : -1);
+    if (d == 0) {
+        double *dst = X + TileCfg<LM>::xat(pos);
+#pragma unroll
+        for (int k = 0; k < L; k++) dst[k] = tv[k];
+        xi[pos] = code;
+    } else if (d == 1) {
+#pragma unroll
+        for (int k = 0; k < L; k++) Y[pos][k] = tv[k];
     } else {
-        for (int k = 0; k <= lp; k++) T[k] = 0.0;
-        code[pos] = d == 0 ? BIG : -1;
+        if (Z) {
+#pragma unroll
+            for (int k = 0; k < L; k++) Z[pos][k] = tv[k];
+        }
+        if constexpr (WITH_CXY) {
+            // z-stage: cxy[z][(lyp,lxp)] = sum_lzp coef[(lzp,lyp,lxp)] Z^lzp
+#pragma unroll
+            for (int lyp = 0; lyp <= lp; lyp++)
+#pragma unroll
+                for (int lxp = 0; lxp <= lp - lyp; lxp++) {
+                    double a = 0.0;
+#pragma unroll
+                    for (int lzp = 0; lzp <= lp - lyp - lxp; lzp++)
+                        a += coef[cidx(lp, lzp, lyp, lxp)] * tv[lzp];
+                    cxy[pos][c2idx(lp, lyp, lxp)] = ok ? a : 0.0;
+                }
+        }
     }
-    const unsigned long long m = __ballot(ok && d == 0);
-    if (lane == 0) {
-        meta.lp = lp;
-        meta.prof_off = sp->prof_off;
-        meta.prof_stride = sp->prof_stride;
-        meta.valid = m != 0ull;
+    // x blocks: max code (BIG if any point invalid) and min code over 8 lanes
+    int mx = code, mn = code;
+#pragma unroll
+    for (int o = 1; o < XB; o <<= 1) {
+        mx = max(mx, __shfl_xor(mx, o, XB));
+        mn = min(mn, __shfl_xor(mn, o, XB));
     }
+    if (d == 0 && (lane % XB) == 0) {
+        bmax[lane / XB] = mx;
+        bmin[lane / XB] = mn;
+    }
+    // thresholds of the 256 tile rows (row = z * TY + y), 4 per lane
+#pragma unroll
+    for (int rr = 0; rr < NROW / 64; rr++) {
+        const int row = lane + 64 * rr;
+        const int kk = __shfl(code, TX + TY + row / TY);
+        const int jj = __shfl(code, TX + row % TY);
+        T[row] = row_T(kk, jj, tg, R.R2);
+    }
+    (void)lp_add;
 }
 
-// Phase 2 helpers: row thresholds and x-block classes of pair slot q.
-// T[row] (row = z*TY + y): offsets with fold_off(i) <= T[row] are inside.
-// bmax/bmin over a block: max fold (BIG if any x invalid) / min over valid x.
-__device__ __forceinline__ void tile_rows(const int *__restrict__ prof, const TileMeta &meta,
-                                          const int *yj, const int *zk, const int *xi, int *T,
-                                          int *bmax, int *bmin, int t) {
-    const int z = t / TY, y = t % TY;
-    T[t] = meta.valid ? row_threshold(prof, meta.prof_off, meta.prof_stride, zk[z], yj[y]) : -1;
-    if (t < NXG) {
-        int mx = 0, mn = BIG;
-        for (int x = 0; x < XB; x++) {
-            const int v = xi[t * XB + x];
-            mx = max(mx, v);
-            mn = min(mn, v);
-        }
-        bmax[t] = mx;
-        bmin[t] = mn;
+#define GPW_LP_DISPATCH(LM, CALL)                                                   \
+    switch (lp) {                                                                   \
+    case 0: CALL(1); break;                                                         \
+    case 1: CALL(2); break;                                                         \
+    case 2: CALL(3); break;                                                         \
+    case 3: CALL(4); break;                                                         \
+    default:                                                                        \
+        if constexpr (LM > 4) {                                                     \
+            switch (lp) {                                                           \
+            case 4: CALL(5); break;                                                 \
+            case 5: CALL(6); break;                                                 \
+            case 6: CALL(7); break;                                                 \
+            case 7: CALL(8); break;                                                 \
+            default:                                                                \
+                if constexpr (LM > 8) {                                             \
+                    switch (lp) {                                                   \
+                    case 8: CALL(9); break;                                         \
+                    case 9: CALL(10); break;                                        \
+                    case 10: CALL(11); break;                                       \
+                    case 11: CALL(12); break;                                       \
+                    default:                                                        \
+                        if constexpr (LM > 12) {                                    \
+                            switch (lp) {                                           \
+                            case 12: CALL(13); break;                               \
+                            case 13: CALL(14); break;                               \
+                            case 14: CALL(15); break;                               \
+                            case 15: CALL(16); break;                               \
+                            default: CALL(17); break;                               \
+                            }                                                       \
+                        }                                                           \
+                        break;                                                      \
+                    }                                                               \
+                }                                                                   \
+                break;                                                              \
+            }                                                                       \
+        }                                                                           \
+        break;                                                                      \
     }
-}
 
 template <int LM>
 struct CollocSmem {
-    double X[Q][XLayout<LM>::SIZE];
-    double Y[Q][TY][LM];
-    double Z[Q][TZ][LM];
-    double cxy[Q][TZ][LM * (LM + 1) / 2];
-    int xi[Q][TX];
-    int yj[Q][TY];
-    int zk[Q][TZ];
-    int T[Q][NROW];
-    int bmax[Q][NXG], bmin[Q][NXG];
-    TileMeta meta[Q];
+    using C = TileCfg<LM>;
+    EntryRec rec[3][C::Q];
+    double coef[2][C::Q][C::C3M];
+    double X[C::Q][C::XS];
+    double Y[C::Q][TY][LM];
+    double cxy[C::Q][TZ][C::C2M];
+    int xi[C::Q][TX];
+    int T[C::Q][NROW];
+    int bmax[C::Q][NXG], bmin[C::Q][NXG];
+    int lp[C::Q];
 };
 
 // register block of one thread for one pair: rows y = 4*yb + r of plane z,
@@ -334,73 +429,30 @@ __device__ __forceinline__ void colloc_block(const CollocSmem<LM> &sm, int q, in
             for (int lxp = 0; lxp <= lp - lyp; lxp++) cx[r][lxp] += cxy[c2idx(lp, lyp, lxp)] * y;
         }
     }
-    const double *Xg = sm.X[q] + xg * XLayout<LM>::GS;
-    if (full) {
+    const double *Xg = sm.X[q] + xg * TileCfg<LM>::GS;
 #pragma unroll
-        for (int j = 0; j < XB; j++) {
-            double xv[L];
+    for (int j = 0; j < XB; j++) {
+        double xv[L];
 #pragma unroll
-            for (int l = 0; l < L; l++) xv[l] = Xg[j * LM + l];
+        for (int l = 0; l < L; l++) xv[l] = Xg[j * LM + l];
+        const int code = sm.xi[q][xg * XB + j];
 #pragma unroll
-            for (int r = 0; r < RB; r++) {
-                double v = 0.0;
+        for (int r = 0; r < RB; r++) {
+            double v = 0.0;
 #pragma unroll
-                for (int l = 0; l < L; l++) v += cx[r][l] * xv[l];
-                acc[r][j] += v;
-            }
-        }
-    } else {
-#pragma unroll
-        for (int j = 0; j < XB; j++) {
-            double xv[L];
-#pragma unroll
-            for (int l = 0; l < L; l++) xv[l] = Xg[j * LM + l];
-            const int code = sm.xi[q][xg * XB + j];
-#pragma unroll
-            for (int r = 0; r < RB; r++) {
-                double v = 0.0;
-#pragma unroll
-                for (int l = 0; l < L; l++) v += cx[r][l] * xv[l];
-                acc[r][j] += code <= Tr[r] ? v : 0.0;
-            }
+            for (int l = 0; l < L; l++) v += cx[r][l] * xv[l];
+            acc[r][j] += (full || code <= Tr[r]) ? v : 0.0;
         }
     }
 }
 
-#define GPW_LP_DISPATCH(LM, CALL)                                                   \
-    switch (lp) {                                                                   \
-    case 0: CALL(1); break;                                                         \
-    case 1: CALL(2); break;                                                         \
-    case 2: CALL(3); break;                                                         \
-    case 3: CALL(4); break;                                                         \
-    case 4: CALL(5); break;                                                         \
-    case 5: CALL(6); break;                                                         \
-    case 6: CALL(7); break;                                                         \
-    case 7: CALL(8); break;                                                         \
-    default:                                                                        \
-        if constexpr (LM > 8) {                                                     \
-            switch (lp) {                                                           \
-            case 8: CALL(9); break;                                                 \
-            case 9: CALL(10); break;                                                \
-            case 10: CALL(11); break;                                               \
-            case 11: CALL(12); break;                                               \
-            case 12: CALL(13); break;                                               \
-            case 13: CALL(14); break;                                               \
-            case 14: CALL(15); break;                                               \
-            case 15: CALL(16); break;                                               \
-            default: CALL(17); break;                                               \
-            }                                                                       \
-        }                                                                           \
-        break;                                                                      \
-    }
-
 // out_mode 0: grid += acc (splits == 1); 1: partial[split] = acc; 2: partial[split] += acc
 template <int LM>
 __global__ __launch_bounds__(NROW, 2) void k_collocate_tiles(
-    TileGrid tg, const SubDev *__restrict__ subs, const int *__restrict__ tile_start,
-    const int *__restrict__ tile_list, const int *__restrict__ ent_sub,
-    const double *__restrict__ coef, const int *__restrict__ prof, int splits,
-    double *__restrict__ out, int out_mode) {
+    TileGrid tg, const EntryRec *__restrict__ recs, const int *__restrict__ tile_start,
+    const double *__restrict__ coef, int splits, double *__restrict__ out, int out_mode) {
+    using C = TileCfg<LM>;
+    constexpr int Q = C::Q;
     __shared__ CollocSmem<LM> sm;
     const int tile = blockIdx.x / splits, split = blockIdx.x % splits;
     const int tix = tile % tg.ntx, tiy = (tile / tg.ntx) % tg.nty, tiz = tile / (tg.ntx * tg.nty);
@@ -410,6 +462,22 @@ __global__ __launch_bounds__(NROW, 2) void k_collocate_tiles(
     const int per = (end - beg + splits - 1) / splits;
     const int mybeg = min(end, beg + split * per), myend = min(end, mybeg + per);
     if (out_mode != 1 && mybeg >= myend) return;
+    const int *rec_words = reinterpret_cast<const int *>(recs);
+
+    // prologue: records of chunks 0 and 1, then coefficients of chunk 0
+    for (int w = t; w < 2 * Q * RECW; w += NROW) {
+        const int item = mybeg + w / RECW;
+        if (item < myend)
+            reinterpret_cast<int *>(&sm.rec[w / (Q * RECW)][0])[w % (Q * RECW)] =
+                rec_words[(long long)item * RECW + w % RECW];
+    }
+    __syncthreads();
+    for (int w = t; w < Q * C::C3M; w += NROW) {
+        const int q = w / C::C3M, idx = w % C::C3M;
+        if (mybeg + q < myend && idx < ncoset(sm.rec[0][q].lp))
+            sm.coef[0][q][idx] = coef[sm.rec[0][q].coef_off + idx];
+    }
+    __syncthreads();
 
     double acc[RB][XB];
 #pragma unroll
@@ -417,34 +485,38 @@ __global__ __launch_bounds__(NROW, 2) void k_collocate_tiles(
 #pragma unroll
         for (int j = 0; j < XB; j++) acc[r][j] = 0.0;
 
-    for (int c = mybeg; c < myend; c += Q) {
+    int k = 0;
+    for (int c = mybeg; c < myend; c += Q, k++) {
         const int nq = min(Q, myend - c);
+        const int rs = k % 3, cs = k & 1;
+        // prefetch: records of chunk k+2, coefficients of chunk k+1
+        int pf_rec = 0;
+        if (t < Q * RECW && c + 2 * Q + t / RECW < myend)
+            pf_rec = rec_words[(long long)(c + 2 * Q) * RECW + t];
+        double pf_coef[C::NPF];
+#pragma unroll
+        for (int j = 0; j < C::NPF; j++) {
+            pf_coef[j] = 0.0;
+            const int w = t + j * NROW;
+            if (w < Q * C::C3M) {
+                const int q = w / C::C3M, idx = w % C::C3M;
+                const EntryRec &Rn = sm.rec[(k + 1) % 3][q];
+                if (c + Q + q < myend && idx < ncoset(Rn.lp)) pf_coef[j] = coef[Rn.coef_off + idx];
+            }
+        }
+        // table phase: wave q -> pair q
         {
             const int q = t / 64, lane = t % 64;
-            if (q < nq)
-                tile_tables<LM>(subs + ent_sub[tile_list[c + q]], 0, lane, x0, y0, z0, tg, sm.X[q],
-                                sm.Y[q], sm.Z[q], sm.xi[q], sm.yj[q], sm.zk[q], sm.meta[q]);
-        }
-        __syncthreads();
-#pragma unroll 1
-        for (int q = 0; q < nq; q++) {
-            // z-stage: cxy[q][z][(lyp,lxp)] = sum_lzp coef * Z^lzp
-            const int lp = sm.meta[q].lp;
-            const int n2 = ncoef2(lp);
-            const int coff = subs[ent_sub[tile_list[c + q]]].coef_off;
-            for (int w = t; w < TZ * n2; w += NROW) {
-                const int zz = w / n2, c2 = w % n2;
-                int lyp, lxp;
-                decode_c2(lp, c2, lyp, lxp);
-                double a = 0.0;
-                if (sm.zk[q][zz] >= 0) {
-                    for (int lzp = 0; lzp <= lp - lyp - lxp; lzp++)
-                        a += coef[coff + cidx(lp, lzp, lyp, lxp)] * sm.Z[q][zz][lzp];
-                }
-                sm.cxy[q][zz][c2] = a;
+            if (q < nq) {
+                const EntryRec &R = sm.rec[rs][q];
+                const int lp = R.lp;
+                if (lane == 0) sm.lp[q] = lp;
+#define GPW_P1_CALL(L)                                                                        \
+    tile_phase1<L, LM, true>(R, 0, lane, x0, y0, z0, tg, sm.coef[cs][q], sm.X[q], sm.Y[q],     \
+                             nullptr, sm.cxy[q], sm.xi[q], sm.T[q], sm.bmax[q], sm.bmin[q])
+                GPW_LP_DISPATCH(LM, GPW_P1_CALL)
+#undef GPW_P1_CALL
             }
-            tile_rows(prof, sm.meta[q], sm.yj[q], sm.zk[q], sm.xi[q], sm.T[q], sm.bmax[q],
-                      sm.bmin[q], t);
         }
         __syncthreads();
 #pragma unroll 1
@@ -458,10 +530,17 @@ __global__ __launch_bounds__(NROW, 2) void k_collocate_tiles(
             }
             if (tmax < 0 || sm.bmin[q][xg] > tmax) continue;
             const bool full = sm.bmax[q][xg] <= tmin;
-            const int lp = sm.meta[q].lp;
+            const int lp = sm.lp[q];
 #define GPW_COLLOC_CALL(L) colloc_block<L, LM>(sm, q, xg, yb, z, Tr, full, acc)
             GPW_LP_DISPATCH(LM, GPW_COLLOC_CALL)
 #undef GPW_COLLOC_CALL
+        }
+        // commit the prefetched data to the LDS rings
+        if (t < Q * RECW) reinterpret_cast<int *>(&sm.rec[(k + 2) % 3][0])[t] = pf_rec;
+#pragma unroll
+        for (int j = 0; j < C::NPF; j++) {
+            const int w = t + j * NROW;
+            if (w < Q * C::C3M) sm.coef[cs ^ 1][w / C::C3M][w % C::C3M] = pf_coef[j];
         }
         __syncthreads();
     }
@@ -522,16 +601,16 @@ __device__ __forceinline__ double row_sum16(double v) {
 
 template <int LM>
 struct IntegSmem {
-    double X[QI][XLayout<LM>::SIZE];
-    double Y[QI][TY][LM];
-    double Z[QI][TZ][LM];
-    double uz[QI][TZ][LM * (LM + 1) / 2];  // per-plane (lyp, lxp) moments
-    int xi[QI][TX];
-    int yj[QI][TY];
-    int zk[QI][TZ];
-    int T[QI][NROW];
-    int bmax[QI][NXG], bmin[QI][NXG];
-    TileMeta meta[QI];
+    using C = TileCfg<LM>;
+    EntryRec rec[2][C::Q];
+    double X[C::Q][C::XS];
+    double Y[C::Q][TY][LM];
+    double Z[C::Q][TZ][LM];
+    double uz[C::Q][TZ][C::C2M];  // per-plane (lyp, lxp) moments
+    int xi[C::Q][TX];
+    int T[C::Q][NROW];
+    int bmax[C::Q][NXG], bmin[C::Q][NXG];
+    int lp[C::Q];
 };
 
 // x stage on the register block, local y contraction, DPP sum over the z plane
@@ -550,7 +629,7 @@ __device__ __forceinline__ void integ_block(IntegSmem<LM> &sm, int q, int xg, in
         for (int r = 0; r < RB; r++)
 #pragma unroll
             for (int l = 0; l < L; l++) tx[r][l] = 0.0;
-        const double *Xg = sm.X[q] + xg * XLayout<LM>::GS;
+        const double *Xg = sm.X[q] + xg * TileCfg<LM>::GS;
 #pragma unroll
         for (int j = 0; j < XB; j++) {
             double xv[L];
@@ -582,13 +661,13 @@ __device__ __forceinline__ void integ_block(IntegSmem<LM> &sm, int q, int xg, in
     }
 }
 
-// partial moments of every (sub-box, tile) item: partial[ent_off[e] + c3]
+// partial moments of every (sub-box, tile) item: partial[rec.ent_off + c3]
 template <int LM>
 __global__ __launch_bounds__(NROW, 2) void k_integrate_tiles(
-    TileGrid tg, const SubDev *__restrict__ subs, const int *__restrict__ tile_start,
-    const int *__restrict__ tile_list, const int *__restrict__ ent_sub,
-    const int *__restrict__ prof, const double *__restrict__ grid, int lp_add,
-    const long long *__restrict__ ent_off, int splits, double *__restrict__ partial) {
+    TileGrid tg, const EntryRec *__restrict__ recs, const int *__restrict__ tile_start,
+    const double *__restrict__ grid, int lp_add, int splits, double *__restrict__ partial) {
+    using C = TileCfg<LM>;
+    constexpr int Q = C::Q;
     __shared__ IntegSmem<LM> sm;
     const int tile = blockIdx.x / splits, split = blockIdx.x % splits;
     const int tix = tile % tg.ntx, tiy = (tile / tg.ntx) % tg.nty, tiz = tile / (tg.ntx * tg.nty);
@@ -598,6 +677,13 @@ __global__ __launch_bounds__(NROW, 2) void k_integrate_tiles(
     const int per = (end - beg + splits - 1) / splits;
     const int mybeg = min(end, beg + split * per), myend = min(end, mybeg + per);
     if (mybeg >= myend) return;
+    const int *rec_words = reinterpret_cast<const int *>(recs);
+
+    for (int w = t; w < Q * RECW; w += NROW) {
+        const int item = mybeg + w / RECW;
+        if (item < myend)
+            reinterpret_cast<int *>(&sm.rec[0][0])[w] = rec_words[(long long)item * RECW + w % RECW];
+    }
 
     double v[RB][XB];
     {
@@ -612,21 +698,28 @@ __global__ __launch_bounds__(NROW, 2) void k_integrate_tiles(
                 v[r][j] = (rowok && x0 + xg * XB + j < tg.nx) ? grid[row + j] : 0.0;
         }
     }
+    __syncthreads();
 
-    for (int c = mybeg; c < myend; c += QI) {
-        const int nq = min(QI, myend - c);
+    int k = 0;
+    for (int c = mybeg; c < myend; c += Q, k++) {
+        const int nq = min(Q, myend - c);
+        const int rs = k & 1;
+        int pf_rec = 0;
+        if (t < Q * RECW && c + Q + t / RECW < myend)
+            pf_rec = rec_words[(long long)(c + Q) * RECW + t];
         {
             const int q = t / 64, lane = t % 64;
-            if (q < nq)
-                tile_tables<LM>(subs + ent_sub[tile_list[c + q]], lp_add, lane, x0, y0, z0, tg,
-                                sm.X[q], sm.Y[q], sm.Z[q], sm.xi[q], sm.yj[q], sm.zk[q],
-                                sm.meta[q]);
+            if (q < nq) {
+                const EntryRec &R = sm.rec[rs][q];
+                const int lp = R.lp + lp_add;
+                if (lane == 0) sm.lp[q] = lp;
+#define GPW_P1_CALL(L)                                                                         \
+    tile_phase1<L, LM, false>(R, lp_add, lane, x0, y0, z0, tg, nullptr, sm.X[q], sm.Y[q],       \
+                              sm.Z[q], nullptr, sm.xi[q], sm.T[q], sm.bmax[q], sm.bmin[q])
+                GPW_LP_DISPATCH(LM, GPW_P1_CALL)
+#undef GPW_P1_CALL
+            }
         }
-        __syncthreads();
-#pragma unroll 1
-        for (int q = 0; q < nq; q++)
-            tile_rows(prof, sm.meta[q], sm.yj[q], sm.zk[q], sm.xi[q], sm.T[q], sm.bmax[q],
-                      sm.bmin[q], t);
         __syncthreads();
 #pragma unroll 1
         for (int q = 0; q < nq; q++) {
@@ -642,7 +735,7 @@ __global__ __launch_bounds__(NROW, 2) void k_integrate_tiles(
                 mode = 0;
             else if (sm.bmax[q][xg] <= tmin)
                 mode = 2;
-            const int lp = sm.meta[q].lp;
+            const int lp = sm.lp[q];
 #define GPW_INTEG_CALL(L) integ_block<L, LM>(sm, q, xg, yb, z, Tr, mode, v, t)
             GPW_LP_DISPATCH(LM, GPW_INTEG_CALL)
 #undef GPW_INTEG_CALL
@@ -651,8 +744,8 @@ __global__ __launch_bounds__(NROW, 2) void k_integrate_tiles(
         // z-stage: M[(lzp,lyp,lxp)] = sum_z uz[q][z][(lyp,lxp)] Z[z][lzp]
 #pragma unroll 1
         for (int q = 0; q < nq; q++) {
-            const int lp = sm.meta[q].lp, n3 = ncoset(lp);
-            const long long off = ent_off[tile_list[c + q]];
+            const int lp = sm.lp[q], n3 = ncoset(lp);
+            const long long off = sm.rec[rs][q].ent_off;
             for (int w = t; w < n3; w += NROW) {
                 int lzp, lyp, lxp;
                 decode_c3(lp, w, lzp, lyp, lxp);
@@ -663,6 +756,7 @@ __global__ __launch_bounds__(NROW, 2) void k_integrate_tiles(
                 partial[off + w] = a;
             }
         }
+        if (t < Q * RECW) reinterpret_cast<int *>(&sm.rec[rs ^ 1][0])[t] = pf_rec;
         __syncthreads();
     }
 }
@@ -1007,26 +1101,23 @@ __global__ __launch_bounds__(256) void k_integrate_core(int n0, int n1, int n2, 
 // launchers (host side)
 
 template <int LM>
-static hipError_t launch_collocate_tiles_lm(const TileGrid &tg, const SubDev *subs,
-                                            const int *tile_start, const int *tile_list,
-                                            const int *ent_sub, const double *coef,
-                                            const int *prof, int ntiles, int splits, double *out,
-                                            int out_mode, hipStream_t st) {
-    hipLaunchKernelGGL((k_collocate_tiles<LM>), dim3(ntiles * splits), dim3(NROW), 0, st, tg,
-                       subs, tile_start, tile_list, ent_sub, coef, prof, splits, out, out_mode);
+static hipError_t launch_collocate_tiles_lm(const TileGrid &tg, const EntryRec *recs,
+                                            const int *tile_start, const double *coef, int ntiles,
+                                            int splits, double *out, int out_mode, hipStream_t st) {
+    hipLaunchKernelGGL((k_collocate_tiles<LM>), dim3(ntiles * splits), dim3(NROW), 0, st, tg, recs,
+                       tile_start, coef, splits, out, out_mode);
     return hipGetLastError();
 }
 
 // class c handles L = lp + 1 in (4c, 4c + 4], the last class up to 17
-hipError_t launch_collocate_tiles(int cls, const TileGrid &tg, const SubDev *subs,
-                                  const int *tile_start, const int *tile_list, const int *ent_sub,
-                                  const double *coef, const int *prof, int ntiles, int splits,
+hipError_t launch_collocate_tiles(int cls, const TileGrid &tg, const EntryRec *recs,
+                                  const int *tile_start, const double *coef, int ntiles, int splits,
                                   double *out, int out_mode, hipStream_t st) {
     switch (cls) {
-    case 0: return launch_collocate_tiles_lm<4>(tg, subs, tile_start, tile_list, ent_sub, coef, prof, ntiles, splits, out, out_mode, st);
-    case 1: return launch_collocate_tiles_lm<8>(tg, subs, tile_start, tile_list, ent_sub, coef, prof, ntiles, splits, out, out_mode, st);
-    case 2: return launch_collocate_tiles_lm<12>(tg, subs, tile_start, tile_list, ent_sub, coef, prof, ntiles, splits, out, out_mode, st);
-    default: return launch_collocate_tiles_lm<17>(tg, subs, tile_start, tile_list, ent_sub, coef, prof, ntiles, splits, out, out_mode, st);
+    case 0: return launch_collocate_tiles_lm<4>(tg, recs, tile_start, coef, ntiles, splits, out, out_mode, st);
+    case 1: return launch_collocate_tiles_lm<8>(tg, recs, tile_start, coef, ntiles, splits, out, out_mode, st);
+    case 2: return launch_collocate_tiles_lm<12>(tg, recs, tile_start, coef, ntiles, splits, out, out_mode, st);
+    default: return launch_collocate_tiles_lm<17>(tg, recs, tile_start, coef, ntiles, splits, out, out_mode, st);
     }
 }
 
@@ -1038,28 +1129,23 @@ hipError_t launch_reduce_partials(const double *partial, int splits, long long n
 }
 
 template <int LM>
-static hipError_t launch_integrate_tiles_lm(const TileGrid &tg, const SubDev *subs,
-                                            const int *tile_start, const int *tile_list,
-                                            const int *ent_sub, const int *prof,
-                                            const double *grid, int lp_add,
-                                            const long long *ent_off, int ntiles, int splits,
-                                            double *partial, hipStream_t st) {
-    hipLaunchKernelGGL((k_integrate_tiles<LM>), dim3(ntiles * splits), dim3(NROW), 0, st, tg,
-                       subs, tile_start, tile_list, ent_sub, prof, grid, lp_add, ent_off, splits,
-                       partial);
+static hipError_t launch_integrate_tiles_lm(const TileGrid &tg, const EntryRec *recs,
+                                            const int *tile_start, const double *grid, int lp_add,
+                                            int ntiles, int splits, double *partial,
+                                            hipStream_t st) {
+    hipLaunchKernelGGL((k_integrate_tiles<LM>), dim3(ntiles * splits), dim3(NROW), 0, st, tg, recs,
+                       tile_start, grid, lp_add, splits, partial);
     return hipGetLastError();
 }
 
-hipError_t launch_integrate_tiles(int cls, const TileGrid &tg, const SubDev *subs,
-                                  const int *tile_start, const int *tile_list, const int *ent_sub,
-                                  const int *prof, const double *grid, int lp_add,
-                                  const long long *ent_off, int ntiles, int splits,
-                                  double *partial, hipStream_t st) {
+hipError_t launch_integrate_tiles(int cls, const TileGrid &tg, const EntryRec *recs,
+                                  const int *tile_start, const double *grid, int lp_add,
+                                  int ntiles, int splits, double *partial, hipStream_t st) {
     switch (cls) {
-    case 0: return launch_integrate_tiles_lm<4>(tg, subs, tile_start, tile_list, ent_sub, prof, grid, lp_add, ent_off, ntiles, splits, partial, st);
-    case 1: return launch_integrate_tiles_lm<8>(tg, subs, tile_start, tile_list, ent_sub, prof, grid, lp_add, ent_off, ntiles, splits, partial, st);
-    case 2: return launch_integrate_tiles_lm<12>(tg, subs, tile_start, tile_list, ent_sub, prof, grid, lp_add, ent_off, ntiles, splits, partial, st);
-    default: return launch_integrate_tiles_lm<17>(tg, subs, tile_start, tile_list, ent_sub, prof, grid, lp_add, ent_off, ntiles, splits, partial, st);
+    case 0: return launch_integrate_tiles_lm<4>(tg, recs, tile_start, grid, lp_add, ntiles, splits, partial, st);
+    case 1: return launch_integrate_tiles_lm<8>(tg, recs, tile_start, grid, lp_add, ntiles, splits, partial, st);
+    case 2: return launch_integrate_tiles_lm<12>(tg, recs, tile_start, grid, lp_add, ntiles, splits, partial, st);
+    default: return launch_integrate_tiles_lm<17>(tg, recs, tile_start, grid, lp_add, ntiles, splits, partial, st);
     }
 }
 

@@ -25,17 +25,14 @@
 #include "gpw_internal.hpp"
 
 namespace gpw {
-hipError_t launch_collocate_tiles(int cls, const TileGrid &tg, const SubDev *subs,
-                                  const int *tile_start, const int *tile_list, const int *ent_sub,
-                                  const double *coef, const int *prof, int ntiles, int splits,
+hipError_t launch_collocate_tiles(int cls, const TileGrid &tg, const EntryRec *recs,
+                                  const int *tile_start, const double *coef, int ntiles, int splits,
                                   double *out, int out_mode, hipStream_t st);
 hipError_t launch_reduce_partials(const double *partial, int splits, long long ng, double *grid,
                                   hipStream_t st);
-hipError_t launch_integrate_tiles(int cls, const TileGrid &tg, const SubDev *subs,
-                                  const int *tile_start, const int *tile_list, const int *ent_sub,
-                                  const int *prof, const double *grid, int lp_add,
-                                  const long long *ent_off, int ntiles, int splits,
-                                  double *partial, hipStream_t st);
+hipError_t launch_integrate_tiles(int cls, const TileGrid &tg, const EntryRec *recs,
+                                  const int *tile_start, const double *grid, int lp_add,
+                                  int ntiles, int splits, double *partial, hipStream_t st);
 hipError_t launch_coef(const PairDev *pairs, int pair0, int npairs, int tau, int lam_max,
                        int lbm_max, const double *pab, double *coef, hipStream_t st);
 hipError_t launch_hab(const PairDev *pairs, int pair0, int npairs, int tau, int with_forces,
@@ -81,6 +78,7 @@ static int imod_h(int a, int m) { return (a % m + m) % m; }
 struct Profile {
     int n[3];              // table covers fold offsets 0..n[d]
     std::vector<int> T;    // (n[2]+1) x (n[1]+1)
+    double R2 = 0.0;       // squared sphere radius when built from a radius
 };
 
 static int sphere_extent(double base, double h, double R2) {
@@ -103,6 +101,7 @@ static void profile_from_radius(double radius, const double dr[3], Profile &P, i
     if (imr < 1) imr = 1;
     const double R = imr * drmin;
     const double R2 = R * R;
+    P.R2 = R2;
     for (int d = 0; d < 3; d++) {
         const int n = sphere_extent(0.0, dr[d], R2);
         P.n[d] = n;
@@ -189,8 +188,7 @@ struct gpw_tasklist {
     SubDev *d_subs = nullptr;
     int *d_prof = nullptr;
     int *d_tile_start = nullptr;  // [2][NCLS][ntiles + 1]
-    int *d_tile_list = nullptr;   // entry ids, segments per (lp_add, class)
-    int *d_ent_sub = nullptr;
+    EntryRec *d_recs = nullptr;   // item records in tile-list order, per (lp_add, class)
     int *d_pe_start = nullptr;
     long long *d_ent_off0 = nullptr, *d_ent_off2 = nullptr;
     double *d_coef = nullptr;
@@ -206,7 +204,7 @@ struct gpw_tasklist {
     int events_used = 0;
 
     ~gpw_tasklist() {
-        void *ptrs[] = {d_pairs, d_subs, d_prof, d_tile_start, d_tile_list, d_ent_sub, d_pe_start,
+        void *ptrs[] = {d_pairs, d_subs, d_prof, d_tile_start, d_recs, d_pe_start,
                         d_ent_off0, d_ent_off2, d_coef, d_partial, d_cpartial, d_Mbuf};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
@@ -387,7 +385,7 @@ struct HostPlan {
     std::vector<int> ent_sub, pe_start;
     std::vector<long long> off0, off2;
     std::vector<int> tile_start;  // [2][NCLS][ntiles + 1]
-    std::vector<int> tile_list;
+    std::vector<EntryRec> recs;   // tile-list order, segments per (lp_add, class)
     long long coef_total = 0, partial_total = 0;
 };
 
@@ -435,9 +433,18 @@ static gpw_status plan_tasklist(gpw_tasklist *tl, const std::vector<gpw_pair_des
     tg.ntx = (tg.nx + TX - 1) / TX;
     tg.nty = (tg.ny + TY - 1) / TY;
     tg.ntz = (tg.nz + TZ - 1) / TZ;
-    for (int d = 0; d < 3; d++) tg.dr[d] = g.dh[d][d];
+    for (int d = 0; d < 3; d++) {
+        tg.dr[d] = g.dh[d][d];
+        tg.period[d] = g.periodic[d] ? g.npts[d] : BIG;
+    }
     tl->ntiles = tg.ntx * tg.nty * tg.ntz;
     tl->splits = std::max(1, splits);
+    // squared sphere radius of every sub-box (from its pair's profile)
+    std::vector<double> sub_R2(tl->subs_h.size(), 0.0);
+    for (size_t si = 0; si < tl->subs_h.size(); si++) {
+        const int pi = geos[tl->subs_h[si].pair].prof;
+        sub_R2[si] = pi >= 0 ? profiles[pi].R2 : 0.0;
+    }
     // entries, pair-major
     std::vector<int> ent_tile;
     std::vector<int> tx, ty, tz;
@@ -474,11 +481,11 @@ static gpw_status plan_tasklist(gpw_tasklist *tl, const std::vector<gpw_pair_des
         acc2 += ncoset(lp + 2);
     }
     hp.partial_total = acc2;
-    // per (lp_add, class): CSR of entry ids over tiles
+    // per (lp_add, class): CSR over tiles of the item records
     const int nt1 = tl->ntiles + 1;
     hp.tile_start.assign((size_t)2 * NCLS * nt1, 0);
-    hp.tile_list.clear();
-    hp.tile_list.reserve(2 * ne);
+    hp.recs.clear();
+    hp.recs.reserve(2 * ne);
     for (int a = 0; a < 2; a++) {
         std::vector<int> cls(ne);
         for (size_t e = 0; e < ne; e++) cls[e] = class_of(tl->subs_h[hp.ent_sub[e]].lp + 2 * a);
@@ -487,13 +494,28 @@ static gpw_status plan_tasklist(gpw_tasklist *tl, const std::vector<gpw_pair_des
             for (size_t e = 0; e < ne; e++)
                 if (cls[e] == c) start[ent_tile[e] + 1]++;
             for (int t = 0; t < tl->ntiles; t++) start[t + 1] += start[t];
-            tl->list_off[a][c] = (long long)hp.tile_list.size();
+            tl->list_off[a][c] = (long long)hp.recs.size();
             tl->class_entries[a][c] = start[tl->ntiles];
-            hp.tile_list.resize(hp.tile_list.size() + start[tl->ntiles]);
-            int *list = hp.tile_list.data() + tl->list_off[a][c];
+            hp.recs.resize(hp.recs.size() + start[tl->ntiles]);
+            EntryRec *list = hp.recs.data() + tl->list_off[a][c];
             std::vector<int> fillp(start, start + tl->ntiles);
-            for (size_t e = 0; e < ne; e++)
-                if (cls[e] == c) list[fillp[ent_tile[e]]++] = (int)e;
+            for (size_t e = 0; e < ne; e++) {
+                if (cls[e] != c) continue;
+                const SubDev &sd = tl->subs_h[hp.ent_sub[e]];
+                EntryRec r{};
+                for (int d = 0; d < 3; d++) {
+                    r.roff[d] = sd.roff[d];
+                    r.base[d] = sd.base[d];
+                    r.lb[d] = sd.lb[d];
+                    r.ub[d] = sd.ub[d];
+                }
+                r.zetp = sd.zetp;
+                r.R2 = sub_R2[hp.ent_sub[e]];
+                r.ent_off = a ? hp.off2[e] : hp.off0[e];
+                r.lp = sd.lp;
+                r.coef_off = sd.coef_off;
+                list[fillp[ent_tile[e]]++] = r;
+            }
         }
     }
     tl->mstride = ncoset(tl->lp_max + 2);
@@ -506,8 +528,7 @@ static gpw_status upload_tasklist(gpw_tasklist *tl, const HostPlan &hp) {
     if ((st = upload(&tl->d_subs, tl->subs_h))) return st;
     if ((st = upload(&tl->d_prof, hp.prof_flat))) return st;
     if ((st = upload(&tl->d_tile_start, hp.tile_start))) return st;
-    if ((st = upload(&tl->d_tile_list, hp.tile_list))) return st;
-    if ((st = upload(&tl->d_ent_sub, hp.ent_sub))) return st;
+    if ((st = upload(&tl->d_recs, hp.recs))) return st;
     if ((st = upload(&tl->d_pe_start, hp.pe_start))) return st;
     if ((st = upload(&tl->d_ent_off0, hp.off0))) return st;
     if ((st = upload(&tl->d_ent_off2, hp.off2))) return st;
@@ -708,14 +729,13 @@ gpw_status gpw_tasklist_collocate(gpw_tasklist *tl, const double *pab, double *g
         for (int c = 0; c < NCLS; c++) {
             if (tl->class_entries[0][c] == 0) continue;
             const int *start = tl->d_tile_start + (size_t)c * nt1;
-            const int *list = tl->d_tile_list + tl->list_off[0][c];
+            const EntryRec *recs = tl->d_recs + tl->list_off[0][c];
             if (tl->splits == 1) {
-                HIP_TRY(launch_collocate_tiles(c, tl->tg, tl->d_subs, start, list, tl->d_ent_sub,
-                                               tl->d_coef, tl->d_prof, tl->ntiles, 1, grid, 0, st));
+                HIP_TRY(launch_collocate_tiles(c, tl->tg, recs, start, tl->d_coef, tl->ntiles, 1,
+                                               grid, 0, st));
             } else {
-                HIP_TRY(launch_collocate_tiles(c, tl->tg, tl->d_subs, start, list, tl->d_ent_sub,
-                                               tl->d_coef, tl->d_prof, tl->ntiles, tl->splits,
-                                               tl->d_cpartial, first ? 1 : 2, st));
+                HIP_TRY(launch_collocate_tiles(c, tl->tg, recs, start, tl->d_coef, tl->ntiles,
+                                               tl->splits, tl->d_cpartial, first ? 1 : 2, st));
                 first = false;
             }
         }
@@ -743,10 +763,9 @@ gpw_status gpw_tasklist_integrate(gpw_tasklist *tl, const double *grid, const do
         for (int c = 0; c < NCLS; c++) {
             if (tl->class_entries[wf][c] == 0) continue;
             const int *start = tl->d_tile_start + (size_t)(wf * NCLS + c) * nt1;
-            const int *list = tl->d_tile_list + tl->list_off[wf][c];
-            HIP_TRY(launch_integrate_tiles(c, tl->tg, tl->d_subs, start, list, tl->d_ent_sub,
-                                           tl->d_prof, grid, lp_add, off, tl->ntiles, tl->splits,
-                                           tl->d_partial, st));
+            const EntryRec *recs = tl->d_recs + tl->list_off[wf][c];
+            HIP_TRY(launch_integrate_tiles(c, tl->tg, recs, start, grid, lp_add, tl->ntiles,
+                                           tl->splits, tl->d_partial, st));
         }
     }
     {
