@@ -36,11 +36,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--splits", type=int, default=int(os.environ.get("GPW_SPLITS", "2")))
+    ap.add_argument("--splits", type=int, default=int(os.environ.get("GPW_SPLITS", "4")))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain GPU leg")
     ap.add_argument("--cpu-pairs", type=int, default=3000)
     ap.add_argument("--plain-pairs", type=int, default=2000)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a device copy of 1 GiB (x3) to calibrate HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
@@ -54,6 +56,15 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.copy_calibration:
+        # a plain copy of known size: reads and writes 2^30 bytes per launch
+        src = torch.ones(1 << 27, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": 1 << 30, "launches": 3}), flush=True)
+        return
     cfg = Wk.config(2)
     npairs = len(cfg["pairs"]["la_max"])
     V_h = Wk.potential(cfg)

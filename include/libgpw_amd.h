@@ -175,6 +175,13 @@ gpw_status gpw_collocate_core(const int *length, int L, const double *co_dev,
 gpw_status gpw_integrate_core(const int *length, int L, const double *P_dev, int ldp,
                               const double *V_dev, double *Velems_dev, void *stream);
 
+/* The same two contractions on host arrays (the library stages them through
+ * device memory): drop-in for the reference's CPU templates. */
+gpw_status gpw_collocate_core_host(const int *length, int L, const double *co, const double *P,
+                                   int ldp, double *V);
+gpw_status gpw_integrate_core_host(const int *length, int L, const double *P, int ldp,
+                                   const double *V, double *Velems);
+
 /* last HIP error string recorded by the library (thread-local) */
 const char *gpw_last_error(void);
 

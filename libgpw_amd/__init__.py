@@ -28,7 +28,8 @@ EXPORTS = (
     "gpw_tasklist_collocate", "gpw_tasklist_integrate",
     "gpw_tasklist_collocate_plain", "gpw_tasklist_integrate_plain",
     "gpw_tasklist_set_timing", "gpw_tasklist_timings",
-    "gpw_collocate_core", "gpw_integrate_core", "gpw_last_error",
+    "gpw_collocate_core", "gpw_integrate_core", "gpw_collocate_core_host",
+    "gpw_integrate_core_host", "gpw_last_error",
 )
 PHASES = ("coef", "collocate", "reduce", "integrate", "moments", "hab")
 
@@ -97,6 +98,8 @@ def lib():
         L.gpw_tasklist_timings.argtypes = [vp, dp, C.c_int, ip]
         L.gpw_collocate_core.argtypes = [ip, C.c_int, vp, vp, C.c_int, vp, vp]
         L.gpw_integrate_core.argtypes = [ip, C.c_int, vp, C.c_int, vp, vp, vp]
+        L.gpw_collocate_core_host.argtypes = [ip, C.c_int, dp, dp, C.c_int, dp]
+        L.gpw_integrate_core_host.argtypes = [ip, C.c_int, dp, C.c_int, dp, dp]
         L.gpw_last_error.argtypes = []
         L.gpw_last_error.restype = C.c_char_p
         for n in EXPORTS:

@@ -55,6 +55,12 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp])
     os.replace(tmp, LIB)
+    # task-file replay driver: plain C++ against the drop-in header
+    exe = os.path.join(LIBDIR, "gpw_replay")
+    _run([os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall",
+          "-I" + os.path.join(ROOT, "include"), os.path.join(CSRC, "gpw_replay_main.cpp"),
+          "-o", exe + ".tmp", "-L" + LIBDIR, "-lgpw_amd", "-Wl,-rpath,$ORIGIN"])
+    os.replace(exe + ".tmp", exe)
     return LIB
 
 

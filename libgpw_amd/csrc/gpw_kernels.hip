@@ -243,7 +243,8 @@ struct TileCfg {
 // operations (no contraction) as the host rule that generates the cube.
 __device__ __forceinline__ int sphere_extent_dev(double b, double h, double R2) {
     const double rem = R2 - b;
-    int n = rem > 0.0 ? (int)floor(sqrt(rem) / h) : 0;
+    // initial guess only; the exact predicate below decides
+    int n = rem > 0.0 ? (int)(sqrtf((float)rem) / (float)h) : 0;
     while (n > 0) {
         const double x = __dmul_rn((double)n, h);
         if (__dadd_rn(b, __dmul_rn(x, x)) <= R2) break;
@@ -575,28 +576,22 @@ __global__ void k_reduce_partials(const double *__restrict__ partial, int splits
     }
 }
 
-// sum of a double over the 16 lanes of a DPP row; every lane gets the sum
-__device__ __forceinline__ double row_sum16(double v) {
-    auto step = [](double a, int ctrl) -> double {
-        const int lo = __double2loint(a), hi = __double2hiint(a);
-        int plo, phi;
-        switch (ctrl) {
-        case 0: plo = __builtin_amdgcn_update_dpp(0, lo, 0xB1, 0xF, 0xF, false);
-                phi = __builtin_amdgcn_update_dpp(0, hi, 0xB1, 0xF, 0xF, false); break;
-        case 1: plo = __builtin_amdgcn_update_dpp(0, lo, 0x4E, 0xF, 0xF, false);
-                phi = __builtin_amdgcn_update_dpp(0, hi, 0x4E, 0xF, 0xF, false); break;
-        case 2: plo = __builtin_amdgcn_update_dpp(0, lo, 0x124, 0xF, 0xF, false);
-                phi = __builtin_amdgcn_update_dpp(0, hi, 0x124, 0xF, 0xF, false); break;
-        default: plo = __builtin_amdgcn_update_dpp(0, lo, 0x128, 0xF, 0xF, false);
-                 phi = __builtin_amdgcn_update_dpp(0, hi, 0x128, 0xF, 0xF, false); break;
-        }
-        return a + __hiloint2double(phi, plo);
-    };
-    v = step(v, 0);  // quad_perm [1,0,3,2]
-    v = step(v, 1);  // quad_perm [2,3,0,1]
-    v = step(v, 2);  // row_ror:4
-    v = step(v, 3);  // row_ror:8
-    return v;
+// DPP move of a double (two 32-bit halves); CTRL is the dpp_ctrl code
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double a) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(a), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(a), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+// sum over the 4 lanes of a quad (quad_perm [1,0,3,2] then [2,3,0,1])
+__device__ __forceinline__ double quad_sum(double v) {
+    v += dpp_f64<0xB1>(v);
+    return v + dpp_f64<0x4E>(v);
+}
+// sum over lanes l, l+4, l+8, l+12 of a DPP row (row_ror:4 then row_ror:8)
+__device__ __forceinline__ double stride4_sum(double v) {
+    v += dpp_f64<0x124>(v);
+    return v + dpp_f64<0x128>(v);
 }
 
 template <int LM>
@@ -612,6 +607,9 @@ struct IntegSmem {
     int bmax[C::Q][NXG], bmin[C::Q][NXG];
     int lp[C::Q];
 };
+
+// sum of a double over the 16 lanes of a DPP row; every lane gets the sum
+__device__ __forceinline__ double row_sum16(double v) { return stride4_sum(quad_sum(v)); }
 
 // x stage on the register block, local y contraction, DPP sum over the z plane
 template <int L, int LM>
@@ -782,10 +780,18 @@ __global__ void k_sum_entries(const PairDev *__restrict__ pairs, int tau, int lp
     const int n3 = ncoset(lp);
     const int b = pe_start[p], e = pe_start[p + 1];
     const double *src = b < e ? partial + ent_off[b] : partial;
+    const int ne = e - b;
     for (int c = threadIdx.x; c < n3; c += blockDim.x) {
-        double a = 0.0;
-        for (int k = 0; k < e - b; k++) a += src[(long long)k * n3 + c];
-        Mbuf[(long long)p * mstride + c] = a;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        int k = 0;
+        for (; k + 4 <= ne; k += 4) {
+            a0 += src[(long long)k * n3 + c];
+            a1 += src[(long long)(k + 1) * n3 + c];
+            a2 += src[(long long)(k + 2) * n3 + c];
+            a3 += src[(long long)(k + 3) * n3 + c];
+        }
+        for (; k < ne; k++) a0 += src[(long long)k * n3 + c];
+        Mbuf[(long long)p * mstride + c] = (a0 + a1) + (a2 + a3);
     }
 }
 

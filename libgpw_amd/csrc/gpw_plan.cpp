@@ -862,6 +862,49 @@ gpw_status gpw_integrate_core(const int *length, int L, const double *P, int ldp
     return GPW_OK;
 }
 
+// host-array variants: stage through device memory
+static gpw_status core_host(bool collocate, const int *length, int L, const double *co,
+                            const double *P, int ldp, const double *V, double *out) {
+    if (!length || !P || !out || (collocate ? !co : !V) || L < 1 || L > LPMAX + 1 ||
+        length[0] < 0 || length[1] < 0 || length[2] < 0 ||
+        ldp < std::max(length[0], std::max(length[1], length[2])))
+        return fail(GPW_ERR_ARG, "bad arguments");
+    const size_t nv = (size_t)length[0] * length[1] * length[2];
+    const size_t np = (size_t)3 * L * ldp, nc = (size_t)L * L * L;
+    double *d_P = nullptr, *d_a = nullptr, *d_b = nullptr;
+    gpw_status rc = GPW_OK;
+    hipError_t e;
+    const size_t na = collocate ? nc : nv, nb = collocate ? nv : nc;
+    if ((e = hipMalloc((void **)&d_P, np * sizeof(double))) != hipSuccess ||
+        (e = hipMalloc((void **)&d_a, std::max<size_t>(na, 1) * sizeof(double))) != hipSuccess ||
+        (e = hipMalloc((void **)&d_b, std::max<size_t>(nb, 1) * sizeof(double))) != hipSuccess) {
+        rc = fail(GPW_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    } else if ((e = hipMemcpy(d_P, P, np * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess ||
+               (e = hipMemcpy(d_a, collocate ? co : V, na * sizeof(double), hipMemcpyHostToDevice)) !=
+                   hipSuccess) {
+        rc = fail(GPW_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+    } else {
+        rc = collocate ? gpw_collocate_core(length, L, d_a, d_P, ldp, d_b, nullptr)
+                       : gpw_integrate_core(length, L, d_P, ldp, d_a, d_b, nullptr);
+        if (rc == GPW_OK &&
+            (e = hipMemcpy(out, d_b, nb * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess)
+            rc = fail(GPW_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+    }
+    for (double *p : {d_P, d_a, d_b})
+        if (p) (void)hipFree(p);
+    return rc;
+}
+
+gpw_status gpw_collocate_core_host(const int *length, int L, const double *co, const double *P,
+                                   int ldp, double *V) {
+    return core_host(true, length, L, co, P, ldp, nullptr, V);
+}
+
+gpw_status gpw_integrate_core_host(const int *length, int L, const double *P, int ldp,
+                                   const double *V, double *Velems) {
+    return core_host(false, length, L, nullptr, P, ldp, V, Velems);
+}
+
 // ---- single task (drop-in) ------------------------------------------------
 
 struct SingleTask {

@@ -214,3 +214,36 @@ def test_libgpw_core(gpw, oracle, torch_cuda, shape):
     out = gpw.integrate_core((i, j, k), dev(torch_cuda, P), dev(torch_cuda, Wg)).cpu().numpy()
     ref = oracle.integrate_core((i, j, k), P, Wg)
     assert np.abs(out - ref).max() < 1e-13 * max(1, np.abs(ref).max())
+
+
+@pytest.mark.parametrize("name", ["collocate_ortho_density", "collocate_ortho_tau"])
+def test_replay_driver_on_fixture(gpw, torch_cuda, name):
+    import os
+    import subprocess
+    say("replay driver", name)
+    exe = os.path.join(os.path.dirname(gpw.LIB_PATH), "gpw_replay")
+    here = os.path.dirname(os.path.abspath(__file__))
+    r = subprocess.run([exe, os.path.join(here, "golden", name + ".task"), "3"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
+def test_libgpw_core_host_arrays(gpw, oracle, torch_cuda):
+    import ctypes as C
+    say("libgpw core host arrays")
+    rng = np.random.default_rng(9)
+    length, L = (13, 45, 29), 7
+    P = rng.uniform(-1, 1, (3, L, 45))
+    co = rng.uniform(-1, 1, (L, L, L))
+    V = np.zeros(length)
+    dp = C.POINTER(C.c_double)
+    lng = (C.c_int * 3)(*length)
+    assert gpw.lib().gpw_collocate_core_host(lng, L, co.ctypes.data_as(dp), P.ctypes.data_as(dp), 45,
+                                             V.ctypes.data_as(dp)) == 0
+    ref = oracle.collocate_core(length, co, P)
+    assert np.abs(V - ref).max() < 1e-14 * np.abs(ref).max()
+    out = np.zeros((L, L, L))
+    assert gpw.lib().gpw_integrate_core_host(lng, L, P.ctypes.data_as(dp), 45, V.ctypes.data_as(dp),
+                                             out.ctypes.data_as(dp)) == 0
+    ref2 = oracle.integrate_core(length, P, V)
+    assert np.abs(out - ref2).max() < 1e-13 * np.abs(ref2).max()

@@ -159,3 +159,46 @@ def test_taskfile_roundtrip(tmp_path, name):
         assert u[k] == t[k]
     assert np.array_equal(u["pab"], t["pab"]) and np.array_equal(u["dh"], t["dh"])
     assert u["grid_entries"] == t["grid_entries"]
+
+
+def test_replay_driver_cli(gpw, tmp_path):
+    exe = os.path.join(os.path.dirname(gpw.LIB_PATH), "gpw_replay")
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 1 and "Usage" in r.stdout
+    bad = tmp_path / "bad.task"
+    bad.write_text("#Grid collocate task v1\ncompute_tau x\n")
+    r = subprocess.run([exe, str(bad)], capture_output=True, text=True)
+    assert r.returncode == 2 and "compute_tau" in r.stderr
+
+
+def test_cxx_header_compiles(gpw, tmp_path):
+    # the drop-in header accepts a reference-style row-major array type
+    src = tmp_path / "use.cpp"
+    src.write_text(r'''
+#include <vector>
+#include "libgpw_amd.hpp"
+template <int N> struct A { std::vector<double> d; int n[N];
+  double &operator()(int i, int j) { return d[i * n[1] + j]; }
+  const double &operator()(int i, int j) const { return d[i * n[1] + j]; }
+  double &operator()(int i, int j, int k) { return d[(i * n[1] + j) * n[2] + k]; }
+  const double &operator()(int i, int j, int k) const { return d[(i * n[1] + j) * n[2] + k]; }
+  size_t size(int i) const { return n[i]; } };
+int main() {
+  A<2> pab{std::vector<double>(16), {4, 4}}; A<3> grid{std::vector<double>(8), {2, 2, 2}};
+  double dh[3][3] = {{1,0,0},{0,1,0},{0,0,1}}, ra[3] = {0,0,0}, rab[3] = {0,0,0};
+  int n[3] = {2, 2, 2}, lb[3] = {0, 0, 0}, lbc[3] = {-1,-1,-1}, ubc[3] = {2,2,2};
+  bool per[3] = {true, true, true};
+  std::vector<int> sb = {0};
+  try {  // lmax = 0 < la_max = 1: must come back as GPW_ERR_LMAX
+    grid_collocate_pgf_product_rspace(false, true, 1, 0, 0, 0, 1.0, 1.0, 1.0, dh, dh, ra, rab, n, n,
+                                      lb, per, 0, 1.0, lbc, ubc, 1, sb, 4, 0, 0, pab, grid);
+  } catch (const gpw::error &e) { return e.code == GPW_ERR_LMAX ? 0 : 5; }
+  return 6;
+}
+''')
+    exe = tmp_path / "use"
+    inc = os.path.join(ROOT, "include")
+    libdir = os.path.dirname(gpw.LIB_PATH)
+    subprocess.run(["g++", "-std=c++17", "-I" + inc, str(src), "-o", str(exe), "-L" + libdir,
+                    "-lgpw_amd", "-Wl,-rpath," + libdir], check=True)
+    assert subprocess.run([str(exe)]).returncode == 0
