@@ -247,3 +247,105 @@ def test_libgpw_core_host_arrays(gpw, oracle, torch_cuda):
                                              out.ctypes.data_as(dp)) == 0
     ref2 = oracle.integrate_core(length, P, V)
     assert np.abs(out - ref2).max() < 1e-13 * np.abs(ref2).max()
+
+
+# ---------------------------------------------------------------- autograd, graphs
+
+
+def test_autograd_matches_oracle_and_fd(gpw, oracle, torch_cuda):
+    """Differentiation workload (4,096 pairs, 96x96x112): grad pab = integrate,
+    grad centres = forces; forces also against a per-pair central finite
+    difference (h = 1e-5 bohr) of <W, collocate>."""
+    from libgpw_amd import workloads as Wk
+    from libgpw_amd.autograd import collocate
+    torch = torch_cuda
+    say("autograd config3 4096 pairs")
+    cfg = Wk.config(3)
+    p = cfg["pairs"]
+    W = Wk.potential(cfg)
+    Wd = dev(torch, W)
+    pab = dev(torch, cfg["pab"]).requires_grad_(True)
+    ra = torch.tensor(p["ra"], dtype=torch.float64, requires_grad=True)
+    rb = torch.tensor(p["ra"] + p["rab"], dtype=torch.float64, requires_grad=True)
+    grid = collocate(pab, ra, rb, p, cfg["npts"], cfg["dh"])
+    loss = (grid * Wd).sum()
+    gpab, gra, grb = torch.autograd.grad(loss, (pab, ra, rb))
+    ref = oracle_batch(oracle, cfg, W=W, forces=True)
+    assert rel_err(grid.detach().cpu().numpy(), ref["grid"]) < TOL
+    assert rel_err(gpab.cpu().numpy(), ref["hab"]) < TOL
+    f = np.concatenate([gra.numpy(), grb.numpy()], axis=1).reshape(-1, 2, 3)
+    assert rel_err(f, ref["forces"]) < TOL
+
+    # per-pair energies E_p = <pab_p, integrate_p(W)> at displaced centres
+    g = gpw.grid_desc(cfg["npts"], cfg["dh"])
+    pab_h = cfg["pab"]
+    off = p["block_offset"]
+    sizes = [gpw.ncoset(int(b)) * gpw.ncoset(int(a)) for a, b in zip(p["la_max"], p["lb_max"])]
+    seg = np.repeat(np.arange(len(sizes)), sizes)
+
+    def energies(ra_, rab_):
+        q = dict(p, ra=ra_, rab=rab_)
+        tl = gpw.TaskList(g, q)
+        hab = torch.zeros(cfg["nblock"], dtype=torch.float64, device="cuda")
+        tl.integrate(Wd, hab)
+        h = hab.cpu().numpy()
+        tl.close()
+        return np.bincount(seg, weights=pab_h * h, minlength=len(sizes))
+
+    h = 1e-5
+    dh_inv = np.linalg.inv(cfg["dh"])
+    fr = p["zetb"] / (p["zeta"] + p["zetb"])
+
+    def centres(ra_, rab_):
+        rp = ra_ + fr[:, None] * rab_
+        return np.floor(rp @ dh_inv).astype(int)
+
+    checked = 0
+    scale = np.abs(f).max()
+    for which in range(2):
+        for d in range(3):
+            e = np.zeros(3)
+            e[d] = h
+            if which == 0:
+                ap, am = (p["ra"] + e, p["rab"] - e), (p["ra"] - e, p["rab"] + e)
+            else:
+                ap, am = (p["ra"], p["rab"] + e), (p["ra"], p["rab"] - e)
+            same = np.all(centres(*ap) == centres(*am), axis=1)
+            fd = (energies(*ap) - energies(*am)) / (2 * h)
+            err = np.abs(fd - f[:, which, d])[same]
+            assert err.max() <= 1e-6 * scale, (which, d, err.max(), scale)
+            checked += int(same.sum())
+    assert checked > 6 * 4000
+
+
+def test_hip_graph_replays(gpw, torch_cuda):
+    """collocate + integrate + forces captured in one HIP graph: every replay,
+    with eager work in between, equals the eager result bit for bit."""
+    from libgpw_amd import workloads as Wk
+    from libgpw_amd.autograd import capture_step
+    torch = torch_cuda
+    say("hip graph capture 1500 pairs")
+    cfg = Wk.subset(Wk.config(2), np.arange(1500))
+    tl = gpw.TaskList(gpw.grid_desc(cfg["npts"], cfg["dh"]), cfg["pairs"], splits=2)
+    n = cfg["npts"]
+    pab = dev(torch, cfg["pab"])
+    pot = dev(torch, Wk.potential(cfg))
+    grid = torch.zeros((n[2], n[1], n[0]), dtype=torch.float64, device="cuda")
+    hab = torch.zeros(cfg["nblock"], dtype=torch.float64, device="cuda")
+    forces = torch.zeros((1500, 6), dtype=torch.float64, device="cuda")
+    # eager reference
+    tl.collocate(pab, grid)
+    tl.integrate(pot, hab, pab=pab, forces=forces)
+    torch.cuda.synchronize()
+    ref = (grid.clone(), hab.clone(), forces.clone())
+    graph = capture_step(tl, pab, grid, pot, hab, forces)
+    for rep in range(4):
+        grid.fill_(7.0)  # eager work between replays
+        hab.fill_(-3.0)
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(grid, ref[0]), rep
+        assert torch.equal(hab, ref[1]), rep
+        assert torch.equal(forces, ref[2]), rep
+    tl.close()
