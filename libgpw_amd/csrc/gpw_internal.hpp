@@ -16,6 +16,10 @@ constexpr int BIG = 0x3fffffff;           // "outside" marker for offsets
 // along x per row, TY x TZ rows, one row per thread.
 constexpr int TX = 32, TY = 16, TZ = 16;
 constexpr int NROW = TY * TZ;             // 256 threads per workgroup
+// k_hab keeps heff / eff in LDS up to this size, else in a global workspace
+// of HAB_WS_BLOCKS slices (one per resident workgroup)
+constexpr size_t HAB_LDS_MAX = 64 * 1024;
+constexpr int HAB_WS_BLOCKS = 4096;
 
 __host__ __device__ constexpr int ncoset(int l) {
     return l < 0 ? 0 : ((l + 1) * (l + 2) * (l + 3)) / 6;
@@ -54,7 +58,10 @@ struct PairDev {
 
 // One sub-box of a pair's cube that does not overlap itself under the
 // periodic map. Offsets i along axis d lie in [lb[d], ub[d]]; grid index g
-// maps back to i = lb + mod(g - base, period).
+// maps back to i = lb + mod(g - base, period). On orthorhombic grids i is the
+// offset from the cube centre and the displacement is i*dr - roff; on general
+// grids i is the absolute lattice index and roff holds the centre in lattice
+// coordinates (displacement i - roff).
 struct SubDev {
     double roff[3];   // ortho: displacement X = i*dr - roff
     double zetp;
@@ -67,6 +74,7 @@ struct SubDev {
     int prof_off;     // offset of the sphere profile (ints), -1: full box
     int prof_stride;  // row stride of the profile (n_y + 1)
     int pad;
+    double R2;        // general cells: squared radius of the row bounds
 };
 
 // One (sub-box, tile) work item as the tile kernels read it: stored in the
@@ -74,9 +82,10 @@ struct SubDev {
 struct EntryRec {
     double roff[3];      // displacement of offset i along d: i*dr[d] - roff[d]
     double zetp;
-    double R2;           // squared sphere radius (imr * min spacing)^2
+    double R2;           // ortho: (imr * min spacing)^2; general: radius^2
     long long ent_off;   // offset of the item's partial moments (integrate)
     int base[3];         // grid index g -> offset i = lb + mod(g - base, period)
+                         // (general cells: roff is the lattice centre gp)
     int lb[3], ub[3];
     int lp;              // product l (without the force extension)
     int coef_off;
@@ -89,7 +98,9 @@ struct TileGrid {
     int nx, ny, nz;          // local grid dims
     int ntx, nty, ntz;       // tiles per axis
     int period[3];           // npts on periodic axes, BIG otherwise
+    int general;             // non-orthorhombic cell (lattice-coordinate path)
     double dr[3];
+    double dh[3][3];         // step vectors (rows), used by the general path
 };
 
 }  // namespace gpw

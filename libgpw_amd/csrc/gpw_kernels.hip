@@ -164,9 +164,13 @@ __device__ void build_eff(const PairDev &p, int tau, const double *__restrict__ 
 }
 
 // one workgroup per pair; dynamic LDS holds alpha and the effective block
+// mtrans (general cells): for each lp a dense C3(lp) x C3(lp) matrix T,
+// coef_ijk = T coef_xyz (multinomial expansion of grid_base_ref_c.cc:647-698)
 __global__ __launch_bounds__(64) void k_coef(const PairDev *__restrict__ pairs, int pair0,
                                              int tau, const double *__restrict__ pab,
-                                             double *__restrict__ coef) {
+                                             double *__restrict__ coef,
+                                             const double *__restrict__ mtrans,
+                                             const long long *__restrict__ mt_off) {
     extern __shared__ double sm_coef[];
     const PairDev p = pairs[pair0 + blockIdx.x];
     const int lam = p.la_max + tau, lbm = p.lb_max + tau;
@@ -176,10 +180,12 @@ __global__ __launch_bounds__(64) void k_coef(const PairDev *__restrict__ pairs, 
     const int nalpha = 3 * (lbm + 1) * (lam + 1) * (lp + 1);
     double *eff = sm_coef + nalpha;
     const int nla = ncoset(lam), nlb = ncoset(lbm);
+    double *cxyz = eff + nla * nlb;
     fill_alpha(p, A, threadIdx.x, blockDim.x);
     build_eff(p, tau, pab, eff, nla, nlb, nla, threadIdx.x, blockDim.x);
     const int na_lo = ncoset(lam_lo - 1), nb_lo = ncoset(lbm_lo - 1);
-    for (int c = threadIdx.x; c < ncoset(lp); c += blockDim.x) {
+    const int n3 = ncoset(lp);
+    for (int c = threadIdx.x; c < n3; c += blockDim.x) {
         int lzp, lyp, lxp;
         decode_c3(lp, c, lzp, lyp, lxp);
         double acc = 0.0;
@@ -199,7 +205,18 @@ __global__ __launch_bounds__(64) void k_coef(const PairDev *__restrict__ pairs, 
                 acc += e * A.at(0, bx, ax, lxp) * A.at(1, by, ay, lyp) * A.at(2, bz, az, lzp);
             }
         }
-        coef[p.coef_off + c] = p.prefactor * acc;
+        if (mtrans)
+            cxyz[c] = p.prefactor * acc;
+        else
+            coef[p.coef_off + c] = p.prefactor * acc;
+    }
+    if (!mtrans) return;
+    __syncthreads();
+    const double *T = mtrans + mt_off[lp];
+    for (int r = threadIdx.x; r < n3; r += blockDim.x) {
+        double a = 0.0;
+        for (int c = 0; c < n3; c++) a += T[(long long)r * n3 + c] * cxyz[c];
+        coef[p.coef_off + r] = a;
     }
 }
 
@@ -273,16 +290,61 @@ __device__ __forceinline__ int row_T(int kk, int jj, const TileGrid &tg, double 
     return sphere_extent_dev(b2, tg.dr[0], R2);
 }
 
+// General cells: i-range of lattice row (j, k) inside the sphere, with the
+// operation order of grid_base_ref_c.cc:790-811 and no contraction, so the
+// index set equals the host's and the oracle's; qb, qc: the row's quadratic
+// scaled by -zetp (exponent of point i: (qa*i + qb)*i + qc).
+__device__ __forceinline__ void general_row_dev(const TileGrid &tg, double gp0, double dj, double dk,
+                                                double R2, double zetp, int &lo, int &hi,
+                                                double &qb, double &qc) {
+    double a = 0.0, b = 0.0, c = 0.0;
+#pragma unroll
+    for (int x = 0; x < 3; x++) {
+        const double pointk = __dmul_rn(tg.dh[2][x], dk);
+        const double pointj = __dadd_rn(pointk, __dmul_rn(tg.dh[1][x], dj));
+        const double v = __dsub_rn(pointj, __dmul_rn(gp0, tg.dh[0][x]));
+        a = __dadd_rn(a, __dmul_rn(tg.dh[0][x], tg.dh[0][x]));
+        b = __dadd_rn(b, __dmul_rn(__dmul_rn(2.0, v), tg.dh[0][x]));
+        c = __dadd_rn(c, __dmul_rn(v, v));
+    }
+    double d = __dsub_rn(__dmul_rn(b, b), __dmul_rn(__dmul_rn(4.0, a), __dsub_rn(c, R2)));
+    if (d < 0.0) {
+        lo = 1;
+        hi = 0;
+        qb = qc = 0.0;
+        return;
+    }
+    d = __dsqrt_rn(d);
+    const double a2 = __dmul_rn(2.0, a);
+    lo = (int)ceil(__ddiv_rn(__dsub_rn(-b, d), a2));
+    hi = (int)floor(__ddiv_rn(__dadd_rn(-b, d), a2));
+    qb = b * -zetp;
+    qc = c * -zetp;
+}
+
+// per-row data of the general path (unused storage on orthorhombic grids)
+template <int Q, bool GEN>
+struct GenRows {
+    int lo[Q][NROW], hi[Q][NROW];
+    double qb[Q][NROW], qc[Q][NROW];
+    double qa[Q];
+    int ball[Q][NXG];  // 1: every point of the x block is in the sub-box
+};
+template <int Q>
+struct GenRows<Q, false> {};
+
 // Table phase of pair slot q, run by one whole wave (lane = t % 64).
 // Lanes 0..31: x entries, 32..47: y entries, 48..63: z entries.
 // WITH_CXY: z lanes also contract coef with their Z powers (collocate).
-template <int L, int LM, bool WITH_CXY>
-__device__ __forceinline__ void tile_phase1(const EntryRec &R, int lp_add, int lane, int x0,
-                                            int y0, int z0, const TileGrid &tg,
+// GEN (general cells): tables are pure powers of the lattice displacement
+// i - gp; the Gaussian is applied per point from the row quadratic.
+template <int L, int LM, bool WITH_CXY, bool GEN, int Q>
+__device__ __forceinline__ void tile_phase1(const EntryRec &R, int q, int lane, int x0, int y0,
+                                            int z0, const TileGrid &tg,
                                             const double *__restrict__ coef, double *X,
                                             double (*Y)[LM], double (*Z)[LM],
                                             double (*cxy)[TileCfg<LM>::C2M], int *xi, int *T,
-                                            int *bmax, int *bmin) {
+                                            int *bmax, int *bmin, GenRows<Q, GEN> &gr) {
     constexpr int lp = L - 1;
     const int d = lane < TX ? 0 : lane < TX + TY ? 1 : 2;
     const int pos = d == 0 ? lane : d == 1 ? lane - TX : lane - TX - TY;
@@ -298,15 +360,19 @@ __device__ __forceinline__ void tile_phase1(const EntryRec &R, int lp_add, int l
     }
     double tv[L];
     {
-        const double r = i * dr - R.roff[d];
-        double v = ok ? exp(-R.zetp * r * r) : 0.0;
+        const double r = GEN ? (double)i - R.roff[d] : i * dr - R.roff[d];
+        double v = ok ? (GEN ? 1.0 : exp(-R.zetp * r * r)) : 0.0;
 #pragma unroll
         for (int k = 0; k < L; k++) {
             tv[k] = v;
             v *= r;
         }
     }
-    const int code = ok ? fold_off(i) : (d == 0 ? BIG : -1);
+    int code;
+    if constexpr (GEN)
+        code = ok ? i : (d == 0 ? BIG : -BIG);
+    else
+        code = ok ? fold_off(i) : (d == 0 ? BIG : -1);
     if (d == 0) {
         double *dst = X + TileCfg<LM>::xat(pos);
 #pragma unroll
@@ -334,27 +400,71 @@ __device__ __forceinline__ void tile_phase1(const EntryRec &R, int lp_add, int l
                 }
         }
     }
-    // x blocks: max code (BIG if any point invalid) and min code over 8 lanes
-    int mx = code, mn = code;
+    if constexpr (!GEN) {
+        // x blocks: max code (BIG if any point invalid) and min code over 8 lanes
+        int mx = code, mn = code;
 #pragma unroll
-    for (int o = 1; o < XB; o <<= 1) {
-        mx = max(mx, __shfl_xor(mx, o, XB));
-        mn = min(mn, __shfl_xor(mn, o, XB));
-    }
-    if (d == 0 && (lane % XB) == 0) {
-        bmax[lane / XB] = mx;
-        bmin[lane / XB] = mn;
-    }
-    // thresholds of the 256 tile rows (row = z * TY + y), 4 per lane
+        for (int o = 1; o < XB; o <<= 1) {
+            mx = max(mx, __shfl_xor(mx, o, XB));
+            mn = min(mn, __shfl_xor(mn, o, XB));
+        }
+        if (d == 0 && (lane % XB) == 0) {
+            bmax[lane / XB] = mx;
+            bmin[lane / XB] = mn;
+        }
+        // thresholds of the 256 tile rows (row = z * TY + y), 4 per lane
 #pragma unroll
-    for (int rr = 0; rr < NROW / 64; rr++) {
-        const int row = lane + 64 * rr;
-        const int kk = __shfl(code, TX + TY + row / TY);
-        const int jj = __shfl(code, TX + row % TY);
-        T[row] = row_T(kk, jj, tg, R.R2);
+        for (int rr = 0; rr < NROW / 64; rr++) {
+            const int row = lane + 64 * rr;
+            const int kk = __shfl(code, TX + TY + row / TY);
+            const int jj = __shfl(code, TX + row % TY);
+            T[row] = row_T(kk, jj, tg, R.R2);
+        }
+    } else {
+        // x blocks: min / max lattice index of the valid points, all valid?
+        int mn = ok ? i : BIG, mx = ok ? i : -BIG, all = ok ? 1 : 0;
+#pragma unroll
+        for (int o = 1; o < XB; o <<= 1) {
+            mx = max(mx, __shfl_xor(mx, o, XB));
+            mn = min(mn, __shfl_xor(mn, o, XB));
+            all = min(all, __shfl_xor(all, o, XB));
+        }
+        if (d == 0 && (lane % XB) == 0) {
+            bmax[lane / XB] = mx;
+            bmin[lane / XB] = mn;
+            gr.ball[q][lane / XB] = all;
+        }
+#pragma unroll
+        for (int rr = 0; rr < NROW / 64; rr++) {
+            const int row = lane + 64 * rr;
+            const int kidx = __shfl(code, TX + TY + row / TY);
+            const int jidx = __shfl(code, TX + row % TY);
+            int lo = 1, hi = 0;
+            double qb = 0.0, qc = 0.0;
+            if (kidx != -BIG && jidx != -BIG)
+                general_row_dev(tg, R.roff[0], __dsub_rn((double)jidx, R.roff[1]),
+                                __dsub_rn((double)kidx, R.roff[2]), R.R2, R.zetp, lo, hi, qb, qc);
+            gr.lo[q][row] = lo;
+            gr.hi[q][row] = hi;
+            gr.qb[q][row] = qb;
+            gr.qc[q][row] = qc;
+        }
+        if (lane == 0) {
+            double a = 0.0;
+#pragma unroll
+            for (int x = 0; x < 3; x++) a = __dadd_rn(a, __dmul_rn(tg.dh[0][x], tg.dh[0][x]));
+            gr.qa[q] = a * -R.zetp;
+        }
+        (void)T;
     }
-    (void)lp_add;
 }
+
+// Phase-3 row data of one thread's register block on the general path.
+struct GenBlock {
+    int lo[RB], hi[RB];
+    double qb[RB], qc[RB];
+    double qa;
+};
 
 #define GPW_LP_DISPATCH(LM, CALL)                                                   \
     switch (lp) {                                                                   \
@@ -395,9 +505,10 @@ __device__ __forceinline__ void tile_phase1(const EntryRec &R, int lp_add, int l
         break;                                                                      \
     }
 
-template <int LM>
+template <int LM, bool GEN>
 struct CollocSmem {
     using C = TileCfg<LM>;
+    GenRows<C::Q, GEN> gr;
     EntryRec rec[3][C::Q];
     double coef[2][C::Q][C::C3M];
     double X[C::Q][C::XS];
@@ -411,10 +522,10 @@ struct CollocSmem {
 
 // register block of one thread for one pair: rows y = 4*yb + r of plane z,
 // points x = 8*xg + j
-template <int L, int LM>
-__device__ __forceinline__ void colloc_block(const CollocSmem<LM> &sm, int q, int xg, int yb, int z,
-                                             const int (&Tr)[RB], bool full,
-                                             double (&acc)[RB][XB]) {
+template <int L, int LM, bool GEN>
+__device__ __forceinline__ void colloc_block(const CollocSmem<LM, GEN> &sm, int q, int xg, int yb,
+                                             int z, const int (&Tr)[RB], bool full,
+                                             const GenBlock &gb, double (&acc)[RB][XB]) {
     constexpr int lp = L - 1;
     double cx[RB][L];
     const double *cxy = sm.cxy[q][z];
@@ -442,19 +553,61 @@ __device__ __forceinline__ void colloc_block(const CollocSmem<LM> &sm, int q, in
             double v = 0.0;
 #pragma unroll
             for (int l = 0; l < L; l++) v += cx[r][l] * xv[l];
-            acc[r][j] += (full || code <= Tr[r]) ? v : 0.0;
+            if constexpr (GEN) {
+                const double ci = (double)code;
+                const double e = exp((gb.qa * ci + gb.qb[r]) * ci + gb.qc[r]);
+                acc[r][j] += (full || (code >= gb.lo[r] && code <= gb.hi[r])) ? v * e : 0.0;
+            } else {
+                acc[r][j] += (full || code <= Tr[r]) ? v : 0.0;
+            }
         }
     }
 }
 
+// classification of a thread's register block for pair q: returns false when
+// no point of the block is inside; sets full when every point is inside.
+template <bool GEN, class Smem>
+__device__ __forceinline__ bool classify_block(const Smem &sm, int q, int xg, int yb, int z,
+                                               int (&Tr)[RB], GenBlock &gb, bool &full) {
+    const int bmn = sm.bmin[q][xg], bmx = sm.bmax[q][xg];
+    if constexpr (!GEN) {
+        int tmin = BIG, tmax = -1;
+#pragma unroll
+        for (int r = 0; r < RB; r++) {
+            Tr[r] = sm.T[q][z * TY + yb * RB + r];
+            tmin = min(tmin, Tr[r]);
+            tmax = max(tmax, Tr[r]);
+        }
+        if (tmax < 0 || bmn > tmax) return false;
+        full = bmx <= tmin;
+        return true;
+    } else {
+        bool any = false;
+        full = sm.gr.ball[q][xg] != 0;
+#pragma unroll
+        for (int r = 0; r < RB; r++) {
+            const int row = z * TY + yb * RB + r;
+            gb.lo[r] = sm.gr.lo[q][row];
+            gb.hi[r] = sm.gr.hi[q][row];
+            gb.qb[r] = sm.gr.qb[q][row];
+            gb.qc[r] = sm.gr.qc[q][row];
+            any = any || (gb.lo[r] <= gb.hi[r] && gb.lo[r] <= bmx && gb.hi[r] >= bmn);
+            full = full && gb.lo[r] <= bmn && bmx <= gb.hi[r];
+            Tr[r] = 0;
+        }
+        gb.qa = sm.gr.qa[q];
+        return any && bmn <= bmx;
+    }
+}
+
 // out_mode 0: grid += acc (splits == 1); 1: partial[split] = acc; 2: partial[split] += acc
-template <int LM>
+template <int LM, bool GEN>
 __global__ __launch_bounds__(NROW, 2) void k_collocate_tiles(
     TileGrid tg, const EntryRec *__restrict__ recs, const int *__restrict__ tile_start,
     const double *__restrict__ coef, int splits, double *__restrict__ out, int out_mode) {
     using C = TileCfg<LM>;
     constexpr int Q = C::Q;
-    __shared__ CollocSmem<LM> sm;
+    __shared__ CollocSmem<LM, GEN> sm;
     const int tile = blockIdx.x / splits, split = blockIdx.x % splits;
     const int tix = tile % tg.ntx, tiy = (tile / tg.ntx) % tg.nty, tiz = tile / (tg.ntx * tg.nty);
     const int x0 = tix * TX, y0 = tiy * TY, z0 = tiz * TZ;
@@ -513,8 +666,9 @@ __global__ __launch_bounds__(NROW, 2) void k_collocate_tiles(
                 const int lp = R.lp;
                 if (lane == 0) sm.lp[q] = lp;
 #define GPW_P1_CALL(L)                                                                        \
-    tile_phase1<L, LM, true>(R, 0, lane, x0, y0, z0, tg, sm.coef[cs][q], sm.X[q], sm.Y[q],     \
-                             nullptr, sm.cxy[q], sm.xi[q], sm.T[q], sm.bmax[q], sm.bmin[q])
+    tile_phase1<L, LM, true, GEN, Q>(R, q, lane, x0, y0, z0, tg, sm.coef[cs][q], sm.X[q],      \
+                                     sm.Y[q], nullptr, sm.cxy[q], sm.xi[q], sm.T[q],          \
+                                     sm.bmax[q], sm.bmin[q], sm.gr)
                 GPW_LP_DISPATCH(LM, GPW_P1_CALL)
 #undef GPW_P1_CALL
             }
@@ -522,17 +676,12 @@ __global__ __launch_bounds__(NROW, 2) void k_collocate_tiles(
         __syncthreads();
 #pragma unroll 1
         for (int q = 0; q < nq; q++) {
-            int Tr[RB], tmin = BIG, tmax = -1;
-#pragma unroll
-            for (int r = 0; r < RB; r++) {
-                Tr[r] = sm.T[q][z * TY + yb * RB + r];
-                tmin = min(tmin, Tr[r]);
-                tmax = max(tmax, Tr[r]);
-            }
-            if (tmax < 0 || sm.bmin[q][xg] > tmax) continue;
-            const bool full = sm.bmax[q][xg] <= tmin;
+            int Tr[RB];
+            GenBlock gb;
+            bool full;
+            if (!classify_block<GEN>(sm, q, xg, yb, z, Tr, gb, full)) continue;
             const int lp = sm.lp[q];
-#define GPW_COLLOC_CALL(L) colloc_block<L, LM>(sm, q, xg, yb, z, Tr, full, acc)
+#define GPW_COLLOC_CALL(L) colloc_block<L, LM, GEN>(sm, q, xg, yb, z, Tr, full, gb, acc)
             GPW_LP_DISPATCH(LM, GPW_COLLOC_CALL)
 #undef GPW_COLLOC_CALL
         }
@@ -594,9 +743,10 @@ __device__ __forceinline__ double stride4_sum(double v) {
     return v + dpp_f64<0x128>(v);
 }
 
-template <int LM>
+template <int LM, bool GEN>
 struct IntegSmem {
     using C = TileCfg<LM>;
+    GenRows<C::Q, GEN> gr;
     EntryRec rec[2][C::Q];
     double X[C::Q][C::XS];
     double Y[C::Q][TY][LM];
@@ -612,9 +762,9 @@ struct IntegSmem {
 __device__ __forceinline__ double row_sum16(double v) { return stride4_sum(quad_sum(v)); }
 
 // x stage on the register block, local y contraction, DPP sum over the z plane
-template <int L, int LM>
-__device__ __forceinline__ void integ_block(IntegSmem<LM> &sm, int q, int xg, int yb, int z,
-                                            const int (&Tr)[RB], int mode,
+template <int L, int LM, bool GEN>
+__device__ __forceinline__ void integ_block(IntegSmem<LM, GEN> &sm, int q, int xg, int yb, int z,
+                                            const int (&Tr)[RB], int mode, const GenBlock &gb,
                                             const double (&v)[RB][XB], int t) {
     constexpr int lp = L - 1;
     constexpr int N2 = L * (L + 1) / 2;
@@ -636,7 +786,14 @@ __device__ __forceinline__ void integ_block(IntegSmem<LM> &sm, int q, int xg, in
             const int code = sm.xi[q][xg * XB + j];
 #pragma unroll
             for (int r = 0; r < RB; r++) {
-                const double w = (mode == 2 || code <= Tr[r]) ? v[r][j] : 0.0;
+                double w;
+                if constexpr (GEN) {
+                    const double ci = (double)code;
+                    const double e = exp((gb.qa * ci + gb.qb[r]) * ci + gb.qc[r]);
+                    w = (mode == 2 || (code >= gb.lo[r] && code <= gb.hi[r])) ? v[r][j] * e : 0.0;
+                } else {
+                    w = (mode == 2 || code <= Tr[r]) ? v[r][j] : 0.0;
+                }
 #pragma unroll
                 for (int l = 0; l < L; l++) tx[r][l] += w * xv[l];
             }
@@ -660,13 +817,13 @@ __device__ __forceinline__ void integ_block(IntegSmem<LM> &sm, int q, int xg, in
 }
 
 // partial moments of every (sub-box, tile) item: partial[rec.ent_off + c3]
-template <int LM>
+template <int LM, bool GEN>
 __global__ __launch_bounds__(NROW, 2) void k_integrate_tiles(
     TileGrid tg, const EntryRec *__restrict__ recs, const int *__restrict__ tile_start,
     const double *__restrict__ grid, int lp_add, int splits, double *__restrict__ partial) {
     using C = TileCfg<LM>;
     constexpr int Q = C::Q;
-    __shared__ IntegSmem<LM> sm;
+    __shared__ IntegSmem<LM, GEN> sm;
     const int tile = blockIdx.x / splits, split = blockIdx.x % splits;
     const int tix = tile % tg.ntx, tiy = (tile / tg.ntx) % tg.nty, tiz = tile / (tg.ntx * tg.nty);
     const int x0 = tix * TX, y0 = tiy * TY, z0 = tiz * TZ;
@@ -712,8 +869,9 @@ __global__ __launch_bounds__(NROW, 2) void k_integrate_tiles(
                 const int lp = R.lp + lp_add;
                 if (lane == 0) sm.lp[q] = lp;
 #define GPW_P1_CALL(L)                                                                         \
-    tile_phase1<L, LM, false>(R, lp_add, lane, x0, y0, z0, tg, nullptr, sm.X[q], sm.Y[q],       \
-                              sm.Z[q], nullptr, sm.xi[q], sm.T[q], sm.bmax[q], sm.bmin[q])
+    tile_phase1<L, LM, false, GEN, Q>(R, q, lane, x0, y0, z0, tg, nullptr, sm.X[q], sm.Y[q],    \
+                                      sm.Z[q], nullptr, sm.xi[q], sm.T[q], sm.bmax[q],          \
+                                      sm.bmin[q], sm.gr)
                 GPW_LP_DISPATCH(LM, GPW_P1_CALL)
 #undef GPW_P1_CALL
             }
@@ -721,20 +879,12 @@ __global__ __launch_bounds__(NROW, 2) void k_integrate_tiles(
         __syncthreads();
 #pragma unroll 1
         for (int q = 0; q < nq; q++) {
-            int Tr[RB], tmin = BIG, tmax = -1;
-#pragma unroll
-            for (int r = 0; r < RB; r++) {
-                Tr[r] = sm.T[q][z * TY + yb * RB + r];
-                tmin = min(tmin, Tr[r]);
-                tmax = max(tmax, Tr[r]);
-            }
-            int mode = 1;
-            if (tmax < 0 || sm.bmin[q][xg] > tmax)
-                mode = 0;
-            else if (sm.bmax[q][xg] <= tmin)
-                mode = 2;
+            int Tr[RB];
+            GenBlock gb;
+            bool full = false;
+            const int mode = classify_block<GEN>(sm, q, xg, yb, z, Tr, gb, full) ? (full ? 2 : 1) : 0;
             const int lp = sm.lp[q];
-#define GPW_INTEG_CALL(L) integ_block<L, LM>(sm, q, xg, yb, z, Tr, mode, v, t)
+#define GPW_INTEG_CALL(L) integ_block<L, LM, GEN>(sm, q, xg, yb, z, Tr, mode, gb, v, t)
             GPW_LP_DISPATCH(LM, GPW_INTEG_CALL)
 #undef GPW_INTEG_CALL
         }
@@ -795,16 +945,23 @@ __global__ void k_sum_entries(const PairDev *__restrict__ pairs, int tau, int lp
     }
 }
 
-// one workgroup per pair: heff = prefactor * alpha^T M over the extended block,
-// then hab (rho copy / tau adjoint) and forces (d phi/dA = 2 zeta phi_{a+1} - a phi_{a-1})
-__global__ __launch_bounds__(64) void k_hab(const PairDev *__restrict__ pairs, int pair0, int tau,
-                                            int with_forces, const double *__restrict__ Mbuf,
+// heff = prefactor * alpha^T M over the extended block, then hab (rho copy /
+// tau adjoint) and forces (d phi/dA = 2 zeta phi_{a+1} - a phi_{a-1}).
+// Workgroups loop over pairs (pair0 + blockIdx.x, stride gridDim.x). heff and
+// eff live in LDS, or, when gws is given (blocks too large for LDS), in the
+// workgroup's slice gws + blockIdx.x * ws_stride of a global workspace.
+__global__ __launch_bounds__(64) void k_hab(const PairDev *__restrict__ pairs, int pair0, int npairs,
+                                            int tau, int with_forces, const double *__restrict__ Mbuf,
                                             int mstride, const double *__restrict__ pab,
-                                            double *__restrict__ hab, double *__restrict__ forces) {
+                                            double *__restrict__ hab, double *__restrict__ forces,
+                                            const double *__restrict__ mtrans,
+                                            const long long *__restrict__ mt_off,
+                                            double *__restrict__ gws, long long ws_stride) {
     extern __shared__ double sm_hab[];
-    const int pi = pair0 + blockIdx.x;
-    const PairDev p = pairs[pi];
     const int fx = with_forces ? 1 : 0;
+    for (int pi = pair0 + blockIdx.x; pi < pair0 + npairs; pi += gridDim.x) {
+    __syncthreads();  // LDS / workspace reuse across pairs
+    const PairDev p = pairs[pi];
     const int lam = p.la_max + tau, lbm = p.lb_max + tau;          // local
     const int lamx = lam + fx, lbmx = lbm + fx;                    // extended
     const int lamx_lo = max(max(p.la_min - tau, 0) - fx, 0);
@@ -813,13 +970,31 @@ __global__ __launch_bounds__(64) void k_hab(const PairDev *__restrict__ pairs, i
     AlphaView A{sm_hab, lamx, lbmx, lpx};
     const int nalpha = 3 * (lbmx + 1) * (lamx + 1) * (lpx + 1);
     const int nlax = ncoset(lamx), nlbx = ncoset(lbmx);
-    double *heff = sm_hab + nalpha;             // nlbx x nlax
-    double *M = heff + nlax * nlbx;             // ncoset(lpx)
-    double *eff = M + ncoset(lpx);              // ncoset(lbm) x ncoset(lam)
-    double *red = eff + ncoset(lam) * ncoset(lbm);  // 6 x 64
+    const int n3x = ncoset(lpx);
+    double *M = sm_hab + nalpha;                    // ncoset(lpx)
+    double *Mijk = M + n3x;                         // general cells: lattice moments
+    double *red = Mijk + n3x;                       // 6 x 64
+    double *heff, *eff;                             // nlbx x nlax, ncoset(lbm) x ncoset(lam)
+    if (gws) {
+        heff = gws + blockIdx.x * ws_stride;
+    } else {
+        heff = red + 6 * 64;
+    }
+    eff = heff + nlax * nlbx;
     fill_alpha(p, A, threadIdx.x, blockDim.x);
-    for (int c = threadIdx.x; c < ncoset(lpx); c += blockDim.x)
-        M[c] = Mbuf[(long long)pi * mstride + c];
+    if (!mtrans) {
+        for (int c = threadIdx.x; c < n3x; c += blockDim.x) M[c] = Mbuf[(long long)pi * mstride + c];
+    } else {
+        // M_xyz = T^T M_ijk (adjoint of the lattice transform)
+        for (int c = threadIdx.x; c < n3x; c += blockDim.x) Mijk[c] = Mbuf[(long long)pi * mstride + c];
+        __syncthreads();
+        const double *T = mtrans + mt_off[lpx];
+        for (int c = threadIdx.x; c < n3x; c += blockDim.x) {
+            double a = 0.0;
+            for (int r = 0; r < n3x; r++) a += T[(long long)r * n3x + c] * Mijk[r];
+            M[c] = a;
+        }
+    }
     __syncthreads();
     const int na_lo = ncoset(lamx_lo - 1), nb_lo = ncoset(lbmx_lo - 1);
     for (int w = threadIdx.x; w < nlax * nlbx; w += blockDim.x) {
@@ -878,7 +1053,7 @@ __global__ __launch_bounds__(64) void k_hab(const PairDev *__restrict__ pairs, i
         }
         hab[p.block_off + (long long)jco * p.ld + ico] += acc;
     }
-    if (!with_forces) return;
+    if (!with_forces) continue;
     const int nla = ncoset(lam), nlb = ncoset(lbm);
     build_eff(p, tau, pab, eff, nla, nlb, nla, threadIdx.x, blockDim.x);
     double f[6] = {0, 0, 0, 0, 0, 0};
@@ -920,6 +1095,7 @@ __global__ __launch_bounds__(64) void k_hab(const PairDev *__restrict__ pairs, i
         for (int i = 0; i < 64; i++) s += red[threadIdx.x * 64 + i];
         forces[(long long)pi * 6 + threadIdx.x] += s;
     }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -935,12 +1111,13 @@ __global__ __launch_bounds__(PR *PL) void k_collocate_plain(TileGrid tg, const S
     __shared__ double cxy[ncoef2(LPMAX)];
     __shared__ double zt[LPMAX + 1];
     const SubDev s = subs[si];
+    const bool gen = tg.general != 0;
     const int lp = s.lp;
     const int k = s.lb[2] + blockIdx.x;
     const int t = threadIdx.x, r = t / PL, lane = t % PL;
+    const double rz = gen ? (double)k - s.roff[2] : k * tg.dr[2] - s.roff[2];
     if (t == 0) {
-        const double rz = k * tg.dr[2] - s.roff[2];
-        double v = exp(-s.zetp * rz * rz);
+        double v = gen ? 1.0 : exp(-s.zetp * rz * rz);
         for (int l = 0; l <= lp; l++) {
             zt[l] = v;
             v *= rz;
@@ -957,13 +1134,24 @@ __global__ __launch_bounds__(PR *PL) void k_collocate_plain(TileGrid tg, const S
     __syncthreads();
     const int j = s.lb[1] + blockIdx.y * PR + r;
     if (j > s.ub[1]) return;
-    const int T = row_threshold(prof, s.prof_off, s.prof_stride, fold_off(k), fold_off(j));
-    if (T < 0) return;
+    int ilo = s.lb[0], ihi = s.ub[0], T = BIG;
+    double qa = 0.0, qb = 0.0, qc = 0.0;
+    const double ry = gen ? (double)j - s.roff[1] : j * tg.dr[1] - s.roff[1];
+    if (gen) {
+        int lo, hi;
+        general_row_dev(tg, s.roff[0], ry, rz, s.R2, s.zetp, lo, hi, qb, qc);
+        ilo = max(ilo, lo);
+        ihi = min(ihi, hi);
+        for (int x = 0; x < 3; x++) qa = __dadd_rn(qa, __dmul_rn(tg.dh[0][x], tg.dh[0][x]));
+        qa = qa * -s.zetp;
+    } else {
+        T = row_threshold(prof, s.prof_off, s.prof_stride, fold_off(k), fold_off(j));
+        if (T < 0) return;
+    }
+    if (ilo > ihi) return;
     double cx[LPMAX + 1];
-    const double ry = j * tg.dr[1] - s.roff[1];
-    const double ey = exp(-s.zetp * ry * ry);
     for (int l = 0; l <= lp; l++) cx[l] = 0.0;
-    double yp = ey;
+    double yp = gen ? 1.0 : exp(-s.zetp * ry * ry);
     for (int lyp = 0; lyp <= lp; lyp++) {
         for (int lxp = 0; lxp <= lp - lyp; lxp++) cx[lxp] += cxy[c2idx(lp, lyp, lxp)] * yp;
         yp *= ry;
@@ -971,14 +1159,15 @@ __global__ __launch_bounds__(PR *PL) void k_collocate_plain(TileGrid tg, const S
     const int gz = imod(s.base[2] + (k - s.lb[2]), s.period[2]);
     const int gy = imod(s.base[1] + (j - s.lb[1]), s.period[1]);
     double *row = grid + ((long long)gz * tg.ny + gy) * tg.nx;
-    for (int i = s.lb[0] + lane; i <= s.ub[0]; i += PL) {
-        if (fold_off(i) > T) continue;
-        const double rx = i * tg.dr[0] - s.roff[0];
-        double xp = exp(-s.zetp * rx * rx), v = 0.0;
+    for (int i = ilo + lane; i <= ihi; i += PL) {
+        if (!gen && fold_off(i) > T) continue;
+        const double rx = gen ? (double)i - s.roff[0] : i * tg.dr[0] - s.roff[0];
+        double xp = gen ? 1.0 : exp(-s.zetp * rx * rx), v = 0.0;
         for (int l = 0; l <= lp; l++) {
             v += cx[l] * xp;
             xp *= rx;
         }
+        if (gen) v *= exp((qa * i + qb) * i + qc);
         row[imod(s.base[0] + (i - s.lb[0]), s.period[0])] += v;
     }
 }
@@ -991,25 +1180,40 @@ __global__ __launch_bounds__(PR *PL) void k_moments_plain(TileGrid tg, const Sub
                                                           int mstride, double *__restrict__ Mbuf) {
     __shared__ double tyl[PR][ncoef2(LPMAX)];
     const SubDev s = subs[si];
+    const bool gen = tg.general != 0;
     const int lp = s.lp + lp_add;
-    const int n2 = ncoef2(lp);
     const int k = s.lb[2] + blockIdx.x;
     const int t = threadIdx.x, r = t / PL, lane = t % PL;
     const int j = s.lb[1] + blockIdx.y * PR + r;
     double tx[LPMAX + 1];
     for (int l = 0; l <= lp; l++) tx[l] = 0.0;
-    double ry = 0.0;
-    const int T = j <= s.ub[1] ? row_threshold(prof, s.prof_off, s.prof_stride, fold_off(k), fold_off(j))
-                               : -1;
-    if (T >= 0) {
-        ry = j * tg.dr[1] - s.roff[1];
+    const double rz = gen ? (double)k - s.roff[2] : k * tg.dr[2] - s.roff[2];
+    const double ry = gen ? (double)j - s.roff[1] : j * tg.dr[1] - s.roff[1];
+    bool rowok = j <= s.ub[1];
+    int ilo = s.lb[0], ihi = s.ub[0], T = BIG;
+    double qa = 0.0, qb = 0.0, qc = 0.0;
+    if (rowok) {
+        if (gen) {
+            int lo, hi;
+            general_row_dev(tg, s.roff[0], ry, rz, s.R2, s.zetp, lo, hi, qb, qc);
+            ilo = max(ilo, lo);
+            ihi = min(ihi, hi);
+            for (int x = 0; x < 3; x++) qa = __dadd_rn(qa, __dmul_rn(tg.dh[0][x], tg.dh[0][x]));
+            qa = qa * -s.zetp;
+        } else {
+            T = row_threshold(prof, s.prof_off, s.prof_stride, fold_off(k), fold_off(j));
+            rowok = T >= 0;
+        }
+    }
+    if (rowok) {
         const int gz = imod(s.base[2] + (k - s.lb[2]), s.period[2]);
         const int gy = imod(s.base[1] + (j - s.lb[1]), s.period[1]);
         const double *row = grid + ((long long)gz * tg.ny + gy) * tg.nx;
-        for (int i = s.lb[0] + lane; i <= s.ub[0]; i += PL) {
-            if (fold_off(i) > T) continue;
-            const double rx = i * tg.dr[0] - s.roff[0];
-            double xp = exp(-s.zetp * rx * rx) * row[imod(s.base[0] + (i - s.lb[0]), s.period[0])];
+        for (int i = ilo + lane; i <= ihi; i += PL) {
+            if (!gen && fold_off(i) > T) continue;
+            const double rx = gen ? (double)i - s.roff[0] : i * tg.dr[0] - s.roff[0];
+            double xp = (gen ? exp((qa * i + qb) * i + qc) : exp(-s.zetp * rx * rx)) *
+                        row[imod(s.base[0] + (i - s.lb[0]), s.period[0])];
             for (int l = 0; l <= lp; l++) {
                 tx[l] += xp;
                 xp *= rx;
@@ -1023,16 +1227,14 @@ __global__ __launch_bounds__(PR *PL) void k_moments_plain(TileGrid tg, const Sub
         tx[l] = v;
     }
     if (lane == 0) {
-        const double ey = T >= 0 ? exp(-s.zetp * ry * ry) : 0.0;
-        double yp = ey;
+        double yp = rowok ? (gen ? 1.0 : exp(-s.zetp * ry * ry)) : 0.0;
         for (int lyp = 0; lyp <= lp; lyp++) {
             for (int lxp = 0; lxp <= lp - lyp; lxp++) tyl[r][c2idx(lp, lyp, lxp)] = tx[lxp] * yp;
             yp *= ry;
         }
     }
     __syncthreads();
-    const double rz = k * tg.dr[2] - s.roff[2];
-    const double ez = exp(-s.zetp * rz * rz);
+    const double ez = gen ? 1.0 : exp(-s.zetp * rz * rz);
     for (int c = t; c < ncoset(lp); c += blockDim.x) {
         int lzp, lyp, lxp;
         decode_c3(lp, c, lzp, lyp, lxp);
@@ -1110,8 +1312,12 @@ template <int LM>
 static hipError_t launch_collocate_tiles_lm(const TileGrid &tg, const EntryRec *recs,
                                             const int *tile_start, const double *coef, int ntiles,
                                             int splits, double *out, int out_mode, hipStream_t st) {
-    hipLaunchKernelGGL((k_collocate_tiles<LM>), dim3(ntiles * splits), dim3(NROW), 0, st, tg, recs,
-                       tile_start, coef, splits, out, out_mode);
+    if (tg.general)
+        hipLaunchKernelGGL((k_collocate_tiles<LM, true>), dim3(ntiles * splits), dim3(NROW), 0, st,
+                           tg, recs, tile_start, coef, splits, out, out_mode);
+    else
+        hipLaunchKernelGGL((k_collocate_tiles<LM, false>), dim3(ntiles * splits), dim3(NROW), 0, st,
+                           tg, recs, tile_start, coef, splits, out, out_mode);
     return hipGetLastError();
 }
 
@@ -1139,8 +1345,12 @@ static hipError_t launch_integrate_tiles_lm(const TileGrid &tg, const EntryRec *
                                             const int *tile_start, const double *grid, int lp_add,
                                             int ntiles, int splits, double *partial,
                                             hipStream_t st) {
-    hipLaunchKernelGGL((k_integrate_tiles<LM>), dim3(ntiles * splits), dim3(NROW), 0, st, tg, recs,
-                       tile_start, grid, lp_add, splits, partial);
+    if (tg.general)
+        hipLaunchKernelGGL((k_integrate_tiles<LM, true>), dim3(ntiles * splits), dim3(NROW), 0, st,
+                           tg, recs, tile_start, grid, lp_add, splits, partial);
+    else
+        hipLaunchKernelGGL((k_integrate_tiles<LM, false>), dim3(ntiles * splits), dim3(NROW), 0, st,
+                           tg, recs, tile_start, grid, lp_add, splits, partial);
     return hipGetLastError();
 }
 
@@ -1157,29 +1367,51 @@ hipError_t launch_integrate_tiles(int cls, const TileGrid &tg, const EntryRec *r
 
 size_t coef_smem_bytes(int lam, int lbm) {
     const int lp = lam + lbm;
-    return sizeof(double) * (size_t)(3 * (lbm + 1) * (lam + 1) * (lp + 1) + ncoset(lam) * ncoset(lbm));
+    return sizeof(double) * (size_t)(3 * (lbm + 1) * (lam + 1) * (lp + 1) + ncoset(lam) * ncoset(lbm) +
+                                     ncoset(lp));
 }
 
 hipError_t launch_coef(const PairDev *pairs, int pair0, int npairs, int tau, int lam_max,
-                       int lbm_max, const double *pab, double *coef, hipStream_t st) {
+                       int lbm_max, const double *pab, double *coef, const double *mtrans,
+                       const long long *mt_off, hipStream_t st) {
     const size_t smem = coef_smem_bytes(lam_max + tau, lbm_max + tau);
-    hipLaunchKernelGGL(k_coef, dim3(npairs), dim3(64), smem, st, pairs, pair0, tau, pab, coef);
+    hipLaunchKernelGGL(k_coef, dim3(npairs), dim3(64), smem, st, pairs, pair0, tau, pab, coef, mtrans,
+                       mt_off);
     return hipGetLastError();
 }
 
-size_t hab_smem_bytes(int lamx, int lbmx, int lam, int lbm) {
+size_t hab_ws_doubles(int lamx, int lbmx, int lam, int lbm) {
+    return (size_t)ncoset(lamx) * ncoset(lbmx) + (size_t)ncoset(lam) * ncoset(lbm);
+}
+
+// LDS of k_hab: alpha, M, Mijk, the force reduction and (LDS mode) heff, eff
+size_t hab_smem_bytes(int lamx, int lbmx, int lam, int lbm, bool lds_blocks) {
     const int lpx = lamx + lbmx;
-    return sizeof(double) * (size_t)(3 * (lbmx + 1) * (lamx + 1) * (lpx + 1) + ncoset(lamx) * ncoset(lbmx) +
-                                     ncoset(lpx) + ncoset(lam) * ncoset(lbm) + 6 * 64);
+    size_t n = 3 * (size_t)(lbmx + 1) * (lamx + 1) * (lpx + 1) + 2 * (size_t)ncoset(lpx) + 6 * 64;
+    if (lds_blocks) n += hab_ws_doubles(lamx, lbmx, lam, lbm);
+    return sizeof(double) * n;
+}
+
+bool hab_needs_workspace(int lamx, int lbmx, int lam, int lbm) {
+    return hab_smem_bytes(lamx, lbmx, lam, lbm, true) > HAB_LDS_MAX;
 }
 
 hipError_t launch_hab(const PairDev *pairs, int pair0, int npairs, int tau, int with_forces,
                       int la_max, int lb_max, const double *Mbuf, int mstride, const double *pab,
-                      double *hab, double *forces, hipStream_t st) {
+                      double *hab, double *forces, const double *mtrans, const long long *mt_off,
+                      double *ws, long long ws_stride, int ws_blocks, hipStream_t st) {
+    if (npairs <= 0) return hipSuccess;
     const int fx = with_forces ? 1 : 0;
-    const size_t smem = hab_smem_bytes(la_max + tau + fx, lb_max + tau + fx, la_max + tau, lb_max + tau);
-    hipLaunchKernelGGL(k_hab, dim3(npairs), dim3(64), smem, st, pairs, pair0, tau, with_forces,
-                       Mbuf, mstride, pab, hab, forces);
+    const int lamx = la_max + tau + fx, lbmx = lb_max + tau + fx;
+    const bool big = hab_needs_workspace(lamx, lbmx, la_max + tau, lb_max + tau);
+    if (big && (!ws || ws_stride < (long long)hab_ws_doubles(lamx, lbmx, la_max + tau, lb_max + tau) ||
+                ws_blocks < 1))
+        return hipErrorInvalidValue;
+    const size_t smem = hab_smem_bytes(lamx, lbmx, la_max + tau, lb_max + tau, !big);
+    const int nblk = big ? std::min(npairs, ws_blocks) : npairs;
+    hipLaunchKernelGGL(k_hab, dim3(nblk), dim3(64), smem, st, pairs, pair0, npairs, tau, with_forces,
+                       Mbuf, mstride, pab, hab, forces, mtrans, mt_off, big ? ws : nullptr,
+                       big ? ws_stride : 0LL);
     return hipGetLastError();
 }
 

@@ -34,10 +34,14 @@ hipError_t launch_integrate_tiles(int cls, const TileGrid &tg, const EntryRec *r
                                   const int *tile_start, const double *grid, int lp_add,
                                   int ntiles, int splits, double *partial, hipStream_t st);
 hipError_t launch_coef(const PairDev *pairs, int pair0, int npairs, int tau, int lam_max,
-                       int lbm_max, const double *pab, double *coef, hipStream_t st);
+                       int lbm_max, const double *pab, double *coef, const double *mtrans,
+                       const long long *mt_off, hipStream_t st);
 hipError_t launch_hab(const PairDev *pairs, int pair0, int npairs, int tau, int with_forces,
                       int la_max, int lb_max, const double *Mbuf, int mstride, const double *pab,
-                      double *hab, double *forces, hipStream_t st);
+                      double *hab, double *forces, const double *mtrans, const long long *mt_off,
+                      double *ws, long long ws_stride, int ws_blocks, hipStream_t st);
+size_t hab_ws_doubles(int lamx, int lbmx, int lam, int lbm);
+bool hab_needs_workspace(int lamx, int lbmx, int lam, int lbm);
 hipError_t launch_sum_entries(const PairDev *pairs, int npairs, int tau, int lp_add,
                               const int *pe_start, const long long *ent_off,
                               const double *partial, int mstride, double *Mbuf, hipStream_t st);
@@ -151,9 +155,10 @@ static gpw_status profile_from_bounds(const int lb[3], const int ub[3], int nsph
 
 struct PairGeom {
     int cc[3];
-    double roff[3];
+    double roff[3];  // general cells: lattice centre gp
     int lb[3], ub[3];
-    int prof;  // index into profile list, -1 none
+    int prof;        // index into profile list, -1 none
+    double R2 = 0.0; // general cells: radius^2
 };
 
 // work-item classes by polynomial length L = lp + 1 (one kernel instantiation
@@ -171,6 +176,7 @@ struct gpw_tasklist {
     gpw_grid_desc grid{};
     int compute_tau = 0;
     int npairs = 0;
+    int force_general = 0;  // single tasks with use_ortho = 0 on any cell
     int lp_max = 0;
     int la_max = 0, lb_max = 0;  // largest given la_max / lb_max
     TileGrid tg{};
@@ -196,6 +202,11 @@ struct gpw_tasklist {
     double *d_cpartial = nullptr;  // collocate partial grids (splits > 1)
     double *d_Mbuf = nullptr;
     int mstride = 0;
+    double *d_mtrans = nullptr;      // general cells: lattice transform per lp
+    long long *d_mt_off = nullptr;
+    double *d_hab_ws = nullptr;      // k_hab workspace for blocks too large for LDS
+    long long hab_ws_stride = 0;
+    int hab_ws_blocks = 0;
 
     // per-phase device timing (events recorded on the launch stream)
     bool timing = false;
@@ -205,7 +216,8 @@ struct gpw_tasklist {
 
     ~gpw_tasklist() {
         void *ptrs[] = {d_pairs, d_subs, d_prof, d_tile_start, d_recs, d_pe_start,
-                        d_ent_off0, d_ent_off2, d_coef, d_partial, d_cpartial, d_Mbuf};
+                        d_ent_off0, d_ent_off2, d_coef, d_partial, d_cpartial, d_Mbuf,
+                        d_mtrans, d_mt_off, d_hab_ws};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
@@ -288,6 +300,55 @@ static void cube_centre(const gpw_grid_desc &g, const double rp[3], int cc[3], d
     }
 }
 
+// General cell (grid_base_ref_c.cc:636-644, 700-721): lattice centre gp and
+// the lattice box around the sphere of the pair's radius, from the 27 points
+// rp + {-1,0,1}^3 * radius.
+static void general_geometry(const gpw_grid_desc &g, const double rp[3], double radius,
+                             PairGeom &geo) {
+    for (int i = 0; i < 3; i++) {
+        geo.cc[i] = 0;
+        double v = 0.0;
+        for (int j = 0; j < 3; j++) v += g.dh_inv[j][i] * rp[j];
+        geo.roff[i] = v;
+        geo.lb[i] = 2147483647;
+        geo.ub[i] = -2147483647 - 1;
+    }
+    for (int i = -1; i <= 1; i++)
+        for (int j = -1; j <= 1; j++)
+            for (int k = -1; k <= 1; k++) {
+                const double x = rp[0] + i * radius;
+                const double y = rp[1] + j * radius;
+                const double z = rp[2] + k * radius;
+                for (int d = 0; d < 3; d++) {
+                    const double r = g.dh_inv[0][d] * x + g.dh_inv[1][d] * y + g.dh_inv[2][d] * z;
+                    geo.lb[d] = std::min(geo.lb[d], (int)std::floor(r));
+                    geo.ub[d] = std::max(geo.ub[d], (int)std::ceil(r));
+                }
+            }
+    geo.prof = -1;
+    geo.R2 = radius * radius;
+}
+
+// i-range of the lattice row (j, k) inside the sphere (grid_base_ref_c.cc:790-811)
+static bool general_row_host(const gpw_grid_desc &g, double gp0, double dj, double dk, double R2,
+                             int &lo, int &hi) {
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int i = 0; i < 3; i++) {
+        const double pointk = g.dh[2][i] * dk;
+        const double pointj = pointk + g.dh[1][i] * dj;
+        const double v = pointj - gp0 * g.dh[0][i];
+        a += g.dh[0][i] * g.dh[0][i];
+        b += 2.0 * v * g.dh[0][i];
+        c += v * v;
+    }
+    double d = b * b - 4 * a * (c - R2);
+    if (d < 0.0) return false;
+    d = std::sqrt(d);
+    lo = (int)std::ceil((-b - d) / (2.0 * a));
+    hi = (int)std::floor((-b + d) / (2.0 * a));
+    return true;
+}
+
 // Split the pair's cube into sub-boxes that fit the period, append SubDevs.
 static gpw_status make_subs(const gpw_tasklist &tl, const PairDev &pd, int pair_index,
                             const PairGeom &geo, int prof_off, int prof_stride,
@@ -329,6 +390,7 @@ static gpw_status make_subs(const gpw_tasklist &tl, const PairDev &pd, int pair_
                 s.lp = pd.lp;
                 s.pair = pair_index;
                 s.coef_off = pd.coef_off;
+                s.R2 = geo.R2;
                 s.prof_off = prof_off;
                 s.prof_stride = prof_stride;
                 out.push_back(s);
@@ -357,6 +419,20 @@ static void axis_tiles(const SubDev &s, int d, int n, int T, int ntile, std::vec
         if (hit[t]) out.push_back(t);
 }
 
+// number of points of a general-cell sub-box inside its sphere
+static long long sub_points_general(const gpw_grid_desc &g, const SubDev &s) {
+    long long n = 0;
+    for (int k = s.lb[2]; k <= s.ub[2]; k++)
+        for (int j = s.lb[1]; j <= s.ub[1]; j++) {
+            int lo, hi;
+            if (!general_row_host(g, s.roff[0], j - s.roff[1], k - s.roff[2], s.R2, lo, hi)) continue;
+            lo = std::max(lo, s.lb[0]);
+            hi = std::min(hi, s.ub[0]);
+            if (hi >= lo) n += hi - lo + 1;
+        }
+    return n;
+}
+
 // number of sphere points in a sub-box (for roofline accounting)
 static long long sub_points(const SubDev &s, const std::vector<int> &prof) {
     auto fo = [](int i) { return i <= 0 ? -i : i - 1; };
@@ -374,6 +450,50 @@ static long long sub_points(const SubDev &s, const std::vector<int> &prof) {
             if (hi >= lo) n += hi - lo + 1;
         }
     return n;
+}
+
+// Lattice transform of a general cell (grid_base_ref_c.cc:647-698): for
+// (x - xp)^lx (y - yp)^ly (z - zp)^lz expanded in lattice displacements
+// (i - ip)^il (j - jp)^jl (k - kp)^kl, T[ijk][xyz] with both indices packed
+// as (z-like, y-like, x-like) triples of total degree <= lp.
+static void multinomial_matrix(const double dh[3][3], int lp, double *T) {
+    const int n = ncoset(lp);
+    std::vector<double> hp((size_t)(lp + 1) * 9);  // hp[k][j][i] = dh[j][i]^k
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            hp[(0 * 3 + j) * 3 + i] = 1.0;
+            for (int k = 1; k <= lp; k++) hp[(k * 3 + j) * 3 + i] = hp[((k - 1) * 3 + j) * 3 + i] * dh[j][i];
+        }
+    auto H = [&](int k, int j, int i) { return hp[(k * 3 + j) * 3 + i]; };
+    std::vector<double> fac(lp + 1, 1.0);
+    for (int k = 1; k <= lp; k++) fac[k] = fac[k - 1] * k;
+    for (size_t q = 0; q < (size_t)n * n; q++) T[q] = 0.0;
+    for (int klx = 0; klx <= lp; klx++)
+        for (int jlx = 0; jlx <= lp - klx; jlx++)
+            for (int ilx = 0; ilx <= lp - klx - jlx; ilx++) {
+                const int lx = ilx + jlx + klx;
+                for (int kly = 0; kly <= lp - lx; kly++)
+                    for (int jly = 0; jly <= lp - lx - kly; jly++)
+                        for (int ily = 0; ily <= lp - lx - kly - jly; ily++) {
+                            const int ly = ily + jly + kly;
+                            for (int klz = 0; klz <= lp - lx - ly; klz++)
+                                for (int jlz = 0; jlz <= lp - lx - ly - klz; jlz++)
+                                    for (int ilz = 0; ilz <= lp - lx - ly - klz - jlz; ilz++) {
+                                        const int lz = ilz + jlz + klz;
+                                        const double w =
+                                            H(ilx, 0, 0) * H(jlx, 1, 0) * H(klx, 2, 0) *
+                                            H(ily, 0, 1) * H(jly, 1, 1) * H(kly, 2, 1) *
+                                            H(ilz, 0, 2) * H(jlz, 1, 2) * H(klz, 2, 2) *
+                                            fac[lx] * fac[ly] * fac[lz] /
+                                            (fac[ilx] * fac[ily] * fac[ilz] * fac[jlx] * fac[jly] *
+                                             fac[jlz] * fac[klx] * fac[kly] * fac[klz]);
+                                        const int ijk = cidx(lp, klx + kly + klz, jlx + jly + jlz,
+                                                             ilx + ily + ilz);
+                                        const int xyz = cidx(lp, lz, ly, lx);
+                                        T[(size_t)ijk * n + xyz] += w;
+                                    }
+                        }
+            }
 }
 
 // Host part of a task list: profiles, pairs, sub-boxes, tile work lists.
@@ -433,9 +553,11 @@ static gpw_status plan_tasklist(gpw_tasklist *tl, const std::vector<gpw_pair_des
     tg.ntx = (tg.nx + TX - 1) / TX;
     tg.nty = (tg.ny + TY - 1) / TY;
     tg.ntz = (tg.nz + TZ - 1) / TZ;
+    tg.general = (tl->force_general || !is_ortho(g)) ? 1 : 0;
     for (int d = 0; d < 3; d++) {
         tg.dr[d] = g.dh[d][d];
         tg.period[d] = g.periodic[d] ? g.npts[d] : BIG;
+        for (int e = 0; e < 3; e++) tg.dh[d][e] = g.dh[d][e];
     }
     tl->ntiles = tg.ntx * tg.nty * tg.ntz;
     tl->splits = std::max(1, splits);
@@ -443,7 +565,7 @@ static gpw_status plan_tasklist(gpw_tasklist *tl, const std::vector<gpw_pair_des
     std::vector<double> sub_R2(tl->subs_h.size(), 0.0);
     for (size_t si = 0; si < tl->subs_h.size(); si++) {
         const int pi = geos[tl->subs_h[si].pair].prof;
-        sub_R2[si] = pi >= 0 ? profiles[pi].R2 : 0.0;
+        sub_R2[si] = tl->tg.general ? tl->subs_h[si].R2 : pi >= 0 ? profiles[pi].R2 : 0.0;
     }
     // entries, pair-major
     std::vector<int> ent_tile;
@@ -462,7 +584,7 @@ static gpw_status plan_tasklist(gpw_tasklist *tl, const std::vector<gpw_pair_des
                     hp.ent_sub.push_back((int)si);
                     hp.pe_start[s.pair + 1]++;
                 }
-        tl->cube_points += sub_points(s, prof_flat);
+        tl->cube_points += tl->tg.general ? sub_points_general(g, s) : sub_points(s, prof_flat);
     }
     for (size_t p = 0; p < pairs.size(); p++) hp.pe_start[p + 1] += hp.pe_start[p];
     const size_t ne = ent_tile.size();
@@ -540,6 +662,30 @@ static gpw_status upload_tasklist(gpw_tasklist *tl, const HostPlan &hp) {
         const long long ng = (long long)tl->tg.nx * tl->tg.ny * tl->tg.nz;
         HIP_TRY(hipMalloc((void **)&tl->d_cpartial, sizeof(double) * ng * tl->splits));
     }
+    {
+        // k_hab workspace, sized for the force extension of the largest block
+        const int t = tl->compute_tau;
+        const int lamx = tl->la_max + t + 1, lbmx = tl->lb_max + t + 1;
+        if (hab_needs_workspace(lamx, lbmx, tl->la_max + t, tl->lb_max + t)) {
+            tl->hab_ws_stride = (long long)hab_ws_doubles(lamx, lbmx, tl->la_max + t, tl->lb_max + t);
+            tl->hab_ws_blocks = std::min(tl->npairs, HAB_WS_BLOCKS);
+            HIP_TRY(hipMalloc((void **)&tl->d_hab_ws,
+                              sizeof(double) * tl->hab_ws_stride * tl->hab_ws_blocks));
+        }
+    }
+    if (tl->tg.general) {
+        const int lpm = std::min(tl->lp_max + 2, LPMAX);
+        std::vector<long long> off(LPMAX + 1, 0);
+        long long total = 0;
+        for (int lp = 0; lp <= lpm; lp++) {
+            off[lp] = total;
+            total += (long long)ncoset(lp) * ncoset(lp);
+        }
+        std::vector<double> T((size_t)total);
+        for (int lp = 0; lp <= lpm; lp++) multinomial_matrix(tl->grid.dh, lp, T.data() + off[lp]);
+        if ((st = upload(&tl->d_mtrans, T))) return st;
+        if ((st = upload(&tl->d_mt_off, off))) return st;
+    }
     return GPW_OK;
 }
 
@@ -559,7 +705,6 @@ static gpw_status check_grid(const gpw_grid_desc *g) {
         if (g->periodic[d] && g->ngrid[d] != g->npts[d])
             return fail(GPW_ERR_ARG, "periodic axes need ngrid == npts");
     }
-    if (!is_ortho(*g)) return fail(GPW_ERR_UNSUPPORTED, "non-orthorhombic grids are not supported yet");
     return GPW_OK;
 }
 
@@ -584,6 +729,10 @@ static gpw_status radius_geometry(const gpw_grid_desc *grid, int npairs, const g
         if (!(pairs[i].radius > 0.0)) return fail(GPW_ERR_ARG, "radius must be > 0");
         const PairDev pd = make_pair_dev(pairs[i], tau);
         PairGeom &geo = geos[i];
+        if (!is_ortho(*grid)) {
+            general_geometry(*grid, pd.rp, pairs[i].radius, geo);
+            continue;
+        }
         cube_centre(*grid, pd.rp, geo.cc, geo.roff);
         long long imr = (long long)std::ceil(pairs[i].radius / drmin);
         if (imr < 1) imr = 1;
@@ -659,13 +808,17 @@ gpw_status gpw_pair_geometry(const gpw_grid_desc *grid, const gpw_pair_desc *pai
     std::vector<Profile> profiles;
     if ((st = radius_geometry(grid, 1, pair, compute_tau ? 1 : 0, geos, profiles))) return st;
     const PairGeom &g = geos[0];
-    const Profile &P = profiles[g.prof];
     for (int d = 0; d < 3; d++) {
         cube_centre_out[d] = g.cc[d];
         roffset_out[d] = g.roff[d];
         lb_cube_out[d] = g.lb[d];
         ub_cube_out[d] = g.ub[d];
     }
+    if (g.prof < 0) {  // general cell: lattice box, no sphere_bounds
+        *nspheres = 0;
+        return GPW_OK;
+    }
+    const Profile &P = profiles[g.prof];
     // re-encode the profile in the reference's sphere_bounds layout
     int n = 0;
     auto put = [&](int v) {
@@ -709,7 +862,7 @@ gpw_status gpw_tasklist_stats_get(const gpw_tasklist *tl, gpw_tasklist_stats *o)
 static gpw_status run_coef(gpw_tasklist *tl, const double *pab, int pair0, int n, hipStream_t st) {
     if (n <= 0) return GPW_OK;
     HIP_TRY(launch_coef(tl->d_pairs, pair0, n, tl->compute_tau, tl->la_max, tl->lb_max, pab,
-                        tl->d_coef, st));
+                        tl->d_coef, tl->d_mtrans, tl->d_mt_off, st));
     return GPW_OK;
 }
 
@@ -776,7 +929,8 @@ gpw_status gpw_tasklist_integrate(gpw_tasklist *tl, const double *grid, const do
     {
         PhaseTimer ph(tl, st, PH_HAB);
         HIP_TRY(launch_hab(tl->d_pairs, 0, tl->npairs, tl->compute_tau, wf, tl->la_max, tl->lb_max,
-                           tl->d_Mbuf, tl->mstride, pab, hab, forces, st));
+                           tl->d_Mbuf, tl->mstride, pab, hab, forces, tl->d_mtrans, tl->d_mt_off,
+                           tl->d_hab_ws, tl->hab_ws_stride, tl->hab_ws_blocks, st));
     }
     return GPW_OK;
 }
@@ -840,7 +994,9 @@ gpw_status gpw_tasklist_integrate_plain(gpw_tasklist *tl, const double *grid, co
                                          tl->mstride, tl->d_Mbuf, st));
         }
         HIP_TRY(launch_hab(tl->d_pairs, p, 1, tl->compute_tau, wf, tl->pairs_h[p].la_max,
-                           tl->pairs_h[p].lb_max, tl->d_Mbuf, tl->mstride, pab, hab, forces, st));
+                           tl->pairs_h[p].lb_max, tl->d_Mbuf, tl->mstride, pab, hab, forces,
+                           tl->d_mtrans, tl->d_mt_off, tl->d_hab_ws, tl->hab_ws_stride,
+                           tl->hab_ws_blocks, st));
     }
     return GPW_OK;
 }
@@ -934,8 +1090,8 @@ static gpw_status single_setup(int compute_tau, int use_ortho, int la_max, int l
         return fail(GPW_ERR_LMAX, "lmax below the local angular momentum");
     if (o1 < 0 || o2 < 0 || o1 + ncoset(la_max) > maxco || o2 + ncoset(lb_max) > maxco)
         return fail(GPW_ERR_ARG, "pab block outside maxco");
-    if (!use_ortho) return fail(GPW_ERR_UNSUPPORTED, "general (non-orthorhombic) path not built yet");
-    if (!lb_cube || !ub_cube) return fail(GPW_ERR_ARG, "ortho path needs lb_cube/ub_cube");
+    if (use_ortho && (!lb_cube || !ub_cube)) return fail(GPW_ERR_ARG, "ortho path needs lb_cube/ub_cube");
+    if (!use_ortho && !(radius > 0.0)) return fail(GPW_ERR_ARG, "general path needs radius > 0");
     gpw_grid_desc g{};
     for (int i = 0; i < 3; i++) {
         g.npts[i] = npts[i];
@@ -974,19 +1130,25 @@ static gpw_status single_setup(int compute_tau, int use_ortho, int la_max, int l
     tl->grid = g;
     tl->compute_tau = tau;
     tl->npairs = 1;
-    std::vector<Profile> profiles(1);
-    if ((st = profile_from_bounds(lb_cube, ub_cube, nspheres, sphere_bounds, profiles[0]))) {
-        delete tl;
-        return st;
-    }
+    tl->force_general = use_ortho ? 0 : 1;
+    std::vector<Profile> profiles;
     std::vector<PairGeom> geos(1);
     const PairDev pd = make_pair_dev(p, tau);
-    cube_centre(g, pd.rp, geos[0].cc, geos[0].roff);
-    for (int d = 0; d < 3; d++) {
-        geos[0].lb[d] = lb_cube[d];
-        geos[0].ub[d] = ub_cube[d];
+    if (use_ortho) {
+        profiles.resize(1);
+        if ((st = profile_from_bounds(lb_cube, ub_cube, nspheres, sphere_bounds, profiles[0]))) {
+            delete tl;
+            return st;
+        }
+        cube_centre(g, pd.rp, geos[0].cc, geos[0].roff);
+        for (int d = 0; d < 3; d++) {
+            geos[0].lb[d] = lb_cube[d];
+            geos[0].ub[d] = ub_cube[d];
+        }
+        geos[0].prof = 0;
+    } else {
+        general_geometry(g, pd.rp, radius, geos[0]);
     }
-    geos[0].prof = 0;
     std::vector<gpw_pair_desc> pv(1, p);
     if ((st = finish_tasklist(tl, pv, geos, profiles, 1))) {
         delete tl;

@@ -30,11 +30,6 @@ def test_dropin_fixtures(gpw, oracle, torch_cuda, name):
     say("dropin fixture", name)
     t = fixture(name)
     grid = np.zeros((t["ngrid"][2], t["ngrid"][1], t["ngrid"][0]))
-    if not t["use_ortho"]:
-        with pytest.raises(gpw.GpwError) as e:
-            gpw.collocate_pgf_product_rspace(t, t["pab"], grid)
-        assert e.value.code == 7  # general path not built in the product yet
-        return
     gpw.collocate_pgf_product_rspace(t, t["pab"], grid)
     ref = oracle.collocate(t, t["pab"])
     assert np.count_nonzero(grid) == np.count_nonzero(ref)
@@ -61,6 +56,35 @@ def test_dropin_random(gpw, oracle, torch_cuda, tau, window, la, lb):
         t["ra"] = np.array([4.5, 3.0, 4.7]); t["rab"] = np.array([0.2, -0.3, 0.1])
         t["lb_grid"] = [-2, 0, -5]
     ref = oracle.collocate(t, pab)
+    grid = np.zeros_like(ref)
+    gpw.collocate_pgf_product_rspace(t, pab, grid)
+    assert np.count_nonzero(grid) == np.count_nonzero(ref)
+    assert rel_err(grid, ref) < TOL
+    W = rng.uniform(-1, 1, ref.shape)
+    h_ref, f_ref = oracle.integrate(t, W, pab=pab, forces=True)
+    h, f = gpw.integrate_pgf_product_rspace(t, W, pab=pab, forces=True)
+    assert rel_err(h, h_ref) < TOL
+    assert rel_err(f, f_ref) < TOL
+
+
+@pytest.mark.parametrize("tau", [False, True])
+@pytest.mark.parametrize("shear,window", [(0.0, False), (0.35, False), (-0.4, True)])
+@pytest.mark.parametrize("la,lb", [((0, 1), (0, 0)), ((2, 3), (1, 3))])
+def test_dropin_random_general(gpw, oracle, torch_cuda, tau, shear, window, la, lb):
+    # use_ortho = 0: the general path, on a sheared cell or (shear 0) forced
+    # on an orthorhombic one
+    say("dropin general", tau, shear, window, la, lb)
+    rng = np.random.default_rng(300 + 10 * tau + int(100 * shear) + la[1] + 4 * lb[1])
+    kw = dict(ortho=False, shear=shear)
+    if window:
+        kw.update(npts=(30, 20, 32), cell=(9.0, 6.0, 9.5), periodic=(0, 1, 0),
+                  window=(28, 20, 28), zeta=2.0, zetb=2.0)
+    t, pab = random_task(rng, oracle, la=la, lb=lb, tau=tau, **kw)
+    if window:
+        t["ra"] = np.array([4.5, 3.0, 4.7]); t["rab"] = np.array([0.2, -0.3, 0.1])
+        t["lb_grid"] = [-5, 0, -5]  # the sheared box spans 25 x-planes
+    ref = oracle.collocate(t, pab)
+    assert np.count_nonzero(ref) > 100
     grid = np.zeros_like(ref)
     gpw.collocate_pgf_product_rspace(t, pab, grid)
     assert np.count_nonzero(grid) == np.count_nonzero(ref)
@@ -105,6 +129,36 @@ def oracle_batch(oracle, cfg, W=None, forces=False, tau=False):
         out["hab"] = hab
         out["forces"] = f
     return out
+
+
+@pytest.mark.parametrize("level", [0, 1, 2, 3])
+def test_config5_triclinic_levels(gpw, oracle, torch_cuda, level):
+    # multigrid workload on its triclinic cell (gamma = 80 deg): each level's
+    # task list (collocate, integrate, forces), batched and plain, vs the oracle
+    from libgpw_amd import workloads as Wk
+    cfg = Wk.level(Wk.config(5, npairs=1500), level)
+    say("config5 level", level, len(cfg["pairs"]["la_max"]), "pairs")
+    W = Wk.potential(cfg)
+    ref = oracle_batch(oracle, cfg, W=W, forces=True)
+    assert np.count_nonzero(ref["grid"]) > 0
+    for plain, splits in ((False, 1), (False, 4), (True, 1)):
+        got = run_tasklist(gpw, torch_cuda, cfg, plain=plain, splits=splits, forces=True, W=W)
+        assert np.count_nonzero(got["grid"]) == np.count_nonzero(ref["grid"]), (plain, splits)
+        assert rel_err(got["grid"], ref["grid"]) < TOL, (plain, splits)
+        assert rel_err(got["hab"], ref["hab"]) < TOL, (plain, splits)
+        assert rel_err(got["forces"], ref["forces"]) < TOL, (plain, splits)
+
+
+def test_config5_tau_general(gpw, oracle, torch_cuda):
+    from libgpw_amd import workloads as Wk
+    cfg = Wk.level(Wk.config(5, npairs=800), 1)
+    say("config5 tau level 1", len(cfg["pairs"]["la_max"]), "pairs")
+    W = Wk.potential(cfg)
+    ref = oracle_batch(oracle, cfg, W=W, forces=True, tau=True)
+    got = run_tasklist(gpw, torch_cuda, cfg, splits=2, forces=True, W=W, tau=True)
+    assert rel_err(got["grid"], ref["grid"]) < TOL
+    assert rel_err(got["hab"], ref["hab"]) < TOL
+    assert rel_err(got["forces"], ref["forces"]) < TOL
 
 
 @pytest.mark.parametrize("plain", [False, True])
@@ -216,7 +270,7 @@ def test_libgpw_core(gpw, oracle, torch_cuda, shape):
     assert np.abs(out - ref).max() < 1e-13 * max(1, np.abs(ref).max())
 
 
-@pytest.mark.parametrize("name", ["collocate_ortho_density", "collocate_ortho_tau"])
+@pytest.mark.parametrize("name", FIXTURES)
 def test_replay_driver_on_fixture(gpw, torch_cuda, name):
     import os
     import subprocess

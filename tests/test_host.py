@@ -89,6 +89,58 @@ def test_plan_stats_counts_sphere_points(gpw, oracle):
     assert st["tile_points"] == st["nentries"] * 32 * 16 * 16
 
 
+def test_plan_stats_general_cell_counts_oracle_points(gpw, oracle):
+    # triclinic cell: the planner's per-row quadratic bounds select the same
+    # points as the oracle's general path (count of points it writes)
+    from libgpw_amd import workloads as W
+    cfg = W.level(W.config(5, npairs=1200), 0)
+    p = cfg["pairs"]
+    # pairs whose Gaussian-product prefactor does not underflow (every point
+    # of the sphere then holds a non-zero value)
+    mu = p["zeta"] * p["zetb"] / (p["zeta"] + p["zetb"])
+    keep = np.flatnonzero(mu * (p["rab"] ** 2).sum(1) < 200.0)
+    idx = keep[::3]
+    sub = W.subset(cfg, idx)
+    g = gpw.grid_desc(sub["npts"], sub["dh"])
+    st = gpw.plan_stats(g, sub["pairs"])
+    og = oracle.make_grid(sub["npts"], sub["dh"])
+    total = 0
+    for i in range(len(idx)):
+        one = W.subset(sub, [i])
+        grid = oracle.batch_collocate(og, oracle.make_pairs(one["pairs"]), one["pab"])
+        total += np.count_nonzero(grid)
+    assert len(idx) >= 25 and total > 0
+    assert st["cube_points"] == total
+
+
+def test_pair_geometry_general_cell_box(gpw):
+    # lattice box of a general cell holds every lattice point of the sphere
+    from libgpw_amd import workloads as W
+    cfg = W.level(W.config(5, npairs=200), 1)
+    g = gpw.grid_desc(cfg["npts"], cfg["dh"])
+    p = cfg["pairs"]
+    dh = np.asarray(cfg["dh"])
+    for i in range(0, len(p["la_max"]), 5):
+        geo = gpw.pair_geometry(g, p, i)
+        assert geo["sphere_bounds"].size == 0
+        f = p["zetb"][i] / (p["zeta"][i] + p["zetb"][i])
+        rp = p["ra"][i] + f * p["rab"][i]
+        gp = rp @ np.linalg.inv(dh)
+        assert np.allclose(geo["roffset"], gp, rtol=0, atol=1e-12)
+        lb, ub = np.array(geo["lb_cube"]), np.array(geo["ub_cube"])
+        # brute force: lattice points within the radius, searched in a wide box
+        c = np.floor(gp).astype(int)
+        span = int(np.ceil(p["radius"][i] / np.min(np.abs(np.linalg.eigvalsh(dh @ dh.T))) ** 0.5)) + 3
+        ax = [np.arange(c[d] - span, c[d] + span + 1) for d in range(3)]
+        K, J, I = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+        pts = np.stack([I, J, K], -1).reshape(-1, 3)
+        r2 = (((pts - gp) @ dh) ** 2).sum(-1)
+        inside = pts[r2 <= p["radius"][i] ** 2]
+        assert len(inside) > 0
+        assert (inside.min(0) >= lb).all() and (inside.max(0) <= ub).all()
+        assert (ub - lb).max() <= 2 * span
+
+
 def test_plan_splits_wide_cubes(gpw):
     # a cube wider than the period is cut into sub-boxes that each fit it
     from libgpw_amd import workloads as W
@@ -122,6 +174,7 @@ def test_dropin_argument_errors(gpw, oracle):
     bad_sb[0] = t["lb_cube"][2] - 1
     cases.append((dict(t, sphere_bounds=bad_sb), 4))
     cases.append((dict(t, sphere_bounds=np.array(t["sphere_bounds"][:3])), 4))
+    cases.append((dict(t, use_ortho=False, radius=0.0), 1))   # general path needs a radius
     for tt, code in cases:
         with pytest.raises(gpw.GpwError) as e:
             _single(tt, pab, gpw)
