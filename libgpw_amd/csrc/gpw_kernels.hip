@@ -743,6 +743,15 @@ __device__ __forceinline__ double stride4_sum(double v) {
     return v + dpp_f64<0x128>(v);
 }
 
+// Per-wave buffer of the integrate transposition: the x-stage results of the
+// wave's 64 rows (4 z-planes x 16 y), row (zl, y) at zl * ZS + y * LM; the
+// plane stride ZS is 64 B modulo 256 B so the 4 planes read disjoint banks.
+template <int LM>
+struct RowT1 {
+    static constexpr int ZS = TY * LM + ((LM % 2) ? 24 : 8);
+    static constexpr int SZ = 4 * ZS;
+};
+
 template <int LM, bool GEN>
 struct IntegSmem {
     using C = TileCfg<LM>;
@@ -752,6 +761,7 @@ struct IntegSmem {
     double Y[C::Q][TY][LM];
     double Z[C::Q][TZ][LM];
     double uz[C::Q][TZ][C::C2M];  // per-plane (lyp, lxp) moments
+    double t1[NROW / 64][RowT1<LM>::SZ];
     int xi[C::Q][TX];
     int T[C::Q][NROW];
     int bmax[C::Q][NXG], bmin[C::Q][NXG];
@@ -761,23 +771,35 @@ struct IntegSmem {
 // sum of a double over the 16 lanes of a DPP row; every lane gets the sum
 __device__ __forceinline__ double row_sum16(double v) { return stride4_sum(quad_sum(v)); }
 
-// x stage on the register block, local y contraction, DPP sum over the z plane
+// Register block (rows y = 4*yb + r of plane z, points x = 8*xg + j) of pair
+// q: x stage in registers, sum of the four x blocks of each row over the lane
+// quad (DPP), then a wave-local transposition through LDS for the y stage:
+// lane (zl, c) of the wave sums Y[y][lyp] * tx[zl][y][lxp] over the 16 rows
+// of its plane for the (lyp, lxp) terms c, c + 16, ...  A wave holds four
+// whole z-planes, so the transposition needs no workgroup barrier (LDS
+// operations of one wave complete in order).
 template <int L, int LM, bool GEN>
 __device__ __forceinline__ void integ_block(IntegSmem<LM, GEN> &sm, int q, int xg, int yb, int z,
                                             const int (&Tr)[RB], int mode, const GenBlock &gb,
                                             const double (&v)[RB][XB], int t) {
     constexpr int lp = L - 1;
     constexpr int N2 = L * (L + 1) / 2;
-    double u[N2];
+    constexpr int ZS = RowT1<LM>::ZS;
+    const int lane = t & 63, wave = t >> 6;
+    const int zl = lane >> 4, cq = lane & 15;
+    double *uzp = sm.uz[q][wave * 4 + zl];
+    if (!__any(mode != 0)) {  // no point of the wave's four planes is inside
+        for (int c = cq; c < N2; c += 16) uzp[c] = 0.0;
+        return;
+    }
+    double tx[RB][L];
 #pragma unroll
-    for (int c = 0; c < N2; c++) u[c] = 0.0;
-    if (mode != 0) {  // 0: nothing inside, 1: masked, 2: whole block inside
-        double tx[RB][L];
+    for (int r = 0; r < RB; r++)
 #pragma unroll
-        for (int r = 0; r < RB; r++)
-#pragma unroll
-            for (int l = 0; l < L; l++) tx[r][l] = 0.0;
+        for (int l = 0; l < L; l++) tx[r][l] = 0.0;
+    if (mode != 0) {  // 1: masked, 2: whole block inside
         const double *Xg = sm.X[q] + xg * TileCfg<LM>::GS;
+        const bool wfull = __all(mode == 2);  // wave-uniform among the active lanes
 #pragma unroll
         for (int j = 0; j < XB; j++) {
             double xv[L];
@@ -786,34 +808,44 @@ __device__ __forceinline__ void integ_block(IntegSmem<LM, GEN> &sm, int q, int x
             const int code = sm.xi[q][xg * XB + j];
 #pragma unroll
             for (int r = 0; r < RB; r++) {
-                double w;
+                double w = v[r][j];
                 if constexpr (GEN) {
                     const double ci = (double)code;
-                    const double e = exp((gb.qa * ci + gb.qb[r]) * ci + gb.qc[r]);
-                    w = (mode == 2 || (code >= gb.lo[r] && code <= gb.hi[r])) ? v[r][j] * e : 0.0;
+                    w *= exp((gb.qa * ci + gb.qb[r]) * ci + gb.qc[r]);
+                    if (!wfull) w = (code >= gb.lo[r] && code <= gb.hi[r]) ? w : 0.0;
                 } else {
-                    w = (mode == 2 || code <= Tr[r]) ? v[r][j] : 0.0;
+                    if (!wfull) w = code <= Tr[r] ? w : 0.0;
                 }
 #pragma unroll
                 for (int l = 0; l < L; l++) tx[r][l] += w * xv[l];
             }
         }
+    }
+    // rows of the quad summed over its four x blocks; lane xg stores row r = xg
+    double *t1 = sm.t1[wave] + zl * ZS;
 #pragma unroll
-        for (int r = 0; r < RB; r++) {
-            const double *Yr = sm.Y[q][yb * RB + r];
+    for (int r = 0; r < RB; r++) {
 #pragma unroll
-            for (int lyp = 0; lyp <= lp; lyp++) {
-                const double y = Yr[lyp];
+        for (int l = 0; l < L; l++) tx[r][l] = quad_sum(tx[r][l]);
+        if (xg == r) {
 #pragma unroll
-                for (int lxp = 0; lxp <= lp - lyp; lxp++) u[c2idx(lp, lyp, lxp)] += tx[r][lxp] * y;
-            }
+            for (int l = 0; l < L; l++) t1[(yb * RB + r) * LM + l] = tx[r][l];
         }
     }
+    __builtin_amdgcn_wave_barrier();
+    const double *Yq = &sm.Y[q][0][0];
+    for (int c = cq; c < N2; c += 16) {
+        int lyp = 0, lxp = c;
+        while (lxp > lp - lyp) {
+            lxp -= lp - lyp + 1;
+            lyp++;
+        }
+        double a = 0.0;
 #pragma unroll
-    for (int c = 0; c < N2; c++) {
-        const double s = row_sum16(u[c]);
-        if ((t & 15) == 0) sm.uz[q][z][c] = s;
+        for (int y = 0; y < TY; y++) a += Yq[y * LM + lyp] * t1[y * LM + lxp];
+        uzp[c] = a;
     }
+    __builtin_amdgcn_wave_barrier();
 }
 
 // partial moments of every (sub-box, tile) item: partial[rec.ent_off + c3]
@@ -889,19 +921,38 @@ __global__ __launch_bounds__(NROW, 2) void k_integrate_tiles(
 #undef GPW_INTEG_CALL
         }
         __syncthreads();
-        // z-stage: M[(lzp,lyp,lxp)] = sum_z uz[q][z][(lyp,lxp)] Z[z][lzp]
-#pragma unroll 1
-        for (int q = 0; q < nq; q++) {
-            const int lp = sm.lp[q], n3 = ncoset(lp);
-            const long long off = sm.rec[rs][q].ent_off;
-            for (int w = t; w < n3; w += NROW) {
+        // z-stage of all pairs of the chunk at once, thread -> (q, c3):
+        // M[(lzp,lyp,lxp)] = sum_z uz[q][z][(lyp,lxp)] Z[z][lzp]
+        {
+            int n3q[Q];
+            int tot = 0;
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                n3q[q] = q < nq ? ncoset(sm.lp[q]) : 0;
+                tot += n3q[q];
+            }
+            for (int w = t; w < tot; w += NROW) {
+                int q = 0, c3 = w;
+#pragma unroll
+                for (int qq = 0; qq < Q - 1; qq++)
+                    if (q == qq && c3 >= n3q[qq]) {
+                        c3 -= n3q[qq];
+                        q++;
+                    }
+                const int lp = sm.lp[q];
                 int lzp, lyp, lxp;
-                decode_c3(lp, w, lzp, lyp, lxp);
-                const int c2 = c2idx(lp, lyp, lxp);
-                double a = 0.0;
-#pragma unroll 4
-                for (int zz = 0; zz < TZ; zz++) a += sm.uz[q][zz][c2] * sm.Z[q][zz][lzp];
-                partial[off + w] = a;
+                decode_c3(lp, c3, lzp, lyp, lxp);
+                const double *uzq = &sm.uz[q][0][c2idx(lp, lyp, lxp)];
+                const double *zq = &sm.Z[q][0][lzp];
+                double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+#pragma unroll
+                for (int zz = 0; zz < TZ; zz += 4) {
+                    a0 += uzq[(zz + 0) * C::C2M] * zq[(zz + 0) * LM];
+                    a1 += uzq[(zz + 1) * C::C2M] * zq[(zz + 1) * LM];
+                    a2 += uzq[(zz + 2) * C::C2M] * zq[(zz + 2) * LM];
+                    a3 += uzq[(zz + 3) * C::C2M] * zq[(zz + 3) * LM];
+                }
+                partial[sm.rec[rs][q].ent_off + c3] = (a0 + a1) + (a2 + a3);
             }
         }
         if (t < Q * RECW) reinterpret_cast<int *>(&sm.rec[rs ^ 1][0])[t] = pf_rec;
