@@ -39,6 +39,8 @@ def main():
     ap.add_argument("--splits", type=int, default=int(os.environ.get("GPW_SPLITS", "4")))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain GPU leg")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep --splits for every kernel class instead of measured choices")
     ap.add_argument("--cpu-pairs", type=int, default=3000)
     ap.add_argument("--plain-pairs", type=int, default=2000)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -72,6 +74,12 @@ def main():
     t0 = time.perf_counter()
     tl = G.TaskList(grid_desc, cfg["pairs"], splits=args.splits)
     plan_s = time.perf_counter() - t0
+    tune = None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        before, after = tl.autotune(max_splits=8, reps=3)
+        tune = {"seconds": round(time.perf_counter() - t0, 3), "kernel_ms_before": round(before, 4),
+                "kernel_ms_after": round(after, 4)}
     st = tl.stats()
 
     n = cfg["npts"]
@@ -132,7 +140,9 @@ def main():
            "data": "synthetic",
            "config": {"workload": cfg["name"], "pairs": npairs, "grid": list(n), "cell_bohr": 15.0,
                       "l_range": [0, 3], "exponents": [0.1, 50.0], "eps": Wk.EPS,
-                      "splits": args.splits, "tile": st["tile"], "sub_boxes": st["nsub"],
+                      "splits": args.splits, "autotune": tune,
+                      "collocate_splits": st["csplits"], "integrate_splits": st["isplits"][0],
+                      "tile": st["tile"], "sub_boxes": st["nsub"],
                       "work_items": st["nentries"], "sphere_points": pts, "plan_s": round(plan_s, 3),
                       "phase_ms": {k: round(v, 4) for k, v in phase_ms.items()}},
            "roofline": roof}

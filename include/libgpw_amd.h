@@ -75,7 +75,9 @@ typedef struct gpw_tasklist_stats {
     long long tile_points;    /* points swept by the tile kernels                       */
     int lp_max;
     int tile[3];              /* tile shape (x, y, z)                                   */
-    int splits;               /* work-list splits per tile                              */
+    int splits;               /* work-list splits per tile given at creation            */
+    int csplits[4];           /* workgroups per tile of the collocate kernel, per class */
+    int isplits[2][4];        /* same for integrate, without / with forces              */
 } gpw_tasklist_stats;
 
 /* ---- single task, host memory (drop-in for the reference's collocate) ---- */
@@ -111,7 +113,9 @@ gpw_status gpw_integrate_pgf_product_rspace(
 /* Builds the per-pair geometry (cube centre, cube and sphere bounds from each
  * pair's radius, splitting of cubes larger than the period) and the tile
  * work lists on the host, and uploads them. compute_tau applies to all pairs.
- * tile: NULL for the default shape, else {tx, ty, tz} (tx*... see DESIGN). */
+ * tile: NULL or {32, 16, 16} (the only tile shape: it is fixed by the
+ * kernels' 4 x 8 register blocks); splits: workgroups per tile for every
+ * kernel class (gpw_tasklist_autotune replaces them by measured choices). */
 gpw_status gpw_tasklist_create(const gpw_grid_desc *grid, int npairs,
                                const gpw_pair_desc *pairs, int compute_tau,
                                const int *tile, int splits, gpw_tasklist **out);
@@ -131,6 +135,18 @@ gpw_status gpw_pair_geometry(const gpw_grid_desc *grid, const gpw_pair_desc *pai
                              int compute_tau, int cube_centre[3], double roffset[3],
                              int lb_cube[3], int ub_cube[3], int *sphere_bounds, int cap,
                              int *nspheres);
+
+/* Autotuner (no counterpart in the reference): times the tile kernels of
+ * every class with 1, 2, 3, 4, 6, 8, 12, 16 (<= max_splits) workgroups per
+ * tile on scratch inputs, best of `reps` runs after a warm-up, and keeps the
+ * fastest choice per (collocate / integrate without / with forces, class);
+ * collocate also counts the reduction of its partial grids. Allocates a
+ * scratch grid and the partial grids, so call it at set-up, not inside
+ * captured or timed regions. ms_before / ms_after (may be NULL): summed
+ * kernel times of collocate + integrate with the previous and the chosen
+ * splits. */
+gpw_status gpw_tasklist_autotune(gpw_tasklist *tl, int max_splits, int reps, void *stream,
+                                 double *ms_before, double *ms_after);
 
 /* grid_dev += collocation of every pair (pab_dev: packed blocks). */
 gpw_status gpw_tasklist_collocate(gpw_tasklist *tl, const double *pab_dev, double *grid_dev,

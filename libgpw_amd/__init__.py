@@ -29,7 +29,7 @@ EXPORTS = (
     "gpw_tasklist_collocate_plain", "gpw_tasklist_integrate_plain",
     "gpw_tasklist_set_timing", "gpw_tasklist_timings",
     "gpw_collocate_core", "gpw_integrate_core", "gpw_collocate_core_host",
-    "gpw_integrate_core_host", "gpw_last_error",
+    "gpw_integrate_core_host", "gpw_tasklist_autotune", "gpw_last_error",
 )
 PHASES = ("coef", "collocate", "reduce", "integrate", "moments", "hab")
 
@@ -57,7 +57,7 @@ class TaskListStats(C.Structure):
     _fields_ = [("npairs", C.c_longlong), ("nsub", C.c_longlong), ("ntiles", C.c_longlong),
                 ("nentries", C.c_longlong), ("cube_points", C.c_longlong),
                 ("tile_points", C.c_longlong), ("lp_max", C.c_int), ("tile", C.c_int * 3),
-                ("splits", C.c_int)]
+                ("splits", C.c_int), ("csplits", C.c_int * 4), ("isplits", (C.c_int * 4) * 2)]
 
 
 _lib = None
@@ -100,6 +100,7 @@ def lib():
         L.gpw_integrate_core.argtypes = [ip, C.c_int, vp, C.c_int, vp, vp, vp]
         L.gpw_collocate_core_host.argtypes = [ip, C.c_int, dp, dp, C.c_int, dp]
         L.gpw_integrate_core_host.argtypes = [ip, C.c_int, dp, C.c_int, dp, dp]
+        L.gpw_tasklist_autotune.argtypes = [vp, C.c_int, C.c_int, vp, dp, dp]
         L.gpw_last_error.argtypes = []
         L.gpw_last_error.restype = C.c_char_p
         for n in EXPORTS:
@@ -151,7 +152,11 @@ def pair_array(p: dict):
 
 
 def _stats_dict(s: TaskListStats) -> dict:
-    return {k: (list(getattr(s, k)) if k == "tile" else getattr(s, k)) for k, _ in s._fields_}
+    out = {k: getattr(s, k) for k, _ in s._fields_}
+    out["tile"] = list(s.tile)
+    out["csplits"] = list(s.csplits)
+    out["isplits"] = [list(r) for r in s.isplits]
+    return out
 
 
 def plan_stats(grid: GridDesc, pairs: dict, compute_tau: bool = False, splits: int = 1) -> dict:
@@ -234,6 +239,14 @@ class TaskList:
         calls = (C.c_int * len(PHASES))()
         _check(lib().gpw_tasklist_timings(self._h, ms, len(PHASES), calls))
         return {p: (ms[i], calls[i]) for i, p in enumerate(PHASES)}
+
+    def autotune(self, max_splits: int = 8, reps: int = 3, stream=None) -> tuple:
+        """Measure and keep the best workgroups-per-tile per kernel class;
+        returns (ms before, ms after) of collocate + integrate kernels."""
+        before, after = C.c_double(0.0), C.c_double(0.0)
+        _check(lib().gpw_tasklist_autotune(self._h, int(max_splits), int(reps), _stream(stream),
+                                           C.byref(before), C.byref(after)))
+        return before.value, after.value
 
     def collocate(self, pab, grid, stream=None, plain=False):
         f = lib().gpw_tasklist_collocate_plain if plain else lib().gpw_tasklist_collocate

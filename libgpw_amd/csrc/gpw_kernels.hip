@@ -553,12 +553,16 @@ __device__ __forceinline__ void colloc_block(const CollocSmem<LM, GEN> &sm, int 
             double v = 0.0;
 #pragma unroll
             for (int l = 0; l < L; l++) v += cx[r][l] * xv[l];
+            // the sphere mask as a 0/1 factor of one FMA (cheaper than a
+            // 64-bit select and an add)
             if constexpr (GEN) {
                 const double ci = (double)code;
                 const double e = exp((gb.qa * ci + gb.qb[r]) * ci + gb.qc[r]);
-                acc[r][j] += (full || (code >= gb.lo[r] && code <= gb.hi[r])) ? v * e : 0.0;
+                const double m = (full || (code >= gb.lo[r] && code <= gb.hi[r])) ? e : 0.0;
+                acc[r][j] = __builtin_fma(v, m, acc[r][j]);
             } else {
-                acc[r][j] += (full || code <= Tr[r]) ? v : 0.0;
+                const double m = (full || code <= Tr[r]) ? 1.0 : 0.0;
+                acc[r][j] = __builtin_fma(v, m, acc[r][j]);
             }
         }
     }

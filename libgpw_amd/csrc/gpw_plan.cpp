@@ -181,6 +181,11 @@ struct gpw_tasklist {
     int la_max = 0, lb_max = 0;  // largest given la_max / lb_max
     TileGrid tg{};
     int ntiles = 0, splits = 1;
+    // workgroups per tile for each kernel class: collocate, integrate
+    // (without / with the force extension); set from `splits`, then by autotune
+    int csplit[NCLS] = {1, 1, 1, 1};
+    int isplit[2][NCLS] = {{1, 1, 1, 1}, {1, 1, 1, 1}};
+    int cpart_cap = 0;             // partial grids allocated in d_cpartial
     long long nentries = 0;
     long long cube_points = 0, tile_points = 0;
     long long class_entries[2][NCLS] = {};  // [lp_add/2][class]
@@ -204,6 +209,7 @@ struct gpw_tasklist {
     int mstride = 0;
     double *d_mtrans = nullptr;      // general cells: lattice transform per lp
     long long *d_mt_off = nullptr;
+    long long coef_total = 0;        // doubles of d_coef
     double *d_hab_ws = nullptr;      // k_hab workspace for blocks too large for LDS
     long long hab_ws_stride = 0;
     int hab_ws_blocks = 0;
@@ -561,6 +567,10 @@ static gpw_status plan_tasklist(gpw_tasklist *tl, const std::vector<gpw_pair_des
     }
     tl->ntiles = tg.ntx * tg.nty * tg.ntz;
     tl->splits = std::max(1, splits);
+    for (int c = 0; c < NCLS; c++) {
+        tl->csplit[c] = tl->splits;
+        tl->isplit[0][c] = tl->isplit[1][c] = tl->splits;
+    }
     // squared sphere radius of every sub-box (from its pair's profile)
     std::vector<double> sub_R2(tl->subs_h.size(), 0.0);
     for (size_t si = 0; si < tl->subs_h.size(); si++) {
@@ -655,12 +665,14 @@ static gpw_status upload_tasklist(gpw_tasklist *tl, const HostPlan &hp) {
     if ((st = upload(&tl->d_ent_off0, hp.off0))) return st;
     if ((st = upload(&tl->d_ent_off2, hp.off2))) return st;
     HIP_TRY(hipMalloc((void **)&tl->d_coef, sizeof(double) * std::max<long long>(hp.coef_total, 1)));
+    tl->coef_total = hp.coef_total;
     HIP_TRY(hipMalloc((void **)&tl->d_partial, sizeof(double) * std::max<long long>(hp.partial_total, 1)));
     HIP_TRY(hipMalloc((void **)&tl->d_Mbuf,
                       sizeof(double) * std::max<long long>((long long)tl->npairs * tl->mstride, 1)));
     if (tl->splits > 1) {
         const long long ng = (long long)tl->tg.nx * tl->tg.ny * tl->tg.nz;
         HIP_TRY(hipMalloc((void **)&tl->d_cpartial, sizeof(double) * ng * tl->splits));
+        tl->cpart_cap = tl->splits;
     }
     {
         // k_hab workspace, sized for the force extension of the largest block
@@ -856,6 +868,212 @@ gpw_status gpw_tasklist_stats_get(const gpw_tasklist *tl, gpw_tasklist_stats *o)
     o->tile[1] = TY;
     o->tile[2] = TZ;
     o->splits = tl->splits;
+    for (int c = 0; c < NCLS; c++) {
+        o->csplits[c] = tl->csplit[c];
+        o->isplits[0][c] = tl->isplit[0][c];
+        o->isplits[1][c] = tl->isplit[1][c];
+    }
+    return GPW_OK;
+}
+
+// largest collocate split over the classes that have work
+static int collocate_splits_max(const gpw_tasklist *tl) {
+    int smax = 1;
+    for (int c = 0; c < NCLS; c++)
+        if (tl->class_entries[0][c]) smax = std::max(smax, tl->csplit[c]);
+    return smax;
+}
+
+// Collocate tile launches of every class. With smax == 1 the kernels add into
+// the grid; otherwise into smax partial grids (reduced afterwards): the class
+// with the most splits runs first and writes all smax partials (workgroups
+// without items write zeros), the others add into theirs.
+static hipError_t collocate_classes(gpw_tasklist *tl, double *grid, int smax, hipStream_t st) {
+    const int nt1 = tl->ntiles + 1;
+    int order[NCLS] = {0, 1, 2, 3};
+    std::stable_sort(order, order + NCLS, [&](int a, int b) { return tl->csplit[a] > tl->csplit[b]; });
+    bool first = true;
+    for (int k = 0; k < NCLS; k++) {
+        const int c = order[k];
+        if (tl->class_entries[0][c] == 0) continue;
+        const int *start = tl->d_tile_start + (size_t)c * nt1;
+        const EntryRec *recs = tl->d_recs + tl->list_off[0][c];
+        hipError_t e;
+        if (smax == 1) {
+            e = launch_collocate_tiles(c, tl->tg, recs, start, tl->d_coef, tl->ntiles, 1, grid, 0, st);
+        } else {
+            e = launch_collocate_tiles(c, tl->tg, recs, start, tl->d_coef, tl->ntiles, tl->csplit[c],
+                                       tl->d_cpartial, first ? 1 : 2, st);
+            first = false;
+        }
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// ---------------------------------------------------------------------------
+// autotuner: workgroups per tile for every (kernel, class), timed on the
+// device on scratch inputs (run times depend on the index sets, not on values)
+
+gpw_status gpw_tasklist_autotune(gpw_tasklist *tl, int max_splits, int reps, void *stream,
+                                 double *ms_before, double *ms_after) {
+    if (!tl) return fail(GPW_ERR_ARG, "NULL argument");
+    if (tl->npairs == 0) return GPW_OK;
+    max_splits = std::min(std::max(max_splits, 1), 16);
+    reps = std::min(std::max(reps, 1), 20);
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<int> cand;
+    for (int s : {1, 2, 3, 4, 6, 8, 12, 16})
+        if (s <= max_splits) cand.push_back(s);
+    const int cmax = cand.back();
+    const long long ng = (long long)tl->tg.nx * tl->tg.ny * tl->tg.nz;
+    const int nt1 = tl->ntiles + 1;
+    double *grid = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&]() {
+        if (grid) (void)hipFree(grid);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    };
+#define AT_TRY(x)                                                          \
+    do {                                                                   \
+        hipError_t err_ = (x);                                             \
+        if (err_ != hipSuccess) {                                          \
+            cleanup();                                                     \
+            return fail(GPW_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(err_)); \
+        }                                                                  \
+    } while (0)
+    AT_TRY(hipMalloc((void **)&grid, sizeof(double) * ng));
+    if (tl->cpart_cap < cmax) {
+        if (tl->d_cpartial) AT_TRY(hipFree(tl->d_cpartial));
+        tl->d_cpartial = nullptr;
+        tl->cpart_cap = 0;
+        AT_TRY(hipMalloc((void **)&tl->d_cpartial, sizeof(double) * ng * cmax));
+        tl->cpart_cap = cmax;
+    }
+    AT_TRY(hipEventCreate(&e0));
+    AT_TRY(hipEventCreate(&e1));
+    AT_TRY(launch_fill(grid, ng, 0.0, st));
+    AT_TRY(launch_fill(tl->d_coef, std::max<long long>(tl->coef_total, 1), 0.0, st));
+    // best-of-reps time of one launch sequence, after one warm-up
+    auto timed = [&](auto &&launch, float &ms) -> hipError_t {
+        hipError_t e = launch();
+        if (e != hipSuccess) return e;
+        ms = 1e30f;
+        for (int r = 0; r < reps; r++) {
+            if ((e = hipEventRecord(e0, st)) != hipSuccess) return e;
+            if ((e = launch()) != hipSuccess) return e;
+            if ((e = hipEventRecord(e1, st)) != hipSuccess) return e;
+            if ((e = hipEventSynchronize(e1)) != hipSuccess) return e;
+            float t = 0.0f;
+            if ((e = hipEventElapsedTime(&t, e0, e1)) != hipSuccess) return e;
+            ms = std::min(ms, t);
+        }
+        return hipSuccess;
+    };
+    const int nc = (int)cand.size();
+    // collocate: per class and split, plus the partial-grid reduction per split
+    std::vector<float> tc((size_t)NCLS * nc, 0.0f), tr(nc, 0.0f);
+    for (int c = 0; c < NCLS; c++) {
+        if (tl->class_entries[0][c] == 0) continue;
+        const int *start = tl->d_tile_start + (size_t)c * nt1;
+        const EntryRec *recs = tl->d_recs + tl->list_off[0][c];
+        for (int i = 0; i < nc; i++) {
+            const int sp = cand[i];
+            AT_TRY(timed([&] {
+                return sp == 1 ? launch_collocate_tiles(c, tl->tg, recs, start, tl->d_coef, tl->ntiles, 1,
+                                                        grid, 0, st)
+                               : launch_collocate_tiles(c, tl->tg, recs, start, tl->d_coef, tl->ntiles,
+                                                        sp, tl->d_cpartial, 1, st);
+            }, tc[(size_t)c * nc + i]));
+        }
+    }
+    for (int i = 1; i < nc; i++)
+        AT_TRY(timed([&] { return launch_reduce_partials(tl->d_cpartial, cand[i], ng, grid, st); }, tr[i]));
+    // choose the largest split smax, then each class's best split <= smax
+    double best = 1e300;
+    int bestc[NCLS] = {1, 1, 1, 1};
+    auto cost_of = [&](int imax, int *pick) {
+        double tot = imax > 0 ? tr[imax] : 0.0;
+        for (int c = 0; c < NCLS; c++) {
+            pick[c] = cand[imax];
+            if (tl->class_entries[0][c] == 0) continue;
+            int bi = 0;
+            for (int i = 1; i <= imax; i++)
+                if (tc[(size_t)c * nc + i] < tc[(size_t)c * nc + bi]) bi = i;
+            pick[c] = cand[bi];
+            tot += tc[(size_t)c * nc + bi];
+        }
+        return tot;
+    };
+    for (int imax = 0; imax < nc; imax++) {
+        int pick[NCLS];
+        const double tot = cost_of(imax, pick);
+        if (tot < best) {
+            best = tot;
+            std::copy(pick, pick + NCLS, bestc);
+        }
+    }
+    // cost of the splits in use before tuning (collocate + integrate)
+    double before = 0.0, after = best;
+    {
+        const int smax0 = collocate_splits_max(tl);
+        int i0 = 0;
+        while (i0 + 1 < nc && cand[i0] < smax0) i0++;
+        before += (smax0 > 1 && cand[i0] == smax0) ? tr[i0] : 0.0;
+        for (int c = 0; c < NCLS; c++) {
+            if (tl->class_entries[0][c] == 0) continue;
+            for (int i = 0; i < nc; i++)
+                if (cand[i] == tl->csplit[c]) before += tc[(size_t)c * nc + i];
+        }
+    }
+    std::copy(bestc, bestc + NCLS, tl->csplit);
+    // integrate, without and with the force extension
+    for (int wf = 0; wf < 2; wf++) {
+        for (int c = 0; c < NCLS; c++) {
+            if (tl->class_entries[wf][c] == 0) continue;
+            const int *start = tl->d_tile_start + (size_t)(wf * NCLS + c) * nt1;
+            const EntryRec *recs = tl->d_recs + tl->list_off[wf][c];
+            float bt = 1e30f, t0 = 0.0f;
+            int bs = 1;
+            for (int i = 0; i < nc; i++) {
+                float t;
+                AT_TRY(timed([&] {
+                    return launch_integrate_tiles(c, tl->tg, recs, start, grid, 2 * wf, tl->ntiles, cand[i],
+                                                  tl->d_partial, st);
+                }, t));
+                if (cand[i] == tl->isplit[wf][c]) t0 = t;
+                if (t < bt) {
+                    bt = t;
+                    bs = cand[i];
+                }
+            }
+            if (wf == 0) {
+                before += t0;
+                after += bt;
+            }
+            tl->isplit[wf][c] = bs;
+        }
+    }
+#undef AT_TRY
+    // keep only the partial grids the chosen collocate splits use
+    const int smax = collocate_splits_max(tl);
+    if (tl->cpart_cap != (smax > 1 ? smax : 0)) {
+        if (tl->d_cpartial) (void)hipFree(tl->d_cpartial);
+        tl->d_cpartial = nullptr;
+        tl->cpart_cap = 0;
+        if (smax > 1) {
+            hipError_t e = hipMalloc((void **)&tl->d_cpartial, sizeof(double) * ng * smax);
+            if (e != hipSuccess) {
+                cleanup();
+                return fail(GPW_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+            }
+            tl->cpart_cap = smax;
+        }
+    }
+    cleanup();
+    if (ms_before) *ms_before = before;
+    if (ms_after) *ms_after = after;
     return GPW_OK;
 }
 
@@ -875,28 +1093,15 @@ gpw_status gpw_tasklist_collocate(gpw_tasklist *tl, const double *pab, double *g
         gpw_status s = run_coef(tl, pab, 0, tl->npairs, st);
         if (s) return s;
     }
-    const int nt1 = tl->ntiles + 1;
+    const int smax = collocate_splits_max(tl);
     {
         PhaseTimer ph(tl, st, PH_CTILES);
-        bool first = true;
-        for (int c = 0; c < NCLS; c++) {
-            if (tl->class_entries[0][c] == 0) continue;
-            const int *start = tl->d_tile_start + (size_t)c * nt1;
-            const EntryRec *recs = tl->d_recs + tl->list_off[0][c];
-            if (tl->splits == 1) {
-                HIP_TRY(launch_collocate_tiles(c, tl->tg, recs, start, tl->d_coef, tl->ntiles, 1,
-                                               grid, 0, st));
-            } else {
-                HIP_TRY(launch_collocate_tiles(c, tl->tg, recs, start, tl->d_coef, tl->ntiles,
-                                               tl->splits, tl->d_cpartial, first ? 1 : 2, st));
-                first = false;
-            }
-        }
+        HIP_TRY(collocate_classes(tl, grid, smax, st));
     }
-    if (tl->splits > 1) {
+    if (smax > 1) {
         PhaseTimer ph(tl, st, PH_CREDUCE);
         const long long ng = (long long)tl->tg.nx * tl->tg.ny * tl->tg.nz;
-        HIP_TRY(launch_reduce_partials(tl->d_cpartial, tl->splits, ng, grid, st));
+        HIP_TRY(launch_reduce_partials(tl->d_cpartial, smax, ng, grid, st));
     }
     return GPW_OK;
 }
@@ -918,7 +1123,7 @@ gpw_status gpw_tasklist_integrate(gpw_tasklist *tl, const double *grid, const do
             const int *start = tl->d_tile_start + (size_t)(wf * NCLS + c) * nt1;
             const EntryRec *recs = tl->d_recs + tl->list_off[wf][c];
             HIP_TRY(launch_integrate_tiles(c, tl->tg, recs, start, grid, lp_add, tl->ntiles,
-                                           tl->splits, tl->d_partial, st));
+                                           tl->isplit[wf][c], tl->d_partial, st));
         }
     }
     {

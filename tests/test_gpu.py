@@ -99,9 +99,13 @@ def test_dropin_random_general(gpw, oracle, torch_cuda, tau, shear, window, la, 
 # ---------------------------------------------------------------- task lists
 
 
-def run_tasklist(gpw, torch, cfg, plain=False, splits=1, forces=False, tau=False, W=None):
+def run_tasklist(gpw, torch, cfg, plain=False, splits=1, forces=False, tau=False, W=None,
+                 autotune=False):
     g = gpw.grid_desc(cfg["npts"], cfg["dh"])
     tl = gpw.TaskList(g, cfg["pairs"], compute_tau=tau, splits=splits)
+    if autotune:
+        before, after = tl.autotune(max_splits=8, reps=2)
+        assert after > 0.0 and after <= before * 1.10 + 0.05, (before, after)
     n = cfg["npts"]
     grid = torch.zeros((n[2], n[1], n[0]), dtype=torch.float64, device="cuda")
     pab = dev(torch, cfg["pab"])
@@ -189,6 +193,24 @@ def test_config2_subset_batched_and_plain(gpw, oracle, torch_cuda, splits):
         p = run_tasklist(gpw, torch_cuda, cfg, plain=True, W=W)
         assert rel_err(p["grid"], ref["grid"]) < TOL
         assert rel_err(p["hab"], ref["hab"]) < TOL
+
+
+def test_autotuned_splits_keep_results(gpw, oracle, torch_cuda):
+    # measured per-class splits (mixed, so partial-grid slots are shared
+    # between classes with different splits) give the oracle's results
+    from libgpw_amd import workloads as Wk
+    cfg = Wk.subset(Wk.config(2), np.arange(3000))
+    say("autotune config2 subset", len(cfg["pairs"]["la_max"]))
+    W = Wk.potential(cfg)
+    ref = oracle_batch(oracle, cfg, W=W, forces=True)
+    got = run_tasklist(gpw, torch_cuda, cfg, splits=1, forces=True, W=W, autotune=True)
+    st = got["stats"]
+    allowed = {1, 2, 3, 4, 6, 8}
+    assert set(st["csplits"]) <= allowed and set(st["isplits"][0] + st["isplits"][1]) <= allowed
+    assert np.count_nonzero(got["grid"]) == np.count_nonzero(ref["grid"])
+    assert rel_err(got["grid"], ref["grid"]) < TOL
+    assert rel_err(got["hab"], ref["hab"]) < TOL
+    assert rel_err(got["forces"], ref["forces"]) < TOL
 
 
 def test_config2_full_size(gpw, oracle, torch_cuda):
