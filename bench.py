@@ -1,17 +1,22 @@
 #!/usr/bin/env python3
 """Benchmark: pairs/s of batched collocate + integrate on one MI355X.
 
-Workload: 20,000 primitive pairs, la, lb uniform in 0..3, exponents
-log-uniform in [0.1, 50] bohr^-2, centres uniform in a 15 bohr cubic cell,
-128^3 periodic grid, coefficients uniform[-1, 1], cube radii from eps = 1e-10
-(synthetic, seeded). One step = zero the grid, collocate every pair onto it,
-zero the blocks, integrate a fixed uniform[-1, 1] potential back onto every
-pair's block; all inputs resident in HBM. Prints one JSON line.
+Default workload (--config 2): 20,000 primitive pairs, la, lb uniform in
+0..3, exponents log-uniform in [0.1, 50] bohr^-2, centres uniform in a 15 bohr
+cubic cell, 128^3 periodic grid, coefficients uniform[-1, 1], cube radii from
+eps = 1e-10 (synthetic, seeded). One step = zero the grid, collocate every
+pair onto it, zero the blocks, integrate a fixed uniform[-1, 1] potential back
+onto every pair's block; all inputs resident in HBM. Prints one JSON line.
 
 Also measured in the same run: the plain one-launch-sequence-per-pair GPU
 form on the same task list, the CPU oracle (OpenMP) on a bounded sample of
 the pairs (cpu_baseline), and the error of the GPU result against the
 oracle on that sample.
+
+--config 1, 3, 4 run the other single-grid workloads the same way (3 and 4
+with forces in the integrate step); --config 5 runs the four-level triclinic
+multigrid workload (500,000 pairs) as one HIP graph of collocate + integrate
++ forces over all levels.
 """
 from __future__ import annotations
 
@@ -31,44 +36,53 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--splits", type=int, default=int(os.environ.get("GPW_SPLITS", "4")))
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-plain", action="store_true", help="skip the plain GPU leg")
-    ap.add_argument("--no-autotune", action="store_true",
-                    help="keep --splits for every kernel class instead of measured choices")
-    ap.add_argument("--cpu-pairs", type=int, default=3000)
-    ap.add_argument("--plain-pairs", type=int, default=2000)
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="only run a device copy of 1 GiB (x3) to calibrate HBM byte counters")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
-              file=sys.stderr)
-        sys.exit(2)
+def roofline(pts, coll_ms, integ_ms, ms_per_step):
+    """Algorithmic bytes: collocate reads + writes every sphere point (16 B),
+    integrate reads it (8 B); the dominant kernel's achieved rate."""
+    coll_bytes, integ_bytes = 16.0 * pts, 8.0 * pts
+    if coll_ms >= integ_ms:
+        dom = ("k_collocate_tiles", coll_bytes, coll_ms)
+    else:
+        dom = ("k_integrate_tiles", integ_bytes, integ_ms)
+    achieved = dom[1] / (dom[2] * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_128cubed_20000pairs.json")
+    if os.path.exists(tpath):
+        try:
+            traffic = json.load(open(tpath)).get(dom[0])
+        except (OSError, ValueError):
+            traffic = None
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom[0],
+            "kernel_ms": round(dom[2], 4), "algorithmic_bytes": dom[1],
+            "step_frac": round((coll_bytes + integ_bytes) / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
-    import torch
 
-    import libgpw_amd as G
-    from libgpw_amd import workloads as Wk
+def cpu_leg(O, legs, nthreads, budget_s=10.0):
+    """Time the oracle on [(grid, pairs, pab, potential, nblock)] (collocate +
+    integrate), repeated until ~budget_s; returns (seconds per pass, outputs, reps)."""
+    reps, outs = 0, None
+    t0 = time.perf_counter()
+    while True:
+        outs = []
+        for og, op, pab, pot, nblock in legs:
+            g = O.batch_collocate(og, op, pab, nthreads=nthreads)
+            h, _ = O.batch_integrate(og, op, pot, nblock, nthreads=nthreads)
+            outs.append((g, h))
+        reps += 1
+        if time.perf_counter() - t0 > budget_s or reps >= 5:
+            break
+    return (time.perf_counter() - t0) / reps, outs, reps
 
-    dev = torch.device("cuda:0")
-    torch.cuda.set_device(dev)
-    if args.copy_calibration:
-        # a plain copy of known size: reads and writes 2^30 bytes per launch
-        src = torch.ones(1 << 27, dtype=torch.float64, device=dev)
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_calibration_bytes": 1 << 30, "launches": 3}), flush=True)
-        return
-    cfg = Wk.config(2)
+
+def rel(a, b):
+    s = np.abs(b).max()
+    return float(np.abs(a - b).max() / s) if s > 0 else float(np.abs(a).max())
+
+
+def run_single(args, torch, G, Wk, dev, cfg):
     npairs = len(cfg["pairs"]["la_max"])
+    with_forces = cfg["index"] in (3, 4)
     V_h = Wk.potential(cfg)
     grid_desc = G.grid_desc(cfg["npts"], cfg["dh"])
     t0 = time.perf_counter()
@@ -87,12 +101,15 @@ def main():
     V = torch.from_numpy(V_h).to(dev)
     grid = torch.zeros((n[2], n[1], n[0]), dtype=torch.float64, device=dev)
     hab = torch.zeros(cfg["nblock"], dtype=torch.float64, device=dev)
+    forces = torch.zeros((npairs, 6), dtype=torch.float64, device=dev) if with_forces else None
 
     def step():
         grid.zero_()
         hab.zero_()
+        if forces is not None:
+            forces.zero_()
         tl.collocate(pab, grid)
-        tl.integrate(V, hab)
+        tl.integrate(V, hab, pab=pab if forces is not None else None, forces=forces)
 
     for _ in range(args.warmup):
         step()
@@ -108,40 +125,21 @@ def main():
     ph = tl.timings()
     tl.set_timing(False)
     phase_ms = {k: v[0] / max(v[1], 1) for k, v in ph.items()}
-    coll_ms = phase_ms["collocate"]
-    integ_ms = phase_ms["integrate"]
     ms_per_step = wall * 1e3 / args.steps
     value = npairs * args.steps / wall
-
-    # algorithmic bytes: collocate reads+writes every sphere point (16 B),
-    # integrate reads it (8 B)
     pts = st["cube_points"]
-    coll_bytes, integ_bytes = 16.0 * pts, 8.0 * pts
-    if coll_ms >= integ_ms:
-        dom = ("k_collocate_tiles", coll_bytes, coll_ms)
-    else:
-        dom = ("k_integrate_tiles", integ_bytes, integ_ms)
-    achieved = dom[1] / (dom[2] * 1e-3) / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_128cubed_20000pairs.json")
-    if os.path.exists(tpath):
-        try:
-            traffic = json.load(open(tpath)).get(dom[0])
-        except (OSError, ValueError):
-            traffic = None
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom[0],
-            "kernel_ms": round(dom[2], 4), "algorithmic_bytes": dom[1],
-            "step_frac": round((coll_bytes + integ_bytes) / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    roof = roofline(pts, phase_ms["collocate"], phase_ms["integrate"], ms_per_step)
 
     out = {"metric": METRIC, "value": round(value, 1), "unit": "pairs/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
            "data": "synthetic",
-           "config": {"workload": cfg["name"], "pairs": npairs, "grid": list(n), "cell_bohr": 15.0,
-                      "l_range": [0, 3], "exponents": [0.1, 50.0], "eps": Wk.EPS,
+           "config": {"workload": cfg["name"], "pairs": npairs, "grid": list(n),
+                      "cell_bohr": [round(float(x), 6) for x in np.diag(cfg["dh"]) * np.asarray(n)],
+                      "forces": with_forces, "eps": Wk.EPS,
                       "splits": args.splits, "autotune": tune,
-                      "collocate_splits": st["csplits"], "integrate_splits": st["isplits"][0],
+                      "collocate_splits": st["csplits"],
+                      "integrate_splits": st["isplits"][1 if with_forces else 0],
                       "tile": st["tile"], "sub_boxes": st["nsub"],
                       "work_items": st["nentries"], "sphere_points": pts, "plan_s": round(plan_s, 3),
                       "phase_ms": {k: round(v, 4) for k, v in phase_ms.items()}},
@@ -178,15 +176,8 @@ def main():
         sub = Wk.subset(cfg, idx)
         og = O.make_grid(sub["npts"], sub["dh"])
         op = O.make_pairs(sub["pairs"])
-        reps = 0
-        t0 = time.perf_counter()
-        while True:
-            ref_grid = O.batch_collocate(og, op, sub["pab"], nthreads=ncpu)
-            ref_hab, _ = O.batch_integrate(og, op, V_h, sub["nblock"], nthreads=ncpu)
-            reps += 1
-            if time.perf_counter() - t0 > 10.0 or reps >= 5:
-                break
-        tc = (time.perf_counter() - t0) / reps
+        tc, outs, reps = cpu_leg(O, [(og, op, sub["pab"], V_h, sub["nblock"])], ncpu)
+        ref_grid, ref_hab = outs[0]
         out["cpu_baseline"] = {"value": round(len(idx) / tc, 1), "unit": "pairs/s", "cores": ncpu,
                                "kind": "port",
                                "sample": f"{len(idx)} random pairs of the workload (seed 7), "
@@ -197,13 +188,150 @@ def main():
         tls.collocate(torch.from_numpy(sub["pab"]).to(dev), g_s)
         tls.integrate(V, h_s)
         torch.cuda.synchronize()
-        gg, hh = g_s.cpu().numpy(), h_s.cpu().numpy()
-        out["rel_err"] = {"grid": float(np.abs(gg - ref_grid).max() / np.abs(ref_grid).max()),
-                          "hab": float(np.abs(hh - ref_hab).max() / np.abs(ref_hab).max()),
-                          "sample": "the cpu_baseline pairs"}
+        out["rel_err"] = {"grid": rel(g_s.cpu().numpy(), ref_grid),
+                          "hab": rel(h_s.cpu().numpy(), ref_hab), "sample": "the cpu_baseline pairs"}
         tls.close()
-
     tl.close()
+    return out
+
+
+def run_multigrid(args, torch, G, Wk, dev, cfg):
+    from libgpw_amd.multigrid import MultiGrid
+    t0 = time.perf_counter()
+    mg = MultiGrid(cfg, splits=args.splits)
+    plan_s = time.perf_counter() - t0
+    tune = None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        res = mg.autotune(max_splits=8, reps=3)
+        tune = {"seconds": round(time.perf_counter() - t0, 3),
+                "kernel_ms_before": round(sum(r[0] for r in res), 4),
+                "kernel_ms_after": round(sum(r[1] for r in res), 4)}
+    pab = torch.from_numpy(cfg["pab"]).to(dev)
+    grids = mg.grids(dev)
+    pots_h = [np.random.default_rng(cfg["seed"] + 7 + k).uniform(-1.0, 1.0, tuple(g.shape))
+              for k, g in enumerate(grids)]
+    pots = [torch.from_numpy(p).to(dev) for p in pots_h]
+    hab = torch.zeros(cfg["nblock"], dtype=torch.float64, device=dev)
+    forces = mg.forces_buffers(dev)
+    # per-level phase times from one timed eager step
+    for lv in mg.levels:
+        lv["tl"].set_timing(True)
+    torch.cuda.synchronize()
+    mg.step(pab, grids, pots, hab, forces)
+    torch.cuda.synchronize()
+    level_ms, pts, coll, integ = [], 0, 0.0, 0.0
+    for lv in mg.levels:
+        ph = lv["tl"].timings()
+        lv["tl"].set_timing(False)
+        st = lv["tl"].stats()
+        pts += st["cube_points"]
+        pm = {k: round(v[0] / max(v[1], 1), 4) for k, v in ph.items()}
+        coll += pm["collocate"] + pm["reduce"]
+        integ += pm["integrate"]
+        level_ms.append({"grid": list(lv["npts"]), "pairs": int(len(lv["index"])),
+                         "sphere_points": st["cube_points"], "collocate_splits": st["csplits"],
+                         "integrate_splits": st["isplits"][1], "phase_ms": pm})
+    graph = mg.capture(pab, grids, pots, hab, forces)
+    for _ in range(args.warmup):
+        graph.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        graph.replay()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    value = mg.npairs * args.steps / wall
+    out = {"metric": METRIC, "value": round(value, 1), "unit": "pairs/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic",
+           "config": {"workload": cfg["name"], "pairs": mg.npairs, "forces": True, "eps": Wk.EPS,
+                      "hip_graph": True, "autotune": tune, "plan_s": round(plan_s, 3),
+                      "levels": level_ms},
+           "roofline": roofline(pts, coll, integ, ms_per_step)}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        O.build()
+        ncpu = min(16, os.cpu_count() or 1)
+        rng = np.random.default_rng(7)
+        frac = min(1.0, args.cpu_pairs / mg.npairs)
+        legs, tls, sample = [], [], 0
+        for k, lv in enumerate(mg.levels):
+            m = max(1, int(round(frac * len(lv["index"]))))
+            idx = np.sort(rng.choice(lv["index"], size=m, replace=False))
+            sub = {kk: np.asarray(v)[idx] for kk, v in cfg["pairs"].items()}
+            og = O.make_grid(lv["npts"], cfg["levels"][k]["dh"])
+            legs.append((og, O.make_pairs(sub), cfg["pab"], pots_h[k], cfg["nblock"]))
+            tls.append(G.TaskList(lv["grid_desc"], sub))
+            sample += m
+        tc, outs, reps = cpu_leg(O, legs, ncpu)
+        out["cpu_baseline"] = {"value": round(sample / tc, 1), "unit": "pairs/s", "cores": ncpu,
+                               "kind": "port",
+                               "sample": f"{sample} pairs drawn per level in proportion (seed 7), "
+                                         f"collocate+integrate x{reps}, OpenMP C oracle"}
+        eg = 0.0
+        h_s = torch.zeros(cfg["nblock"], dtype=torch.float64, device=dev)
+        h_ref = np.zeros(cfg["nblock"])
+        for k, (tl, (g_ref, h_r)) in enumerate(zip(tls, outs)):
+            g_s = torch.zeros_like(grids[k])
+            tl.collocate(pab, g_s)
+            tl.integrate(pots[k], h_s)
+            torch.cuda.synchronize()
+            eg = max(eg, rel(g_s.cpu().numpy(), g_ref))
+            h_ref += h_r
+            tl.close()
+        out["rel_err"] = {"grid": eg, "hab": rel(h_s.cpu().numpy(), h_ref),
+                          "sample": "the cpu_baseline pairs"}
+    mg.close()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5],
+                    help="workload (2: the 20,000-pair 128^3 configuration of the metric)")
+    ap.add_argument("--pairs", type=int, default=None, help="override the pair count")
+    ap.add_argument("--splits", type=int, default=int(os.environ.get("GPW_SPLITS", "4")))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain GPU leg")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep --splits for every kernel class instead of measured choices")
+    ap.add_argument("--cpu-pairs", type=int, default=3000)
+    ap.add_argument("--plain-pairs", type=int, default=2000)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a device copy of 1 GiB (x3) to calibrate HBM byte counters")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import libgpw_amd as G
+    from libgpw_amd import workloads as Wk
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    if args.copy_calibration:
+        # a plain copy of known size: reads and writes 2^30 bytes per launch
+        src = torch.ones(1 << 27, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": 1 << 30, "launches": 3}), flush=True)
+        return
+    cfg = Wk.config(args.config, npairs=args.pairs)
+    if args.config == 5:
+        out = run_multigrid(args, torch, G, Wk, dev, cfg)
+    else:
+        out = run_single(args, torch, G, Wk, dev, cfg)
     print(json.dumps(out), flush=True)
 
 

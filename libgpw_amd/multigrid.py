@@ -1,0 +1,94 @@
+"""Multigrid collocate / integrate: one task list per grid level.
+
+CP2K keeps a hierarchy of grids of decreasing resolution and maps every
+product Gaussian to one level by its exponent (the relative-cutoff rule,
+``workloads.level_of``). Each level here is an independent task list on its
+own grid; the pairs of all levels share one packed coefficient buffer (pab)
+and one hab buffer, addressed by each pair's block_offset, so the levels
+write disjoint hab blocks. Forces come back per level (pairs in level order).
+
+``MultiGrid.capture`` records collocate + integrate (+ forces) of every level
+in one HIP graph (torch.cuda.CUDAGraph); task-list calls allocate nothing and
+clear memory with fill kernels, so they are legal inside capture.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import TaskList, grid_desc
+
+
+class MultiGrid:
+    def __init__(self, cfg: dict, compute_tau: bool = False, splits: int = 1):
+        """cfg: a multigrid configuration (``workloads.config(5)``): ``pairs``
+        (every pair, global block offsets) and ``levels`` (npts, dh, index)."""
+        self.levels = []
+        p = cfg["pairs"]
+        for lv in cfg["levels"]:
+            idx = np.asarray(lv["index"])
+            if idx.size == 0:
+                continue
+            sub = {k: np.asarray(v)[idx] for k, v in p.items()}
+            g = grid_desc(lv["npts"], lv["dh"])
+            self.levels.append({"npts": tuple(lv["npts"]), "index": idx, "grid_desc": g,
+                                "tl": TaskList(g, sub, compute_tau=compute_tau, splits=splits)})
+        self.npairs = int(sum(len(lv["index"]) for lv in self.levels))
+
+    def autotune(self, max_splits: int = 8, reps: int = 3):
+        return [lv["tl"].autotune(max_splits=max_splits, reps=reps) for lv in self.levels]
+
+    def grids(self, device="cuda"):
+        """Zeroed level grids (z, y, x)."""
+        return [torch.zeros((n[2], n[1], n[0]), dtype=torch.float64, device=device)
+                for n in (lv["npts"] for lv in self.levels)]
+
+    def forces_buffers(self, device="cuda"):
+        return [torch.zeros((len(lv["index"]), 6), dtype=torch.float64, device=device)
+                for lv in self.levels]
+
+    def collocate(self, pab, grids):
+        for lv, g in zip(self.levels, grids):
+            lv["tl"].collocate(pab, g)
+        return grids
+
+    def integrate(self, pots, hab, pab=None, forces=None):
+        for k, (lv, v) in enumerate(zip(self.levels, pots)):
+            lv["tl"].integrate(v, hab, pab=pab, forces=None if forces is None else forces[k])
+        return hab
+
+    def step(self, pab, grids, pots, hab, forces=None):
+        """One collocate + integrate (+ forces) of every level."""
+        for g in grids:
+            g.zero_()
+        hab.zero_()
+        if forces is not None:
+            for f in forces:
+                f.zero_()
+        self.collocate(pab, grids)
+        self.integrate(pots, hab, pab=pab if forces is not None else None, forces=forces)
+
+    def capture(self, pab, grids, pots, hab, forces=None):
+        """HIP graph of ``step``; replaying it recomputes grids, hab and forces
+        from the current contents of pab and pots."""
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):  # warm-up outside the graph
+            self.step(pab, grids, pots, hab, forces)
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            self.step(pab, grids, pots, hab, forces)
+        return graph
+
+    def forces_by_pair(self, forces) -> np.ndarray:
+        """Level force buffers back in global pair order: (npairs, 2, 3)."""
+        out = np.zeros((self.npairs, 6))
+        for lv, f in zip(self.levels, forces):
+            out[lv["index"]] = f.cpu().numpy()
+        return out.reshape(-1, 2, 3)
+
+    def close(self):
+        for lv in self.levels:
+            lv["tl"].close()
