@@ -54,15 +54,17 @@ struct AlphaView {
     }
 };
 
-__device__ void fill_alpha(const PairDev &p, AlphaView A, int tid, int nthr) {
+// pp points to the pair in global memory: its per-axis fields are read with a
+// run-time axis index, which on a register copy would force it to scratch
+__device__ void fill_alpha(const PairDev *__restrict__ pp, AlphaView A, int tid, int nthr) {
     const int n = 3 * (A.lbm + 1) * (A.lam + 1);
     for (int w = tid; w < n; w += nthr) {
         const int la = w % (A.lam + 1);
         const int lb = (w / (A.lam + 1)) % (A.lbm + 1);
         const int d = w / ((A.lam + 1) * (A.lbm + 1));
         for (int k = 0; k <= A.lpm; k++) A.at(d, lb, la, k) = 0.0;
-        const double pa = p.rp[d] - p.ra[d];
-        const double pb = p.rp[d] - p.rb[d];
+        const double pa = pp->rp[d] - pp->ra[d];
+        const double pb = pp->rp[d] - pp->rb[d];
         double ca = 1.0, powa = 1.0;
         for (int i = 0; i <= la; i++) {
             double cb = 1.0, powb = 1.0;
@@ -163,60 +165,100 @@ __device__ void build_eff(const PairDev &p, int tau, const double *__restrict__ 
     __syncthreads();
 }
 
-// one workgroup per pair; dynamic LDS holds alpha and the effective block
-// mtrans (general cells): for each lp a dense C3(lp) x C3(lp) matrix T,
-// coef_ijk = T coef_xyz (multinomial expansion of grid_base_ref_c.cc:647-698)
+// General cells: apply the lattice transform of degree-preserving blocks.
+// T (C3(lp) x C3(lp), both indices cidx(lp, z, y, x)) only couples monomials
+// of the same total degree, so row r of degree d only visits the C2(d)
+// columns of degree d. transpose: out[c] = sum_r T[r][c] in[r] instead.
+__device__ void lattice_transform(const double *__restrict__ T, int lp, const double *in, double *out,
+                                  bool transpose, int tid, int nthr) {
+    const int n3 = ncoset(lp);
+    for (int r = tid; r < n3; r += nthr) {
+        int lz, ly, lx;
+        decode_c3(lp, r, lz, ly, lx);
+        const int d = lz + ly + lx;
+        double a = 0.0;
+        for (int kz = 0; kz <= d; kz++)
+            for (int ky = 0; ky <= d - kz; ky++) {
+                const int c = cidx(lp, kz, ky, d - kz - ky);
+                a += (transpose ? T[(long long)c * n3 + r] : T[(long long)r * n3 + c]) * in[c];
+            }
+        out[r] = a;
+    }
+}
+
+// cab -> cxyz of one pair per workgroup, factorised over the Cartesian axes
+// (the three-step contraction of grid_base_ref_c.cc:234-315, in two passes):
+//   Ty[az][bz][(ly,lx)] = sum_{ay,by} sum_{ax,bx} eff[b][a] alpha_x(bx,ax,lx) alpha_y(by,ay,ly)
+//   coef[(lz,ly,lx)]    = prefactor sum_{az,bz} Ty[az][bz][(ly,lx)] alpha_z(bz,az,lz)
+// with (ax, bx) over the shells of the (tau-shifted) block; Ty holds only
+// ly + lx <= (lam - az) + (lbm - bz). Dynamic LDS: alpha, eff, Ty (+ cxyz).
+// mtrans (general cells): coef_ijk = T coef_xyz, T the multinomial expansion
+// of grid_base_ref_c.cc:647-698 for each lp.
+// Workgroups loop over pairs (pair0 + blockIdx.x, stride gridDim.x); eff, Ty
+// and cxyz live in LDS after alpha, or, when gws is given (blocks too large
+// for LDS), in the workgroup's slice gws + blockIdx.x * ws_stride.
 __global__ __launch_bounds__(64) void k_coef(const PairDev *__restrict__ pairs, int pair0,
-                                             int tau, const double *__restrict__ pab,
+                                             int npairs, int tau, const double *__restrict__ pab,
                                              double *__restrict__ coef,
                                              const double *__restrict__ mtrans,
-                                             const long long *__restrict__ mt_off) {
+                                             const long long *__restrict__ mt_off,
+                                             double *__restrict__ gws, long long ws_stride) {
     extern __shared__ double sm_coef[];
-    const PairDev p = pairs[pair0 + blockIdx.x];
+    for (int pi = pair0 + blockIdx.x; pi < pair0 + npairs; pi += gridDim.x) {
+    __syncthreads();  // LDS / workspace reuse across pairs
+    const PairDev p = pairs[pi];
     const int lam = p.la_max + tau, lbm = p.lb_max + tau;
     const int lam_lo = max(p.la_min - tau, 0), lbm_lo = max(p.lb_min - tau, 0);
     const int lp = lam + lbm;
     AlphaView A{sm_coef, lam, lbm, lp};
     const int nalpha = 3 * (lbm + 1) * (lam + 1) * (lp + 1);
-    double *eff = sm_coef + nalpha;
     const int nla = ncoset(lam), nlb = ncoset(lbm);
-    double *cxyz = eff + nla * nlb;
-    fill_alpha(p, A, threadIdx.x, blockDim.x);
-    build_eff(p, tau, pab, eff, nla, nlb, nla, threadIdx.x, blockDim.x);
-    const int na_lo = ncoset(lam_lo - 1), nb_lo = ncoset(lbm_lo - 1);
-    const int n3 = ncoset(lp);
-    for (int c = threadIdx.x; c < n3; c += blockDim.x) {
-        int lzp, lyp, lxp;
-        decode_c3(lp, c, lzp, lyp, lxp);
+    const int n2 = ncoef2(lp), n3 = ncoset(lp);
+    double *eff = gws ? gws + blockIdx.x * ws_stride : sm_coef + nalpha;
+    double *Ty = eff + nla * nlb;
+    double *cxyz = Ty + (lam + 1) * (lbm + 1) * n2;
+    fill_alpha(&pairs[pi], A, threadIdx.x, blockDim.x);
+    build_eff(p, tau, pab, eff, nla, nlb, nla, threadIdx.x, blockDim.x);  // ends with a barrier
+    const int nzz = (lam + 1) * (lbm + 1);
+    for (int w = threadIdx.x; w < nzz * n2; w += blockDim.x) {
+        const int zz = w / n2, az = zz / (lbm + 1), bz = zz % (lbm + 1);
+        int ly, lx;
+        decode_c2(lp, w % n2, ly, lx);
+        if (ly + lx > (lam - az) + (lbm - bz)) continue;
         double acc = 0.0;
-        for (int jco = nb_lo; jco < nlb; jco++) {
-            int lb = 0;
-            while (ncoset(lb) <= jco) lb++;
-            int bx, by, bz;
-            decode_coset(lb, jco - ncoset(lb - 1), bx, by, bz);
-            for (int ico = na_lo; ico < nla; ico++) {
-                const double e = eff[jco * nla + ico];
-                if (e == 0.0) continue;
-                int la = 0;
-                while (ncoset(la) <= ico) la++;
-                int ax, ay, az;
-                decode_coset(la, ico - ncoset(la - 1), ax, ay, az);
-                if (lxp > ax + bx || lyp > ay + by || lzp > az + bz) continue;
-                acc += e * A.at(0, bx, ax, lxp) * A.at(1, by, ay, lyp) * A.at(2, bz, az, lzp);
+        for (int ay = 0; ay <= lam - az; ay++) {
+            const int ax_lo = max(0, lam_lo - ay - az), ax_hi = lam - ay - az;
+            for (int by = 0; by <= lbm - bz; by++) {
+                if (ly > ay + by) continue;
+                const double cy = A.at(1, by, ay, ly);
+                const int bx_lo = max(0, lbm_lo - by - bz), bx_hi = lbm - by - bz;
+                for (int ax = ax_lo; ax <= ax_hi; ax++) {
+                    const double *e = eff + coset(ax, ay, az);
+                    for (int bx = max(bx_lo, lx - ax); bx <= bx_hi; bx++)
+                        acc += e[coset(bx, by, bz) * nla] * A.at(0, bx, ax, lx) * cy;
+                }
             }
         }
-        if (mtrans)
-            cxyz[c] = p.prefactor * acc;
-        else
-            coef[p.coef_off + c] = p.prefactor * acc;
+        Ty[w] = acc;
     }
-    if (!mtrans) return;
     __syncthreads();
-    const double *T = mtrans + mt_off[lp];
-    for (int r = threadIdx.x; r < n3; r += blockDim.x) {
-        double a = 0.0;
-        for (int c = 0; c < n3; c++) a += T[(long long)r * n3 + c] * cxyz[c];
-        coef[p.coef_off + r] = a;
+    double *dst = mtrans ? cxyz : coef + p.coef_off;
+    for (int c = threadIdx.x; c < n3; c += blockDim.x) {
+        int lz, ly, lx;
+        decode_c3(lp, c, lz, ly, lx);
+        const int c2 = c2idx(lp, ly, lx);
+        double acc = 0.0;
+        for (int az = 0; az <= lam; az++)
+            for (int bz = max(0, lz - az); bz <= lbm; bz++) {
+                if (ly + lx > (lam - az) + (lbm - bz)) break;
+                acc += Ty[(az * (lbm + 1) + bz) * n2 + c2] * A.at(2, bz, az, lz);
+            }
+        dst[c] = p.prefactor * acc;
+    }
+    if (!mtrans) continue;
+    __syncthreads();
+    lattice_transform(mtrans + mt_off[lp], lp, cxyz, coef + p.coef_off, false, threadIdx.x,
+                      blockDim.x);
     }
 }
 
@@ -1026,29 +1068,36 @@ __global__ __launch_bounds__(64) void k_hab(const PairDev *__restrict__ pairs, i
     const int nalpha = 3 * (lbmx + 1) * (lamx + 1) * (lpx + 1);
     const int nlax = ncoset(lamx), nlbx = ncoset(lbmx);
     const int n3x = ncoset(lpx);
+    const int n2x = ncoef2(lpx);
     double *M = sm_hab + nalpha;                    // ncoset(lpx)
     double *Mijk = M + n3x;                         // general cells: lattice moments
     double *red = Mijk + n3x;                       // 6 x 64
-    double *heff, *eff;                             // nlbx x nlax, ncoset(lbm) x ncoset(lam)
-    if (gws) {
-        heff = gws + blockIdx.x * ws_stride;
-    } else {
-        heff = red + 6 * 64;
-    }
-    eff = heff + nlax * nlbx;
-    fill_alpha(p, A, threadIdx.x, blockDim.x);
+    // heff (nlbx x nlax), eff (ncoset(lbm) x ncoset(lam)), U ((lamx+1)(lbmx+1) x C2(lpx))
+    double *heff = gws ? gws + blockIdx.x * ws_stride : red + 6 * 64;
+    double *eff = heff + nlax * nlbx;
+    double *U = eff + ncoset(lam) * ncoset(lbm);
+    fill_alpha(&pairs[pi], A, threadIdx.x, blockDim.x);
     if (!mtrans) {
         for (int c = threadIdx.x; c < n3x; c += blockDim.x) M[c] = Mbuf[(long long)pi * mstride + c];
     } else {
         // M_xyz = T^T M_ijk (adjoint of the lattice transform)
         for (int c = threadIdx.x; c < n3x; c += blockDim.x) Mijk[c] = Mbuf[(long long)pi * mstride + c];
         __syncthreads();
-        const double *T = mtrans + mt_off[lpx];
-        for (int c = threadIdx.x; c < n3x; c += blockDim.x) {
-            double a = 0.0;
-            for (int r = 0; r < n3x; r++) a += T[(long long)r * n3x + c] * Mijk[r];
-            M[c] = a;
-        }
+        lattice_transform(mtrans + mt_off[lpx], lpx, Mijk, M, true, threadIdx.x, blockDim.x);
+    }
+    __syncthreads();
+    // adjoint of the factorised cab -> cxyz transform (see k_coef):
+    //   U[az][bz][(ly,lx)] = sum_lz M[(lz,ly,lx)] alpha_z(bz,az,lz)
+    //   heff[b][a] = prefactor sum_{ly,lx} U[az][bz][(ly,lx)] alpha_y(by,ay,ly) alpha_x(bx,ax,lx)
+    const int nzz = (lamx + 1) * (lbmx + 1);
+    for (int w = threadIdx.x; w < nzz * n2x; w += blockDim.x) {
+        const int zz = w / n2x, az = zz / (lbmx + 1), bz = zz % (lbmx + 1);
+        int ly, lx;
+        decode_c2(lpx, w % n2x, ly, lx);
+        if (ly + lx > (lamx - az) + (lbmx - bz)) continue;
+        double acc = 0.0;
+        for (int lz = 0; lz <= az + bz; lz++) acc += M[cidx(lpx, lz, ly, lx)] * A.at(2, bz, az, lz);
+        U[w] = acc;
     }
     __syncthreads();
     const int na_lo = ncoset(lamx_lo - 1), nb_lo = ncoset(lbmx_lo - 1);
@@ -1062,13 +1111,11 @@ __global__ __launch_bounds__(64) void k_hab(const PairDev *__restrict__ pairs, i
             int ax, ay, az, bx, by, bz;
             decode_coset(la, ico - ncoset(la - 1), ax, ay, az);
             decode_coset(lb, jco - ncoset(lb - 1), bx, by, bz);
-            for (int lzp = 0; lzp <= az + bz; lzp++) {
-                const double cz = A.at(2, bz, az, lzp);
-                for (int lyp = 0; lyp <= ay + by; lyp++) {
-                    const double cyz = cz * A.at(1, by, ay, lyp);
-                    for (int lxp = 0; lxp <= ax + bx; lxp++)
-                        acc += M[cidx(lpx, lzp, lyp, lxp)] * cyz * A.at(0, bx, ax, lxp);
-                }
+            const double *Uz = U + (az * (lbmx + 1) + bz) * n2x;
+            for (int ly = 0; ly <= ay + by; ly++) {
+                double sx = 0.0;
+                for (int lx = 0; lx <= ax + bx; lx++) sx += Uz[c2idx(lpx, ly, lx)] * A.at(0, bx, ax, lx);
+                acc += sx * A.at(1, by, ay, ly);
             }
         }
         heff[jco * nlax + ico] = p.prefactor * acc;
@@ -1420,23 +1467,41 @@ hipError_t launch_integrate_tiles(int cls, const TileGrid &tg, const EntryRec *r
     }
 }
 
-size_t coef_smem_bytes(int lam, int lbm) {
+// k_coef: per-pair blocks (eff, Ty, cxyz) and the LDS with / without them
+size_t coef_ws_doubles(int lam, int lbm) {
     const int lp = lam + lbm;
-    return sizeof(double) * (size_t)(3 * (lbm + 1) * (lam + 1) * (lp + 1) + ncoset(lam) * ncoset(lbm) +
-                                     ncoset(lp));
+    return (size_t)ncoset(lam) * ncoset(lbm) + (size_t)(lam + 1) * (lbm + 1) * ncoef2(lp) +
+           (size_t)ncoset(lp);
 }
+
+size_t coef_smem_bytes(int lam, int lbm, bool lds_blocks) {
+    const int lp = lam + lbm;
+    size_t n = 3 * (size_t)(lbm + 1) * (lam + 1) * (lp + 1);
+    if (lds_blocks) n += coef_ws_doubles(lam, lbm);
+    return sizeof(double) * n;
+}
+
+bool coef_needs_workspace(int lam, int lbm) { return coef_smem_bytes(lam, lbm, true) > HAB_LDS_MAX; }
 
 hipError_t launch_coef(const PairDev *pairs, int pair0, int npairs, int tau, int lam_max,
                        int lbm_max, const double *pab, double *coef, const double *mtrans,
-                       const long long *mt_off, hipStream_t st) {
-    const size_t smem = coef_smem_bytes(lam_max + tau, lbm_max + tau);
-    hipLaunchKernelGGL(k_coef, dim3(npairs), dim3(64), smem, st, pairs, pair0, tau, pab, coef, mtrans,
-                       mt_off);
+                       const long long *mt_off, double *ws, long long ws_stride, int ws_blocks,
+                       hipStream_t st) {
+    if (npairs <= 0) return hipSuccess;
+    const int lam = lam_max + tau, lbm = lbm_max + tau;
+    const bool big = coef_needs_workspace(lam, lbm);
+    if (big && (!ws || ws_stride < (long long)coef_ws_doubles(lam, lbm) || ws_blocks < 1))
+        return hipErrorInvalidValue;
+    const size_t smem = coef_smem_bytes(lam, lbm, !big);
+    const int nblk = big ? std::min(npairs, ws_blocks) : npairs;
+    hipLaunchKernelGGL(k_coef, dim3(nblk), dim3(64), smem, st, pairs, pair0, npairs, tau, pab, coef,
+                       mtrans, mt_off, big ? ws : nullptr, big ? ws_stride : 0LL);
     return hipGetLastError();
 }
 
 size_t hab_ws_doubles(int lamx, int lbmx, int lam, int lbm) {
-    return (size_t)ncoset(lamx) * ncoset(lbmx) + (size_t)ncoset(lam) * ncoset(lbm);
+    return (size_t)ncoset(lamx) * ncoset(lbmx) + (size_t)ncoset(lam) * ncoset(lbm) +
+           (size_t)(lamx + 1) * (lbmx + 1) * ncoef2(lamx + lbmx);
 }
 
 // LDS of k_hab: alpha, M, Mijk, the force reduction and (LDS mode) heff, eff

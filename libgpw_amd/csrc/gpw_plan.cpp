@@ -35,7 +35,10 @@ hipError_t launch_integrate_tiles(int cls, const TileGrid &tg, const EntryRec *r
                                   int ntiles, int splits, double *partial, hipStream_t st);
 hipError_t launch_coef(const PairDev *pairs, int pair0, int npairs, int tau, int lam_max,
                        int lbm_max, const double *pab, double *coef, const double *mtrans,
-                       const long long *mt_off, hipStream_t st);
+                       const long long *mt_off, double *ws, long long ws_stride, int ws_blocks,
+                       hipStream_t st);
+size_t coef_ws_doubles(int lam, int lbm);
+bool coef_needs_workspace(int lam, int lbm);
 hipError_t launch_hab(const PairDev *pairs, int pair0, int npairs, int tau, int with_forces,
                       int la_max, int lb_max, const double *Mbuf, int mstride, const double *pab,
                       double *hab, double *forces, const double *mtrans, const long long *mt_off,
@@ -210,7 +213,7 @@ struct gpw_tasklist {
     double *d_mtrans = nullptr;      // general cells: lattice transform per lp
     long long *d_mt_off = nullptr;
     long long coef_total = 0;        // doubles of d_coef
-    double *d_hab_ws = nullptr;      // k_hab workspace for blocks too large for LDS
+    double *d_hab_ws = nullptr;      // k_coef / k_hab workspace for blocks too large for LDS
     long long hab_ws_stride = 0;
     int hab_ws_blocks = 0;
 
@@ -675,14 +678,18 @@ static gpw_status upload_tasklist(gpw_tasklist *tl, const HostPlan &hp) {
         tl->cpart_cap = tl->splits;
     }
     {
-        // k_hab workspace, sized for the force extension of the largest block
+        // workspace of k_coef / k_hab (stream-ordered, never used by both at
+        // once), sized for the largest block and the force extension
         const int t = tl->compute_tau;
-        const int lamx = tl->la_max + t + 1, lbmx = tl->lb_max + t + 1;
-        if (hab_needs_workspace(lamx, lbmx, tl->la_max + t, tl->lb_max + t)) {
-            tl->hab_ws_stride = (long long)hab_ws_doubles(lamx, lbmx, tl->la_max + t, tl->lb_max + t);
+        const int lam = tl->la_max + t, lbm = tl->lb_max + t;
+        long long stride = 0;
+        if (coef_needs_workspace(lam, lbm)) stride = (long long)coef_ws_doubles(lam, lbm);
+        if (hab_needs_workspace(lam + 1, lbm + 1, lam, lbm))
+            stride = std::max(stride, (long long)hab_ws_doubles(lam + 1, lbm + 1, lam, lbm));
+        if (stride > 0) {
+            tl->hab_ws_stride = stride;
             tl->hab_ws_blocks = std::min(tl->npairs, HAB_WS_BLOCKS);
-            HIP_TRY(hipMalloc((void **)&tl->d_hab_ws,
-                              sizeof(double) * tl->hab_ws_stride * tl->hab_ws_blocks));
+            HIP_TRY(hipMalloc((void **)&tl->d_hab_ws, sizeof(double) * stride * tl->hab_ws_blocks));
         }
     }
     if (tl->tg.general) {
@@ -1080,7 +1087,8 @@ gpw_status gpw_tasklist_autotune(gpw_tasklist *tl, int max_splits, int reps, voi
 static gpw_status run_coef(gpw_tasklist *tl, const double *pab, int pair0, int n, hipStream_t st) {
     if (n <= 0) return GPW_OK;
     HIP_TRY(launch_coef(tl->d_pairs, pair0, n, tl->compute_tau, tl->la_max, tl->lb_max, pab,
-                        tl->d_coef, tl->d_mtrans, tl->d_mt_off, st));
+                        tl->d_coef, tl->d_mtrans, tl->d_mt_off, tl->d_hab_ws, tl->hab_ws_stride,
+                        tl->hab_ws_blocks, st));
     return GPW_OK;
 }
 
