@@ -91,7 +91,7 @@ def run_single(args, torch, G, Wk, dev, cfg):
     tune = None
     if not args.no_autotune:
         t0 = time.perf_counter()
-        before, after = tl.autotune(max_splits=8, reps=3)
+        before, after = tl.autotune(max_splits=0, reps=3)
         tune = {"seconds": round(time.perf_counter() - t0, 3), "kernel_ms_before": round(before, 4),
                 "kernel_ms_after": round(after, 4)}
     st = tl.stats()
@@ -203,7 +203,7 @@ def run_multigrid(args, torch, G, Wk, dev, cfg):
     tune = None
     if not args.no_autotune:
         t0 = time.perf_counter()
-        res = mg.autotune(max_splits=8, reps=3)
+        res = mg.autotune(max_splits=0, reps=3)
         tune = {"seconds": round(time.perf_counter() - t0, 3),
                 "kernel_ms_before": round(sum(r[0] for r in res), 4),
                 "kernel_ms_after": round(sum(r[1] for r in res), 4)}

@@ -137,8 +137,10 @@ gpw_status gpw_pair_geometry(const gpw_grid_desc *grid, const gpw_pair_desc *pai
                              int *nspheres);
 
 /* Autotuner (no counterpart in the reference): times the tile kernels of
- * every class with 1, 2, 3, 4, 6, 8, 12, 16 (<= max_splits) workgroups per
- * tile on scratch inputs, best of `reps` runs after a warm-up, and keeps the
+ * every class with 1, 2, 3, 4, 6, 8, 12, 16, 24, ..., 128 (<= max_splits)
+ * workgroups per tile on scratch inputs, best of `reps` runs after a warm-up
+ * (max_splits <= 0: 8, raised on grids of few tiles so that up to 4096
+ * workgroups are tried; collocate partial grids stay within 2 GiB), and keeps the
  * fastest choice per (collocate / integrate without / with forces, class);
  * collocate also counts the reduction of its partial grids. Allocates a
  * scratch grid and the partial grids, so call it at set-up, not inside

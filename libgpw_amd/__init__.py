@@ -240,9 +240,10 @@ class TaskList:
         _check(lib().gpw_tasklist_timings(self._h, ms, len(PHASES), calls))
         return {p: (ms[i], calls[i]) for i, p in enumerate(PHASES)}
 
-    def autotune(self, max_splits: int = 8, reps: int = 3, stream=None) -> tuple:
-        """Measure and keep the best workgroups-per-tile per kernel class;
-        returns (ms before, ms after) of collocate + integrate kernels."""
+    def autotune(self, max_splits: int = 0, reps: int = 3, stream=None) -> tuple:
+        """Measure and keep the best workgroups-per-tile per kernel class
+        (max_splits 0: automatic range); returns (ms before, ms after) of the
+        collocate + integrate kernels."""
         before, after = C.c_double(0.0), C.c_double(0.0)
         _check(lib().gpw_tasklist_autotune(self._h, int(max_splits), int(reps), _stream(stream),
                                            C.byref(before), C.byref(after)))

@@ -197,15 +197,17 @@ __device__ void lattice_transform(const double *__restrict__ T, int lp, const do
 // Workgroups loop over pairs (pair0 + blockIdx.x, stride gridDim.x); eff, Ty
 // and cxyz live in LDS after alpha, or, when gws is given (blocks too large
 // for LDS), in the workgroup's slice gws + blockIdx.x * ws_stride.
-__global__ __launch_bounds__(64) void k_coef(const PairDev *__restrict__ pairs, int pair0,
+__global__ __launch_bounds__(64) void k_coef(const PairDev *__restrict__ pairs,
+                                             const int *__restrict__ order, int pair0,
                                              int npairs, int tau, const double *__restrict__ pab,
                                              double *__restrict__ coef,
                                              const double *__restrict__ mtrans,
                                              const long long *__restrict__ mt_off,
                                              double *__restrict__ gws, long long ws_stride) {
     extern __shared__ double sm_coef[];
-    for (int pi = pair0 + blockIdx.x; pi < pair0 + npairs; pi += gridDim.x) {
+    for (int ip = pair0 + blockIdx.x; ip < pair0 + npairs; ip += gridDim.x) {
     __syncthreads();  // LDS / workspace reuse across pairs
+    const int pi = order ? order[ip] : ip;
     const PairDev p = pairs[pi];
     const int lam = p.la_max + tau, lbm = p.lb_max + tau;
     const int lam_lo = max(p.la_min - tau, 0), lbm_lo = max(p.lb_min - tau, 0);
@@ -1047,7 +1049,8 @@ __global__ void k_sum_entries(const PairDev *__restrict__ pairs, int tau, int lp
 // Workgroups loop over pairs (pair0 + blockIdx.x, stride gridDim.x). heff and
 // eff live in LDS, or, when gws is given (blocks too large for LDS), in the
 // workgroup's slice gws + blockIdx.x * ws_stride of a global workspace.
-__global__ __launch_bounds__(64) void k_hab(const PairDev *__restrict__ pairs, int pair0, int npairs,
+__global__ __launch_bounds__(64) void k_hab(const PairDev *__restrict__ pairs,
+                                            const int *__restrict__ order, int pair0, int npairs,
                                             int tau, int with_forces, const double *__restrict__ Mbuf,
                                             int mstride, const double *__restrict__ pab,
                                             double *__restrict__ hab, double *__restrict__ forces,
@@ -1056,8 +1059,9 @@ __global__ __launch_bounds__(64) void k_hab(const PairDev *__restrict__ pairs, i
                                             double *__restrict__ gws, long long ws_stride) {
     extern __shared__ double sm_hab[];
     const int fx = with_forces ? 1 : 0;
-    for (int pi = pair0 + blockIdx.x; pi < pair0 + npairs; pi += gridDim.x) {
+    for (int ip = pair0 + blockIdx.x; ip < pair0 + npairs; ip += gridDim.x) {
     __syncthreads();  // LDS / workspace reuse across pairs
+    const int pi = order ? order[ip] : ip;
     const PairDev p = pairs[pi];
     const int lam = p.la_max + tau, lbm = p.lb_max + tau;          // local
     const int lamx = lam + fx, lbmx = lbm + fx;                    // extended
@@ -1483,10 +1487,12 @@ size_t coef_smem_bytes(int lam, int lbm, bool lds_blocks) {
 
 bool coef_needs_workspace(int lam, int lbm) { return coef_smem_bytes(lam, lbm, true) > HAB_LDS_MAX; }
 
-hipError_t launch_coef(const PairDev *pairs, int pair0, int npairs, int tau, int lam_max,
-                       int lbm_max, const double *pab, double *coef, const double *mtrans,
-                       const long long *mt_off, double *ws, long long ws_stride, int ws_blocks,
-                       hipStream_t st) {
+// pairs order[pair0 .. pair0 + npairs) (or pair0 .. when order is NULL), whose
+// la_max / lb_max are at most la_max / lb_max (LDS sizing)
+hipError_t launch_coef(const PairDev *pairs, const int *order, int pair0, int npairs, int tau,
+                       int lam_max, int lbm_max, const double *pab, double *coef,
+                       const double *mtrans, const long long *mt_off, double *ws,
+                       long long ws_stride, int ws_blocks, hipStream_t st) {
     if (npairs <= 0) return hipSuccess;
     const int lam = lam_max + tau, lbm = lbm_max + tau;
     const bool big = coef_needs_workspace(lam, lbm);
@@ -1494,8 +1500,8 @@ hipError_t launch_coef(const PairDev *pairs, int pair0, int npairs, int tau, int
         return hipErrorInvalidValue;
     const size_t smem = coef_smem_bytes(lam, lbm, !big);
     const int nblk = big ? std::min(npairs, ws_blocks) : npairs;
-    hipLaunchKernelGGL(k_coef, dim3(nblk), dim3(64), smem, st, pairs, pair0, npairs, tau, pab, coef,
-                       mtrans, mt_off, big ? ws : nullptr, big ? ws_stride : 0LL);
+    hipLaunchKernelGGL(k_coef, dim3(nblk), dim3(64), smem, st, pairs, order, pair0, npairs, tau, pab,
+                       coef, mtrans, mt_off, big ? ws : nullptr, big ? ws_stride : 0LL);
     return hipGetLastError();
 }
 
@@ -1516,10 +1522,11 @@ bool hab_needs_workspace(int lamx, int lbmx, int lam, int lbm) {
     return hab_smem_bytes(lamx, lbmx, lam, lbm, true) > HAB_LDS_MAX;
 }
 
-hipError_t launch_hab(const PairDev *pairs, int pair0, int npairs, int tau, int with_forces,
-                      int la_max, int lb_max, const double *Mbuf, int mstride, const double *pab,
-                      double *hab, double *forces, const double *mtrans, const long long *mt_off,
-                      double *ws, long long ws_stride, int ws_blocks, hipStream_t st) {
+hipError_t launch_hab(const PairDev *pairs, const int *order, int pair0, int npairs, int tau,
+                      int with_forces, int la_max, int lb_max, const double *Mbuf, int mstride,
+                      const double *pab, double *hab, double *forces, const double *mtrans,
+                      const long long *mt_off, double *ws, long long ws_stride, int ws_blocks,
+                      hipStream_t st) {
     if (npairs <= 0) return hipSuccess;
     const int fx = with_forces ? 1 : 0;
     const int lamx = la_max + tau + fx, lbmx = lb_max + tau + fx;
@@ -1529,9 +1536,9 @@ hipError_t launch_hab(const PairDev *pairs, int pair0, int npairs, int tau, int 
         return hipErrorInvalidValue;
     const size_t smem = hab_smem_bytes(lamx, lbmx, la_max + tau, lb_max + tau, !big);
     const int nblk = big ? std::min(npairs, ws_blocks) : npairs;
-    hipLaunchKernelGGL(k_hab, dim3(nblk), dim3(64), smem, st, pairs, pair0, npairs, tau, with_forces,
-                       Mbuf, mstride, pab, hab, forces, mtrans, mt_off, big ? ws : nullptr,
-                       big ? ws_stride : 0LL);
+    hipLaunchKernelGGL(k_hab, dim3(nblk), dim3(64), smem, st, pairs, order, pair0, npairs, tau,
+                       with_forces, Mbuf, mstride, pab, hab, forces, mtrans, mt_off,
+                       big ? ws : nullptr, big ? ws_stride : 0LL);
     return hipGetLastError();
 }
 

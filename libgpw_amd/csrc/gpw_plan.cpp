@@ -33,16 +33,18 @@ hipError_t launch_reduce_partials(const double *partial, int splits, long long n
 hipError_t launch_integrate_tiles(int cls, const TileGrid &tg, const EntryRec *recs,
                                   const int *tile_start, const double *grid, int lp_add,
                                   int ntiles, int splits, double *partial, hipStream_t st);
-hipError_t launch_coef(const PairDev *pairs, int pair0, int npairs, int tau, int lam_max,
-                       int lbm_max, const double *pab, double *coef, const double *mtrans,
-                       const long long *mt_off, double *ws, long long ws_stride, int ws_blocks,
-                       hipStream_t st);
+hipError_t launch_coef(const PairDev *pairs, const int *order, int pair0, int npairs, int tau,
+                       int lam_max, int lbm_max, const double *pab, double *coef,
+                       const double *mtrans, const long long *mt_off, double *ws,
+                       long long ws_stride, int ws_blocks, hipStream_t st);
+size_t coef_smem_bytes(int lam, int lbm, bool lds_blocks);
 size_t coef_ws_doubles(int lam, int lbm);
 bool coef_needs_workspace(int lam, int lbm);
-hipError_t launch_hab(const PairDev *pairs, int pair0, int npairs, int tau, int with_forces,
-                      int la_max, int lb_max, const double *Mbuf, int mstride, const double *pab,
-                      double *hab, double *forces, const double *mtrans, const long long *mt_off,
-                      double *ws, long long ws_stride, int ws_blocks, hipStream_t st);
+hipError_t launch_hab(const PairDev *pairs, const int *order, int pair0, int npairs, int tau,
+                      int with_forces, int la_max, int lb_max, const double *Mbuf, int mstride,
+                      const double *pab, double *hab, double *forces, const double *mtrans,
+                      const long long *mt_off, double *ws, long long ws_stride, int ws_blocks,
+                      hipStream_t st);
 size_t hab_ws_doubles(int lamx, int lbmx, int lam, int lbm);
 bool hab_needs_workspace(int lamx, int lbmx, int lam, int lbm);
 hipError_t launch_sum_entries(const PairDev *pairs, int npairs, int tau, int lp_add,
@@ -213,6 +215,13 @@ struct gpw_tasklist {
     double *d_mtrans = nullptr;      // general cells: lattice transform per lp
     long long *d_mt_off = nullptr;
     long long coef_total = 0;        // doubles of d_coef
+    // pairs grouped by block size for the per-pair transform kernels (one
+    // launch per group, LDS sized for the group): d_order[start .. start+count)
+    struct PairGroup {
+        int start, count, la_max, lb_max;
+    };
+    std::vector<PairGroup> groups;
+    int *d_order = nullptr;
     double *d_hab_ws = nullptr;      // k_coef / k_hab workspace for blocks too large for LDS
     long long hab_ws_stride = 0;
     int hab_ws_blocks = 0;
@@ -226,7 +235,7 @@ struct gpw_tasklist {
     ~gpw_tasklist() {
         void *ptrs[] = {d_pairs, d_subs, d_prof, d_tile_start, d_recs, d_pe_start,
                         d_ent_off0, d_ent_off2, d_coef, d_partial, d_cpartial, d_Mbuf,
-                        d_mtrans, d_mt_off, d_hab_ws};
+                        d_mtrans, d_mt_off, d_hab_ws, d_order};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
@@ -657,9 +666,44 @@ static gpw_status plan_tasklist(gpw_tasklist *tl, const std::vector<gpw_pair_des
     return GPW_OK;
 }
 
+// Groups of pairs by the LDS size class of their transform (powers of two
+// from 8 KiB), so that small blocks do not run at the occupancy of the
+// largest one.
+static void group_pairs(gpw_tasklist *tl, std::vector<int> &order) {
+    const int t = tl->compute_tau;
+    std::vector<std::vector<int>> byc;
+    std::vector<int> cls(tl->npairs);
+    for (int i = 0; i < tl->npairs; i++) {
+        const PairDev &p = tl->pairs_h[i];
+        size_t b = coef_smem_bytes(p.la_max + t, p.lb_max + t, true);
+        int c = 0;
+        while (c < 12 && (size_t(8192) << c) < b) c++;
+        cls[i] = c;
+        if ((int)byc.size() <= c) byc.resize(c + 1);
+        byc[c].push_back(i);
+    }
+    order.clear();
+    tl->groups.clear();
+    for (auto &v : byc) {
+        if (v.empty()) continue;
+        gpw_tasklist::PairGroup g{(int)order.size(), (int)v.size(), 0, 0};
+        for (int i : v) {
+            order.push_back(i);
+            g.la_max = std::max(g.la_max, tl->pairs_h[i].la_max);
+            g.lb_max = std::max(g.lb_max, tl->pairs_h[i].lb_max);
+        }
+        tl->groups.push_back(g);
+    }
+}
+
 static gpw_status upload_tasklist(gpw_tasklist *tl, const HostPlan &hp) {
     gpw_status st;
     if ((st = upload(&tl->d_pairs, tl->pairs_h))) return st;
+    {
+        std::vector<int> order;
+        group_pairs(tl, order);
+        if ((st = upload(&tl->d_order, order))) return st;
+    }
     if ((st = upload(&tl->d_subs, tl->subs_h))) return st;
     if ((st = upload(&tl->d_prof, hp.prof_flat))) return st;
     if ((st = upload(&tl->d_tile_start, hp.tile_start))) return st;
@@ -926,14 +970,18 @@ gpw_status gpw_tasklist_autotune(gpw_tasklist *tl, int max_splits, int reps, voi
                                  double *ms_before, double *ms_after) {
     if (!tl) return fail(GPW_ERR_ARG, "NULL argument");
     if (tl->npairs == 0) return GPW_OK;
-    max_splits = std::min(std::max(max_splits, 1), 16);
+    const long long ng = (long long)tl->tg.nx * tl->tg.ny * tl->tg.nz;
+    if (max_splits <= 0)  // automatic: 8, more on grids of few tiles (>= 4096 workgroups)
+        max_splits = std::max(8, std::min(128, 4096 / std::max(tl->ntiles, 1)));
+    max_splits = std::min(max_splits, 128);
+    // partial grids of collocate stay within 2 GiB
+    max_splits = (int)std::max<long long>(1, std::min<long long>(max_splits, (2LL << 30) / (8 * ng)));
     reps = std::min(std::max(reps, 1), 20);
     hipStream_t st = (hipStream_t)stream;
     std::vector<int> cand;
-    for (int s : {1, 2, 3, 4, 6, 8, 12, 16})
+    for (int s : {1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128})
         if (s <= max_splits) cand.push_back(s);
     const int cmax = cand.back();
-    const long long ng = (long long)tl->tg.nx * tl->tg.ny * tl->tg.nz;
     const int nt1 = tl->ntiles + 1;
     double *grid = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1086,7 +1134,14 @@ gpw_status gpw_tasklist_autotune(gpw_tasklist *tl, int max_splits, int reps, voi
 
 static gpw_status run_coef(gpw_tasklist *tl, const double *pab, int pair0, int n, hipStream_t st) {
     if (n <= 0) return GPW_OK;
-    HIP_TRY(launch_coef(tl->d_pairs, pair0, n, tl->compute_tau, tl->la_max, tl->lb_max, pab,
+    if (pair0 == 0 && n == tl->npairs) {  // the whole list: one launch per size group
+        for (const auto &g : tl->groups)
+            HIP_TRY(launch_coef(tl->d_pairs, tl->d_order, g.start, g.count, tl->compute_tau, g.la_max,
+                                g.lb_max, pab, tl->d_coef, tl->d_mtrans, tl->d_mt_off, tl->d_hab_ws,
+                                tl->hab_ws_stride, tl->hab_ws_blocks, st));
+        return GPW_OK;
+    }
+    HIP_TRY(launch_coef(tl->d_pairs, nullptr, pair0, n, tl->compute_tau, tl->la_max, tl->lb_max, pab,
                         tl->d_coef, tl->d_mtrans, tl->d_mt_off, tl->d_hab_ws, tl->hab_ws_stride,
                         tl->hab_ws_blocks, st));
     return GPW_OK;
@@ -1141,9 +1196,11 @@ gpw_status gpw_tasklist_integrate(gpw_tasklist *tl, const double *grid, const do
     }
     {
         PhaseTimer ph(tl, st, PH_HAB);
-        HIP_TRY(launch_hab(tl->d_pairs, 0, tl->npairs, tl->compute_tau, wf, tl->la_max, tl->lb_max,
-                           tl->d_Mbuf, tl->mstride, pab, hab, forces, tl->d_mtrans, tl->d_mt_off,
-                           tl->d_hab_ws, tl->hab_ws_stride, tl->hab_ws_blocks, st));
+        for (const auto &g : tl->groups)
+            HIP_TRY(launch_hab(tl->d_pairs, tl->d_order, g.start, g.count, tl->compute_tau, wf,
+                               g.la_max, g.lb_max, tl->d_Mbuf, tl->mstride, pab, hab, forces,
+                               tl->d_mtrans, tl->d_mt_off, tl->d_hab_ws, tl->hab_ws_stride,
+                               tl->hab_ws_blocks, st));
     }
     return GPW_OK;
 }
@@ -1206,7 +1263,7 @@ gpw_status gpw_tasklist_integrate_plain(gpw_tasklist *tl, const double *grid, co
                                          sd.ub[1] - sd.lb[1] + 1, 2 * wf, tl->d_prof, grid,
                                          tl->mstride, tl->d_Mbuf, st));
         }
-        HIP_TRY(launch_hab(tl->d_pairs, p, 1, tl->compute_tau, wf, tl->pairs_h[p].la_max,
+        HIP_TRY(launch_hab(tl->d_pairs, nullptr, p, 1, tl->compute_tau, wf, tl->pairs_h[p].la_max,
                            tl->pairs_h[p].lb_max, tl->d_Mbuf, tl->mstride, pab, hab, forces,
                            tl->d_mtrans, tl->d_mt_off, tl->d_hab_ws, tl->hab_ws_stride,
                            tl->hab_ws_blocks, st));

@@ -35,7 +35,7 @@ class MultiGrid:
                                 "tl": TaskList(g, sub, compute_tau=compute_tau, splits=splits)})
         self.npairs = int(sum(len(lv["index"]) for lv in self.levels))
 
-    def autotune(self, max_splits: int = 8, reps: int = 3):
+    def autotune(self, max_splits: int = 0, reps: int = 3):
         return [lv["tl"].autotune(max_splits=max_splits, reps=reps) for lv in self.levels]
 
     def grids(self, device="cuda"):
