@@ -1049,8 +1049,13 @@ __global__ void k_sum_entries(const PairDev *__restrict__ pairs, int tau, int lp
 // Workgroups loop over pairs (pair0 + blockIdx.x, stride gridDim.x). heff and
 // eff live in LDS, or, when gws is given (blocks too large for LDS), in the
 // workgroup's slice gws + blockIdx.x * ws_stride of a global workspace.
+// Work unit: a segment of pairs that share one hab block (segs: CSR over
+// positions of `order`, segments seg0 .. seg0 + nseg); one workgroup walks a
+// segment's pairs in order, so the += into the shared block is race-free and
+// deterministic. segs NULL: one pair per segment, positions seg0 .. seg0 + nseg.
 __global__ __launch_bounds__(64) void k_hab(const PairDev *__restrict__ pairs,
-                                            const int *__restrict__ order, int pair0, int npairs,
+                                            const int *__restrict__ order,
+                                            const int *__restrict__ segs, int seg0, int nseg,
                                             int tau, int with_forces, const double *__restrict__ Mbuf,
                                             int mstride, const double *__restrict__ pab,
                                             double *__restrict__ hab, double *__restrict__ forces,
@@ -1059,7 +1064,9 @@ __global__ __launch_bounds__(64) void k_hab(const PairDev *__restrict__ pairs,
                                             double *__restrict__ gws, long long ws_stride) {
     extern __shared__ double sm_hab[];
     const int fx = with_forces ? 1 : 0;
-    for (int ip = pair0 + blockIdx.x; ip < pair0 + npairs; ip += gridDim.x) {
+    for (int sg = seg0 + blockIdx.x; sg < seg0 + nseg; sg += gridDim.x) {
+    const int ip_end = segs ? segs[sg + 1] : sg + 1;
+    for (int ip = segs ? segs[sg] : sg; ip < ip_end; ip++) {
     __syncthreads();  // LDS / workspace reuse across pairs
     const int pi = order ? order[ip] : ip;
     const PairDev p = pairs[pi];
@@ -1200,6 +1207,7 @@ __global__ __launch_bounds__(64) void k_hab(const PairDev *__restrict__ pairs,
         double s = 0.0;
         for (int i = 0; i < 64; i++) s += red[threadIdx.x * 64 + i];
         forces[(long long)pi * 6 + threadIdx.x] += s;
+    }
     }
     }
 }
@@ -1522,12 +1530,13 @@ bool hab_needs_workspace(int lamx, int lbmx, int lam, int lbm) {
     return hab_smem_bytes(lamx, lbmx, lam, lbm, true) > HAB_LDS_MAX;
 }
 
-hipError_t launch_hab(const PairDev *pairs, const int *order, int pair0, int npairs, int tau,
-                      int with_forces, int la_max, int lb_max, const double *Mbuf, int mstride,
-                      const double *pab, double *hab, double *forces, const double *mtrans,
-                      const long long *mt_off, double *ws, long long ws_stride, int ws_blocks,
-                      hipStream_t st) {
-    if (npairs <= 0) return hipSuccess;
+hipError_t launch_hab(const PairDev *pairs, const int *order, const int *segs, int seg0, int nseg,
+                      int tau, int with_forces, int la_max, int lb_max, const double *Mbuf,
+                      int mstride, const double *pab, double *hab, double *forces,
+                      const double *mtrans, const long long *mt_off, double *ws,
+                      long long ws_stride, int ws_blocks, hipStream_t st) {
+    if (nseg <= 0) return hipSuccess;
+    const int npairs = nseg;
     const int fx = with_forces ? 1 : 0;
     const int lamx = la_max + tau + fx, lbmx = lb_max + tau + fx;
     const bool big = hab_needs_workspace(lamx, lbmx, la_max + tau, lb_max + tau);
@@ -1536,7 +1545,7 @@ hipError_t launch_hab(const PairDev *pairs, const int *order, int pair0, int npa
         return hipErrorInvalidValue;
     const size_t smem = hab_smem_bytes(lamx, lbmx, la_max + tau, lb_max + tau, !big);
     const int nblk = big ? std::min(npairs, ws_blocks) : npairs;
-    hipLaunchKernelGGL(k_hab, dim3(nblk), dim3(64), smem, st, pairs, order, pair0, npairs, tau,
+    hipLaunchKernelGGL(k_hab, dim3(nblk), dim3(64), smem, st, pairs, order, segs, seg0, nseg, tau,
                        with_forces, Mbuf, mstride, pab, hab, forces, mtrans, mt_off,
                        big ? ws : nullptr, big ? ws_stride : 0LL);
     return hipGetLastError();

@@ -40,11 +40,11 @@ hipError_t launch_coef(const PairDev *pairs, const int *order, int pair0, int np
 size_t coef_smem_bytes(int lam, int lbm, bool lds_blocks);
 size_t coef_ws_doubles(int lam, int lbm);
 bool coef_needs_workspace(int lam, int lbm);
-hipError_t launch_hab(const PairDev *pairs, const int *order, int pair0, int npairs, int tau,
-                      int with_forces, int la_max, int lb_max, const double *Mbuf, int mstride,
-                      const double *pab, double *hab, double *forces, const double *mtrans,
-                      const long long *mt_off, double *ws, long long ws_stride, int ws_blocks,
-                      hipStream_t st);
+hipError_t launch_hab(const PairDev *pairs, const int *order, const int *segs, int seg0, int nseg,
+                      int tau, int with_forces, int la_max, int lb_max, const double *Mbuf,
+                      int mstride, const double *pab, double *hab, double *forces,
+                      const double *mtrans, const long long *mt_off, double *ws,
+                      long long ws_stride, int ws_blocks, hipStream_t st);
 size_t hab_ws_doubles(int lamx, int lbmx, int lam, int lbm);
 bool hab_needs_workspace(int lamx, int lbmx, int lam, int lbm);
 hipError_t launch_sum_entries(const PairDev *pairs, int npairs, int tau, int lp_add,
@@ -217,11 +217,15 @@ struct gpw_tasklist {
     long long coef_total = 0;        // doubles of d_coef
     // pairs grouped by block size for the per-pair transform kernels (one
     // launch per group, LDS sized for the group): d_order[start .. start+count)
+    // within a group, pairs sharing a coefficient block (contracted shell
+    // pairs: one block, one primitive pair per exponent pair) are adjacent and
+    // form one segment, walked by one workgroup of k_hab
     struct PairGroup {
-        int start, count, la_max, lb_max;
+        int start, count, la_max, lb_max, seg0, nseg;
     };
     std::vector<PairGroup> groups;
     int *d_order = nullptr;
+    int *d_segs = nullptr;
     double *d_hab_ws = nullptr;      // k_coef / k_hab workspace for blocks too large for LDS
     long long hab_ws_stride = 0;
     int hab_ws_blocks = 0;
@@ -235,7 +239,7 @@ struct gpw_tasklist {
     ~gpw_tasklist() {
         void *ptrs[] = {d_pairs, d_subs, d_prof, d_tile_start, d_recs, d_pe_start,
                         d_ent_off0, d_ent_off2, d_coef, d_partial, d_cpartial, d_Mbuf,
-                        d_mtrans, d_mt_off, d_hab_ws, d_order};
+                        d_mtrans, d_mt_off, d_hab_ws, d_order, d_segs};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
@@ -668,42 +672,75 @@ static gpw_status plan_tasklist(gpw_tasklist *tl, const std::vector<gpw_pair_des
 
 // Groups of pairs by the LDS size class of their transform (powers of two
 // from 8 KiB), so that small blocks do not run at the occupancy of the
-// largest one.
-static void group_pairs(gpw_tasklist *tl, std::vector<int> &order) {
+// largest one; inside a group pairs are sorted by block offset and pairs of
+// one block form a segment. Blocks may be shared only whole (same offset,
+// same la_max and lb_max); any other overlap is an argument error.
+static gpw_status group_pairs(gpw_tasklist *tl, std::vector<int> &order, std::vector<int> &segs) {
     const int t = tl->compute_tau;
+    // overlap check over all blocks
+    {
+        std::vector<int> idx(tl->npairs);
+        for (int i = 0; i < tl->npairs; i++) idx[i] = i;
+        std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
+            return tl->pairs_h[a].block_off < tl->pairs_h[b].block_off;
+        });
+        long long end = -1;
+        const PairDev *prev = nullptr;
+        for (int i : idx) {
+            const PairDev &p = tl->pairs_h[i];
+            const long long sz = (long long)ncoset(p.lb_max) * ncoset(p.la_max);
+            if (prev && p.block_off == prev->block_off) {
+                if (p.la_max != prev->la_max || p.lb_max != prev->lb_max)
+                    return fail(GPW_ERR_ARG, "pairs sharing a block need equal la_max and lb_max");
+                continue;
+            }
+            if (p.block_off < end) return fail(GPW_ERR_ARG, "coefficient blocks overlap");
+            end = p.block_off + sz;
+            prev = &p;
+        }
+    }
     std::vector<std::vector<int>> byc;
-    std::vector<int> cls(tl->npairs);
     for (int i = 0; i < tl->npairs; i++) {
         const PairDev &p = tl->pairs_h[i];
         size_t b = coef_smem_bytes(p.la_max + t, p.lb_max + t, true);
         int c = 0;
         while (c < 12 && (size_t(8192) << c) < b) c++;
-        cls[i] = c;
         if ((int)byc.size() <= c) byc.resize(c + 1);
         byc[c].push_back(i);
     }
     order.clear();
+    segs.clear();
     tl->groups.clear();
     for (auto &v : byc) {
         if (v.empty()) continue;
-        gpw_tasklist::PairGroup g{(int)order.size(), (int)v.size(), 0, 0};
+        std::stable_sort(v.begin(), v.end(), [&](int a, int b) {
+            return tl->pairs_h[a].block_off < tl->pairs_h[b].block_off;
+        });
+        gpw_tasklist::PairGroup g{(int)order.size(), (int)v.size(), 0, 0, (int)segs.size(), 0};
+        long long last = -1;
         for (int i : v) {
+            if (tl->pairs_h[i].block_off != last) {
+                segs.push_back((int)order.size());
+                g.nseg++;
+                last = tl->pairs_h[i].block_off;
+            }
             order.push_back(i);
             g.la_max = std::max(g.la_max, tl->pairs_h[i].la_max);
             g.lb_max = std::max(g.lb_max, tl->pairs_h[i].lb_max);
         }
         tl->groups.push_back(g);
     }
+    segs.push_back((int)order.size());
+    return GPW_OK;
 }
 
 static gpw_status upload_tasklist(gpw_tasklist *tl, const HostPlan &hp) {
     gpw_status st;
+    std::vector<int> order, segs;
+    if ((st = group_pairs(tl, order, segs))) return st;  // argument checks before device work
     if ((st = upload(&tl->d_pairs, tl->pairs_h))) return st;
-    {
-        std::vector<int> order;
-        group_pairs(tl, order);
-        if ((st = upload(&tl->d_order, order))) return st;
-    }
+    if ((st = upload(&tl->d_order, order))) return st;
+    if ((st = upload(&tl->d_segs, segs))) return st;
     if ((st = upload(&tl->d_subs, tl->subs_h))) return st;
     if ((st = upload(&tl->d_prof, hp.prof_flat))) return st;
     if ((st = upload(&tl->d_tile_start, hp.tile_start))) return st;
@@ -1197,8 +1234,8 @@ gpw_status gpw_tasklist_integrate(gpw_tasklist *tl, const double *grid, const do
     {
         PhaseTimer ph(tl, st, PH_HAB);
         for (const auto &g : tl->groups)
-            HIP_TRY(launch_hab(tl->d_pairs, tl->d_order, g.start, g.count, tl->compute_tau, wf,
-                               g.la_max, g.lb_max, tl->d_Mbuf, tl->mstride, pab, hab, forces,
+            HIP_TRY(launch_hab(tl->d_pairs, tl->d_order, tl->d_segs, g.seg0, g.nseg, tl->compute_tau,
+                               wf, g.la_max, g.lb_max, tl->d_Mbuf, tl->mstride, pab, hab, forces,
                                tl->d_mtrans, tl->d_mt_off, tl->d_hab_ws, tl->hab_ws_stride,
                                tl->hab_ws_blocks, st));
     }
@@ -1263,7 +1300,7 @@ gpw_status gpw_tasklist_integrate_plain(gpw_tasklist *tl, const double *grid, co
                                          sd.ub[1] - sd.lb[1] + 1, 2 * wf, tl->d_prof, grid,
                                          tl->mstride, tl->d_Mbuf, st));
         }
-        HIP_TRY(launch_hab(tl->d_pairs, nullptr, p, 1, tl->compute_tau, wf, tl->pairs_h[p].la_max,
+        HIP_TRY(launch_hab(tl->d_pairs, nullptr, nullptr, p, 1, tl->compute_tau, wf, tl->pairs_h[p].la_max,
                            tl->pairs_h[p].lb_max, tl->d_Mbuf, tl->mstride, pab, hab, forces,
                            tl->d_mtrans, tl->d_mt_off, tl->d_hab_ws, tl->hab_ws_stride,
                            tl->hab_ws_blocks, st));

@@ -188,3 +188,26 @@ def subset(cfg: dict, idx) -> dict:
     out = dict(cfg)
     out.update(pairs=q, pab=pab, nblock=nblock, name=cfg["name"] + f"-subset{len(idx)}")
     return out
+
+
+def contracted_shells(n: int, seed: int = 77, cell=(12.0, 12.0, 12.0), lmax: int = 2,
+                      nprim=(3, 2)):
+    """n random contracted shell pairs (for tests of the contracted front end):
+    la, lb in 0..lmax, nprim primitives per side with exponents log-uniform in
+    [0.3, 30] and contraction coefficients uniform [0.2, 1]; returns (shells,
+    cell matrix)."""
+    rng = np.random.default_rng(seed)
+    hmat = np.diag(np.asarray(cell, dtype=np.float64))
+    shells = []
+    for _ in range(n):
+        na, nb = int(nprim[0]), int(nprim[1])
+        shells.append({
+            "ra": rng.uniform(0.0, 1.0, 3) @ hmat,
+            "rb": rng.uniform(0.0, 1.0, 3) @ hmat,
+            "la": int(rng.integers(0, lmax + 1)), "lb": int(rng.integers(0, lmax + 1)),
+            "exps_a": np.exp(rng.uniform(math.log(0.3), math.log(30.0), na)),
+            "coefs_a": rng.uniform(0.2, 1.0, na),
+            "exps_b": np.exp(rng.uniform(math.log(0.3), math.log(30.0), nb)),
+            "coefs_b": rng.uniform(0.2, 1.0, nb),
+        })
+    return shells, hmat

@@ -475,3 +475,57 @@ def test_multigrid_graph_matches_oracle(gpw, oracle, torch_cuda):
     assert rel_err(hab.cpu().numpy(), hab_ref) < TOL
     assert rel_err(mg.forces_by_pair(forces), f_ref) < TOL
     mg.close()
+
+
+@pytest.mark.parametrize("plain", [False, True])
+def test_contracted_shell_pairs(gpw, oracle, torch_cuda, plain):
+    """Contracted front end: primitive pairs share their shell pair's block
+    with rscale = c_a c_b. Against the oracle on the expanded form (own block
+    per primitive pair, coefficients c_a c_b P, rscale 1): grid, contracted
+    hab = sum over primitives of c_a c_b hab_prim, and per-primitive forces."""
+    from libgpw_amd import workloads as Wk
+    from libgpw_amd.contracted import ShellPairs
+    torch = torch_cuda
+    shells, cell = Wk.contracted_shells(60, lmax=2)
+    sp = ShellPairs(shells, cell)
+    say("contracted", sp.nshells, "shell pairs", sp.nprimitive, "primitive pairs", "plain" if plain else "")
+    npts = (48, 48, 48)
+    dh = cell / np.asarray(npts)[:, None]
+    rng = np.random.default_rng(5)
+    P = rng.uniform(-1.0, 1.0, sp.nblock)
+    W = rng.uniform(-1.0, 1.0, (npts[2], npts[1], npts[0]))
+    # expanded form for the oracle
+    ex = {k: np.array(v, copy=True) for k, v in sp.pairs.items()}
+    sizes = np.array([(la + 1) * (la + 2) * (la + 3) // 6 * ((lb + 1) * (lb + 2) * (lb + 3) // 6)
+                      for la, lb in zip(ex["la_max"], ex["lb_max"])])
+    ex_off = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    pab_ex = np.zeros(int(sizes.sum()))
+    for i in range(sp.nprimitive):
+        bo = sp.pairs["block_offset"][i]
+        pab_ex[ex_off[i]:ex_off[i] + sizes[i]] = sp.pairs["rscale"][i] * P[bo:bo + sizes[i]]
+    ex["rscale"] = np.ones(sp.nprimitive)
+    ex["block_offset"] = ex_off
+    og = oracle.make_grid(npts, dh)
+    op = oracle.make_pairs(ex)
+    grid_ref = oracle.batch_collocate(og, op, pab_ex)
+    hab_ex, f_ref = oracle.batch_integrate(og, op, W, int(sizes.sum()), pab=pab_ex, forces=True)
+    hab_ref = np.zeros(sp.nblock)
+    for i in range(sp.nprimitive):
+        bo = sp.pairs["block_offset"][i]
+        hab_ref[bo:bo + sizes[i]] += sp.pairs["rscale"][i] * hab_ex[ex_off[i]:ex_off[i] + sizes[i]]
+    # product
+    tl = sp.tasklist(gpw.grid_desc(npts, dh), splits=2)
+    grid = torch.zeros((npts[2], npts[1], npts[0]), dtype=torch.float64, device="cuda")
+    hab = torch.zeros(sp.nblock, dtype=torch.float64, device="cuda")
+    forces = torch.zeros((sp.nprimitive, 6), dtype=torch.float64, device="cuda")
+    Pd = dev(torch, P)
+    tl.collocate(Pd, grid, plain=plain)
+    tl.integrate(dev(torch, W), hab, pab=Pd, forces=forces, plain=plain)
+    torch.cuda.synchronize()
+    assert npts_set(grid.cpu().numpy()) == npts_set(grid_ref)
+    assert rel_err(grid.cpu().numpy(), grid_ref) < TOL
+    assert rel_err(hab.cpu().numpy(), hab_ref) < TOL
+    assert rel_err(forces.cpu().numpy().reshape(-1, 2, 3), f_ref) < TOL
+    fs = sp.forces_by_shell(forces)
+    assert fs.shape == (sp.nshells, 2, 3)
+    tl.close()

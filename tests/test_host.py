@@ -181,6 +181,46 @@ def test_dropin_argument_errors(gpw, oracle):
         assert e.value.code == code, (e.value, code)
 
 
+def test_tasklist_rejects_overlapping_blocks(gpw):
+    # blocks may be shared whole (contracted shell pairs), never partially
+    from libgpw_amd import workloads as W
+    cfg = W.config(2, npairs=3)
+    g = gpw.grid_desc(cfg["npts"], cfg["dh"])
+    p = {k: np.array(v, copy=True) for k, v in cfg["pairs"].items()}
+    p["la_max"][:] = p["la_min"][:] = 1
+    p["lb_max"][:] = p["lb_min"][:] = 1
+    p["block_offset"][:] = [0, 5, 20]          # 9-entry blocks: 0..8 and 5..13 overlap
+    with pytest.raises(gpw.GpwError) as e:
+        gpw.TaskList(g, p)
+    assert e.value.code == 1 and "overlap" in str(e.value)
+    p["block_offset"][:] = [0, 0, 20]
+    p["la_max"][1] = p["la_min"][1] = 2        # shared block, different shell
+    with pytest.raises(gpw.GpwError) as e:
+        gpw.TaskList(g, p)
+    assert e.value.code == 1 and "sharing" in str(e.value)
+
+
+def test_contracted_expansion(gpw):
+    # primitive pairs of a shell pair: all exponent combinations, rscale =
+    # c_a c_b, one shared block, minimum-image rab
+    from libgpw_amd import workloads as W
+    from libgpw_amd.contracted import ShellPairs
+    shells, cell = W.contracted_shells(4, nprim=(3, 2))
+    sp = ShellPairs(shells, cell)
+    assert sp.nprimitive == 4 * 6
+    for s, sh in enumerate(shells):
+        idx = np.flatnonzero(sp.shell_of == s)
+        assert len(set(sp.pairs["block_offset"][idx])) == 1
+        got = sorted(zip(sp.pairs["zeta"][idx], sp.pairs["zetb"][idx], sp.pairs["rscale"][idx]))
+        want = sorted((a, b, ca * cb) for a, ca in zip(sh["exps_a"], sh["coefs_a"])
+                      for b, cb in zip(sh["exps_b"], sh["coefs_b"]))
+        assert np.allclose(got, want, rtol=1e-15, atol=0)
+        rab = sp.pairs["rab"][idx[0]]
+        assert np.all(np.abs(rab) <= np.diag(cell) / 2 + 1e-12)
+        d = sh["rb"] - sh["ra"] - rab
+        assert np.allclose(d / np.diag(cell), np.round(d / np.diag(cell)), atol=1e-12)
+
+
 def test_tasklist_argument_errors(gpw):
     from libgpw_amd import workloads as W
     cfg = W.config(2, npairs=4)
