@@ -295,6 +295,9 @@ struct TileCfg {
     static constexpr int C2M = LM * (LM + 1) / 2;       // (lyp, lxp) terms
     static constexpr int C3M = LM * (LM + 1) * (LM + 2) / 6;
     static constexpr int GS = XB * LM + 4;              // doubles per x block (+32 B spreads banks)
+    // y-table row stride, odd: the 4 rows 4 * yb + r a wave reads at once
+    // (yb = 0..3) fall into distinct banks
+    static constexpr int YS = LM | 1;
     static constexpr int XS = NXG * GS;
     static constexpr int NPF = (Q * C3M + NROW - 1) / NROW;  // coef prefetch registers
     __device__ static int xat(int x) { return (x / XB) * GS + (x % XB) * LM; }
@@ -386,7 +389,7 @@ template <int L, int LM, bool WITH_CXY, bool GEN, int Q>
 __device__ __forceinline__ void tile_phase1(const EntryRec &R, int q, int lane, int x0, int y0,
                                             int z0, const TileGrid &tg,
                                             const double *__restrict__ coef, double *X,
-                                            double (*Y)[LM], double (*Z)[LM],
+                                            double (*Y)[TileCfg<LM>::YS], double (*Z)[LM],
                                             double (*cxy)[TileCfg<LM>::C2M], int *xi, int *T,
                                             int *bmax, int *bmin, GenRows<Q, GEN> &gr) {
     constexpr int lp = L - 1;
@@ -556,7 +559,7 @@ struct CollocSmem {
     EntryRec rec[3][C::Q];
     double coef[2][C::Q][C::C3M];
     double X[C::Q][C::XS];
-    double Y[C::Q][TY][LM];
+    double Y[C::Q][TY][C::YS];
     double cxy[C::Q][TZ][C::C2M];
     int xi[C::Q][TX];
     int T[C::Q][NROW];
@@ -792,11 +795,13 @@ __device__ __forceinline__ double stride4_sum(double v) {
 }
 
 // Per-wave buffer of the integrate transposition: the x-stage results of the
-// wave's 64 rows (4 z-planes x 16 y), row (zl, y) at zl * ZS + y * LM; the
-// plane stride ZS is 64 B modulo 256 B so the 4 planes read disjoint banks.
+// wave's 64 rows (4 z-planes x 16 y), row (zl, y) at zl * ZS + y * RS. A store
+// writes rows 4 * yb + r (yb, zl = 0..3) of one r at once; RS = 1 (mod 4) and
+// ZS = 1 (mod 16) put those 16 rows on 16 distinct 8-byte bank pairs.
 template <int LM>
 struct RowT1 {
-    static constexpr int ZS = TY * LM + ((LM % 2) ? 24 : 8);
+    static constexpr int RS = LM + ((1 - LM) % 4 + 4) % 4;
+    static constexpr int ZS = TY * RS + 1;
     static constexpr int SZ = 4 * ZS;
 };
 
@@ -806,7 +811,7 @@ struct IntegSmem {
     GenRows<C::Q, GEN> gr;
     EntryRec rec[2][C::Q];
     double X[C::Q][C::XS];
-    double Y[C::Q][TY][LM];
+    double Y[C::Q][TY][C::YS];
     double Z[C::Q][TZ][LM];
     double uz[C::Q][TZ][C::C2M];  // per-plane (lyp, lxp) moments
     double t1[NROW / 64][RowT1<LM>::SZ];
@@ -877,11 +882,12 @@ __device__ __forceinline__ void integ_block(IntegSmem<LM, GEN> &sm, int q, int x
         for (int l = 0; l < L; l++) tx[r][l] = quad_sum(tx[r][l]);
         if (xg == r) {
 #pragma unroll
-            for (int l = 0; l < L; l++) t1[(yb * RB + r) * LM + l] = tx[r][l];
+            for (int l = 0; l < L; l++) t1[(yb * RB + r) * RowT1<LM>::RS + l] = tx[r][l];
         }
     }
     __builtin_amdgcn_wave_barrier();
     const double *Yq = &sm.Y[q][0][0];
+    constexpr int YS = TileCfg<LM>::YS;
     for (int c = cq; c < N2; c += 16) {
         int lyp = 0, lxp = c;
         while (lxp > lp - lyp) {
@@ -890,7 +896,7 @@ __device__ __forceinline__ void integ_block(IntegSmem<LM, GEN> &sm, int q, int x
         }
         double a = 0.0;
 #pragma unroll
-        for (int y = 0; y < TY; y++) a += Yq[y * LM + lyp] * t1[y * LM + lxp];
+        for (int y = 0; y < TY; y++) a += Yq[y * YS + lyp] * t1[y * RowT1<LM>::RS + lxp];
         uzp[c] = a;
     }
     __builtin_amdgcn_wave_barrier();
