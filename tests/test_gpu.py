@@ -529,3 +529,65 @@ def test_contracted_shell_pairs(gpw, oracle, torch_cuda, plain):
     fs = sp.forces_by_shell(forces)
     assert fs.shape == (sp.nshells, 2, 3)
     tl.close()
+
+
+def test_tasklist_windowed_nonperiodic(gpw, oracle, torch_cuda):
+    """Task list on a local window with non-periodic x and z (lb_grid, ngrid <
+    npts), several pairs, l <= 2, collocate + integrate + forces, against the
+    oracle's single-task path pair by pair."""
+    torch = torch_cuda
+    rng = np.random.default_rng(17)
+    npts, ngrid, lb_grid, per = (40, 36, 44), (34, 36, 32), (-3, 0, -6), (0, 1, 0)
+    cell = np.array([10.0, 9.0, 11.0])
+    dh = np.diag(cell / np.asarray(npts))
+    n = 14
+    la = rng.integers(0, 3, n); lb = rng.integers(0, 3, n)
+    zeta = rng.uniform(2.5, 5.0, n); zetb = rng.uniform(2.5, 5.0, n)
+    # centres inside the window, away from its non-periodic edges
+    lo = (12 - np.asarray(lb_grid)) * np.diag(dh)  # window = indices -lb_grid .. -lb_grid + ngrid - 1
+    hi = (np.asarray(ngrid) - 12 - np.asarray(lb_grid)) * np.diag(dh)
+    ra = rng.uniform(lo, hi, (n, 3))
+    rab = rng.normal(0.0, 0.3, (n, 3))
+    radius = np.sqrt(np.log(1e10) / (zeta + zetb))
+    sizes = np.array([gpw.ncoset(int(b)) * gpw.ncoset(int(a)) for a, b in zip(la, lb)])
+    off = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    pab = rng.uniform(-1.0, 1.0, int(sizes.sum()))
+    pairs = {"la_max": la.astype(np.int32), "la_min": la.astype(np.int32),
+             "lb_max": lb.astype(np.int32), "lb_min": lb.astype(np.int32),
+             "zeta": zeta, "zetb": zetb, "rscale": np.ones(n), "ra": ra, "rab": rab,
+             "radius": radius, "block_offset": off}
+    W = rng.uniform(-1.0, 1.0, (ngrid[2], ngrid[1], ngrid[0]))
+    grid_ref = np.zeros_like(W)
+    hab_ref = np.zeros(int(sizes.sum()))
+    f_ref = np.zeros((n, 2, 3))
+    for i in range(n):
+        lbc, ubc, sb = oracle.sphere_bounds(radius[i], np.diag(dh))
+        t = {"compute_tau": False, "use_ortho": True, "la_max": int(la[i]), "la_min": int(la[i]),
+             "lb_max": int(lb[i]), "lb_min": int(lb[i]), "zeta": zeta[i], "zetb": zetb[i],
+             "rscale": 1.0, "dh": dh, "dh_inv": np.linalg.inv(dh), "ra": ra[i], "rab": rab[i],
+             "npts": list(npts), "ngrid": list(ngrid), "lb_grid": list(lb_grid),
+             "periodic": list(per), "lmax": 4, "radius": radius[i], "lb_cube": lbc,
+             "ub_cube": ubc, "sphere_bounds": sb, "nspheres": len(sb),
+             "maxco": gpw.ncoset(int(la[i])), "pab_ld": gpw.ncoset(int(la[i])),
+             "pab_rows": gpw.ncoset(int(lb[i])), "o1": 0, "o2": 0}
+        blk = pab[off[i]:off[i] + sizes[i]]
+        oracle.collocate(t, blk, grid_ref)
+        h, f = oracle.integrate(t, W, pab=blk, forces=True)
+        hab_ref[off[i]:off[i] + sizes[i]] = np.asarray(h).reshape(-1)
+        f_ref[i] = f
+    say("windowed task list", n, "pairs")
+    g = gpw.grid_desc(npts, dh, ngrid=ngrid, lb_grid=lb_grid, periodic=per)
+    for splits in (1, 3):
+        tl = gpw.TaskList(g, pairs, splits=splits)
+        grid = torch.zeros((ngrid[2], ngrid[1], ngrid[0]), dtype=torch.float64, device="cuda")
+        hab = torch.zeros(int(sizes.sum()), dtype=torch.float64, device="cuda")
+        forces = torch.zeros((n, 6), dtype=torch.float64, device="cuda")
+        pd = dev(torch, pab)
+        tl.collocate(pd, grid)
+        tl.integrate(dev(torch, W), hab, pab=pd, forces=forces)
+        torch.cuda.synchronize()
+        assert npts_set(grid.cpu().numpy()) == npts_set(grid_ref), splits
+        assert rel_err(grid.cpu().numpy(), grid_ref) < TOL, splits
+        assert rel_err(hab.cpu().numpy(), hab_ref) < TOL, splits
+        assert rel_err(forces.cpu().numpy().reshape(-1, 2, 3), f_ref) < TOL, splits
+        tl.close()
