@@ -902,15 +902,17 @@ __device__ __forceinline__ void integ_block(IntegSmem<LM, GEN> &sm, int q, int x
     __builtin_amdgcn_wave_barrier();
 }
 
-// partial moments of every (sub-box, tile) item: partial[rec.ent_off + c3]
+// partial moments of every (sub-box, tile) item: partial[rec.ent_off + c3];
+// workgroup bid = tile * splits + split
 template <int LM, bool GEN>
-__global__ __launch_bounds__(NROW, 2) void k_integrate_tiles(
-    TileGrid tg, const EntryRec *__restrict__ recs, const int *__restrict__ tile_start,
-    const double *__restrict__ grid, int lp_add, int splits, double *__restrict__ partial) {
+__device__ __forceinline__ void integrate_tile(IntegSmem<LM, GEN> &sm, int bid, const TileGrid &tg,
+                                               const EntryRec *__restrict__ recs,
+                                               const int *__restrict__ tile_start,
+                                               const double *__restrict__ grid, int lp_add,
+                                               int splits, double *__restrict__ partial) {
     using C = TileCfg<LM>;
     constexpr int Q = C::Q;
-    __shared__ IntegSmem<LM, GEN> sm;
-    const int tile = blockIdx.x / splits, split = blockIdx.x % splits;
+    const int tile = bid / splits, split = bid % splits;
     const int tix = tile % tg.ntx, tiy = (tile / tg.ntx) % tg.nty, tiz = tile / (tg.ntx * tg.nty);
     const int x0 = tix * TX, y0 = tiy * TY, z0 = tiz * TZ;
     const int t = threadIdx.x, xg = t & 3, yb = (t >> 2) & 3, z = t >> 4;
@@ -1012,6 +1014,33 @@ __global__ __launch_bounds__(NROW, 2) void k_integrate_tiles(
         if (t < Q * RECW) reinterpret_cast<int *>(&sm.rec[rs ^ 1][0])[t] = pf_rec;
         __syncthreads();
     }
+}
+
+template <int LM, bool GEN>
+__global__ __launch_bounds__(NROW, 2) void k_integrate_tiles(
+    TileGrid tg, const EntryRec *__restrict__ recs, const int *__restrict__ tile_start,
+    const double *__restrict__ grid, int lp_add, int splits, double *__restrict__ partial) {
+    __shared__ IntegSmem<LM, GEN> sm;
+    integrate_tile<LM, GEN>(sm, blockIdx.x, tg, recs, tile_start, grid, lp_add, splits, partial);
+}
+
+// Two classes in one launch (workgroups of class A first, then class B), so
+// the tail of one class overlaps the other; LDS is the union of both.
+template <int LMA, int LMB, bool GEN>
+__global__ __launch_bounds__(NROW, 2) void k_integrate_tiles2(
+    TileGrid tg, const EntryRec *__restrict__ recsA, const int *__restrict__ startA, int splitsA,
+    const EntryRec *__restrict__ recsB, const int *__restrict__ startB, int splitsB,
+    const double *__restrict__ grid, int lp_add, int ntiles, double *__restrict__ partial) {
+    __shared__ union {
+        IntegSmem<LMA, GEN> a;
+        IntegSmem<LMB, GEN> b;
+    } sm;
+    const int na = ntiles * splitsA;
+    if ((int)blockIdx.x < na)
+        integrate_tile<LMA, GEN>(sm.a, blockIdx.x, tg, recsA, startA, grid, lp_add, splitsA, partial);
+    else
+        integrate_tile<LMB, GEN>(sm.b, blockIdx.x - na, tg, recsB, startB, grid, lp_add, splitsB,
+                                 partial);
 }
 
 // ---------------------------------------------------------------------------
@@ -1471,6 +1500,18 @@ static hipError_t launch_integrate_tiles_lm(const TileGrid &tg, const EntryRec *
     else
         hipLaunchKernelGGL((k_integrate_tiles<LM, false>), dim3(ntiles * splits), dim3(NROW), 0, st,
                            tg, recs, tile_start, grid, lp_add, splits, partial);
+    return hipGetLastError();
+}
+
+// classes 1 (LM 8) and 0 (LM 4) of orthorhombic lists in one launch
+hipError_t launch_integrate_tiles_84(const TileGrid &tg, const EntryRec *recs8, const int *start8,
+                                     int splits8, const EntryRec *recs4, const int *start4,
+                                     int splits4, const double *grid, int lp_add, int ntiles,
+                                     double *partial, hipStream_t st) {
+    if (tg.general) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_integrate_tiles2<8, 4, false>), dim3(ntiles * (splits8 + splits4)),
+                       dim3(NROW), 0, st, tg, recs8, start8, splits8, recs4, start4, splits4, grid,
+                       lp_add, ntiles, partial);
     return hipGetLastError();
 }
 

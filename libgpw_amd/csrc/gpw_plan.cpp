@@ -33,6 +33,10 @@ hipError_t launch_reduce_partials(const double *partial, int splits, long long n
 hipError_t launch_integrate_tiles(int cls, const TileGrid &tg, const EntryRec *recs,
                                   const int *tile_start, const double *grid, int lp_add,
                                   int ntiles, int splits, double *partial, hipStream_t st);
+hipError_t launch_integrate_tiles_84(const TileGrid &tg, const EntryRec *recs8, const int *start8,
+                                     int splits8, const EntryRec *recs4, const int *start4,
+                                     int splits4, const double *grid, int lp_add, int ntiles,
+                                     double *partial, hipStream_t st);
 hipError_t launch_coef(const PairDev *pairs, const int *order, int pair0, int npairs, int tau,
                        int lam_max, int lbm_max, const double *pab, double *coef,
                        const double *mtrans, const long long *mt_off, double *ws,
@@ -1218,6 +1222,15 @@ gpw_status gpw_tasklist_integrate(gpw_tasklist *tl, const double *grid, const do
     const int nt1 = tl->ntiles + 1;
     {
         PhaseTimer ph(tl, st, PH_ITILES);
+        const bool only01 = tl->class_entries[wf][2] == 0 && tl->class_entries[wf][3] == 0;
+        if (!tl->tg.general && only01 && tl->class_entries[wf][0] && tl->class_entries[wf][1]) {
+            // the two light classes together in one launch (class 1 first)
+            HIP_TRY(launch_integrate_tiles_84(
+                tl->tg, tl->d_recs + tl->list_off[wf][1], tl->d_tile_start + (size_t)(wf * NCLS + 1) * nt1,
+                tl->isplit[wf][1], tl->d_recs + tl->list_off[wf][0],
+                tl->d_tile_start + (size_t)(wf * NCLS + 0) * nt1, tl->isplit[wf][0], grid, lp_add,
+                tl->ntiles, tl->d_partial, st));
+        } else
         for (int c = 0; c < NCLS; c++) {
             if (tl->class_entries[wf][c] == 0) continue;
             const int *start = tl->d_tile_start + (size_t)(wf * NCLS + c) * nt1;
