@@ -369,13 +369,21 @@ __device__ __forceinline__ void general_row_dev(const TileGrid &tg, double gp0, 
     qc = c * -zetp;
 }
 
-// per-row data of the general path (unused storage on orthorhombic grids)
+// per-row data of the general path (unused storage on orthorhombic grids).
+// Row (y, z) is the lattice row at displacements dj = j - gp1, dk = k - gp2;
+// lo..hi its points inside the sphere. With u = i - gp0 and
+// w = dj h1 + dk h2, the exponent -zetp |u h0 + w|^2 of point i is
+// qa u^2 + qb u + qc, qb = c01 dj + c02 dk, qc = (c11 dj + c12 dk) dj + c22 dk^2.
+// kap = exp(2 qa) is the step ratio of the exponential recursion along a row.
 template <int Q, bool GEN>
 struct GenRows {
     int lo[Q][NROW], hi[Q][NROW];
-    double qb[Q][NROW], qc[Q][NROW];
-    double qa[Q];
+    double dj[Q][TY], dk[Q][TZ];
+    double qa[Q], kap[Q], gp0[Q], c01[Q], c02[Q], c11[Q], c12[Q], c22[Q];
     int ball[Q][NXG];  // 1: every point of the x block is in the sub-box
+    // c0: when the valid points j of the x block have the lattice indices
+    // c0 + j (no periodic wrap between them), else BIG; vmask: valid points
+    int c0[Q][NXG], vmask[Q][NXG];
 };
 template <int Q>
 struct GenRows<Q, false> {};
@@ -469,17 +477,25 @@ __device__ __forceinline__ void tile_phase1(const EntryRec &R, int q, int lane, 
         }
     } else {
         // x blocks: min / max lattice index of the valid points, all valid?
+        // and whether the valid points share one offset i - j
         int mn = ok ? i : BIG, mx = ok ? i : -BIG, all = ok ? 1 : 0;
+        int omn = ok ? i - (lane & (XB - 1)) : BIG, omx = ok ? omn : -BIG;
+        int vm = ok ? 1 << (lane & (XB - 1)) : 0;
 #pragma unroll
         for (int o = 1; o < XB; o <<= 1) {
             mx = max(mx, __shfl_xor(mx, o, XB));
             mn = min(mn, __shfl_xor(mn, o, XB));
             all = min(all, __shfl_xor(all, o, XB));
+            omx = max(omx, __shfl_xor(omx, o, XB));
+            omn = min(omn, __shfl_xor(omn, o, XB));
+            vm |= __shfl_xor(vm, o, XB);
         }
         if (d == 0 && (lane % XB) == 0) {
             bmax[lane / XB] = mx;
             bmin[lane / XB] = mn;
             gr.ball[q][lane / XB] = all;
+            gr.c0[q][lane / XB] = omn == omx ? omn : BIG;
+            gr.vmask[q][lane / XB] = vm;
         }
 #pragma unroll
         for (int rr = 0; rr < NROW / 64; rr++) {
@@ -487,31 +503,83 @@ __device__ __forceinline__ void tile_phase1(const EntryRec &R, int q, int lane, 
             const int kidx = __shfl(code, TX + TY + row / TY);
             const int jidx = __shfl(code, TX + row % TY);
             int lo = 1, hi = 0;
-            double qb = 0.0, qc = 0.0;
+            double qb, qc;
             if (kidx != -BIG && jidx != -BIG)
                 general_row_dev(tg, R.roff[0], __dsub_rn((double)jidx, R.roff[1]),
                                 __dsub_rn((double)kidx, R.roff[2]), R.R2, R.zetp, lo, hi, qb, qc);
             gr.lo[q][row] = lo;
             gr.hi[q][row] = hi;
-            gr.qb[q][row] = qb;
-            gr.qc[q][row] = qc;
         }
+        // displacements of the tile's y and z lattice rows (0 when invalid:
+        // such rows are empty)
+        if (d == 1) gr.dj[q][pos] = ok ? __dsub_rn((double)i, R.roff[1]) : 0.0;
+        if (d == 2) gr.dk[q][pos] = ok ? __dsub_rn((double)i, R.roff[2]) : 0.0;
         if (lane == 0) {
             double a = 0.0;
 #pragma unroll
             for (int x = 0; x < 3; x++) a = __dadd_rn(a, __dmul_rn(tg.dh[0][x], tg.dh[0][x]));
             gr.qa[q] = a * -R.zetp;
+            gr.kap[q] = exp(2.0 * (a * -R.zetp));
+            gr.gp0[q] = R.roff[0];
+            double h01 = 0.0, h02 = 0.0, h11 = 0.0, h12 = 0.0, h22 = 0.0;
+#pragma unroll
+            for (int x = 0; x < 3; x++) {
+                h01 += tg.dh[0][x] * tg.dh[1][x];
+                h02 += tg.dh[0][x] * tg.dh[2][x];
+                h11 += tg.dh[1][x] * tg.dh[1][x];
+                h12 += tg.dh[1][x] * tg.dh[2][x];
+                h22 += tg.dh[2][x] * tg.dh[2][x];
+            }
+            gr.c01[q] = -2.0 * R.zetp * h01;
+            gr.c02[q] = -2.0 * R.zetp * h02;
+            gr.c11[q] = -R.zetp * h11;
+            gr.c12[q] = -2.0 * R.zetp * h12;
+            gr.c22[q] = -R.zetp * h22;
         }
         (void)T;
     }
 }
 
-// Phase-3 row data of one thread's register block on the general path.
+// Phase-3 data of one thread's register block on the general path.
+// contig: the valid points j of the block have the lattice indices c0 + j;
+// then bit j of bits[r] says whether point j of row r is valid and inside
+// the sphere, and u0 = c0 - gp0.
 struct GenBlock {
-    int lo[RB], hi[RB];
-    double qb[RB], qc[RB];
-    double qa;
+    int bits[RB];
+    double qa, kap, gp0, u0;
+    int bmn, bmx;
+    bool contig;
 };
+
+// exponent coefficients qb, qc of row yb * RB + r of plane z (see GenRows)
+template <int Q>
+__device__ __forceinline__ void gen_row_coef(const GenRows<Q, true> &gr, int q, int z, int y,
+                                             double &qb, double &qc) {
+    const double dk = gr.dk[q][z], dj = gr.dj[q][y];
+    qb = gr.c01[q] * dj + gr.c02[q] * dk;
+    qc = (gr.c11[q] * dj + gr.c12[q] * dk) * dj + gr.c22[q] * dk * dk;
+}
+
+// Gaussian factors of the block's rows by the recursion along the row:
+// e(u+1) = e(u) rho(u), rho(u+1) = rho(u) kap, started at u0 (two exp per
+// row instead of one per point). Returns false when the recursion cannot be
+// used: the block is not contiguous, or the start value of a row with points
+// inside underflows (later factors would be lost).
+template <int Q>
+__device__ __forceinline__ bool gen_row_start(const GenRows<Q, true> &gr, int q, int z, int yb,
+                                              const GenBlock &gb, double (&e)[RB],
+                                              double (&rho)[RB]) {
+    bool fast = gb.contig;
+#pragma unroll
+    for (int r = 0; r < RB; r++) {
+        double qb, qc;
+        gen_row_coef(gr, q, z, yb * RB + r, qb, qc);
+        e[r] = exp((gb.qa * gb.u0 + qb) * gb.u0 + qc);
+        rho[r] = exp(gb.qa * (2.0 * gb.u0 + 1.0) + qb);
+        fast = fast && (gb.bits[r] == 0 || e[r] >= 1e-290);
+    }
+    return fast;
+}
 
 #define GPW_LP_DISPATCH(LM, CALL)                                                   \
     switch (lp) {                                                                   \
@@ -574,6 +642,7 @@ __device__ __forceinline__ void colloc_block(const CollocSmem<LM, GEN> &sm, int 
                                              int z, const int (&Tr)[RB], bool full,
                                              const GenBlock &gb, double (&acc)[RB][XB]) {
     constexpr int lp = L - 1;
+    (void)full;
     double cx[RB][L];
     const double *cxy = sm.cxy[q][z];
 #pragma unroll
@@ -589,25 +658,70 @@ __device__ __forceinline__ void colloc_block(const CollocSmem<LM, GEN> &sm, int 
         }
     }
     const double *Xg = sm.X[q] + xg * TileCfg<LM>::GS;
+    if constexpr (GEN) {
+        double e[RB], rho[RB];
+        if (gen_row_start(sm.gr, q, z, yb, gb, e, rho)) {
 #pragma unroll
-    for (int j = 0; j < XB; j++) {
-        double xv[L];
+            for (int j = 0; j < XB; j++) {
+                double xv[L];
 #pragma unroll
-        for (int l = 0; l < L; l++) xv[l] = Xg[j * LM + l];
-        const int code = sm.xi[q][xg * XB + j];
+                for (int l = 0; l < L; l++) xv[l] = Xg[j * LM + l];
 #pragma unroll
-        for (int r = 0; r < RB; r++) {
-            double v = 0.0;
+                for (int r = 0; r < RB; r++) {
+                    double v = 0.0;
 #pragma unroll
-            for (int l = 0; l < L; l++) v += cx[r][l] * xv[l];
-            // the sphere mask as a 0/1 factor of one FMA (cheaper than a
-            // 64-bit select and an add)
-            if constexpr (GEN) {
-                const double ci = (double)code;
-                const double e = exp((gb.qa * ci + gb.qb[r]) * ci + gb.qc[r]);
-                const double m = (full || (code >= gb.lo[r] && code <= gb.hi[r])) ? e : 0.0;
-                acc[r][j] = __builtin_fma(v, m, acc[r][j]);
-            } else {
+                    for (int l = 0; l < L; l++) v += cx[r][l] * xv[l];
+                    const double m = (gb.bits[r] & (1 << j)) ? e[r] : 0.0;
+                    acc[r][j] = __builtin_fma(v, m, acc[r][j]);
+                    if (j < XB - 1) {
+                        e[r] *= rho[r];
+                        rho[r] *= gb.kap;
+                    }
+                }
+            }
+        } else {
+            // per-point exponentials (a block across a periodic wrap, or far
+            // from a steep Gaussian): one point per iteration, so that only
+            // RB exponentials are live at a time
+#pragma unroll 1
+            for (int j = 0; j < XB; j++) {
+                double xv[L];
+#pragma unroll
+                for (int l = 0; l < L; l++) xv[l] = Xg[j * LM + l];
+                const int code = sm.xi[q][xg * XB + j];
+                const double u = (double)code - gb.gp0;
+                double d[RB];
+#pragma unroll
+                for (int r = 0; r < RB; r++) {
+                    double v = 0.0;
+#pragma unroll
+                    for (int l = 0; l < L; l++) v += cx[r][l] * xv[l];
+                    const int row = z * TY + yb * RB + r;
+                    double qb, qc;
+                    gen_row_coef(sm.gr, q, z, yb * RB + r, qb, qc);
+                    const bool in = code >= sm.gr.lo[q][row] && code <= sm.gr.hi[q][row];
+                    d[r] = in ? v * exp((gb.qa * u + qb) * u + qc) : 0.0;
+                }
+#pragma unroll
+                for (int jj = 0; jj < XB; jj++)
+#pragma unroll
+                    for (int r = 0; r < RB; r++) acc[r][jj] += jj == j ? d[r] : 0.0;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < XB; j++) {
+            double xv[L];
+#pragma unroll
+            for (int l = 0; l < L; l++) xv[l] = Xg[j * LM + l];
+            const int code = sm.xi[q][xg * XB + j];
+#pragma unroll
+            for (int r = 0; r < RB; r++) {
+                double v = 0.0;
+#pragma unroll
+                for (int l = 0; l < L; l++) v += cx[r][l] * xv[l];
+                // the sphere mask as a 0/1 factor of one FMA (cheaper than a
+                // 64-bit select and an add)
                 const double m = (full || code <= Tr[r]) ? 1.0 : 0.0;
                 acc[r][j] = __builtin_fma(v, m, acc[r][j]);
             }
@@ -635,18 +749,25 @@ __device__ __forceinline__ bool classify_block(const Smem &sm, int q, int xg, in
     } else {
         bool any = false;
         full = sm.gr.ball[q][xg] != 0;
+        const int c0 = sm.gr.c0[q][xg], vm = sm.gr.vmask[q][xg];
+        gb.contig = c0 != BIG;
 #pragma unroll
         for (int r = 0; r < RB; r++) {
             const int row = z * TY + yb * RB + r;
-            gb.lo[r] = sm.gr.lo[q][row];
-            gb.hi[r] = sm.gr.hi[q][row];
-            gb.qb[r] = sm.gr.qb[q][row];
-            gb.qc[r] = sm.gr.qc[q][row];
-            any = any || (gb.lo[r] <= gb.hi[r] && gb.lo[r] <= bmx && gb.hi[r] >= bmn);
-            full = full && gb.lo[r] <= bmn && bmx <= gb.hi[r];
+            const int lo = sm.gr.lo[q][row], hi = sm.gr.hi[q][row];
+            any = any || (lo <= hi && lo <= bmx && hi >= bmn);
+            full = full && lo <= bmn && bmx <= hi;
+            // points j = max(lo - c0, 0) .. min(hi - c0, 7) (contiguous blocks)
+            const int a = max(lo - c0, 0), b = min(hi - c0, XB - 1);
+            gb.bits[r] = (gb.contig && a <= b) ? ((0xFF >> (XB - 1 - b)) & (0xFF << a)) & vm : 0;
             Tr[r] = 0;
         }
         gb.qa = sm.gr.qa[q];
+        gb.kap = sm.gr.kap[q];
+        gb.gp0 = sm.gr.gp0[q];
+        gb.bmn = bmn;
+        gb.bmx = bmx;
+        gb.u0 = (double)c0 - gb.gp0;
         return any && bmn <= bmx;
     }
 }
@@ -853,24 +974,65 @@ __device__ __forceinline__ void integ_block(IntegSmem<LM, GEN> &sm, int q, int x
     if (mode != 0) {  // 1: masked, 2: whole block inside
         const double *Xg = sm.X[q] + xg * TileCfg<LM>::GS;
         const bool wfull = __all(mode == 2);  // wave-uniform among the active lanes
+        if constexpr (GEN) {
+            double e[RB], rho[RB];
+            if (gen_row_start(sm.gr, q, z, yb, gb, e, rho)) {
 #pragma unroll
-        for (int j = 0; j < XB; j++) {
-            double xv[L];
+                for (int j = 0; j < XB; j++) {
+                    double xv[L];
 #pragma unroll
-            for (int l = 0; l < L; l++) xv[l] = Xg[j * LM + l];
-            const int code = sm.xi[q][xg * XB + j];
+                    for (int l = 0; l < L; l++) xv[l] = Xg[j * LM + l];
 #pragma unroll
-            for (int r = 0; r < RB; r++) {
-                double w = v[r][j];
-                if constexpr (GEN) {
-                    const double ci = (double)code;
-                    w *= exp((gb.qa * ci + gb.qb[r]) * ci + gb.qc[r]);
-                    if (!wfull) w = (code >= gb.lo[r] && code <= gb.hi[r]) ? w : 0.0;
-                } else {
-                    if (!wfull) w = code <= Tr[r] ? w : 0.0;
+                    for (int r = 0; r < RB; r++) {
+                        const double w = (gb.bits[r] & (1 << j)) ? v[r][j] * e[r] : 0.0;
+#pragma unroll
+                        for (int l = 0; l < L; l++) tx[r][l] += w * xv[l];
+                        if (j < XB - 1) {
+                            e[r] *= rho[r];
+                            rho[r] *= gb.kap;
+                        }
+                    }
                 }
+            } else {
+                // per-point exponentials (a block across a periodic wrap, or
+                // far from a steep Gaussian): one point per iteration, so that
+                // only RB exponentials are live at a time
+#pragma unroll 1
+                for (int j = 0; j < XB; j++) {
+                    double xv[L];
 #pragma unroll
-                for (int l = 0; l < L; l++) tx[r][l] += w * xv[l];
+                    for (int l = 0; l < L; l++) xv[l] = Xg[j * LM + l];
+                    const int code = sm.xi[q][xg * XB + j];
+                    const double u = (double)code - gb.gp0;
+#pragma unroll
+                    for (int r = 0; r < RB; r++) {
+                        double vj = 0.0;
+#pragma unroll
+                        for (int jj = 0; jj < XB; jj++) vj = jj == j ? v[r][jj] : vj;
+                        const int row = z * TY + yb * RB + r;
+                        double qb, qc;
+                        gen_row_coef(sm.gr, q, z, yb * RB + r, qb, qc);
+                        const bool in = code >= sm.gr.lo[q][row] && code <= sm.gr.hi[q][row];
+                        const double w = in ? vj * exp((gb.qa * u + qb) * u + qc) : 0.0;
+#pragma unroll
+                        for (int l = 0; l < L; l++) tx[r][l] += w * xv[l];
+                    }
+                }
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < XB; j++) {
+                double xv[L];
+#pragma unroll
+                for (int l = 0; l < L; l++) xv[l] = Xg[j * LM + l];
+                const int code = sm.xi[q][xg * XB + j];
+#pragma unroll
+                for (int r = 0; r < RB; r++) {
+                    double w = v[r][j];
+                    if (!wfull) w = code <= Tr[r] ? w : 0.0;
+#pragma unroll
+                    for (int l = 0; l < L; l++) tx[r][l] += w * xv[l];
+                }
             }
         }
     }

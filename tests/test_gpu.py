@@ -591,3 +591,46 @@ def test_tasklist_windowed_nonperiodic(gpw, oracle, torch_cuda):
         assert rel_err(hab.cpu().numpy(), hab_ref) < TOL, splits
         assert rel_err(forces.cpu().numpy().reshape(-1, 2, 3), f_ref) < TOL, splits
         tl.close()
+
+
+@pytest.mark.parametrize("case", ["wrap", "steep"])
+def test_general_tasklist_row_recursion_edges(gpw, oracle, torch_cuda, case):
+    """General-cell task lists where the tile kernels cannot run the Gaussian
+    recursion along a whole 8-point x block and take the per-point path:
+    "wrap": x period 20 (not a multiple of 8) and Gaussians wider than the
+    period, so blocks hold points on both sides of a periodic wrap of one
+    sub-box; "steep": zetp |h0|^2 of 18 to 40, so the Gaussian of a block's
+    first point underflows while later points are inside the sphere.
+    Collocate, integrate and forces against the oracle."""
+    torch = torch_cuda
+    rng = np.random.default_rng(41 if case == "wrap" else 42)
+    if case == "wrap":
+        npts, cell, n = (20, 24, 28), np.array([8.0, 9.0, 10.0]), 12
+        zeta = rng.uniform(0.25, 0.5, n); zetb = rng.uniform(0.25, 0.5, n)
+    else:
+        npts, cell, n = (48, 40, 44), np.array([12.0, 10.0, 11.0]), 60
+        zeta = rng.uniform(150.0, 320.0, n); zetb = rng.uniform(150.0, 320.0, n)
+    dh = np.diag(cell / np.asarray(npts))
+    dh[1, 0] = 0.3 * dh[0, 0]
+    dh[2, 1] = -0.2 * dh[1, 1]
+    la = rng.integers(0, 3, n); lb = rng.integers(0, 3, n)
+    ra = rng.uniform(0.0, 1.0, (n, 3)) @ (dh * np.asarray(npts)[:, None])
+    rab = rng.normal(0.0, 0.3 if case == "wrap" else 0.02, (n, 3))
+    sizes = np.array([gpw.ncoset(int(b)) * gpw.ncoset(int(a)) for a, b in zip(la, lb)])
+    off = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    pairs = {"la_max": la.astype(np.int32), "la_min": la.astype(np.int32),
+             "lb_max": lb.astype(np.int32), "lb_min": lb.astype(np.int32),
+             "zeta": zeta, "zetb": zetb, "rscale": np.ones(n), "ra": ra, "rab": rab,
+             "radius": np.sqrt(np.log(1e10) / (zeta + zetb)), "block_offset": off}
+    cfg = {"npts": npts, "dh": dh, "pairs": pairs, "pab": rng.uniform(-1.0, 1.0, int(sizes.sum())),
+           "nblock": int(sizes.sum())}
+    W = rng.uniform(-1.0, 1.0, (npts[2], npts[1], npts[0]))
+    say("general row recursion edges", case, n, "pairs")
+    ref = oracle_batch(oracle, cfg, W=W, forces=True)
+    assert np.count_nonzero(ref["grid"]) > 100
+    for splits in (1, 2):
+        got = run_tasklist(gpw, torch, cfg, splits=splits, forces=True, W=W)
+        assert npts_set(got["grid"]) == npts_set(ref["grid"]), splits
+        assert rel_err(got["grid"], ref["grid"]) < TOL, splits
+        assert rel_err(got["hab"], ref["hab"]) < TOL, splits
+        assert rel_err(got["forces"], ref["forces"]) < TOL, splits
