@@ -16,10 +16,13 @@ constexpr int BIG = 0x3fffffff;           // "outside" marker for offsets
 // along x per row, TY x TZ rows, one row per thread.
 constexpr int TX = 32, TY = 16, TZ = 16;
 constexpr int NROW = TY * TZ;             // 256 threads per workgroup
-// k_hab keeps heff / eff in LDS up to this size, else in a global workspace
-// of HAB_WS_BLOCKS slices (one per resident workgroup)
+// k_coef / k_hab keep their per-pair blocks in LDS up to these sizes, else in
+// a global workspace of HAB_WS_BLOCKS slices (one per resident workgroup);
+// TR_NT: their largest workgroup
+constexpr size_t COEF_LDS_MAX = 24 * 1024;
 constexpr size_t HAB_LDS_MAX = 64 * 1024;
 constexpr int HAB_WS_BLOCKS = 4096;
+constexpr int TR_NT = 256;
 
 __host__ __device__ constexpr int ncoset(int l) {
     return l < 0 ? 0 : ((l + 1) * (l + 2) * (l + 3)) / 6;

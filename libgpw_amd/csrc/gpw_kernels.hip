@@ -197,7 +197,7 @@ __device__ void lattice_transform(const double *__restrict__ T, int lp, const do
 // Workgroups loop over pairs (pair0 + blockIdx.x, stride gridDim.x); eff, Ty
 // and cxyz live in LDS after alpha, or, when gws is given (blocks too large
 // for LDS), in the workgroup's slice gws + blockIdx.x * ws_stride.
-__global__ __launch_bounds__(64) void k_coef(const PairDev *__restrict__ pairs,
+__global__ __launch_bounds__(TR_NT) void k_coef(const PairDev *__restrict__ pairs,
                                              const int *__restrict__ order, int pair0,
                                              int npairs, int tau, const double *__restrict__ pab,
                                              double *__restrict__ coef,
@@ -1250,7 +1250,7 @@ __global__ void k_sum_entries(const PairDev *__restrict__ pairs, int tau, int lp
 // positions of `order`, segments seg0 .. seg0 + nseg); one workgroup walks a
 // segment's pairs in order, so the += into the shared block is race-free and
 // deterministic. segs NULL: one pair per segment, positions seg0 .. seg0 + nseg.
-__global__ __launch_bounds__(64) void k_hab(const PairDev *__restrict__ pairs,
+__global__ __launch_bounds__(TR_NT) void k_hab(const PairDev *__restrict__ pairs,
                                             const int *__restrict__ order,
                                             const int *__restrict__ segs, int seg0, int nseg,
                                             int tau, int with_forces, const double *__restrict__ Mbuf,
@@ -1279,9 +1279,9 @@ __global__ __launch_bounds__(64) void k_hab(const PairDev *__restrict__ pairs,
     const int n2x = ncoef2(lpx);
     double *M = sm_hab + nalpha;                    // ncoset(lpx)
     double *Mijk = M + n3x;                         // general cells: lattice moments
-    double *red = Mijk + n3x;                       // 6 x 64
+    double *red = Mijk + n3x;                       // 6 x (waves per workgroup)
     // heff (nlbx x nlax), eff (ncoset(lbm) x ncoset(lam)), U ((lamx+1)(lbmx+1) x C2(lpx))
-    double *heff = gws ? gws + blockIdx.x * ws_stride : red + 6 * 64;
+    double *heff = gws ? gws + blockIdx.x * ws_stride : red + 6 * (TR_NT / 64);
     double *eff = heff + nlax * nlbx;
     double *U = eff + ncoset(lam) * ncoset(lbm);
     fill_alpha(&pairs[pi], A, threadIdx.x, blockDim.x);
@@ -1398,11 +1398,19 @@ __global__ __launch_bounds__(64) void k_hab(const PairDev *__restrict__ pairs,
             f[3 + d] += e * gb;
         }
     }
-    for (int k = 0; k < 6; k++) red[k * 64 + threadIdx.x] = f[k];
+    // sums over the wave (butterfly), then over the waves
+    const int nw = (int)(blockDim.x / 64);
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        double v = f[k];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        if ((threadIdx.x & 63) == 0) red[k * (TR_NT / 64) + (threadIdx.x >> 6)] = v;
+    }
     __syncthreads();
     if (threadIdx.x < 6) {
         double s = 0.0;
-        for (int i = 0; i < 64; i++) s += red[threadIdx.x * 64 + i];
+        for (int i = 0; i < nw; i++) s += red[threadIdx.x * (TR_NT / 64) + i];
         forces[(long long)pi * 6 + threadIdx.x] += s;
     }
     }
@@ -1688,6 +1696,10 @@ hipError_t launch_integrate_tiles(int cls, const TileGrid &tg, const EntryRec *r
     }
 }
 
+// workgroup size of k_coef / k_hab: one wave for small blocks, four waves
+// (more latency hiding per LDS allocation) from lp = 4 on
+static int transform_threads(int lp) { return lp >= 4 ? TR_NT : 64; }
+
 // k_coef: per-pair blocks (eff, Ty, cxyz) and the LDS with / without them
 size_t coef_ws_doubles(int lam, int lbm) {
     const int lp = lam + lbm;
@@ -1702,7 +1714,7 @@ size_t coef_smem_bytes(int lam, int lbm, bool lds_blocks) {
     return sizeof(double) * n;
 }
 
-bool coef_needs_workspace(int lam, int lbm) { return coef_smem_bytes(lam, lbm, true) > HAB_LDS_MAX; }
+bool coef_needs_workspace(int lam, int lbm) { return coef_smem_bytes(lam, lbm, true) > COEF_LDS_MAX; }
 
 // pairs order[pair0 .. pair0 + npairs) (or pair0 .. when order is NULL), whose
 // la_max / lb_max are at most la_max / lb_max (LDS sizing)
@@ -1717,7 +1729,7 @@ hipError_t launch_coef(const PairDev *pairs, const int *order, int pair0, int np
         return hipErrorInvalidValue;
     const size_t smem = coef_smem_bytes(lam, lbm, !big);
     const int nblk = big ? std::min(npairs, ws_blocks) : npairs;
-    hipLaunchKernelGGL(k_coef, dim3(nblk), dim3(64), smem, st, pairs, order, pair0, npairs, tau, pab,
+    hipLaunchKernelGGL(k_coef, dim3(nblk), dim3(transform_threads(lam + lbm)), smem, st, pairs, order, pair0, npairs, tau, pab,
                        coef, mtrans, mt_off, big ? ws : nullptr, big ? ws_stride : 0LL);
     return hipGetLastError();
 }
@@ -1730,7 +1742,7 @@ size_t hab_ws_doubles(int lamx, int lbmx, int lam, int lbm) {
 // LDS of k_hab: alpha, M, Mijk, the force reduction and (LDS mode) heff, eff
 size_t hab_smem_bytes(int lamx, int lbmx, int lam, int lbm, bool lds_blocks) {
     const int lpx = lamx + lbmx;
-    size_t n = 3 * (size_t)(lbmx + 1) * (lamx + 1) * (lpx + 1) + 2 * (size_t)ncoset(lpx) + 6 * 64;
+    size_t n = 3 * (size_t)(lbmx + 1) * (lamx + 1) * (lpx + 1) + 2 * (size_t)ncoset(lpx) + 6 * (TR_NT / 64);
     if (lds_blocks) n += hab_ws_doubles(lamx, lbmx, lam, lbm);
     return sizeof(double) * n;
 }
@@ -1754,7 +1766,7 @@ hipError_t launch_hab(const PairDev *pairs, const int *order, const int *segs, i
         return hipErrorInvalidValue;
     const size_t smem = hab_smem_bytes(lamx, lbmx, la_max + tau, lb_max + tau, !big);
     const int nblk = big ? std::min(npairs, ws_blocks) : npairs;
-    hipLaunchKernelGGL(k_hab, dim3(nblk), dim3(64), smem, st, pairs, order, segs, seg0, nseg, tau,
+    hipLaunchKernelGGL(k_hab, dim3(nblk), dim3(transform_threads(lamx + lbmx)), smem, st, pairs, order, segs, seg0, nseg, tau,
                        with_forces, Mbuf, mstride, pab, hab, forces, mtrans, mt_off,
                        big ? ws : nullptr, big ? ws_stride : 0LL);
     return hipGetLastError();
