@@ -36,9 +36,11 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
 
-def roofline(pts, coll_ms, integ_ms, ms_per_step):
+def roofline(pts, coll_ms, integ_ms, ms_per_step, workload):
     """Algorithmic bytes: collocate reads + writes every sphere point (16 B),
-    integrate reads it (8 B); the dominant kernel's achieved rate."""
+    integrate reads it (8 B); the dominant kernel's achieved rate. traffic:
+    the measured HBM bytes of that kernel where a counter profile of this
+    workload exists, else null."""
     coll_bytes, integ_bytes = 16.0 * pts, 8.0 * pts
     if coll_ms >= integ_ms:
         dom = ("k_collocate_tiles", coll_bytes, coll_ms)
@@ -49,7 +51,9 @@ def roofline(pts, coll_ms, integ_ms, ms_per_step):
     tpath = os.path.join(ROOT, "profiles", "traffic_128cubed_20000pairs.json")
     if os.path.exists(tpath):
         try:
-            traffic = json.load(open(tpath)).get(dom[0])
+            prof = json.load(open(tpath))
+            if prof.get("workload") == workload:
+                traffic = prof.get(dom[0])
         except (OSError, ValueError):
             traffic = None
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -128,7 +132,7 @@ def run_single(args, torch, G, Wk, dev, cfg):
     ms_per_step = wall * 1e3 / args.steps
     value = npairs * args.steps / wall
     pts = st["cube_points"]
-    roof = roofline(pts, phase_ms["collocate"], phase_ms["integrate"], ms_per_step)
+    roof = roofline(pts, phase_ms["collocate"], phase_ms["integrate"], ms_per_step, cfg["name"])
 
     out = {"metric": METRIC, "value": round(value, 1), "unit": "pairs/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
@@ -250,7 +254,7 @@ def run_multigrid(args, torch, G, Wk, dev, cfg):
            "config": {"workload": cfg["name"], "pairs": mg.npairs, "forces": True, "eps": Wk.EPS,
                       "hip_graph": True, "autotune": tune, "plan_s": round(plan_s, 3),
                       "levels": level_ms},
-           "roofline": roofline(pts, coll, integ, ms_per_step)}
+           "roofline": roofline(pts, coll, integ, ms_per_step, cfg["name"])}
     if not args.no_cpu:
         from oracle import oracle as O
         O.build()
