@@ -773,11 +773,11 @@ __device__ __forceinline__ bool classify_block(const Smem &sm, int q, int xg, in
 }
 
 // out_mode 0: grid += acc (splits == 1); 1: partial[split] = acc; 2: partial[split] += acc
-// General-cell kernels of the larger classes run one workgroup per CU: the
-// row data of the general path would otherwise spill registers in the inner
-// loops (measured faster despite the lower occupancy).
+// General-cell collocate kernels (and the larger integrate classes) run one
+// workgroup per CU: the row data of the general path would otherwise spill
+// registers in the inner loops (measured faster despite the lower occupancy).
 template <int LM, bool GEN>
-__global__ __launch_bounds__(NROW, (GEN && LM >= 8) ? 1 : 2) void k_collocate_tiles(
+__global__ __launch_bounds__(NROW, GEN ? 1 : 2) void k_collocate_tiles(
     TileGrid tg, const EntryRec *__restrict__ recs, const int *__restrict__ tile_start,
     const double *__restrict__ coef, int splits, double *__restrict__ out, int out_mode) {
     using C = TileCfg<LM>;
