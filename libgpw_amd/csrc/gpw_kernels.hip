@@ -112,6 +112,19 @@ __device__ __forceinline__ void decode_c2(int lp, int c, int &lyp, int &lxp) {
     lxp = c;
 }
 
+// Cartesian exponents of every coset index c < NCOSET_LMAX, packed as
+// lx | ly << 4 | lz << 8 (a table in LDS instead of the shell search and
+// decode per lookup)
+__device__ void fill_decode(int *tab, int tid, int nthr) {
+    for (int c = tid; c < NCOSET_LMAX; c += nthr) {
+        int l = 0;
+        while (ncoset(l) <= c) l++;
+        int x, y, z;
+        decode_coset(l, c - ncoset(l - 1), x, y, z);
+        tab[c] = x | (y << 4) | (z << 8);
+    }
+}
+
 // effective block of the local (tau-shifted) ranges, eff[jco*ldE + ico]
 __device__ void build_eff(const PairDev &p, int tau, const double *__restrict__ pab, double *eff,
                           int ldE, int nlb, int nla, int tid, int nthr) {
@@ -1263,6 +1276,8 @@ __global__ __launch_bounds__(TR_NT) void k_hab(const PairDev *__restrict__ pairs
                                             const long long *__restrict__ mt_off,
                                             double *__restrict__ gws, long long ws_stride) {
     extern __shared__ double sm_hab[];
+    __shared__ int dec[NCOSET_LMAX];
+    fill_decode(dec, threadIdx.x, blockDim.x);  // first read after the barrier below
     const int fx = with_forces ? 1 : 0;
     for (int sg = seg0 + blockIdx.x; sg < seg0 + nseg; sg += gridDim.x) {
     const int ip_end = segs ? segs[sg + 1] : sg + 1;
@@ -1312,22 +1327,22 @@ __global__ __launch_bounds__(TR_NT) void k_hab(const PairDev *__restrict__ pairs
     }
     __syncthreads();
     const int na_lo = ncoset(lamx_lo - 1), nb_lo = ncoset(lbmx_lo - 1);
-    for (int w = threadIdx.x; w < nlax * nlbx; w += blockDim.x) {
-        const int ico = w % nlax, jco = w / nlax;
+    // only the shells lamx_lo..lamx, lbmx_lo..lbmx: the hab and force stages
+    // read nothing else
+    const int nAx = nlax - na_lo, nBx = nlbx - nb_lo;
+    for (int w = threadIdx.x; w < nAx * nBx; w += blockDim.x) {
+        const int ico = na_lo + w % nAx, jco = nb_lo + w / nAx;
+        const int da = dec[ico], db = dec[jco];
+        const int ax = da & 15, ay = (da >> 4) & 15, az = da >> 8;
+        const int bx = db & 15, by = (db >> 4) & 15, bz = db >> 8;
+        const double *Uz = U + (az * (lbmx + 1) + bz) * n2x;
+        const double *Ax = &A.at(0, bx, ax, 0), *Ay = &A.at(1, by, ay, 0);
         double acc = 0.0;
-        if (ico >= na_lo && jco >= nb_lo) {
-            int la = 0, lb = 0;
-            while (ncoset(la) <= ico) la++;
-            while (ncoset(lb) <= jco) lb++;
-            int ax, ay, az, bx, by, bz;
-            decode_coset(la, ico - ncoset(la - 1), ax, ay, az);
-            decode_coset(lb, jco - ncoset(lb - 1), bx, by, bz);
-            const double *Uz = U + (az * (lbmx + 1) + bz) * n2x;
-            for (int ly = 0; ly <= ay + by; ly++) {
-                double sx = 0.0;
-                for (int lx = 0; lx <= ax + bx; lx++) sx += Uz[c2idx(lpx, ly, lx)] * A.at(0, bx, ax, lx);
-                acc += sx * A.at(1, by, ay, ly);
-            }
+        for (int ly = 0; ly <= ay + by; ly++) {
+            const double *Ur = Uz + c2idx(lpx, ly, 0);
+            double sx = 0.0;
+            for (int lx = 0; lx <= ax + bx; lx++) sx += Ur[lx] * Ax[lx];
+            acc += sx * Ay[ly];
         }
         heff[jco * nlax + ico] = p.prefactor * acc;
     }
@@ -1342,12 +1357,9 @@ __global__ __launch_bounds__(TR_NT) void k_hab(const PairDev *__restrict__ pairs
         if (!tau) {
             acc = heff[jco * nlax + ico];
         } else {
-            int la = 0, lb = 0;
-            while (ncoset(la) <= ico) la++;
-            while (ncoset(lb) <= jco) lb++;
-            int a[3], b[3];
-            decode_coset(la, ico - ncoset(la - 1), a[0], a[1], a[2]);
-            decode_coset(lb, jco - ncoset(lb - 1), b[0], b[1], b[2]);
+            const int da = dec[ico], db = dec[jco];
+            const int a[3] = {da & 15, (da >> 4) & 15, da >> 8};
+            const int b[3] = {db & 15, (db >> 4) & 15, db >> 8};
             acc = 0.0;
             for (int d = 0; d < 3; d++) {
                 int am[3] = {a[0], a[1], a[2]}, ap[3] = {a[0], a[1], a[2]};
@@ -1376,12 +1388,9 @@ __global__ __launch_bounds__(TR_NT) void k_hab(const PairDev *__restrict__ pairs
         if (ico < la_lo || jco < lb_lo) continue;
         const double e = eff[jco * nla + ico];
         if (e == 0.0) continue;
-        int la = 0, lb = 0;
-        while (ncoset(la) <= ico) la++;
-        while (ncoset(lb) <= jco) lb++;
-        int a[3], b[3];
-        decode_coset(la, ico - ncoset(la - 1), a[0], a[1], a[2]);
-        decode_coset(lb, jco - ncoset(lb - 1), b[0], b[1], b[2]);
+        const int da = dec[ico], db = dec[jco];
+        const int a[3] = {da & 15, (da >> 4) & 15, da >> 8};
+        const int b[3] = {db & 15, (db >> 4) & 15, db >> 8};
         for (int d = 0; d < 3; d++) {
             int ap[3] = {a[0], a[1], a[2]}, am[3] = {a[0], a[1], a[2]};
             int bp[3] = {b[0], b[1], b[2]}, bm[3] = {b[0], b[1], b[2]};
