@@ -125,6 +125,16 @@ __device__ void fill_decode(int *tab, int tid, int nthr) {
     }
 }
 
+// (lyp, lxp) of every packed index c < C2(lp), as lyp | lxp << 8
+constexpr int NC2MAX = ncoef2(LPMAX + 2);
+__device__ void fill_decode2(int *tab, int lp, int tid, int nthr) {
+    for (int c = tid; c < ncoef2(lp); c += nthr) {
+        int y, x;
+        decode_c2(lp, c, y, x);
+        tab[c] = y | (x << 8);
+    }
+}
+
 // effective block of the local (tau-shifted) ranges, eff[jco*ldE + ico]
 __device__ void build_eff(const PairDev &p, int tau, const double *__restrict__ pab, double *eff,
                           int ldE, int nlb, int nla, int tid, int nthr) {
@@ -218,6 +228,7 @@ __global__ __launch_bounds__(TR_NT) void k_coef(const PairDev *__restrict__ pair
                                              const long long *__restrict__ mt_off,
                                              double *__restrict__ gws, long long ws_stride) {
     extern __shared__ double sm_coef[];
+    __shared__ int dec2[NC2MAX];
     for (int ip = pair0 + blockIdx.x; ip < pair0 + npairs; ip += gridDim.x) {
     __syncthreads();  // LDS / workspace reuse across pairs
     const int pi = order ? order[ip] : ip;
@@ -233,12 +244,12 @@ __global__ __launch_bounds__(TR_NT) void k_coef(const PairDev *__restrict__ pair
     double *Ty = eff + nla * nlb;
     double *cxyz = Ty + (lam + 1) * (lbm + 1) * n2;
     fill_alpha(&pairs[pi], A, threadIdx.x, blockDim.x);
+    fill_decode2(dec2, lp, threadIdx.x, blockDim.x);
     build_eff(p, tau, pab, eff, nla, nlb, nla, threadIdx.x, blockDim.x);  // ends with a barrier
     const int nzz = (lam + 1) * (lbm + 1);
     for (int w = threadIdx.x; w < nzz * n2; w += blockDim.x) {
         const int zz = w / n2, az = zz / (lbm + 1), bz = zz % (lbm + 1);
-        int ly, lx;
-        decode_c2(lp, w % n2, ly, lx);
+        const int dc = dec2[w % n2], ly = dc & 255, lx = dc >> 8;
         if (ly + lx > (lam - az) + (lbm - bz)) continue;
         double acc = 0.0;
         for (int ay = 0; ay <= lam - az; ay++) {
@@ -1277,6 +1288,7 @@ __global__ __launch_bounds__(TR_NT) void k_hab(const PairDev *__restrict__ pairs
                                             double *__restrict__ gws, long long ws_stride) {
     extern __shared__ double sm_hab[];
     __shared__ int dec[NCOSET_LMAX];
+    __shared__ int dec2[NC2MAX];
     fill_decode(dec, threadIdx.x, blockDim.x);  // first read after the barrier below
     const int fx = with_forces ? 1 : 0;
     for (int sg = seg0 + blockIdx.x; sg < seg0 + nseg; sg += gridDim.x) {
@@ -1303,6 +1315,7 @@ __global__ __launch_bounds__(TR_NT) void k_hab(const PairDev *__restrict__ pairs
     double *eff = heff + nlax * nlbx;
     double *U = eff + ncoset(lam) * ncoset(lbm);
     fill_alpha(&pairs[pi], A, threadIdx.x, blockDim.x);
+    fill_decode2(dec2, lpx, threadIdx.x, blockDim.x);
     if (!mtrans) {
         for (int c = threadIdx.x; c < n3x; c += blockDim.x) M[c] = Mbuf[(long long)pi * mstride + c];
     } else {
@@ -1318,8 +1331,7 @@ __global__ __launch_bounds__(TR_NT) void k_hab(const PairDev *__restrict__ pairs
     const int nzz = (lamx + 1) * (lbmx + 1);
     for (int w = threadIdx.x; w < nzz * n2x; w += blockDim.x) {
         const int zz = w / n2x, az = zz / (lbmx + 1), bz = zz % (lbmx + 1);
-        int ly, lx;
-        decode_c2(lpx, w % n2x, ly, lx);
+        const int dc = dec2[w % n2x], ly = dc & 255, lx = dc >> 8;
         if (ly + lx > (lamx - az) + (lbmx - bz)) continue;
         double acc = 0.0;
         for (int lz = 0; lz <= az + bz; lz++) acc += M[cidx(lpx, lz, ly, lx)] * A.at(2, bz, az, lz);
