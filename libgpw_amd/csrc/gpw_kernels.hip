@@ -797,9 +797,11 @@ __device__ __forceinline__ bool classify_block(const Smem &sm, int q, int xg, in
 }
 
 // out_mode 0: grid += acc (splits == 1); 1: partial[split] = acc; 2: partial[split] += acc
-// General-cell collocate kernels (and the larger integrate classes) run one
-// workgroup per CU: the row data of the general path would otherwise spill
-// registers in the inner loops (measured faster despite the lower occupancy).
+// General-cell collocate kernels run one workgroup per CU: the row data of
+// the general path would otherwise spill registers in the inner loops
+// (measured faster despite the lower occupancy). The general integrate
+// kernels keep two (their x stage runs in two halves of the powers above
+// L = 8, which measured faster than one workgroup per CU without spills).
 template <int LM, bool GEN>
 __global__ __launch_bounds__(NROW, GEN ? 1 : 2) void k_collocate_tiles(
     TileGrid tg, const EntryRec *__restrict__ recs, const int *__restrict__ tile_start,
@@ -972,6 +974,95 @@ struct IntegSmem {
 // sum of a double over the 16 lanes of a DPP row; every lane gets the sum
 __device__ __forceinline__ double row_sum16(double v) { return stride4_sum(quad_sum(v)); }
 
+// x stage of integ_block for the powers L0 <= l < L1 of the x table, summed
+// over the lane quad and stored to the wave's transposition rows t1
+template <int L0, int L1, int LM, bool GEN>
+__device__ __forceinline__ void integ_x_part(IntegSmem<LM, GEN> &sm, int q, int xg, int yb, int z,
+                                             const int (&Tr)[RB], int mode, const GenBlock &gb,
+                                             const double (&v)[RB][XB], double *t1) {
+    constexpr int NL = L1 - L0;
+    double tx[RB][NL];
+#pragma unroll
+    for (int r = 0; r < RB; r++)
+#pragma unroll
+        for (int l = 0; l < NL; l++) tx[r][l] = 0.0;
+    if (mode != 0) {  // 1: masked, 2: whole block inside
+        const double *Xg = sm.X[q] + xg * TileCfg<LM>::GS;
+        const bool wfull = __all(mode == 2);  // wave-uniform among the active lanes
+        if constexpr (GEN) {
+            double e[RB], rho[RB];
+            if (gen_row_start(sm.gr, q, z, yb, gb, e, rho)) {
+#pragma unroll
+                for (int j = 0; j < XB; j++) {
+                    double xv[NL];
+#pragma unroll
+                    for (int l = 0; l < NL; l++) xv[l] = Xg[j * LM + L0 + l];
+#pragma unroll
+                    for (int r = 0; r < RB; r++) {
+                        const double w = (gb.bits[r] & (1 << j)) ? v[r][j] * e[r] : 0.0;
+#pragma unroll
+                        for (int l = 0; l < NL; l++) tx[r][l] += w * xv[l];
+                        if (j < XB - 1) {
+                            e[r] *= rho[r];
+                            rho[r] *= gb.kap;
+                        }
+                    }
+                }
+            } else {
+                // per-point exponentials (a block across a periodic wrap, or
+                // far from a steep Gaussian): one point per iteration, so that
+                // only RB exponentials are live at a time
+#pragma unroll 1
+                for (int j = 0; j < XB; j++) {
+                    double xv[NL];
+#pragma unroll
+                    for (int l = 0; l < NL; l++) xv[l] = Xg[j * LM + L0 + l];
+                    const int code = sm.xi[q][xg * XB + j];
+                    const double u = (double)code - gb.gp0;
+#pragma unroll
+                    for (int r = 0; r < RB; r++) {
+                        double vj = 0.0;
+#pragma unroll
+                        for (int jj = 0; jj < XB; jj++) vj = jj == j ? v[r][jj] : vj;
+                        const int row = z * TY + yb * RB + r;
+                        double qb, qc;
+                        gen_row_coef(sm.gr, q, z, yb * RB + r, qb, qc);
+                        const bool in = code >= sm.gr.lo[q][row] && code <= sm.gr.hi[q][row];
+                        const double w = in ? vj * exp((gb.qa * u + qb) * u + qc) : 0.0;
+#pragma unroll
+                        for (int l = 0; l < NL; l++) tx[r][l] += w * xv[l];
+                    }
+                }
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < XB; j++) {
+                double xv[NL];
+#pragma unroll
+                for (int l = 0; l < NL; l++) xv[l] = Xg[j * LM + L0 + l];
+                const int code = sm.xi[q][xg * XB + j];
+#pragma unroll
+                for (int r = 0; r < RB; r++) {
+                    double w = v[r][j];
+                    if (!wfull) w = code <= Tr[r] ? w : 0.0;
+#pragma unroll
+                    for (int l = 0; l < NL; l++) tx[r][l] += w * xv[l];
+                }
+            }
+        }
+    }
+    // rows of the quad summed over its four x blocks; lane xg stores row r = xg
+#pragma unroll
+    for (int r = 0; r < RB; r++) {
+#pragma unroll
+        for (int l = 0; l < NL; l++) tx[r][l] = quad_sum(tx[r][l]);
+        if (xg == r) {
+#pragma unroll
+            for (int l = 0; l < NL; l++) t1[(yb * RB + r) * RowT1<LM>::RS + L0 + l] = tx[r][l];
+        }
+    }
+}
+
 // Register block (rows y = 4*yb + r of plane z, points x = 8*xg + j) of pair
 // q: x stage in registers, sum of the four x blocks of each row over the lane
 // quad (DPP), then a wave-local transposition through LDS for the y stage:
@@ -993,87 +1084,12 @@ __device__ __forceinline__ void integ_block(IntegSmem<LM, GEN> &sm, int q, int x
         for (int c = cq; c < N2; c += 16) uzp[c] = 0.0;
         return;
     }
-    double tx[RB][L];
-#pragma unroll
-    for (int r = 0; r < RB; r++)
-#pragma unroll
-        for (int l = 0; l < L; l++) tx[r][l] = 0.0;
-    if (mode != 0) {  // 1: masked, 2: whole block inside
-        const double *Xg = sm.X[q] + xg * TileCfg<LM>::GS;
-        const bool wfull = __all(mode == 2);  // wave-uniform among the active lanes
-        if constexpr (GEN) {
-            double e[RB], rho[RB];
-            if (gen_row_start(sm.gr, q, z, yb, gb, e, rho)) {
-#pragma unroll
-                for (int j = 0; j < XB; j++) {
-                    double xv[L];
-#pragma unroll
-                    for (int l = 0; l < L; l++) xv[l] = Xg[j * LM + l];
-#pragma unroll
-                    for (int r = 0; r < RB; r++) {
-                        const double w = (gb.bits[r] & (1 << j)) ? v[r][j] * e[r] : 0.0;
-#pragma unroll
-                        for (int l = 0; l < L; l++) tx[r][l] += w * xv[l];
-                        if (j < XB - 1) {
-                            e[r] *= rho[r];
-                            rho[r] *= gb.kap;
-                        }
-                    }
-                }
-            } else {
-                // per-point exponentials (a block across a periodic wrap, or
-                // far from a steep Gaussian): one point per iteration, so that
-                // only RB exponentials are live at a time
-#pragma unroll 1
-                for (int j = 0; j < XB; j++) {
-                    double xv[L];
-#pragma unroll
-                    for (int l = 0; l < L; l++) xv[l] = Xg[j * LM + l];
-                    const int code = sm.xi[q][xg * XB + j];
-                    const double u = (double)code - gb.gp0;
-#pragma unroll
-                    for (int r = 0; r < RB; r++) {
-                        double vj = 0.0;
-#pragma unroll
-                        for (int jj = 0; jj < XB; jj++) vj = jj == j ? v[r][jj] : vj;
-                        const int row = z * TY + yb * RB + r;
-                        double qb, qc;
-                        gen_row_coef(sm.gr, q, z, yb * RB + r, qb, qc);
-                        const bool in = code >= sm.gr.lo[q][row] && code <= sm.gr.hi[q][row];
-                        const double w = in ? vj * exp((gb.qa * u + qb) * u + qc) : 0.0;
-#pragma unroll
-                        for (int l = 0; l < L; l++) tx[r][l] += w * xv[l];
-                    }
-                }
-            }
-        } else {
-#pragma unroll
-            for (int j = 0; j < XB; j++) {
-                double xv[L];
-#pragma unroll
-                for (int l = 0; l < L; l++) xv[l] = Xg[j * LM + l];
-                const int code = sm.xi[q][xg * XB + j];
-#pragma unroll
-                for (int r = 0; r < RB; r++) {
-                    double w = v[r][j];
-                    if (!wfull) w = code <= Tr[r] ? w : 0.0;
-#pragma unroll
-                    for (int l = 0; l < L; l++) tx[r][l] += w * xv[l];
-                }
-            }
-        }
-    }
-    // rows of the quad summed over its four x blocks; lane xg stores row r = xg
+    // general cells above L = 8: two passes over half the powers each (half
+    // the accumulator registers: no spills at two workgroups per CU)
+    constexpr int LA = (GEN && L > 8) ? (L + 1) / 2 : L;
     double *t1 = sm.t1[wave] + zl * ZS;
-#pragma unroll
-    for (int r = 0; r < RB; r++) {
-#pragma unroll
-        for (int l = 0; l < L; l++) tx[r][l] = quad_sum(tx[r][l]);
-        if (xg == r) {
-#pragma unroll
-            for (int l = 0; l < L; l++) t1[(yb * RB + r) * RowT1<LM>::RS + l] = tx[r][l];
-        }
-    }
+    integ_x_part<0, LA>(sm, q, xg, yb, z, Tr, mode, gb, v, t1);
+    if constexpr (LA < L) integ_x_part<LA, L>(sm, q, xg, yb, z, Tr, mode, gb, v, t1);
     __builtin_amdgcn_wave_barrier();
     const double *Yq = &sm.Y[q][0][0];
     constexpr int YS = TileCfg<LM>::YS;
@@ -1206,7 +1222,7 @@ __device__ __forceinline__ void integrate_tile(IntegSmem<LM, GEN> &sm, int bid, 
 }
 
 template <int LM, bool GEN>
-__global__ __launch_bounds__(NROW, (GEN && LM >= 12) ? 1 : 2) void k_integrate_tiles(
+__global__ __launch_bounds__(NROW, (GEN && LM >= 17) ? 1 : 2) void k_integrate_tiles(
     TileGrid tg, const EntryRec *__restrict__ recs, const int *__restrict__ tile_start,
     const double *__restrict__ grid, int lp_add, int splits, double *__restrict__ partial) {
     __shared__ IntegSmem<LM, GEN> sm;
