@@ -23,6 +23,11 @@ constexpr size_t COEF_LDS_MAX = 24 * 1024;
 constexpr size_t HAB_LDS_MAX = 64 * 1024;
 constexpr int HAB_WS_BLOCKS = 4096;
 constexpr int TR_NT = 256;
+// pairs whose cube is one sub-box of at most SMALL_NX x SMALL_NY x SMALL_NZ
+// points (general cells; SMALL_NX x SMALL_NYZ_ORTHO^2 on orthorhombic ones,
+// where the tile kernels are faster) are integrated by k_moments_small (one
+// workgroup per pair) instead of the tiles; limits chosen by measurement
+constexpr int SMALL_NX = 64, SMALL_NY = 48, SMALL_NZ = 48, SMALL_NYZ_ORTHO = 32, SMALL_LMAX = 12;
 
 __host__ __device__ constexpr int ncoset(int l) {
     return l < 0 ? 0 : ((l + 1) * (l + 2) * (l + 3)) / 6;

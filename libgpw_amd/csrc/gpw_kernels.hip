@@ -1262,8 +1262,9 @@ __global__ void k_fill(double *__restrict__ p, long long n, double v) {
 __global__ void k_sum_entries(const PairDev *__restrict__ pairs, int tau, int lp_add,
                               const int *__restrict__ pe_start, const long long *__restrict__ ent_off,
                               const double *__restrict__ partial, int mstride,
-                              double *__restrict__ Mbuf) {
+                              const unsigned char *__restrict__ skip, double *__restrict__ Mbuf) {
     const int p = blockIdx.x;
+    if (skip && skip[p]) return;  // moments written by k_moments_small
     const PairDev P = pairs[p];
     const int lp = P.lp + lp_add;
     const int n3 = ncoset(lp);
@@ -1455,6 +1456,150 @@ __global__ __launch_bounds__(TR_NT) void k_hab(const PairDev *__restrict__ pairs
     }
     }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Integrate of small spheres: one workgroup per pair whose cube is a single
+// sub-box of at most SMALL_NX x SMALL_NY x SMALL_NZ points. The tile kernels
+// sweep whole 32 x 16 x 16 tiles per (sub-box, tile) item, which for a sphere
+// a few points wide is mostly overhead; here the threads take the rows of the
+// sub-box instead. x stage along each row in registers, the Gaussian by the
+// recursion e(i+1) = e(i) rho(i), rho(i+1) = rho(i) kappa started at the
+// row's first point inside the sphere (no underflow); then the y and z stages
+// through LDS, in passes of whole z-planes of at most NROW rows. The point set
+// of a row is the tile kernels' (orthorhombic: the sphere profile; general:
+// the reference's quadratic bounds). Writes the pair's moments Mbuf[pair]
+// (k_sum_entries skips these pairs).
+
+template <int LM>
+struct SmallSmem {
+    double rows[NROW][LM];                    // x-stage result per row of the pass
+    double Yt[SMALL_NY][LM], Zt[SMALL_NZ][LM];  // powers of the row displacements
+    double t2[SMALL_NZ][LM * (LM + 1) / 2];   // per plane: (ly, lx) sums over y
+    double M[LM * (LM + 1) * (LM + 2) / 6];
+};
+
+template <int LM>
+__global__ __launch_bounds__(NROW) void k_moments_small(TileGrid tg, const SubDev *__restrict__ subs,
+                                                        const int *__restrict__ list,
+                                                        const int *__restrict__ prof,
+                                                        const double *__restrict__ grid, int lp_add,
+                                                        int mstride, double *__restrict__ Mbuf) {
+    __shared__ SmallSmem<LM> sm;
+    const SubDev s = subs[list[blockIdx.x]];
+    const bool gen = tg.general != 0;
+    const int lp = s.lp + lp_add, L = lp + 1;  // L <= LM (host classification)
+    const int ny = s.ub[1] - s.lb[1] + 1, nz = s.ub[2] - s.lb[2] + 1;
+    const int t = threadIdx.x;
+    const int n2 = ncoef2(lp), n3 = ncoset(lp);
+    for (int w = t; w < ny + nz; w += NROW) {
+        const int d = w < ny ? 1 : 2, idx = w < ny ? w : w - ny;
+        const int o = s.lb[d] + idx;
+        const double r = gen ? __dsub_rn((double)o, s.roff[d]) : o * tg.dr[d] - s.roff[d];
+        double *dst = d == 1 ? sm.Yt[idx] : sm.Zt[idx];
+        double v = 1.0;
+        for (int l = 0; l < L; l++) {
+            dst[l] = v;
+            v *= r;
+        }
+    }
+    for (int c = t; c < n3; c += NROW) sm.M[c] = 0.0;
+    // x recursion: exponent step kap2 between consecutive ratios
+    double qa;
+    if (gen) {
+        double a = 0.0;
+        for (int x = 0; x < 3; x++) a += tg.dh[0][x] * tg.dh[0][x];
+        qa = -s.zetp * a;
+    } else {
+        qa = -s.zetp * tg.dr[0] * tg.dr[0];
+    }
+    const double kap = exp(2.0 * qa);
+    const int P = max(1, min(nz, NROW / ny));  // planes per pass
+    for (int k0 = 0; k0 < nz; k0 += P) {
+        const int np = min(P, nz - k0);
+        double tx[LM];
+#pragma unroll
+        for (int l = 0; l < LM; l++) tx[l] = 0.0;
+        if (t < np * ny) {
+            const int y = t % ny, zp = t / ny;
+            const int j = s.lb[1] + y, k = s.lb[2] + k0 + zp;
+            int ilo = s.lb[0], ihi = s.ub[0];
+            double ex0 = 0.0, d0 = 0.0;
+            if (gen) {
+                const double dj = __dsub_rn((double)j, s.roff[1]), dk = __dsub_rn((double)k, s.roff[2]);
+                int lo, hi;
+                double qb_, qc_;
+                general_row_dev(tg, s.roff[0], dj, dk, s.R2, s.zetp, lo, hi, qb_, qc_);
+                ilo = max(ilo, lo);
+                ihi = min(ihi, hi);
+                // exponent qa u^2 + qb u + qc of the displacement u = i - gp0
+                double b = 0.0, c = 0.0;
+                for (int x = 0; x < 3; x++) {
+                    const double w = tg.dh[1][x] * dj + tg.dh[2][x] * dk;
+                    b += w * tg.dh[0][x];
+                    c += w * w;
+                }
+                const double qb = -2.0 * s.zetp * b, qc = -s.zetp * c;
+                const double u0 = (double)ilo - s.roff[0];
+                ex0 = (qa * u0 + qb) * u0 + qc;
+                d0 = qa * (2.0 * u0 + 1.0) + qb;
+            } else {
+                const int T = row_threshold(prof, s.prof_off, s.prof_stride, fold_off(k), fold_off(j));
+                if (T < 0) {
+                    ihi = ilo - 1;
+                } else {
+                    ilo = max(ilo, -T);
+                    ihi = min(ihi, T + 1);
+                }
+                const double rx = ilo * tg.dr[0] - s.roff[0];
+                const double ry = j * tg.dr[1] - s.roff[1], rz = k * tg.dr[2] - s.roff[2];
+                ex0 = -s.zetp * (rx * rx + ry * ry + rz * rz);
+                d0 = -s.zetp * (2.0 * rx * tg.dr[0] + tg.dr[0] * tg.dr[0]);
+            }
+            if (ilo <= ihi) {
+                const int gz = imod(s.base[2] + (k - s.lb[2]), s.period[2]);
+                const int gy = imod(s.base[1] + (j - s.lb[1]), s.period[1]);
+                const double *row = grid + ((long long)gz * tg.ny + gy) * tg.nx;
+                double e = exp(ex0), rho = exp(d0);
+                for (int i = ilo; i <= ihi; i++) {
+                    const double u = gen ? (double)i - s.roff[0] : i * tg.dr[0] - s.roff[0];
+                    double p = row[imod(s.base[0] + (i - s.lb[0]), s.period[0])] * e;
+#pragma unroll
+                    for (int l = 0; l < LM; l++) {
+                        if (l < L) {
+                            tx[l] += p;
+                            p *= u;
+                        }
+                    }
+                    e *= rho;
+                    rho *= kap;
+                }
+            }
+#pragma unroll
+            for (int l = 0; l < LM; l++)
+                if (l < L) sm.rows[t][l] = tx[l];
+        }
+        __syncthreads();
+        for (int w = t; w < np * n2; w += NROW) {
+            const int zp = w / n2;
+            int ly, lx;
+            decode_c2(lp, w % n2, ly, lx);
+            double a = 0.0;
+            for (int y = 0; y < ny; y++) a += sm.Yt[y][ly] * sm.rows[zp * ny + y][lx];
+            sm.t2[zp][w % n2] = a;
+        }
+        __syncthreads();
+        for (int c = t; c < n3; c += NROW) {
+            int lz, ly, lx;
+            decode_c3(lp, c, lz, ly, lx);
+            const int c2 = c2idx(lp, ly, lx);
+            double a = sm.M[c];
+            for (int zp = 0; zp < np; zp++) a += sm.Zt[k0 + zp][lz] * sm.t2[zp][c2];
+            sm.M[c] = a;
+        }
+        __syncthreads();
+    }
+    for (int c = t; c < n3; c += NROW) Mbuf[(long long)s.pair * mstride + c] = sm.M[c];
 }
 
 // ---------------------------------------------------------------------------
@@ -1814,9 +1959,24 @@ hipError_t launch_hab(const PairDev *pairs, const int *order, const int *segs, i
 
 hipError_t launch_sum_entries(const PairDev *pairs, int npairs, int tau, int lp_add,
                               const int *pe_start, const long long *ent_off,
-                              const double *partial, int mstride, double *Mbuf, hipStream_t st) {
+                              const double *partial, int mstride, const unsigned char *skip,
+                              double *Mbuf, hipStream_t st) {
     hipLaunchKernelGGL(k_sum_entries, dim3(npairs), dim3(64), 0, st, pairs, tau, lp_add, pe_start,
-                       ent_off, partial, mstride, Mbuf);
+                       ent_off, partial, mstride, skip, Mbuf);
+    return hipGetLastError();
+}
+
+// class c: L = lp + lp_add + 1 in (4c, 4c + 4], c <= 2
+hipError_t launch_moments_small(int cls, const TileGrid &tg, const SubDev *subs, const int *list,
+                                int n, const int *prof, const double *grid, int lp_add, int mstride,
+                                double *Mbuf, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    switch (cls) {
+    case 0: hipLaunchKernelGGL(k_moments_small<4>, dim3(n), dim3(NROW), 0, st, tg, subs, list, prof, grid, lp_add, mstride, Mbuf); break;
+    case 1: hipLaunchKernelGGL(k_moments_small<8>, dim3(n), dim3(NROW), 0, st, tg, subs, list, prof, grid, lp_add, mstride, Mbuf); break;
+    case 2: hipLaunchKernelGGL(k_moments_small<12>, dim3(n), dim3(NROW), 0, st, tg, subs, list, prof, grid, lp_add, mstride, Mbuf); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

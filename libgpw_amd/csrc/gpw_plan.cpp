@@ -51,9 +51,13 @@ hipError_t launch_hab(const PairDev *pairs, const int *order, const int *segs, i
                       long long ws_stride, int ws_blocks, hipStream_t st);
 size_t hab_ws_doubles(int lamx, int lbmx, int lam, int lbm);
 bool hab_needs_workspace(int lamx, int lbmx, int lam, int lbm);
+hipError_t launch_moments_small(int cls, const TileGrid &tg, const SubDev *subs, const int *list,
+                                int n, const int *prof, const double *grid, int lp_add, int mstride,
+                                double *Mbuf, hipStream_t st);
 hipError_t launch_sum_entries(const PairDev *pairs, int npairs, int tau, int lp_add,
                               const int *pe_start, const long long *ent_off,
-                              const double *partial, int mstride, double *Mbuf, hipStream_t st);
+                              const double *partial, int mstride, const unsigned char *skip,
+                              double *Mbuf, hipStream_t st);
 hipError_t launch_fill(double *p, long long n, double v, hipStream_t st);
 hipError_t launch_collocate_plain(const TileGrid &tg, const SubDev *subs, int si, int nz_box,
                                   int ny_box, const double *coef, const int *prof, double *grid,
@@ -179,6 +183,9 @@ static int class_of(int lp) {
 }
 
 enum { PH_COEF, PH_CTILES, PH_CREDUCE, PH_ITILES, PH_ISUM, PH_HAB, NPH };
+constexpr int NSET = 3, NSCLS = 3;
+// tile-list set of integrate without (wf = 0) / with (wf = 1) forces
+static inline int iset(int wf) { return wf ? 1 : 2; }
 constexpr int NEVENT = 2048;
 
 struct gpw_tasklist {
@@ -197,8 +204,14 @@ struct gpw_tasklist {
     int cpart_cap = 0;             // partial grids allocated in d_cpartial
     long long nentries = 0;
     long long cube_points = 0, tile_points = 0;
-    long long class_entries[2][NCLS] = {};  // [lp_add/2][class]
-    long long list_off[2][NCLS] = {};       // segment of the class in the tile lists
+    // tile lists: set 0 = collocate (every item), set 1 = integrate with the
+    // force extension (lp_add 2), set 2 = integrate without; the integrate
+    // sets leave out the pairs of the small-sphere kernel (small_*)
+    long long class_entries[NSET][NCLS] = {};
+    long long list_off[NSET][NCLS] = {};    // segment of the class in the tile lists
+    // sub-box ids of the pairs integrated by k_moments_small, per [force
+    // extension][class of L]: d_small[small_off[w][c] .. + small_cnt[w][c])
+    long long small_off[2][NSCLS] = {}, small_cnt[2][NSCLS] = {};
 
     std::vector<PairDev> pairs_h;
     std::vector<SubDev> subs_h;
@@ -207,7 +220,9 @@ struct gpw_tasklist {
     PairDev *d_pairs = nullptr;
     SubDev *d_subs = nullptr;
     int *d_prof = nullptr;
-    int *d_tile_start = nullptr;  // [2][NCLS][ntiles + 1]
+    int *d_tile_start = nullptr;  // [NSET][NCLS][ntiles + 1]
+    int *d_small = nullptr;
+    unsigned char *d_small_mask = nullptr;  // [2][npairs]: pair integrated by k_moments_small
     EntryRec *d_recs = nullptr;   // item records in tile-list order, per (lp_add, class)
     int *d_pe_start = nullptr;
     long long *d_ent_off0 = nullptr, *d_ent_off2 = nullptr;
@@ -241,7 +256,7 @@ struct gpw_tasklist {
     int events_used = 0;
 
     ~gpw_tasklist() {
-        void *ptrs[] = {d_pairs, d_subs, d_prof, d_tile_start, d_recs, d_pe_start,
+        void *ptrs[] = {d_pairs, d_subs, d_prof, d_tile_start, d_small, d_small_mask, d_recs, d_pe_start,
                         d_ent_off0, d_ent_off2, d_coef, d_partial, d_cpartial, d_Mbuf,
                         d_mtrans, d_mt_off, d_hab_ws, d_order, d_segs};
         for (void *p : ptrs)
@@ -530,8 +545,10 @@ struct HostPlan {
     std::vector<int> prof_flat;
     std::vector<int> ent_sub, pe_start;
     std::vector<long long> off0, off2;
-    std::vector<int> tile_start;  // [2][NCLS][ntiles + 1]
-    std::vector<EntryRec> recs;   // tile-list order, segments per (lp_add, class)
+    std::vector<int> tile_start;  // [NSET][NCLS][ntiles + 1]
+    std::vector<EntryRec> recs;   // tile-list order, segments per (set, class)
+    std::vector<int> small;       // sub-box ids for k_moments_small
+    std::vector<unsigned char> small_mask;  // [2][npairs]
     long long coef_total = 0, partial_total = 0;
 };
 
@@ -633,14 +650,46 @@ static gpw_status plan_tasklist(gpw_tasklist *tl, const std::vector<gpw_pair_des
         acc2 += ncoset(lp + 2);
     }
     hp.partial_total = acc2;
-    // per (lp_add, class): CSR over tiles of the item records
+    // pairs for the small-sphere integrate kernel, per force extension
+    std::vector<char> small[2];
+    for (int w = 0; w < 2; w++) small[w].assign(pairs.size(), 0);
+    hp.small.clear();
+    hp.small_mask.assign(2 * pairs.size(), 0);
+    for (int w = 0; w < 2; w++) {
+        std::vector<int> by_cls[NSCLS];
+        for (size_t p = 0; p < pairs.size(); p++) {
+            if (tl->pair_sub_start[p + 1] - tl->pair_sub_start[p] != 1) continue;
+            const int si = tl->pair_sub_start[p];
+            const SubDev &sd = tl->subs_h[si];
+            const int L = sd.lp + 2 * w + 1;
+            const int nyz = tl->tg.general ? SMALL_NY : SMALL_NYZ_ORTHO;
+            if (L > SMALL_LMAX || sd.ub[0] - sd.lb[0] + 1 > SMALL_NX ||
+                sd.ub[1] - sd.lb[1] + 1 > std::min(nyz, SMALL_NY) ||
+                sd.ub[2] - sd.lb[2] + 1 > std::min(nyz, SMALL_NZ))
+                continue;
+            small[w][p] = 1;
+            hp.small_mask[w * pairs.size() + p] = 1;
+            by_cls[(L - 1) / 4].push_back(si);
+        }
+        for (int c = 0; c < NSCLS; c++) {
+            tl->small_off[w][c] = (long long)hp.small.size();
+            tl->small_cnt[w][c] = (long long)by_cls[c].size();
+            hp.small.insert(hp.small.end(), by_cls[c].begin(), by_cls[c].end());
+        }
+    }
+    // per (set, class): CSR over tiles of the item records
     const int nt1 = tl->ntiles + 1;
-    hp.tile_start.assign((size_t)2 * NCLS * nt1, 0);
+    hp.tile_start.assign((size_t)NSET * NCLS * nt1, 0);
     hp.recs.clear();
-    hp.recs.reserve(2 * ne);
-    for (int a = 0; a < 2; a++) {
+    hp.recs.reserve(NSET * ne);
+    for (int a = 0; a < NSET; a++) {
+        const int lp_add = a == 1 ? 2 : 0;
         std::vector<int> cls(ne);
-        for (size_t e = 0; e < ne; e++) cls[e] = class_of(tl->subs_h[hp.ent_sub[e]].lp + 2 * a);
+        for (size_t e = 0; e < ne; e++) {
+            const SubDev &sd = tl->subs_h[hp.ent_sub[e]];
+            const bool skip = a == 1 ? small[1][sd.pair] : a == 2 ? small[0][sd.pair] : false;
+            cls[e] = skip ? -1 : class_of(sd.lp + lp_add);
+        }
         for (int c = 0; c < NCLS; c++) {
             int *start = hp.tile_start.data() + (size_t)(a * NCLS + c) * nt1;
             for (size_t e = 0; e < ne; e++)
@@ -663,7 +712,7 @@ static gpw_status plan_tasklist(gpw_tasklist *tl, const std::vector<gpw_pair_des
                 }
                 r.zetp = sd.zetp;
                 r.R2 = sub_R2[hp.ent_sub[e]];
-                r.ent_off = a ? hp.off2[e] : hp.off0[e];
+                r.ent_off = a == 1 ? hp.off2[e] : hp.off0[e];
                 r.lp = sd.lp;
                 r.coef_off = sd.coef_off;
                 list[fillp[ent_tile[e]]++] = r;
@@ -748,6 +797,8 @@ static gpw_status upload_tasklist(gpw_tasklist *tl, const HostPlan &hp) {
     if ((st = upload(&tl->d_subs, tl->subs_h))) return st;
     if ((st = upload(&tl->d_prof, hp.prof_flat))) return st;
     if ((st = upload(&tl->d_tile_start, hp.tile_start))) return st;
+    if ((st = upload(&tl->d_small, hp.small))) return st;
+    if ((st = upload(&tl->d_small_mask, hp.small_mask))) return st;
     if ((st = upload(&tl->d_recs, hp.recs))) return st;
     if ((st = upload(&tl->d_pe_start, hp.pe_start))) return st;
     if ((st = upload(&tl->d_ent_off0, hp.off0))) return st;
@@ -1126,10 +1177,11 @@ gpw_status gpw_tasklist_autotune(gpw_tasklist *tl, int max_splits, int reps, voi
     std::copy(bestc, bestc + NCLS, tl->csplit);
     // integrate, without and with the force extension
     for (int wf = 0; wf < 2; wf++) {
+        const int is = iset(wf);
         for (int c = 0; c < NCLS; c++) {
-            if (tl->class_entries[wf][c] == 0) continue;
-            const int *start = tl->d_tile_start + (size_t)(wf * NCLS + c) * nt1;
-            const EntryRec *recs = tl->d_recs + tl->list_off[wf][c];
+            if (tl->class_entries[is][c] == 0) continue;
+            const int *start = tl->d_tile_start + (size_t)(is * NCLS + c) * nt1;
+            const EntryRec *recs = tl->d_recs + tl->list_off[is][c];
             float bt = 1e30f, t0 = 0.0f;
             int bs = 1;
             for (int i = 0; i < nc; i++) {
@@ -1220,29 +1272,36 @@ gpw_status gpw_tasklist_integrate(gpw_tasklist *tl, const double *grid, const do
     const int lp_add = 2 * wf;
     const long long *off = wf ? tl->d_ent_off2 : tl->d_ent_off0;
     const int nt1 = tl->ntiles + 1;
+    const int is = iset(wf);
     {
         PhaseTimer ph(tl, st, PH_ITILES);
-        const bool only01 = tl->class_entries[wf][2] == 0 && tl->class_entries[wf][3] == 0;
-        if (!tl->tg.general && only01 && tl->class_entries[wf][0] && tl->class_entries[wf][1]) {
+        const bool only01 = tl->class_entries[is][2] == 0 && tl->class_entries[is][3] == 0;
+        if (!tl->tg.general && only01 && tl->class_entries[is][0] && tl->class_entries[is][1]) {
             // the two light classes together in one launch (class 1 first)
             HIP_TRY(launch_integrate_tiles_84(
-                tl->tg, tl->d_recs + tl->list_off[wf][1], tl->d_tile_start + (size_t)(wf * NCLS + 1) * nt1,
-                tl->isplit[wf][1], tl->d_recs + tl->list_off[wf][0],
-                tl->d_tile_start + (size_t)(wf * NCLS + 0) * nt1, tl->isplit[wf][0], grid, lp_add,
+                tl->tg, tl->d_recs + tl->list_off[is][1], tl->d_tile_start + (size_t)(is * NCLS + 1) * nt1,
+                tl->isplit[wf][1], tl->d_recs + tl->list_off[is][0],
+                tl->d_tile_start + (size_t)(is * NCLS + 0) * nt1, tl->isplit[wf][0], grid, lp_add,
                 tl->ntiles, tl->d_partial, st));
         } else
         for (int c = 0; c < NCLS; c++) {
-            if (tl->class_entries[wf][c] == 0) continue;
-            const int *start = tl->d_tile_start + (size_t)(wf * NCLS + c) * nt1;
-            const EntryRec *recs = tl->d_recs + tl->list_off[wf][c];
+            if (tl->class_entries[is][c] == 0) continue;
+            const int *start = tl->d_tile_start + (size_t)(is * NCLS + c) * nt1;
+            const EntryRec *recs = tl->d_recs + tl->list_off[is][c];
             HIP_TRY(launch_integrate_tiles(c, tl->tg, recs, start, grid, lp_add, tl->ntiles,
                                            tl->isplit[wf][c], tl->d_partial, st));
         }
+        // small spheres: one workgroup per pair, moments straight into Mbuf
+        for (int c = 0; c < NSCLS; c++)
+            HIP_TRY(launch_moments_small(c, tl->tg, tl->d_subs, tl->d_small + tl->small_off[wf][c],
+                                         (int)tl->small_cnt[wf][c], tl->d_prof, grid, lp_add,
+                                         tl->mstride, tl->d_Mbuf, st));
     }
     {
         PhaseTimer ph(tl, st, PH_ISUM);
         HIP_TRY(launch_sum_entries(tl->d_pairs, tl->npairs, tl->compute_tau, lp_add, tl->d_pe_start,
-                                   off, tl->d_partial, tl->mstride, tl->d_Mbuf, st));
+                                   off, tl->d_partial, tl->mstride,
+                                   tl->d_small_mask + (size_t)wf * tl->npairs, tl->d_Mbuf, st));
     }
     {
         PhaseTimer ph(tl, st, PH_HAB);
