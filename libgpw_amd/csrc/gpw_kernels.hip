@@ -1561,9 +1561,11 @@ __global__ __launch_bounds__(NROW) void k_moments_small(TileGrid tg, const SubDe
                 const int gy = imod(s.base[1] + (j - s.lb[1]), s.period[1]);
                 const double *row = grid + ((long long)gz * tg.ny + gy) * tg.nx;
                 double e = exp(ex0), rho = exp(d0);
+                int gx = imod(s.base[0] + (ilo - s.lb[0]), s.period[0]);  // then stepped, wrapped
                 for (int i = ilo; i <= ihi; i++) {
                     const double u = gen ? (double)i - s.roff[0] : i * tg.dr[0] - s.roff[0];
-                    double p = row[imod(s.base[0] + (i - s.lb[0]), s.period[0])] * e;
+                    double p = row[gx] * e;
+                    gx = gx + 1 == s.period[0] ? 0 : gx + 1;
 #pragma unroll
                     for (int l = 0; l < LM; l++) {
                         if (l < L) {
