@@ -1471,11 +1471,13 @@ __global__ __launch_bounds__(TR_NT) void k_hab(const PairDev *__restrict__ pairs
 // the reference's quadratic bounds). Writes the pair's moments Mbuf[pair]
 // (k_sum_entries skips these pairs).
 
+constexpr int SMALL_PMAX = 16;  // z-planes per pass (LDS of t2)
+
 template <int LM>
 struct SmallSmem {
     double rows[NROW][LM];                    // x-stage result per row of the pass
     double Yt[SMALL_NY][LM], Zt[SMALL_NZ][LM];  // powers of the row displacements
-    double t2[SMALL_NZ][LM * (LM + 1) / 2];   // per plane: (ly, lx) sums over y
+    double t2[SMALL_PMAX][LM * (LM + 1) / 2]; // per plane of the pass: (ly, lx) sums over y
     double M[LM * (LM + 1) * (LM + 2) / 6];
 };
 
@@ -1514,7 +1516,7 @@ __global__ __launch_bounds__(NROW) void k_moments_small(TileGrid tg, const SubDe
         qa = -s.zetp * tg.dr[0] * tg.dr[0];
     }
     const double kap = exp(2.0 * qa);
-    const int P = max(1, min(nz, NROW / ny));  // planes per pass
+    const int P = max(1, min(min(nz, SMALL_PMAX), NROW / ny));  // planes per pass
     for (int k0 = 0; k0 < nz; k0 += P) {
         const int np = min(P, nz - k0);
         double tx[LM];
