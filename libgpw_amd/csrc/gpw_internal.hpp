@@ -52,6 +52,18 @@ __host__ __device__ constexpr int c2idx(int lp, int lyp, int lxp) {
     return lyp * (lp + 1) - (lyp * (lyp - 1)) / 2 + lxp;
 }
 
+// Lattice transforms of a general cell in CSR form, for every lp <= LPMAX and
+// both directions (dir 0: coef_ijk = T coef_xyz, dir 1: M_xyz = T^T M_ijk).
+// Rows of (lp, dir) start at rp + rp_off[2 * lp + dir]; their entries are
+// absolute indices into col / val. T is block-diagonal by degree and, for
+// cells with axis-aligned steps, sparse within the blocks.
+struct LatticeCSR {
+    const int *rp = nullptr;
+    const int *col = nullptr;
+    const double *val = nullptr;
+    const long long *rp_off = nullptr;
+};
+
 // Per-pair parameters on the device (one per primitive pair).
 struct PairDev {
     double zeta, zetb, prefactor, zetp;

@@ -188,23 +188,14 @@ __device__ void build_eff(const PairDev &p, int tau, const double *__restrict__ 
     __syncthreads();
 }
 
-// General cells: apply the lattice transform of degree-preserving blocks.
-// T (C3(lp) x C3(lp), both indices cidx(lp, z, y, x)) only couples monomials
-// of the same total degree, so row r of degree d only visits the C2(d)
-// columns of degree d. transpose: out[c] = sum_r T[r][c] in[r] instead.
-__device__ void lattice_transform(const double *__restrict__ T, int lp, const double *in, double *out,
-                                  bool transpose, int tid, int nthr) {
-    const int n3 = ncoset(lp);
-    for (int r = tid; r < n3; r += nthr) {
-        int lz, ly, lx;
-        decode_c3(lp, r, lz, ly, lx);
-        const int d = lz + ly + lx;
+// General cells: out = T in (dir 0) or T^T in (dir 1) for the monomials of
+// total degree <= lp, from the CSR rows of (lp, dir).
+__device__ void lattice_transform(const LatticeCSR &lt, int lp, int dir, const double *in,
+                                  double *out, int tid, int nthr) {
+    const int *rp = lt.rp + lt.rp_off[2 * lp + dir];
+    for (int r = tid; r < ncoset(lp); r += nthr) {
         double a = 0.0;
-        for (int kz = 0; kz <= d; kz++)
-            for (int ky = 0; ky <= d - kz; ky++) {
-                const int c = cidx(lp, kz, ky, d - kz - ky);
-                a += (transpose ? T[(long long)c * n3 + r] : T[(long long)r * n3 + c]) * in[c];
-            }
+        for (int k = rp[r]; k < rp[r + 1]; k++) a += lt.val[k] * in[lt.col[k]];
         out[r] = a;
     }
 }
@@ -215,7 +206,7 @@ __device__ void lattice_transform(const double *__restrict__ T, int lp, const do
 //   coef[(lz,ly,lx)]    = prefactor sum_{az,bz} Ty[az][bz][(ly,lx)] alpha_z(bz,az,lz)
 // with (ax, bx) over the shells of the (tau-shifted) block; Ty holds only
 // ly + lx <= (lam - az) + (lbm - bz). Dynamic LDS: alpha, eff, Ty (+ cxyz).
-// mtrans (general cells): coef_ijk = T coef_xyz, T the multinomial expansion
+// lt.rp set (general cells): coef_ijk = T coef_xyz, T the multinomial expansion
 // of grid_base_ref_c.cc:647-698 for each lp.
 // Workgroups loop over pairs (pair0 + blockIdx.x, stride gridDim.x); eff, Ty
 // and cxyz live in LDS after alpha, or, when gws is given (blocks too large
@@ -224,8 +215,7 @@ __global__ __launch_bounds__(TR_NT) void k_coef(const PairDev *__restrict__ pair
                                              const int *__restrict__ order, int pair0,
                                              int npairs, int tau, const double *__restrict__ pab,
                                              double *__restrict__ coef,
-                                             const double *__restrict__ mtrans,
-                                             const long long *__restrict__ mt_off,
+                                             LatticeCSR lt,
                                              double *__restrict__ gws, long long ws_stride) {
     extern __shared__ double sm_coef[];
     __shared__ int dec2[NC2MAX];
@@ -268,7 +258,7 @@ __global__ __launch_bounds__(TR_NT) void k_coef(const PairDev *__restrict__ pair
         Ty[w] = acc;
     }
     __syncthreads();
-    double *dst = mtrans ? cxyz : coef + p.coef_off;
+    double *dst = lt.rp ? cxyz : coef + p.coef_off;
     for (int c = threadIdx.x; c < n3; c += blockDim.x) {
         int lz, ly, lx;
         decode_c3(lp, c, lz, ly, lx);
@@ -281,10 +271,9 @@ __global__ __launch_bounds__(TR_NT) void k_coef(const PairDev *__restrict__ pair
             }
         dst[c] = p.prefactor * acc;
     }
-    if (!mtrans) continue;
+    if (!lt.rp) continue;
     __syncthreads();
-    lattice_transform(mtrans + mt_off[lp], lp, cxyz, coef + p.coef_off, false, threadIdx.x,
-                      blockDim.x);
+    lattice_transform(lt, lp, 0, cxyz, coef + p.coef_off, threadIdx.x, blockDim.x);
     }
 }
 
@@ -1300,8 +1289,7 @@ __global__ __launch_bounds__(TR_NT) void k_hab(const PairDev *__restrict__ pairs
                                             int tau, int with_forces, const double *__restrict__ Mbuf,
                                             int mstride, const double *__restrict__ pab,
                                             double *__restrict__ hab, double *__restrict__ forces,
-                                            const double *__restrict__ mtrans,
-                                            const long long *__restrict__ mt_off,
+                                            LatticeCSR lt,
                                             double *__restrict__ gws, long long ws_stride) {
     extern __shared__ double sm_hab[];
     __shared__ int dec[NCOSET_LMAX];
@@ -1333,13 +1321,13 @@ __global__ __launch_bounds__(TR_NT) void k_hab(const PairDev *__restrict__ pairs
     double *U = eff + ncoset(lam) * ncoset(lbm);
     fill_alpha(&pairs[pi], A, threadIdx.x, blockDim.x);
     fill_decode2(dec2, lpx, threadIdx.x, blockDim.x);
-    if (!mtrans) {
+    if (!lt.rp) {
         for (int c = threadIdx.x; c < n3x; c += blockDim.x) M[c] = Mbuf[(long long)pi * mstride + c];
     } else {
         // M_xyz = T^T M_ijk (adjoint of the lattice transform)
         for (int c = threadIdx.x; c < n3x; c += blockDim.x) Mijk[c] = Mbuf[(long long)pi * mstride + c];
         __syncthreads();
-        lattice_transform(mtrans + mt_off[lpx], lpx, Mijk, M, true, threadIdx.x, blockDim.x);
+        lattice_transform(lt, lpx, 1, Mijk, M, threadIdx.x, blockDim.x);
     }
     __syncthreads();
     // adjoint of the factorised cab -> cxyz transform (see k_coef):
@@ -1909,7 +1897,7 @@ bool coef_needs_workspace(int lam, int lbm) { return coef_smem_bytes(lam, lbm, t
 // la_max / lb_max are at most la_max / lb_max (LDS sizing)
 hipError_t launch_coef(const PairDev *pairs, const int *order, int pair0, int npairs, int tau,
                        int lam_max, int lbm_max, const double *pab, double *coef,
-                       const double *mtrans, const long long *mt_off, double *ws,
+                       const LatticeCSR &lt, double *ws,
                        long long ws_stride, int ws_blocks, hipStream_t st) {
     if (npairs <= 0) return hipSuccess;
     const int lam = lam_max + tau, lbm = lbm_max + tau;
@@ -1919,7 +1907,7 @@ hipError_t launch_coef(const PairDev *pairs, const int *order, int pair0, int np
     const size_t smem = coef_smem_bytes(lam, lbm, !big);
     const int nblk = big ? std::min(npairs, ws_blocks) : npairs;
     hipLaunchKernelGGL(k_coef, dim3(nblk), dim3(transform_threads(lam + lbm)), smem, st, pairs, order, pair0, npairs, tau, pab,
-                       coef, mtrans, mt_off, big ? ws : nullptr, big ? ws_stride : 0LL);
+                       coef, lt, big ? ws : nullptr, big ? ws_stride : 0LL);
     return hipGetLastError();
 }
 
@@ -1943,7 +1931,7 @@ bool hab_needs_workspace(int lamx, int lbmx, int lam, int lbm) {
 hipError_t launch_hab(const PairDev *pairs, const int *order, const int *segs, int seg0, int nseg,
                       int tau, int with_forces, int la_max, int lb_max, const double *Mbuf,
                       int mstride, const double *pab, double *hab, double *forces,
-                      const double *mtrans, const long long *mt_off, double *ws,
+                      const LatticeCSR &lt, double *ws,
                       long long ws_stride, int ws_blocks, hipStream_t st) {
     if (nseg <= 0) return hipSuccess;
     const int npairs = nseg;
@@ -1956,7 +1944,7 @@ hipError_t launch_hab(const PairDev *pairs, const int *order, const int *segs, i
     const size_t smem = hab_smem_bytes(lamx, lbmx, la_max + tau, lb_max + tau, !big);
     const int nblk = big ? std::min(npairs, ws_blocks) : npairs;
     hipLaunchKernelGGL(k_hab, dim3(nblk), dim3(transform_threads(lamx + lbmx)), smem, st, pairs, order, segs, seg0, nseg, tau,
-                       with_forces, Mbuf, mstride, pab, hab, forces, mtrans, mt_off,
+                       with_forces, Mbuf, mstride, pab, hab, forces, lt,
                        big ? ws : nullptr, big ? ws_stride : 0LL);
     return hipGetLastError();
 }

@@ -39,7 +39,7 @@ hipError_t launch_integrate_tiles_84(const TileGrid &tg, const EntryRec *recs8, 
                                      double *partial, hipStream_t st);
 hipError_t launch_coef(const PairDev *pairs, const int *order, int pair0, int npairs, int tau,
                        int lam_max, int lbm_max, const double *pab, double *coef,
-                       const double *mtrans, const long long *mt_off, double *ws,
+                       const LatticeCSR &lt, double *ws,
                        long long ws_stride, int ws_blocks, hipStream_t st);
 size_t coef_smem_bytes(int lam, int lbm, bool lds_blocks);
 size_t coef_ws_doubles(int lam, int lbm);
@@ -47,7 +47,7 @@ bool coef_needs_workspace(int lam, int lbm);
 hipError_t launch_hab(const PairDev *pairs, const int *order, const int *segs, int seg0, int nseg,
                       int tau, int with_forces, int la_max, int lb_max, const double *Mbuf,
                       int mstride, const double *pab, double *hab, double *forces,
-                      const double *mtrans, const long long *mt_off, double *ws,
+                      const LatticeCSR &lt, double *ws,
                       long long ws_stride, int ws_blocks, hipStream_t st);
 size_t hab_ws_doubles(int lamx, int lbmx, int lam, int lbm);
 bool hab_needs_workspace(int lamx, int lbmx, int lam, int lbm);
@@ -231,8 +231,11 @@ struct gpw_tasklist {
     double *d_cpartial = nullptr;  // collocate partial grids (splits > 1)
     double *d_Mbuf = nullptr;
     int mstride = 0;
-    double *d_mtrans = nullptr;      // general cells: lattice transform per lp
-    long long *d_mt_off = nullptr;
+    // general cells: lattice transforms (CSR) per lp and direction
+    int *d_mt_rp = nullptr, *d_mt_col = nullptr;
+    double *d_mt_val = nullptr;
+    long long *d_mt_rpoff = nullptr;
+    LatticeCSR lat() const { return LatticeCSR{d_mt_rp, d_mt_col, d_mt_val, d_mt_rpoff}; }
     long long coef_total = 0;        // doubles of d_coef
     // pairs grouped by block size for the per-pair transform kernels (one
     // launch per group, LDS sized for the group): d_order[start .. start+count)
@@ -258,7 +261,7 @@ struct gpw_tasklist {
     ~gpw_tasklist() {
         void *ptrs[] = {d_pairs, d_subs, d_prof, d_tile_start, d_small, d_small_mask, d_recs, d_pe_start,
                         d_ent_off0, d_ent_off2, d_coef, d_partial, d_cpartial, d_Mbuf,
-                        d_mtrans, d_mt_off, d_hab_ws, d_order, d_segs};
+                        d_mt_rp, d_mt_col, d_mt_val, d_mt_rpoff, d_hab_ws, d_order, d_segs};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
@@ -838,8 +841,33 @@ static gpw_status upload_tasklist(gpw_tasklist *tl, const HostPlan &hp) {
         }
         std::vector<double> T((size_t)total);
         for (int lp = 0; lp <= lpm; lp++) multinomial_matrix(tl->grid.dh, lp, T.data() + off[lp]);
-        if ((st = upload(&tl->d_mtrans, T))) return st;
-        if ((st = upload(&tl->d_mt_off, off))) return st;
+        // CSR of T (dir 0) and of its transpose (dir 1), exact zeros dropped
+        std::vector<int> rp, col;
+        std::vector<double> val;
+        std::vector<long long> rpoff(2 * (LPMAX + 1), 0);
+        for (int lp = 0; lp <= lpm; lp++) {
+            const int n = ncoset(lp);
+            const double *Tl = T.data() + off[lp];
+            for (int dir = 0; dir < 2; dir++) {
+                rpoff[2 * lp + dir] = (long long)rp.size();
+                for (int r = 0; r < n; r++) {
+                    rp.push_back((int)val.size());
+                    for (int c = 0; c < n; c++) {
+                        const double v = dir == 0 ? Tl[(size_t)r * n + c] : Tl[(size_t)c * n + r];
+                        if (v != 0.0) {
+                            col.push_back(c);
+                            val.push_back(v);
+                        }
+                    }
+                }
+                rp.push_back((int)val.size());
+            }
+        }
+        if (val.size() > 0x7fffffffULL) return fail(GPW_ERR_ARG, "lattice transform too large");
+        if ((st = upload(&tl->d_mt_rp, rp))) return st;
+        if ((st = upload(&tl->d_mt_col, col))) return st;
+        if ((st = upload(&tl->d_mt_val, val))) return st;
+        if ((st = upload(&tl->d_mt_rpoff, rpoff))) return st;
     }
     return GPW_OK;
 }
@@ -1230,12 +1258,12 @@ static gpw_status run_coef(gpw_tasklist *tl, const double *pab, int pair0, int n
     if (pair0 == 0 && n == tl->npairs) {  // the whole list: one launch per size group
         for (const auto &g : tl->groups)
             HIP_TRY(launch_coef(tl->d_pairs, tl->d_order, g.start, g.count, tl->compute_tau, g.la_max,
-                                g.lb_max, pab, tl->d_coef, tl->d_mtrans, tl->d_mt_off, tl->d_hab_ws,
+                                g.lb_max, pab, tl->d_coef, tl->lat(), tl->d_hab_ws,
                                 tl->hab_ws_stride, tl->hab_ws_blocks, st));
         return GPW_OK;
     }
     HIP_TRY(launch_coef(tl->d_pairs, nullptr, pair0, n, tl->compute_tau, tl->la_max, tl->lb_max, pab,
-                        tl->d_coef, tl->d_mtrans, tl->d_mt_off, tl->d_hab_ws, tl->hab_ws_stride,
+                        tl->d_coef, tl->lat(), tl->d_hab_ws, tl->hab_ws_stride,
                         tl->hab_ws_blocks, st));
     return GPW_OK;
 }
@@ -1308,7 +1336,7 @@ gpw_status gpw_tasklist_integrate(gpw_tasklist *tl, const double *grid, const do
         for (const auto &g : tl->groups)
             HIP_TRY(launch_hab(tl->d_pairs, tl->d_order, tl->d_segs, g.seg0, g.nseg, tl->compute_tau,
                                wf, g.la_max, g.lb_max, tl->d_Mbuf, tl->mstride, pab, hab, forces,
-                               tl->d_mtrans, tl->d_mt_off, tl->d_hab_ws, tl->hab_ws_stride,
+                               tl->lat(), tl->d_hab_ws, tl->hab_ws_stride,
                                tl->hab_ws_blocks, st));
     }
     return GPW_OK;
@@ -1374,7 +1402,7 @@ gpw_status gpw_tasklist_integrate_plain(gpw_tasklist *tl, const double *grid, co
         }
         HIP_TRY(launch_hab(tl->d_pairs, nullptr, nullptr, p, 1, tl->compute_tau, wf, tl->pairs_h[p].la_max,
                            tl->pairs_h[p].lb_max, tl->d_Mbuf, tl->mstride, pab, hab, forces,
-                           tl->d_mtrans, tl->d_mt_off, tl->d_hab_ws, tl->hab_ws_stride,
+                           tl->lat(), tl->d_hab_ws, tl->hab_ws_stride,
                            tl->hab_ws_blocks, st));
     }
     return GPW_OK;
