@@ -1345,10 +1345,25 @@ __global__ __launch_bounds__(TR_NT) void k_hab(const PairDev *__restrict__ pairs
     __syncthreads();
     const int na_lo = ncoset(lamx_lo - 1), nb_lo = ncoset(lbmx_lo - 1);
     // only the shells lamx_lo..lamx, lbmx_lo..lbmx: the hab and force stages
-    // read nothing else
+    // read nothing else. Forces without tau read a cross of it: the block's
+    // own b shells with every a shell, and its own a shells with every b
+    // shell (not the corners a +- 1 with b +- 1), so only the cross is built.
     const int nAx = nlax - na_lo, nBx = nlbx - nb_lo;
-    for (int w = threadIdx.x; w < nAx * nBx; w += blockDim.x) {
-        const int ico = na_lo + w % nAx, jco = nb_lo + w / nAx;
+    const bool cross = with_forces && !tau;
+    const int na_mid = ncoset(p.la_min - 1), nAmid = ncoset(p.la_max) - na_mid;
+    const int nb_mid = ncoset(p.lb_min - 1), nBmid = ncoset(p.lb_max) - nb_mid;
+    const int n1 = cross ? nBmid * nAx : nAx * nBx;
+    const int nout = cross ? n1 + (nBx - nBmid) * nAmid : n1;
+    for (int w = threadIdx.x; w < nout; w += blockDim.x) {
+        int ico, jco;
+        if (w < n1) {
+            ico = na_lo + w % nAx;
+            jco = (cross ? nb_mid : nb_lo) + w / nAx;
+        } else {
+            const int w2 = w - n1, r = w2 / nAmid, nlow = nb_mid - nb_lo;
+            ico = na_mid + w2 % nAmid;
+            jco = r < nlow ? nb_lo + r : nb_mid + nBmid + (r - nlow);
+        }
         const int da = dec[ico], db = dec[jco];
         const int ax = da & 15, ay = (da >> 4) & 15, az = da >> 8;
         const int bx = db & 15, by = (db >> 4) & 15, bz = db >> 8;
