@@ -57,7 +57,8 @@ def roofline(pts, coll_ms, integ_ms, ms_per_step, workload):
         except (OSError, ValueError):
             traffic = None
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom[0],
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": dom[0] if dom[0] != "k_integrate_tiles" else "k_integrate_tiles + k_moments_small",
             "kernel_ms": round(dom[2], 4), "algorithmic_bytes": dom[1],
             "step_frac": round((coll_bytes + integ_bytes) / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
