@@ -1338,8 +1338,17 @@ __global__ __launch_bounds__(TR_NT) void k_hab(const PairDev *__restrict__ pairs
         const int zz = w / n2x, az = zz / (lbmx + 1), bz = zz % (lbmx + 1);
         const int dc = dec2[w % n2x], ly = dc & 255, lx = dc >> 8;
         if (ly + lx > (lamx - az) + (lbmx - bz)) continue;
+        // two accumulators: shorter dependency chains
         double acc = 0.0;
-        for (int lz = 0; lz <= az + bz; lz++) acc += M[cidx(lpx, lz, ly, lx)] * A.at(2, bz, az, lz);
+        const double *Az = &A.at(2, bz, az, 0);
+        double acc2 = 0.0;
+        int lz = 0;
+        for (; lz + 1 <= az + bz; lz += 2) {
+            acc += M[cidx(lpx, lz, ly, lx)] * Az[lz];
+            acc2 += M[cidx(lpx, lz + 1, ly, lx)] * Az[lz + 1];
+        }
+        if (lz <= az + bz) acc += M[cidx(lpx, lz, ly, lx)] * Az[lz];
+        acc += acc2;
         U[w] = acc;
     }
     __syncthreads();
@@ -1373,7 +1382,14 @@ __global__ __launch_bounds__(TR_NT) void k_hab(const PairDev *__restrict__ pairs
         for (int ly = 0; ly <= ay + by; ly++) {
             const double *Ur = Uz + c2idx(lpx, ly, 0);
             double sx = 0.0;
-            for (int lx = 0; lx <= ax + bx; lx++) sx += Ur[lx] * Ax[lx];
+            double sx2 = 0.0;
+            int lx = 0;
+            for (; lx + 1 <= ax + bx; lx += 2) {
+                sx += Ur[lx] * Ax[lx];
+                sx2 += Ur[lx + 1] * Ax[lx + 1];
+            }
+            if (lx <= ax + bx) sx += Ur[lx] * Ax[lx];
+            sx += sx2;
             acc += sx * Ay[ly];
         }
         heff[jco * nlax + ico] = p.prefactor * acc;
