@@ -634,3 +634,41 @@ def test_general_tasklist_row_recursion_edges(gpw, oracle, torch_cuda, case):
         assert rel_err(got["grid"], ref["grid"]) < TOL, splits
         assert rel_err(got["hab"], ref["hab"]) < TOL, splits
         assert rel_err(got["forces"], ref["forces"]) < TOL, splits
+
+
+@pytest.mark.parametrize("cell_kind", ["ortho", "general"])
+def test_tasklist_small_sphere_limits(gpw, oracle, torch_cuda, cell_kind):
+    """Pairs whose sub-boxes lie on both sides of the size limits of the
+    small-sphere integrate kernel (32 rows per axis on orthorhombic cells, 48
+    on general ones; several passes of z-planes for the larger ones) in one
+    task list: collocate, integrate and forces against the oracle."""
+    torch = torch_cuda
+    rng = np.random.default_rng(61 if cell_kind == "ortho" else 62)
+    npts, cell, n = (80, 72, 76), np.array([16.0, 14.4, 15.2]), 12
+    dh = np.diag(cell / np.asarray(npts))
+    if cell_kind == "general":
+        dh[1, 0] = 0.25 * dh[0, 0]
+        zp = rng.uniform(0.8, 1.6, n)    # lattice boxes about 40 to 56 rows wide
+    else:
+        zp = rng.uniform(1.9, 3.4, n)    # boxes about 27 to 36 rows wide
+    f = rng.uniform(0.3, 0.7, n)
+    zeta, zetb = zp * f, zp * (1.0 - f)
+    la = rng.integers(0, 3, n); lb = rng.integers(0, 3, n)
+    ra = rng.uniform(0.0, 1.0, (n, 3)) @ (dh * np.asarray(npts)[:, None])
+    rab = rng.normal(0.0, 0.3, (n, 3))
+    sizes = np.array([gpw.ncoset(int(b)) * gpw.ncoset(int(a)) for a, b in zip(la, lb)])
+    off = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    pairs = {"la_max": la.astype(np.int32), "la_min": la.astype(np.int32),
+             "lb_max": lb.astype(np.int32), "lb_min": lb.astype(np.int32),
+             "zeta": zeta, "zetb": zetb, "rscale": np.ones(n), "ra": ra, "rab": rab,
+             "radius": np.sqrt(np.log(1e10) / (zeta + zetb)), "block_offset": off}
+    cfg = {"npts": npts, "dh": dh, "pairs": pairs, "pab": rng.uniform(-1.0, 1.0, int(sizes.sum())),
+           "nblock": int(sizes.sum())}
+    W = rng.uniform(-1.0, 1.0, (npts[2], npts[1], npts[0]))
+    say("small-sphere limits", cell_kind, n, "pairs")
+    ref = oracle_batch(oracle, cfg, W=W, forces=True)
+    got = run_tasklist(gpw, torch, cfg, splits=2, forces=True, W=W)
+    assert npts_set(got["grid"]) == npts_set(ref["grid"])
+    assert rel_err(got["grid"], ref["grid"]) < TOL
+    assert rel_err(got["hab"], ref["hab"]) < TOL
+    assert rel_err(got["forces"], ref["forces"]) < TOL
