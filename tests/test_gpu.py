@@ -531,15 +531,17 @@ def test_contracted_shell_pairs(gpw, oracle, torch_cuda, plain):
     tl.close()
 
 
-def test_tasklist_windowed_nonperiodic(gpw, oracle, torch_cuda):
+@pytest.mark.parametrize("shear", [0.0, 0.2])
+def test_tasklist_windowed_nonperiodic(gpw, oracle, torch_cuda, shear):
     """Task list on a local window with non-periodic x and z (lb_grid, ngrid <
     npts), several pairs, l <= 2, collocate + integrate + forces, against the
-    oracle's single-task path pair by pair."""
+    oracle's single-task path pair by pair; shear > 0: a general cell."""
     torch = torch_cuda
     rng = np.random.default_rng(17)
     npts, ngrid, lb_grid, per = (40, 36, 44), (34, 36, 32), (-3, 0, -6), (0, 1, 0)
     cell = np.array([10.0, 9.0, 11.0])
     dh = np.diag(cell / np.asarray(npts))
+    dh[1, 0] = shear * dh[0, 0]
     n = 14
     la = rng.integers(0, 3, n); lb = rng.integers(0, 3, n)
     zeta = rng.uniform(2.5, 5.0, n); zetb = rng.uniform(2.5, 5.0, n)
@@ -562,7 +564,7 @@ def test_tasklist_windowed_nonperiodic(gpw, oracle, torch_cuda):
     f_ref = np.zeros((n, 2, 3))
     for i in range(n):
         lbc, ubc, sb = oracle.sphere_bounds(radius[i], np.diag(dh))
-        t = {"compute_tau": False, "use_ortho": True, "la_max": int(la[i]), "la_min": int(la[i]),
+        t = {"compute_tau": False, "use_ortho": not shear, "la_max": int(la[i]), "la_min": int(la[i]),
              "lb_max": int(lb[i]), "lb_min": int(lb[i]), "zeta": zeta[i], "zetb": zetb[i],
              "rscale": 1.0, "dh": dh, "dh_inv": np.linalg.inv(dh), "ra": ra[i], "rab": rab[i],
              "npts": list(npts), "ngrid": list(ngrid), "lb_grid": list(lb_grid),
@@ -575,7 +577,7 @@ def test_tasklist_windowed_nonperiodic(gpw, oracle, torch_cuda):
         h, f = oracle.integrate(t, W, pab=blk, forces=True)
         hab_ref[off[i]:off[i] + sizes[i]] = np.asarray(h).reshape(-1)
         f_ref[i] = f
-    say("windowed task list", n, "pairs")
+    say("windowed task list", n, "pairs, shear", shear)
     g = gpw.grid_desc(npts, dh, ngrid=ngrid, lb_grid=lb_grid, periodic=per)
     for splits in (1, 3):
         tl = gpw.TaskList(g, pairs, splits=splits)
