@@ -36,31 +36,98 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
 
-def roofline(pts, coll_ms, integ_ms, ms_per_step, workload):
-    """Algorithmic bytes: collocate reads + writes every sphere point (16 B),
-    integrate reads it (8 B); the dominant kernel's achieved rate. traffic:
-    the measured HBM bytes of that kernel where a counter profile of this
-    workload exists, else null."""
+def hbm_copy_peak(torch, dev, nbytes=1 << 30, reps=5):
+    """Attainable HBM bandwidth measured in this run: best of `reps` device
+    copies of nbytes (read + write = 2 nbytes of traffic), timed with events
+    on the current stream. GB/s."""
+    n = nbytes // 8
+    src = torch.ones(n, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    best = 1e30
+    for _ in range(reps):
+        e0.record()
+        dst.copy_(src)
+        e1.record()
+        e1.synchronize()
+        best = min(best, e0.elapsed_time(e1))
+    del src, dst
+    return 2.0 * nbytes / (best * 1e-3) / 1e9
+
+
+def roofline(st, coll_ms, integ_ms, ms_per_step, workload, forces, peak_measured):
+    """Two rooflines of the dominant kernel and of the whole step.
+
+    Bytes (the contract's "hbm" line): a per-pair algorithmic MODEL --
+    collocate reads and writes every sphere point of every pair (16 B),
+    integrate reads it (8 B). A grid point covered by several pairs counts
+    once per pair, so the model can exceed what crosses HBM (the tile
+    kernels keep a tile in registers while all its pairs pass); `traffic`
+    is the HBM bytes rocprofv3 counted for the same kernel and workload.
+    FLOPs: the algorithmic FP64 flops of the z, y, x contraction order
+    (stats flops_*: C3(lp) FMAs per z-plane, C2(lp) per row, lp + 1 per
+    point of each sphere) against the FP64 peak."""
+    pts = st["cube_points"]
     coll_bytes, integ_bytes = 16.0 * pts, 8.0 * pts
+    coll_fl = st["flops_collocate"]
+    integ_fl = st["flops_integrate"][1 if forces else 0]
     if coll_ms >= integ_ms:
-        dom = ("k_collocate_tiles", coll_bytes, coll_ms)
+        dom = ("k_collocate_tiles", coll_bytes, coll_ms, coll_fl)
     else:
-        dom = ("k_integrate_tiles", integ_bytes, integ_ms)
+        dom = ("k_integrate_tiles", integ_bytes, integ_ms, integ_fl)
     achieved = dom[1] / (dom[2] * 1e-3) / 1e9
-    traffic = None
+    traffic, tsrc = None, None
     tpath = os.path.join(ROOT, "profiles", "traffic_128cubed_20000pairs.json")
     if os.path.exists(tpath):
         try:
             prof = json.load(open(tpath))
             if prof.get("workload") == workload:
                 traffic = prof.get(dom[0])
+                tsrc = prof.get("source")
         except (OSError, ValueError):
             traffic = None
+
+    def tf(fl, ms):
+        return fl / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
+
+    flops = {
+        "collocate": {"tflops": round(tf(coll_fl, coll_ms), 3),
+                      "frac": round(tf(coll_fl, coll_ms) / FP64_PEAK_TFLOPS, 4),
+                      "algorithmic_flops": coll_fl, "kernel_ms": round(coll_ms, 4)},
+        "integrate": {"tflops": round(tf(integ_fl, integ_ms), 3),
+                      "frac": round(tf(integ_fl, integ_ms) / FP64_PEAK_TFLOPS, 4),
+                      "algorithmic_flops": integ_fl, "kernel_ms": round(integ_ms, 4)},
+        "step": {"tflops": round(tf(coll_fl + integ_fl, ms_per_step), 3),
+                 "frac": round(tf(coll_fl + integ_fl, ms_per_step) / FP64_PEAK_TFLOPS, 4)},
+        "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s"}
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "traffic_source": tsrc,
+            "model": "per-pair algorithmic bytes (16 B collocate / 8 B integrate per sphere point); "
+                     "a grid point counts once per pair covering it",
+            "peak_measured": round(peak_measured, 1),
+            "frac_of_measured": round(achieved / peak_measured, 4),
             "kernel": dom[0] if dom[0] != "k_integrate_tiles" else "k_integrate_tiles + k_moments_small",
             "kernel_ms": round(dom[2], 4), "algorithmic_bytes": dom[1],
-            "step_frac": round((coll_bytes + integ_bytes) / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+            "collocate_byte_model": {"achieved": round(coll_bytes / (coll_ms * 1e-3) / 1e9, 1),
+                                     "frac": round(coll_bytes / (coll_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+            "integrate_byte_model": {"achieved": round(integ_bytes / (integ_ms * 1e-3) / 1e9, 1),
+                                     "frac": round(integ_bytes / (integ_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+            "step_frac": round((coll_bytes + integ_bytes) / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "flop_frac": round(tf(dom[3], dom[2]) / FP64_PEAK_TFLOPS, 4),
+            "flops": flops}
+
+
+def host_cores():
+    """CPU cores this process may use: `nproc` (honours the affinity mask and
+    OMP_NUM_THREADS), else the affinity mask."""
+    import subprocess
+    try:
+        return max(1, int(subprocess.run(["nproc"], capture_output=True, text=True,
+                                         check=True).stdout.strip()))
+    except (OSError, ValueError, subprocess.CalledProcessError):
+        return max(1, len(os.sched_getaffinity(0)))
 
 
 def cpu_leg(O, legs, nthreads, budget_s=10.0):
@@ -133,7 +200,8 @@ def run_single(args, torch, G, Wk, dev, cfg):
     ms_per_step = wall * 1e3 / args.steps
     value = npairs * args.steps / wall
     pts = st["cube_points"]
-    roof = roofline(pts, phase_ms["collocate"], phase_ms["integrate"], ms_per_step, cfg["name"])
+    roof = roofline(st, phase_ms["collocate"], phase_ms["integrate"], ms_per_step, cfg["name"],
+                    with_forces, args.hbm_peak)
 
     out = {"metric": METRIC, "value": round(value, 1), "unit": "pairs/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
@@ -148,7 +216,10 @@ def run_single(args, torch, G, Wk, dev, cfg):
                       "tile": st["tile"], "sub_boxes": st["nsub"],
                       "work_items": st["nentries"], "sphere_points": pts, "plan_s": round(plan_s, 3),
                       "phase_ms": {k: round(v, 4) for k, v in phase_ms.items()}},
-           "roofline": roof}
+           "roofline": roof,
+           # one pass including the host + device planning of the task list
+           "end_to_end": {"value": round(npairs / (plan_s + ms_per_step * 1e-3), 1), "unit": "pairs/s",
+                          "plan_s": round(plan_s, 4), "ms_per_step": round(ms_per_step, 4)}}
 
     # ---- plain GPU form (same task-list code family, one launch sequence per pair)
     if not args.no_plain:
@@ -159,15 +230,19 @@ def run_single(args, torch, G, Wk, dev, cfg):
         tlp.collocate(pab_s, grid, plain=True)
         tlp.integrate(V, hab_s, plain=True)
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        grid.zero_(); hab_s.zero_()
-        tlp.collocate(pab_s, grid, plain=True)
-        tlp.integrate(V, hab_s, plain=True)
-        torch.cuda.synchronize()
-        tp = time.perf_counter() - t0
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            grid.zero_(); hab_s.zero_()
+            tlp.collocate(pab_s, grid, plain=True)
+            tlp.integrate(V, hab_s, plain=True)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        tp = float(np.median(times))
         plain_v = len(sub["pairs"]["la_max"]) / tp
         out["plain_gpu"] = {"value": round(plain_v, 1), "unit": "pairs/s",
-                            "sample": f"first {len(sub['pairs']['la_max'])} pairs of the workload, one step",
+                            "sample": f"first {len(sub['pairs']['la_max'])} pairs of the workload, "
+                                      "median of 3 steps",
                             "speedup_batched": round(value / plain_v, 2)}
         tlp.close()
 
@@ -175,18 +250,22 @@ def run_single(args, torch, G, Wk, dev, cfg):
     if not args.no_cpu:
         from oracle import oracle as O
         O.build()
-        ncpu = min(16, os.cpu_count() or 1)
+        ncpu = host_cores()
         rng = np.random.default_rng(7)
         idx = np.sort(rng.choice(npairs, size=min(args.cpu_pairs, npairs), replace=False))
         sub = Wk.subset(cfg, idx)
         og = O.make_grid(sub["npts"], sub["dh"])
         op = O.make_pairs(sub["pairs"])
-        tc, outs, reps = cpu_leg(O, [(og, op, sub["pab"], V_h, sub["nblock"])], ncpu)
+        leg = [(og, op, sub["pab"], V_h, sub["nblock"])]
+        tc, outs, reps = cpu_leg(O, leg, ncpu)
+        t1, _, reps1 = cpu_leg(O, leg, 1, budget_s=0.0)
         ref_grid, ref_hab = outs[0]
         out["cpu_baseline"] = {"value": round(len(idx) / tc, 1), "unit": "pairs/s", "cores": ncpu,
                                "kind": "port",
                                "sample": f"{len(idx)} random pairs of the workload (seed 7), "
-                                         f"collocate+integrate x{reps}, OpenMP C oracle"}
+                                         f"collocate+integrate x{reps}, OpenMP C oracle",
+                               "one_core": {"value": round(len(idx) / t1, 1), "cores": 1,
+                                            "sample": f"the same pairs, x{reps1}"}}
         tls = G.TaskList(grid_desc, sub["pairs"], splits=args.splits)
         g_s = torch.zeros_like(grid)
         h_s = torch.zeros(sub["nblock"], dtype=torch.float64, device=dev)
@@ -225,12 +304,14 @@ def run_multigrid(args, torch, G, Wk, dev, cfg):
     torch.cuda.synchronize()
     mg.step(pab, grids, pots, hab, forces)
     torch.cuda.synchronize()
-    level_ms, pts, coll, integ = [], 0, 0.0, 0.0
+    level_ms, pts, coll, integ, fl_c, fl_i = [], 0, 0.0, 0.0, 0.0, 0.0
     for lv in mg.levels:
         ph = lv["tl"].timings()
         lv["tl"].set_timing(False)
         st = lv["tl"].stats()
         pts += st["cube_points"]
+        fl_c += st["flops_collocate"]
+        fl_i += st["flops_integrate"][1]
         pm = {k: round(v[0] / max(v[1], 1), 4) for k, v in ph.items()}
         coll += pm["collocate"] + pm["reduce"]
         integ += pm["integrate"]
@@ -255,11 +336,16 @@ def run_multigrid(args, torch, G, Wk, dev, cfg):
            "config": {"workload": cfg["name"], "pairs": mg.npairs, "forces": True, "eps": Wk.EPS,
                       "hip_graph": True, "autotune": tune, "plan_s": round(plan_s, 3),
                       "levels": level_ms},
-           "roofline": roofline(pts, coll, integ, ms_per_step, cfg["name"])}
+           "roofline": roofline({"cube_points": pts, "flops_collocate": fl_c,
+                                 "flops_integrate": [fl_i, fl_i]}, coll, integ, ms_per_step,
+                                cfg["name"], True, args.hbm_peak),
+           "end_to_end": {"value": round(mg.npairs / (plan_s + ms_per_step * 1e-3), 1),
+                          "unit": "pairs/s", "plan_s": round(plan_s, 4),
+                          "ms_per_step": round(ms_per_step, 4)}}
     if not args.no_cpu:
         from oracle import oracle as O
         O.build()
-        ncpu = min(16, os.cpu_count() or 1)
+        ncpu = host_cores()
         rng = np.random.default_rng(7)
         frac = min(1.0, args.cpu_pairs / mg.npairs)
         legs, tls, sample = [], [], 0
@@ -272,10 +358,13 @@ def run_multigrid(args, torch, G, Wk, dev, cfg):
             tls.append(G.TaskList(lv["grid_desc"], sub))
             sample += m
         tc, outs, reps = cpu_leg(O, legs, ncpu)
+        t1, _, reps1 = cpu_leg(O, legs, 1, budget_s=0.0)
         out["cpu_baseline"] = {"value": round(sample / tc, 1), "unit": "pairs/s", "cores": ncpu,
                                "kind": "port",
                                "sample": f"{sample} pairs drawn per level in proportion (seed 7), "
-                                         f"collocate+integrate x{reps}, OpenMP C oracle"}
+                                         f"collocate+integrate x{reps}, OpenMP C oracle",
+                               "one_core": {"value": round(sample / t1, 1), "cores": 1,
+                                            "sample": f"the same pairs, x{reps1}"}}
         eg = 0.0
         h_s = torch.zeros(cfg["nblock"], dtype=torch.float64, device=dev)
         h_ref = np.zeros(cfg["nblock"])
@@ -332,6 +421,8 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": 1 << 30, "launches": 3}), flush=True)
         return
+    # attainable HBM bandwidth of this GPU, measured before the workload
+    args.hbm_peak = hbm_copy_peak(torch, dev)
     cfg = Wk.config(args.config, npairs=args.pairs)
     if args.config == 5:
         out = run_multigrid(args, torch, G, Wk, dev, cfg)

@@ -78,6 +78,11 @@ typedef struct gpw_tasklist_stats {
     int splits;               /* work-list splits per tile given at creation            */
     int csplits[4];           /* workgroups per tile of the collocate kernel, per class */
     int isplits[2][4];        /* same for integrate, without / with forces              */
+    /* algorithmic FP64 flops of one pass (2 per FMA) in the contraction order
+     * z, y, x: C3(lp) FMAs per z-plane, C2(lp) per (z, y) row and lp + 1 per
+     * point of each pair's sphere; integrate with forces runs at lp + 2       */
+    double flops_collocate;
+    double flops_integrate[2];  /* without / with forces                          */
 } gpw_tasklist_stats;
 
 /* ---- single task, host memory (drop-in for the reference's collocate) ---- */

@@ -57,7 +57,8 @@ class TaskListStats(C.Structure):
     _fields_ = [("npairs", C.c_longlong), ("nsub", C.c_longlong), ("ntiles", C.c_longlong),
                 ("nentries", C.c_longlong), ("cube_points", C.c_longlong),
                 ("tile_points", C.c_longlong), ("lp_max", C.c_int), ("tile", C.c_int * 3),
-                ("splits", C.c_int), ("csplits", C.c_int * 4), ("isplits", (C.c_int * 4) * 2)]
+                ("splits", C.c_int), ("csplits", C.c_int * 4), ("isplits", (C.c_int * 4) * 2),
+                ("flops_collocate", C.c_double), ("flops_integrate", C.c_double * 2)]
 
 
 _lib = None
@@ -133,21 +134,27 @@ def grid_desc(npts, dh, ngrid=None, lb_grid=(0, 0, 0), periodic=(1, 1, 1)) -> Gr
     return g
 
 
+# numpy view of gpw_pair_desc (same field offsets as the ctypes structure)
+_PAIR_DTYPE = np.dtype({
+    "names": [f for f, _ in PairDesc._fields_],
+    "formats": [np.int32] * 4 + [np.float64] * 3 + [(np.float64, 3), (np.float64, 3), np.float64,
+                                                    np.int64],
+    "offsets": [getattr(PairDesc, f).offset for f, _ in PairDesc._fields_],
+    "itemsize": C.sizeof(PairDesc)})
+
+
 def pair_array(p: dict):
+    """gpw_pair_desc array of a dict of per-pair arrays (filled column-wise)."""
     n = len(p["la_max"])
     arr = (PairDesc * n)()
-    la_max, la_min = p["la_max"], p["la_min"]
-    lb_max, lb_min = p["lb_max"], p["lb_min"]
-    for i in range(n):
-        q = arr[i]
-        q.la_max = int(la_max[i]); q.la_min = int(la_min[i])
-        q.lb_max = int(lb_max[i]); q.lb_min = int(lb_min[i])
-        q.zeta = float(p["zeta"][i]); q.zetb = float(p["zetb"][i])
-        q.rscale = float(p["rscale"][i]); q.radius = float(p["radius"][i])
-        ra, rab = p["ra"][i], p["rab"][i]
-        q.ra[0], q.ra[1], q.ra[2] = float(ra[0]), float(ra[1]), float(ra[2])
-        q.rab[0], q.rab[1], q.rab[2] = float(rab[0]), float(rab[1]), float(rab[2])
-        q.block_offset = int(p["block_offset"][i])
+    if n == 0:
+        return arr
+    v = np.frombuffer(arr, dtype=_PAIR_DTYPE, count=n)
+    for k in ("la_max", "la_min", "lb_max", "lb_min", "zeta", "zetb", "rscale", "radius",
+              "block_offset"):
+        v[k] = np.asarray(p[k])
+    v["ra"] = np.asarray(p["ra"], dtype=np.float64).reshape(n, 3)
+    v["rab"] = np.asarray(p["rab"], dtype=np.float64).reshape(n, 3)
     return arr
 
 
@@ -156,6 +163,7 @@ def _stats_dict(s: TaskListStats) -> dict:
     out["tile"] = list(s.tile)
     out["csplits"] = list(s.csplits)
     out["isplits"] = [list(r) for r in s.isplits]
+    out["flops_integrate"] = list(s.flops_integrate)
     return out
 
 
@@ -182,11 +190,22 @@ def pair_geometry(grid: GridDesc, pairs: dict, i: int, compute_tau: bool = False
             "ub_cube": list(ub), "sphere_bounds": sb[:n.value].copy()}
 
 
-def _ptr(t):
+def _ptr(t, numel=None, name="tensor", exact=False):
+    """Device pointer of a contiguous float64 tensor; with numel, checks that
+    it holds at least (exact: exactly) that many elements, so that a short
+    buffer is an error here and not an out-of-bounds access on the device."""
     if t is None:
         return None
     import torch
-    assert t.is_cuda and t.dtype == torch.float64 and t.is_contiguous(), "need contiguous fp64 device tensor"
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name}: expected a torch tensor, got {type(t).__name__}")
+    if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+        raise ValueError(f"{name}: need a contiguous float64 device tensor "
+                         f"(got {t.dtype} on {t.device}, contiguous={t.is_contiguous()})")
+    if numel is not None:
+        n = t.numel()
+        if (exact and n != numel) or n < numel:
+            raise ValueError(f"{name}: {n} elements, need {'exactly ' if exact else 'at least '}{numel}")
     return C.c_void_p(t.data_ptr())
 
 
@@ -209,6 +228,13 @@ class TaskList:
         self.grid = grid
         self.npairs = len(pairs["la_max"])
         self._pairs = pair_array(pairs)
+        # sizes the device buffers of collocate / integrate must have
+        self.ngrid_points = int(grid.ngrid[0]) * int(grid.ngrid[1]) * int(grid.ngrid[2])
+        self.nblock = 0
+        if self.npairs:
+            ends = [int(bo) + ncoset(int(b)) * ncoset(int(a)) for bo, a, b in
+                    zip(pairs["block_offset"], pairs["la_max"], pairs["lb_max"])]
+            self.nblock = max(ends)
         h = C.c_void_p()
         _check(lib().gpw_tasklist_create(C.byref(grid), self.npairs, self._pairs,
                                          int(compute_tau), None, int(splits), C.byref(h)))
@@ -251,12 +277,17 @@ class TaskList:
 
     def collocate(self, pab, grid, stream=None, plain=False):
         f = lib().gpw_tasklist_collocate_plain if plain else lib().gpw_tasklist_collocate
-        _check(f(self._h, _ptr(pab), _ptr(grid), _stream(stream)))
+        _check(f(self._h, _ptr(pab, self.nblock, "pab"),
+                 _ptr(grid, self.ngrid_points, "grid", exact=True), _stream(stream)))
         return grid
 
     def integrate(self, grid, hab, pab=None, forces=None, stream=None, plain=False):
         f = lib().gpw_tasklist_integrate_plain if plain else lib().gpw_tasklist_integrate
-        _check(f(self._h, _ptr(grid), _ptr(pab), _ptr(hab), _ptr(forces), _stream(stream)))
+        if forces is not None and pab is None:
+            raise ValueError("forces need pab")
+        _check(f(self._h, _ptr(grid, self.ngrid_points, "grid", exact=True),
+                 _ptr(pab, self.nblock, "pab"), _ptr(hab, self.nblock, "hab"),
+                 _ptr(forces, 6 * self.npairs, "forces"), _stream(stream)))
         return hab
 
 
@@ -286,10 +317,24 @@ def _dptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+def _check_single_sizes(t: dict, grid, pab=None, hab=None):
+    """The host arrays of a single task must cover what the library reads or
+    writes: grid ngrid[2] x ngrid[1] x ngrid[0], pab / hab maxco x maxco."""
+    ng = int(t["ngrid"][0]) * int(t["ngrid"][1]) * int(t["ngrid"][2])
+    if grid.size != ng:
+        raise ValueError(f"grid: {grid.size} elements, need ngrid product {ng}")
+    mc = int(t["maxco"]) ** 2
+    for name, a in (("pab", pab), ("hab", hab)):
+        if a is not None and a.size < mc:
+            raise ValueError(f"{name}: {a.size} elements, need maxco^2 = {mc}")
+
+
 def collocate_pgf_product_rspace(t: dict, pab: np.ndarray, grid: np.ndarray) -> np.ndarray:
     """Drop-in for grid_collocate_pgf_product_rspace on host arrays (grid += ...)."""
     pab = np.ascontiguousarray(pab, dtype=np.float64)
-    assert grid.dtype == np.float64 and grid.flags["C_CONTIGUOUS"]
+    if not (isinstance(grid, np.ndarray) and grid.dtype == np.float64 and grid.flags["C_CONTIGUOUS"]):
+        raise ValueError("grid: need a C-contiguous float64 numpy array (it is accumulated in place)")
+    _check_single_sizes(t, grid, pab=pab)
     _check(lib().gpw_collocate_pgf_product_rspace(*_single_args(t), _dptr(pab), _dptr(grid)))
     return grid
 
@@ -300,6 +345,11 @@ def integrate_pgf_product_rspace(t: dict, grid: np.ndarray, pab: np.ndarray | No
     grid = np.ascontiguousarray(grid, dtype=np.float64)
     if hab is None:
         hab = np.zeros((t["maxco"], t["maxco"]))
+    if not (hab.dtype == np.float64 and hab.flags["C_CONTIGUOUS"]):
+        raise ValueError("hab: need a C-contiguous float64 numpy array (it is accumulated in place)")
+    if forces and pab is None:
+        raise ValueError("forces need pab")
+    _check_single_sizes(t, grid, pab=pab, hab=hab)
     f = np.zeros(6) if forces else None
     pab_c = None if pab is None else np.ascontiguousarray(pab, dtype=np.float64)
     _check(lib().gpw_integrate_pgf_product_rspace(
@@ -312,16 +362,22 @@ def collocate_core(length, co, P, stream=None):
     """libgpw collocate_core on device tensors: returns V (length[0..2])."""
     import torch
     L = co.shape[0]
+    if tuple(co.shape) != (L, L, L) or P.dim() != 3 or P.shape[0] != 3 or P.shape[1] != L:
+        raise ValueError(f"collocate_core: co must be L^3 and P 3 x L x ldp (co {tuple(co.shape)}, "
+                         f"P {tuple(P.shape)})")
     V = torch.empty(tuple(int(x) for x in length), dtype=torch.float64, device=co.device)
-    _check(lib().gpw_collocate_core(_ci(length), L, _ptr(co), _ptr(P), int(P.shape[2]), _ptr(V),
-                                    _stream(stream)))
+    _check(lib().gpw_collocate_core(_ci(length), L, _ptr(co, L ** 3, "co"), _ptr(P, P.numel(), "P"),
+                                    int(P.shape[2]), _ptr(V), _stream(stream)))
     return V
 
 
 def integrate_core(length, P, V, stream=None):
     import torch
     L = P.shape[1]
+    if P.dim() != 3 or P.shape[0] != 3:
+        raise ValueError(f"integrate_core: P must be 3 x L x ldp (got {tuple(P.shape)})")
+    nv = int(length[0]) * int(length[1]) * int(length[2])
     out = torch.empty((L, L, L), dtype=torch.float64, device=V.device)
-    _check(lib().gpw_integrate_core(_ci(length), L, _ptr(P), int(P.shape[2]), _ptr(V), _ptr(out),
-                                    _stream(stream)))
+    _check(lib().gpw_integrate_core(_ci(length), L, _ptr(P), int(P.shape[2]),
+                                    _ptr(V, nv, "V", exact=True), _ptr(out), _stream(stream)))
     return out

@@ -316,20 +316,29 @@ struct TileCfg {
     __device__ static int xat(int x) { return (x / XB) * GS + (x % XB) * LM; }
 };
 
+// Index-set predicates. They must round exactly as the host planner (built
+// with -ffp-contract=off) does, so every product is kept apart from the sum
+// that consumes it: the functions below are written with plain operators
+// under "#pragma clang fp contract(off)" (the __dmul_rn / __dadd_rn helpers
+// of the HIP headers are plain operators compiled in the header's scope and
+// are fused into FMAs after inlining, which moves points that lie exactly on
+// a sphere or row bound in or out).
+
 // Largest n >= 0 with b + (n h)^2 <= R2, evaluated with the same IEEE
 // operations (no contraction) as the host rule that generates the cube.
 __device__ __forceinline__ int sphere_extent_dev(double b, double h, double R2) {
+#pragma clang fp contract(off)
     const double rem = R2 - b;
     // initial guess only; the exact predicate below decides
     int n = rem > 0.0 ? (int)(sqrtf((float)rem) / (float)h) : 0;
     while (n > 0) {
-        const double x = __dmul_rn((double)n, h);
-        if (__dadd_rn(b, __dmul_rn(x, x)) <= R2) break;
+        const double x = (double)n * h;
+        if (b + x * x <= R2) break;
         n--;
     }
     for (;;) {
-        const double x = __dmul_rn((double)(n + 1), h);
-        if (__dadd_rn(b, __dmul_rn(x, x)) <= R2)
+        const double x = (double)(n + 1) * h;
+        if (b + x * x <= R2)
             n++;
         else
             return n;
@@ -340,12 +349,13 @@ __device__ __forceinline__ int sphere_extent_dev(double b, double h, double R2) 
 // fold <= T are inside; -1 if the row is empty. Matches the profile the
 // host planner builds for the reference's sphere_bounds layout.
 __device__ __forceinline__ int row_T(int kk, int jj, const TileGrid &tg, double R2) {
+#pragma clang fp contract(off)
     if (kk < 0 || jj < 0) return -1;
-    const double zk = __dmul_rn((double)kk, tg.dr[2]);
-    const double base = __dmul_rn(zk, zk);
+    const double zk = (double)kk * tg.dr[2];
+    const double base = zk * zk;
     if (kk > 0 && base > R2) return -1;
-    const double yj = __dmul_rn((double)jj, tg.dr[1]);
-    const double b2 = __dadd_rn(base, __dmul_rn(yj, yj));
+    const double yj = (double)jj * tg.dr[1];
+    const double b2 = base + yj * yj;
     if (jj > 0 && b2 > R2) return -1;
     return sphere_extent_dev(b2, tg.dr[0], R2);
 }
@@ -357,17 +367,18 @@ __device__ __forceinline__ int row_T(int kk, int jj, const TileGrid &tg, double 
 __device__ __forceinline__ void general_row_dev(const TileGrid &tg, double gp0, double dj, double dk,
                                                 double R2, double zetp, int &lo, int &hi,
                                                 double &qb, double &qc) {
+#pragma clang fp contract(off)
     double a = 0.0, b = 0.0, c = 0.0;
 #pragma unroll
     for (int x = 0; x < 3; x++) {
-        const double pointk = __dmul_rn(tg.dh[2][x], dk);
-        const double pointj = __dadd_rn(pointk, __dmul_rn(tg.dh[1][x], dj));
-        const double v = __dsub_rn(pointj, __dmul_rn(gp0, tg.dh[0][x]));
-        a = __dadd_rn(a, __dmul_rn(tg.dh[0][x], tg.dh[0][x]));
-        b = __dadd_rn(b, __dmul_rn(__dmul_rn(2.0, v), tg.dh[0][x]));
-        c = __dadd_rn(c, __dmul_rn(v, v));
+        const double pointk = tg.dh[2][x] * dk;
+        const double pointj = pointk + tg.dh[1][x] * dj;
+        const double v = pointj - gp0 * tg.dh[0][x];
+        a = a + tg.dh[0][x] * tg.dh[0][x];
+        b = b + (2.0 * v) * tg.dh[0][x];
+        c = c + v * v;
     }
-    double d = __dsub_rn(__dmul_rn(b, b), __dmul_rn(__dmul_rn(4.0, a), __dsub_rn(c, R2)));
+    double d = b * b - (4.0 * a) * (c - R2);
     if (d < 0.0) {
         lo = 1;
         hi = 0;
@@ -375,9 +386,9 @@ __device__ __forceinline__ void general_row_dev(const TileGrid &tg, double gp0, 
         return;
     }
     d = __dsqrt_rn(d);
-    const double a2 = __dmul_rn(2.0, a);
-    lo = (int)ceil(__ddiv_rn(__dsub_rn(-b, d), a2));
-    hi = (int)floor(__ddiv_rn(__dadd_rn(-b, d), a2));
+    const double a2 = 2.0 * a;
+    lo = (int)ceil((-b - d) / a2);
+    hi = (int)floor((-b + d) / a2);
     qb = b * -zetp;
     qc = c * -zetp;
 }

@@ -193,6 +193,10 @@ struct gpw_tasklist {
     int compute_tau = 0;
     int npairs = 0;
     int force_general = 0;  // single tasks with use_ortho = 0 on any cell
+    int force_ortho = 0;    // single tasks with use_ortho = 1: the orthorhombic
+                            // path on dh[i][i] even when dh has off-diagonal
+                            // terms (grid_base_ref_c.cc:981 takes it on the
+                            // caller's word, :546 reads only dh[i][i])
     int lp_max = 0;
     int la_max = 0, lb_max = 0;  // largest given la_max / lb_max
     TileGrid tg{};
@@ -204,6 +208,7 @@ struct gpw_tasklist {
     int cpart_cap = 0;             // partial grids allocated in d_cpartial
     long long nentries = 0;
     long long cube_points = 0, tile_points = 0;
+    double flops_collocate = 0.0, flops_integrate[2] = {0.0, 0.0};
     // tile lists: set 0 = collocate (every item), set 1 = integrate with the
     // force extension (lp_add 2), set 2 = integrate without; the integrate
     // sets leave out the pairs of the small-sphere kernel (small_*)
@@ -463,25 +468,38 @@ static void axis_tiles(const SubDev &s, int d, int n, int T, int ntile, std::vec
         if (hit[t]) out.push_back(t);
 }
 
-// number of points of a general-cell sub-box inside its sphere
-static long long sub_points_general(const gpw_grid_desc &g, const SubDev &s) {
-    long long n = 0;
-    for (int k = s.lb[2]; k <= s.ub[2]; k++)
+// Sphere points of a sub-box, with the rows (z, y) and z-planes that hold
+// any: the algorithmic work of the contraction order z, y, x
+// (grid_base_ref_c.cc:431-494) is C3(lp) FMAs per plane, C2(lp) per row and
+// lp + 1 per point.
+struct SubWork {
+    long long pts = 0, rows = 0, planes = 0;
+};
+
+static SubWork sub_points_general(const gpw_grid_desc &g, const SubDev &s) {
+    SubWork w;
+    for (int k = s.lb[2]; k <= s.ub[2]; k++) {
+        const long long rows0 = w.rows;
         for (int j = s.lb[1]; j <= s.ub[1]; j++) {
             int lo, hi;
             if (!general_row_host(g, s.roff[0], j - s.roff[1], k - s.roff[2], s.R2, lo, hi)) continue;
             lo = std::max(lo, s.lb[0]);
             hi = std::min(hi, s.ub[0]);
-            if (hi >= lo) n += hi - lo + 1;
+            if (hi >= lo) {
+                w.pts += hi - lo + 1;
+                w.rows++;
+            }
         }
-    return n;
+        if (w.rows > rows0) w.planes++;
+    }
+    return w;
 }
 
-// number of sphere points in a sub-box (for roofline accounting)
-static long long sub_points(const SubDev &s, const std::vector<int> &prof) {
+static SubWork sub_points(const SubDev &s, const std::vector<int> &prof) {
     auto fo = [](int i) { return i <= 0 ? -i : i - 1; };
-    long long n = 0;
-    for (int k = s.lb[2]; k <= s.ub[2]; k++)
+    SubWork w;
+    for (int k = s.lb[2]; k <= s.ub[2]; k++) {
+        const long long rows0 = w.rows;
         for (int j = s.lb[1]; j <= s.ub[1]; j++) {
             int T;
             if (s.prof_off < 0)
@@ -491,9 +509,20 @@ static long long sub_points(const SubDev &s, const std::vector<int> &prof) {
             if (T < 0) continue;
             // offsets i in [lb, ub] with fo(i) <= T  <=>  i in [-T, T+1]
             const int lo = std::max(s.lb[0], -T), hi = std::min(s.ub[0], T + 1);
-            if (hi >= lo) n += hi - lo + 1;
+            if (hi >= lo) {
+                w.pts += hi - lo + 1;
+                w.rows++;
+            }
         }
-    return n;
+        if (w.rows > rows0) w.planes++;
+    }
+    return w;
+}
+
+// FP64 flops of one contraction (collocate or integrate) of a sub-box at
+// product angular momentum lp: 2 per FMA
+static double contraction_flops(const SubWork &w, int lp) {
+    return 2.0 * ((double)(lp + 1) * w.pts + (double)ncoef2(lp) * w.rows + (double)ncoset(lp) * w.planes);
 }
 
 // Lattice transform of a general cell (grid_base_ref_c.cc:647-698): for
@@ -599,7 +628,7 @@ static gpw_status plan_tasklist(gpw_tasklist *tl, const std::vector<gpw_pair_des
     tg.ntx = (tg.nx + TX - 1) / TX;
     tg.nty = (tg.ny + TY - 1) / TY;
     tg.ntz = (tg.nz + TZ - 1) / TZ;
-    tg.general = (tl->force_general || !is_ortho(g)) ? 1 : 0;
+    tg.general = tl->force_general ? 1 : tl->force_ortho ? 0 : !is_ortho(g) ? 1 : 0;
     for (int d = 0; d < 3; d++) {
         tg.dr[d] = g.dh[d][d];
         tg.period[d] = g.periodic[d] ? g.npts[d] : BIG;
@@ -622,6 +651,7 @@ static gpw_status plan_tasklist(gpw_tasklist *tl, const std::vector<gpw_pair_des
     std::vector<int> tx, ty, tz;
     hp.pe_start.assign(pairs.size() + 1, 0);
     tl->cube_points = 0;
+    tl->flops_collocate = tl->flops_integrate[0] = tl->flops_integrate[1] = 0.0;
     for (size_t si = 0; si < tl->subs_h.size(); si++) {
         const SubDev &s = tl->subs_h[si];
         axis_tiles(s, 0, tg.nx, TX, tg.ntx, tx);
@@ -634,7 +664,11 @@ static gpw_status plan_tasklist(gpw_tasklist *tl, const std::vector<gpw_pair_des
                     hp.ent_sub.push_back((int)si);
                     hp.pe_start[s.pair + 1]++;
                 }
-        tl->cube_points += tl->tg.general ? sub_points_general(g, s) : sub_points(s, prof_flat);
+        const SubWork w = tl->tg.general ? sub_points_general(g, s) : sub_points(s, prof_flat);
+        tl->cube_points += w.pts;
+        tl->flops_collocate += contraction_flops(w, s.lp);
+        tl->flops_integrate[0] += contraction_flops(w, s.lp);
+        tl->flops_integrate[1] += contraction_flops(w, s.lp + 2);
     }
     for (size_t p = 0; p < pairs.size(); p++) hp.pe_start[p + 1] += hp.pe_start[p];
     const size_t ne = ent_tile.size();
@@ -1039,6 +1073,9 @@ gpw_status gpw_tasklist_stats_get(const gpw_tasklist *tl, gpw_tasklist_stats *o)
     o->tile[1] = TY;
     o->tile[2] = TZ;
     o->splits = tl->splits;
+    o->flops_collocate = tl->flops_collocate;
+    o->flops_integrate[0] = tl->flops_integrate[0];
+    o->flops_integrate[1] = tl->flops_integrate[1];
     for (int c = 0; c < NCLS; c++) {
         o->csplits[c] = tl->csplit[c];
         o->isplits[0][c] = tl->isplit[0][c];
@@ -1538,6 +1575,7 @@ static gpw_status single_setup(int compute_tau, int use_ortho, int la_max, int l
     tl->compute_tau = tau;
     tl->npairs = 1;
     tl->force_general = use_ortho ? 0 : 1;
+    tl->force_ortho = use_ortho ? 1 : 0;
     std::vector<Profile> profiles;
     std::vector<PairGeom> geos(1);
     const PairDev pd = make_pair_dev(p, tau);

@@ -1,0 +1,246 @@
+"""GPU tests of the full-size workloads and of argument handling.
+
+* The 100,000-pair mixed workload (l <= 4, exponents down to 0.05, cubes
+  wider than the 25 bohr cell, 200^3) and the 500,000-pair triclinic
+  multigrid workload run whole on the GPU: the adjoint identity
+  <collocate(P), W> = <P, integrate(W)> over every pair, plus oracle parity
+  of hab and forces on a random sample of the pairs (integrate is per pair,
+  so the sample's blocks of the full run are compared directly) and of the
+  grid of the sample collocated on its own.
+* Task lists whose pairs have la_min < la_max and lb_min < lb_max (rho and
+  tau, with forces) against the oracle.
+* The single-task drop-in with use_ortho = 1 on a cell with off-diagonal
+  steps takes the orthorhombic path on dh[i][i], as the reference does.
+* Short buffers are rejected before any device work.
+Tolerance 1e-10 relative (max|x - ref| / max|ref|).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import random_task, rel_err
+from test_gpu import dev, oracle_batch, run_tasklist, same_support
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def _blocks_of(pairs, idx):
+    """flat indices of the packed blocks of pairs idx, in order"""
+    out = []
+    for i in idx:
+        sz = (int(pairs["la_max"][i]) + 1) * (int(pairs["la_max"][i]) + 2) * (int(pairs["la_max"][i]) + 3) // 6
+        sz *= (int(pairs["lb_max"][i]) + 1) * (int(pairs["lb_max"][i]) + 2) * (int(pairs["lb_max"][i]) + 3) // 6
+        bo = int(pairs["block_offset"][i])
+        out.append(np.arange(bo, bo + sz))
+    return np.concatenate(out)
+
+
+def test_config4_full_size(gpw, oracle, torch_cuda):
+    """All 100,000 pairs (la, lb 0..4, exponents 0.05..100, 200^3, with
+    forces, so L up to 11): adjoint identity over every pair; hab and forces
+    of 3,000 sampled pairs from the full run against the oracle; the
+    sample's grid, collocated on its own, against the oracle."""
+    from libgpw_amd import workloads as Wk
+    torch = torch_cuda
+    cfg = Wk.config(4)
+    npairs = len(cfg["pairs"]["la_max"])
+    say("config4 full", npairs, "pairs")
+    W = Wk.potential(cfg)
+    got = run_tasklist(gpw, torch, cfg, W=W, forces=True, splits=4)
+    st = got["stats"]
+    assert st["npairs"] == npairs and st["lp_max"] == 8
+    # cubes wider than the period are split into sub-boxes
+    assert st["nsub"] > npairs
+    lhs = float((got["grid"] * W).sum())
+    rhs = float((cfg["pab"] * got["hab"]).sum())
+    assert abs(lhs - rhs) < 1e-11 * np.abs(got["grid"]).sum(), (lhs, rhs)
+    rng = np.random.default_rng(44)
+    idx = np.sort(rng.choice(npairs, size=3000, replace=False))
+    sub = Wk.subset(cfg, idx)
+    say("config4 oracle sample", len(idx))
+    ref = oracle_batch(oracle, sub, W=W, forces=True)
+    blk = _blocks_of(cfg["pairs"], idx)
+    assert rel_err(got["hab"][blk], ref["hab"]) < TOL
+    assert rel_err(got["forces"][idx], ref["forces"]) < TOL
+    part = run_tasklist(gpw, torch, sub, splits=2)
+    assert same_support(part["grid"], ref["grid"])
+    assert rel_err(part["grid"], ref["grid"]) < TOL
+
+
+def test_config5_full_size(gpw, oracle, torch_cuda):
+    """All 500,000 pairs of the triclinic multigrid workload (four levels):
+    per level, the adjoint identity over the level's pairs and oracle parity
+    of hab and forces on 500 sampled pairs of the full run; the sample's
+    grid, collocated on its own, against the oracle."""
+    from libgpw_amd import workloads as Wk
+    torch = torch_cuda
+    cfg = Wk.config(5)
+    say("config5 full", len(cfg["pairs"]["la_max"]), "pairs",
+        [len(lv["index"]) for lv in cfg["levels"]])
+    rng = np.random.default_rng(55)
+    for k in range(len(cfg["levels"])):
+        lvc = Wk.level(cfg, k)
+        n = len(lvc["pairs"]["la_max"])
+        if n == 0:
+            continue
+        say("config5 level", k, n, "pairs", lvc["npts"])
+        W = Wk.potential(lvc, seed_offset=20 + k)
+        got = run_tasklist(gpw, torch, lvc, W=W, forces=True, splits=2)
+        lhs = float((got["grid"] * W).sum())
+        rhs = float((lvc["pab"] * got["hab"]).sum())
+        assert abs(lhs - rhs) < 1e-11 * np.abs(got["grid"]).sum(), (k, lhs, rhs)
+        idx = np.sort(rng.choice(n, size=min(500, n), replace=False))
+        sub = Wk.subset(lvc, idx)
+        ref = oracle_batch(oracle, sub, W=W, forces=True)
+        blk = _blocks_of(lvc["pairs"], idx)
+        assert rel_err(got["hab"][blk], ref["hab"]) < TOL, k
+        assert rel_err(got["forces"][idx], ref["forces"]) < TOL, k
+        part = run_tasklist(gpw, torch, sub, splits=1)
+        assert same_support(part["grid"], ref["grid"]), k
+        assert rel_err(part["grid"], ref["grid"]) < TOL, k
+
+
+def _ranged_pairs(rng, n, cell_m):
+    """n pairs with la_min < la_max and lb_min < lb_max (mixed ranges)"""
+    la_max = rng.integers(1, 4, n)
+    lb_max = rng.integers(1, 4, n)
+    la_min = np.array([rng.integers(0, a) for a in la_max])
+    lb_min = np.array([rng.integers(0, b) for b in lb_max])
+    zeta = np.exp(rng.uniform(np.log(0.3), np.log(20.0), n))
+    zetb = np.exp(rng.uniform(np.log(0.3), np.log(20.0), n))
+    ra = rng.uniform(0.0, 1.0, (n, 3)) @ cell_m
+    rab = rng.normal(0.0, 0.7, (n, 3))
+    sizes = np.array([((a + 1) * (a + 2) * (a + 3) // 6) * ((b + 1) * (b + 2) * (b + 3) // 6)
+                      for a, b in zip(la_max, lb_max)])
+    off = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    pairs = {"la_max": la_max.astype(np.int32), "la_min": la_min.astype(np.int32),
+             "lb_max": lb_max.astype(np.int32), "lb_min": lb_min.astype(np.int32),
+             "zeta": zeta, "zetb": zetb, "rscale": rng.uniform(0.5, 2.0, n), "ra": ra, "rab": rab,
+             "radius": np.sqrt(np.log(1e10) / (zeta + zetb)), "block_offset": off}
+    return pairs, int(sizes.sum())
+
+
+@pytest.mark.parametrize("tau", [False, True])
+@pytest.mark.parametrize("shear", [0.0, 0.25])
+def test_tasklist_angular_momentum_ranges(gpw, oracle, torch_cuda, tau, shear):
+    """la_min < la_max and lb_min < lb_max through the tile kernels and the
+    small-sphere integrate kernel (collocate, integrate, forces; rho and
+    tau; orthorhombic and sheared cells), batched with splits and plain."""
+    torch = torch_cuda
+    rng = np.random.default_rng(71 + 2 * tau + int(10 * shear))
+    npts = (40, 44, 48)
+    cell = np.array([9.0, 9.9, 10.8])
+    dh = np.diag(cell / np.asarray(npts))
+    dh[1, 0] = shear * dh[0, 0]
+    cell_m = dh * np.asarray(npts)[:, None]
+    pairs, nblock = _ranged_pairs(rng, 150, cell_m)
+    cfg = {"npts": npts, "dh": dh, "pairs": pairs, "pab": rng.uniform(-1.0, 1.0, nblock),
+           "nblock": nblock}
+    W = rng.uniform(-1.0, 1.0, (npts[2], npts[1], npts[0]))
+    say("l ranges", "tau" if tau else "rho", "shear", shear)
+    ref = oracle_batch(oracle, cfg, W=W, forces=True, tau=tau)
+    assert np.count_nonzero(ref["grid"]) > 1000
+    for plain, splits in ((False, 1), (False, 3), (True, 1)):
+        got = run_tasklist(gpw, torch, cfg, plain=plain, splits=splits, forces=True, W=W, tau=tau)
+        assert same_support(got["grid"], ref["grid"]), (plain, splits)
+        assert rel_err(got["grid"], ref["grid"]) < TOL, (plain, splits)
+        assert rel_err(got["hab"], ref["hab"]) < TOL, (plain, splits)
+        assert rel_err(got["forces"], ref["forces"]) < TOL, (plain, splits)
+
+
+@pytest.mark.parametrize("tau", [False, True])
+def test_dropin_use_ortho_on_sheared_cell(gpw, oracle, torch_cuda, tau):
+    """use_ortho = 1 with off-diagonal dh: the orthorhombic path on the
+    diagonal steps and the caller's sphere_bounds (grid_base_ref_c.cc:981,
+    :546), not the general path, as the oracle does."""
+    say("drop-in use_ortho on sheared cell", tau)
+    rng = np.random.default_rng(808 + tau)
+    t, pab = random_task(rng, oracle, la=(0, 2), lb=(1, 2), tau=tau, ortho=True, shear=0.15)
+    assert t["dh"][1][0] != 0.0 and t["use_ortho"]
+    ref = oracle.collocate(t, pab)
+    # the general path on the same cell writes a different point set
+    gen = oracle.collocate(dict(t, use_ortho=False), pab)
+    assert not same_support(gen, ref) or rel_err(gen, ref) > 1e-3
+    grid = np.zeros_like(ref)
+    gpw.collocate_pgf_product_rspace(t, pab, grid)
+    assert same_support(grid, ref)
+    assert rel_err(grid, ref) < TOL
+    W = rng.uniform(-1, 1, ref.shape)
+    h_ref, f_ref = oracle.integrate(t, W, pab=pab, forces=True)
+    h, f = gpw.integrate_pgf_product_rspace(t, W, pab=pab, forces=True)
+    assert rel_err(h, h_ref) < TOL
+    assert rel_err(f, f_ref) < TOL
+
+
+def test_tasklist_rejects_short_buffers(gpw, torch_cuda):
+    """Buffers shorter than the task list needs raise ValueError before any
+    launch (grid of the wrong size, pab / hab shorter than the largest
+    block end, forces shorter than 6 per pair)."""
+    from libgpw_amd import workloads as Wk
+    torch = torch_cuda
+    say("short buffers")
+    cfg = Wk.subset(Wk.config(2), np.arange(50))
+    tl = gpw.TaskList(gpw.grid_desc(cfg["npts"], cfg["dh"]), cfg["pairs"])
+    n = cfg["npts"]
+    grid = torch.zeros((n[2], n[1], n[0]), dtype=torch.float64, device="cuda")
+    pab = dev(torch, cfg["pab"])
+    hab = torch.zeros(cfg["nblock"], dtype=torch.float64, device="cuda")
+    f = torch.zeros((50, 6), dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError):
+        tl.collocate(pab, grid[:-1])
+    with pytest.raises(ValueError):
+        tl.collocate(pab[:-1], grid)
+    with pytest.raises(ValueError):
+        tl.integrate(grid, hab[:-1])
+    with pytest.raises(ValueError):
+        tl.integrate(grid, hab, pab=pab, forces=f[:-1])
+    with pytest.raises(ValueError):
+        tl.integrate(grid, hab, forces=f)
+    with pytest.raises(ValueError):
+        tl.collocate(pab.float(), grid)
+    # right sizes still work
+    tl.collocate(pab, grid)
+    tl.integrate(grid, hab, pab=pab, forces=f)
+    torch.cuda.synchronize()
+    assert torch.isfinite(hab).all()
+    tl.close()
+
+
+@pytest.mark.parametrize("shear", [0.0, 0.25])
+def test_tile_index_sets_per_pair(gpw, oracle, torch_cuda, shear):
+    """Per pair, the tile kernels write exactly the oracle's point set on a
+    grid whose spacing (0.225 bohr) is not a binary fraction, so that many
+    points lie within rounding of a sphere or row bound: the in-kernel
+    bounds must round as the host rule does (no fused multiply-add). Pairs
+    of several radii, one task list per pair; collocate support compared
+    exactly, and integrate of a random potential against the oracle."""
+    from libgpw_amd import workloads as Wk
+    torch = torch_cuda
+    rng = np.random.default_rng(91 + int(10 * shear))
+    npts = (40, 44, 48)
+    cell = np.array([9.0, 9.9, 10.8])
+    dh = np.diag(cell / np.asarray(npts))
+    dh[1, 0] = shear * dh[0, 0]
+    pairs, nblock = _ranged_pairs(rng, 40, dh * np.asarray(npts)[:, None])
+    cfg = {"npts": npts, "dh": dh, "pairs": pairs, "pab": rng.uniform(-1.0, 1.0, nblock),
+           "nblock": nblock, "name": "index-sets", "seed": 0}
+    W = rng.uniform(-1.0, 1.0, (npts[2], npts[1], npts[0]))
+    say("per-pair index sets, shear", shear)
+    bad = []
+    for i in range(40):
+        sub = Wk.subset(cfg, [i])
+        ref = oracle_batch(oracle, sub, W=W)
+        got = run_tasklist(gpw, torch, sub, W=W)
+        if not same_support(got["grid"], ref["grid"]):
+            bad.append(i)
+            continue
+        assert rel_err(got["grid"], ref["grid"]) < TOL, i
+        assert rel_err(got["hab"], ref["hab"]) < TOL, i
+    assert not bad, bad
