@@ -91,7 +91,12 @@ typedef struct gpw_tasklist_stats {
  * dh, dh_inv: 3x3 row-major; pab: maxco x maxco row-major host array, the
  * sub-block starts at pab[o2][o1]; grid: ngrid[2] x ngrid[1] x ngrid[0] host
  * array, accumulated. lb_cube/ub_cube/sphere_bounds are only read when
- * use_ortho != 0. Runs on the GPU (one pair, plain kernels). */
+ * use_ortho != 0; use_ortho != 0 takes the orthorhombic path on dh[i][i]
+ * whatever the off-diagonal steps (as the reference does). Runs on the GPU
+ * (one pair, plain kernels); only the grid points of the pair's cube cross
+ * PCIe (the whole grid when the cube holds more points than the grid). The
+ * library keeps a per-thread device workspace that grows on demand and is
+ * held until the process ends. */
 gpw_status gpw_collocate_pgf_product_rspace(
     int compute_tau, int use_ortho, int la_max, int la_min, int lb_max, int lb_min,
     double zeta, double zetb, double rscale, const double *dh, const double *dh_inv,
@@ -112,6 +117,29 @@ gpw_status gpw_integrate_pgf_product_rspace(
     const int *lb_grid, const int *periodic, int lmax, double radius, const int *lb_cube,
     const int *ub_cube, int nspheres, const int *sphere_bounds, int maxco, int o1, int o2,
     const double *grid, const double *pab, double *hab, double *forces);
+
+/* The same two calls on device memory (no counterpart in the reference):
+ * pab / hab are maxco x maxco device arrays (block at row o2, column o1),
+ * read and written in place; grid is the ngrid[2] x ngrid[1] x ngrid[0]
+ * device grid (accumulated); forces_dev: NULL or 6 device doubles
+ * (accumulated). Stream ordered on `stream`; the pair's geometry is uploaded
+ * into the calling thread's device workspace (grown with hipMalloc when a
+ * larger cube needs it), so calls of one thread must stay on one stream. */
+gpw_status gpw_collocate_pgf_product_rspace_dev(
+    int compute_tau, int use_ortho, int la_max, int la_min, int lb_max, int lb_min,
+    double zeta, double zetb, double rscale, const double *dh, const double *dh_inv,
+    const double *ra, const double *rab, const int *npts, const int *ngrid,
+    const int *lb_grid, const int *periodic, int lmax, double radius, const int *lb_cube,
+    const int *ub_cube, int nspheres, const int *sphere_bounds, int maxco, int o1, int o2,
+    const double *pab_dev, double *grid_dev, void *stream);
+gpw_status gpw_integrate_pgf_product_rspace_dev(
+    int compute_tau, int use_ortho, int la_max, int la_min, int lb_max, int lb_min,
+    double zeta, double zetb, double rscale, const double *dh, const double *dh_inv,
+    const double *ra, const double *rab, const int *npts, const int *ngrid,
+    const int *lb_grid, const int *periodic, int lmax, double radius, const int *lb_cube,
+    const int *ub_cube, int nspheres, const int *sphere_bounds, int maxco, int o1, int o2,
+    const double *grid_dev, const double *pab_dev, double *hab_dev, double *forces_dev,
+    void *stream);
 
 /* ---- batched task lists (device memory, stream ordered) ------------------ */
 

@@ -23,6 +23,7 @@ STATUS = {0: "GPW_OK", 1: "GPW_ERR_ARG", 2: "GPW_ERR_LMAX", 3: "GPW_ERR_BOUNDS",
 # every symbol declared in include/libgpw_amd.h
 EXPORTS = (
     "gpw_collocate_pgf_product_rspace", "gpw_integrate_pgf_product_rspace",
+    "gpw_collocate_pgf_product_rspace_dev", "gpw_integrate_pgf_product_rspace_dev",
     "gpw_tasklist_create", "gpw_tasklist_destroy", "gpw_tasklist_stats_get",
     "gpw_tasklist_plan_stats", "gpw_pair_geometry",
     "gpw_tasklist_collocate", "gpw_tasklist_integrate",
@@ -83,6 +84,8 @@ def lib():
                   ip, C.c_int, C.c_int, C.c_int]
         L.gpw_collocate_pgf_product_rspace.argtypes = single + [dp, dp]
         L.gpw_integrate_pgf_product_rspace.argtypes = single + [dp, dp, dp, dp]
+        L.gpw_collocate_pgf_product_rspace_dev.argtypes = single + [vp, vp, vp]
+        L.gpw_integrate_pgf_product_rspace_dev.argtypes = single + [vp, vp, vp, vp, vp]
         L.gpw_tasklist_create.argtypes = [C.POINTER(GridDesc), C.c_int, C.POINTER(PairDesc),
                                           C.c_int, ip, C.c_int, C.POINTER(vp)]
         L.gpw_tasklist_destroy.argtypes = [vp]
@@ -356,6 +359,31 @@ def integrate_pgf_product_rspace(t: dict, grid: np.ndarray, pab: np.ndarray | No
         *_single_args(t), _dptr(grid), None if pab_c is None else _dptr(pab_c), _dptr(hab),
         None if f is None else _dptr(f)))
     return hab, (None if f is None else f.reshape(2, 3))
+
+
+def _single_dev_sizes(t: dict, grid, pab=None, hab=None, forces=None):
+    ng = int(t["ngrid"][0]) * int(t["ngrid"][1]) * int(t["ngrid"][2])
+    mc = int(t["maxco"]) ** 2
+    return (_ptr(grid, ng, "grid", exact=True), _ptr(pab, mc, "pab"), _ptr(hab, mc, "hab"),
+            _ptr(forces, 6, "forces"))
+
+
+def collocate_pgf_product_rspace_dev(t: dict, pab, grid, stream=None):
+    """The single-task collocate on device tensors (pab maxco x maxco, grid
+    ngrid[2] x ngrid[1] x ngrid[0], accumulated), stream ordered."""
+    g, p, _, _ = _single_dev_sizes(t, grid, pab=pab)
+    _check(lib().gpw_collocate_pgf_product_rspace_dev(*_single_args(t), p, g, _stream(stream)))
+    return grid
+
+
+def integrate_pgf_product_rspace_dev(t: dict, grid, hab, pab=None, forces=None, stream=None):
+    """The single-task integrate on device tensors: hab (maxco x maxco) and
+    forces (6, optional; needs pab) accumulated."""
+    if forces is not None and pab is None:
+        raise ValueError("forces need pab")
+    g, p, h, f = _single_dev_sizes(t, grid, pab=pab, hab=hab, forces=forces)
+    _check(lib().gpw_integrate_pgf_product_rspace_dev(*_single_args(t), g, p, h, f, _stream(stream)))
+    return hab
 
 
 def collocate_core(length, co, P, stream=None):

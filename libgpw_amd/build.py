@@ -30,6 +30,7 @@ COMMON = [
 SOURCES = [
     ("gpw_kernels.hip", []),
     ("gpw_plan.cpp", ["-ffp-contract=off"]),
+    ("gpw_single.cpp", ["-ffp-contract=off"]),
 ]
 
 

@@ -22,81 +22,22 @@
 #include <string>
 #include <vector>
 
-#include "gpw_internal.hpp"
+#include "gpw_host.hpp"
 
-namespace gpw {
-hipError_t launch_collocate_tiles(int cls, const TileGrid &tg, const EntryRec *recs,
-                                  const int *tile_start, const double *coef, int ntiles, int splits,
-                                  double *out, int out_mode, hipStream_t st);
-hipError_t launch_reduce_partials(const double *partial, int splits, long long ng, double *grid,
-                                  hipStream_t st);
-hipError_t launch_integrate_tiles(int cls, const TileGrid &tg, const EntryRec *recs,
-                                  const int *tile_start, const double *grid, int lp_add,
-                                  int ntiles, int splits, double *partial, hipStream_t st);
-hipError_t launch_integrate_tiles_84(const TileGrid &tg, const EntryRec *recs8, const int *start8,
-                                     int splits8, const EntryRec *recs4, const int *start4,
-                                     int splits4, const double *grid, int lp_add, int ntiles,
-                                     double *partial, hipStream_t st);
-hipError_t launch_coef(const PairDev *pairs, const int *order, int pair0, int npairs, int tau,
-                       int lam_max, int lbm_max, const double *pab, double *coef,
-                       const LatticeCSR &lt, double *ws,
-                       long long ws_stride, int ws_blocks, hipStream_t st);
-size_t coef_smem_bytes(int lam, int lbm, bool lds_blocks);
-size_t coef_ws_doubles(int lam, int lbm);
-bool coef_needs_workspace(int lam, int lbm);
-hipError_t launch_hab(const PairDev *pairs, const int *order, const int *segs, int seg0, int nseg,
-                      int tau, int with_forces, int la_max, int lb_max, const double *Mbuf,
-                      int mstride, const double *pab, double *hab, double *forces,
-                      const LatticeCSR &lt, double *ws,
-                      long long ws_stride, int ws_blocks, hipStream_t st);
-size_t hab_ws_doubles(int lamx, int lbmx, int lam, int lbm);
-bool hab_needs_workspace(int lamx, int lbmx, int lam, int lbm);
-hipError_t launch_moments_small(int cls, const TileGrid &tg, const SubDev *subs, const int *list,
-                                int n, const int *prof, const double *grid, int lp_add, int mstride,
-                                double *Mbuf, hipStream_t st);
-hipError_t launch_sum_entries(const PairDev *pairs, int npairs, int tau, int lp_add,
-                              const int *pe_start, const long long *ent_off,
-                              const double *partial, int mstride, const unsigned char *skip,
-                              double *Mbuf, hipStream_t st);
-hipError_t launch_fill(double *p, long long n, double v, hipStream_t st);
-hipError_t launch_collocate_plain(const TileGrid &tg, const SubDev *subs, int si, int nz_box,
-                                  int ny_box, const double *coef, const int *prof, double *grid,
-                                  hipStream_t st);
-hipError_t launch_moments_plain(const TileGrid &tg, const SubDev *subs, int si, int nz_box,
-                                int ny_box, int lp_add, const int *prof, const double *grid,
-                                int mstride, double *Mbuf, hipStream_t st);
-hipError_t launch_collocate_core(const int *length, int L, const double *co, const double *P,
-                                 int ldp, double *V, hipStream_t st);
-hipError_t launch_integrate_core(const int *length, int L, const double *P, int ldp,
-                                 const double *V, double *Velems, hipStream_t st);
-}  // namespace gpw
 
 using namespace gpw;
 
 static thread_local std::string g_last_error;
 
-static gpw_status fail(gpw_status code, const std::string &msg) {
+gpw_status gpw::fail(gpw_status code, const std::string &msg) {
     g_last_error = msg;
     return code;
 }
 
-#define HIP_TRY(expr)                                                                      \
-    do {                                                                                   \
-        hipError_t _e = (expr);                                                            \
-        if (_e != hipSuccess)                                                              \
-            return fail(GPW_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));   \
-    } while (0)
-
-static int imod_h(int a, int m) { return (a % m + m) % m; }
+int gpw::imod_h(int a, int m) { return (a % m + m) % m; }
 
 // ---------------------------------------------------------------------------
 // sphere profiles: T[kk][jj] = -igmin (>= 0) or -1 for an empty row
-
-struct Profile {
-    int n[3];              // table covers fold offsets 0..n[d]
-    std::vector<int> T;    // (n[2]+1) x (n[1]+1)
-    double R2 = 0.0;       // squared sphere radius when built from a radius
-};
 
 static int sphere_extent(double base, double h, double R2) {
     int n = 0;
@@ -111,7 +52,7 @@ static int sphere_extent(double base, double h, double R2) {
 
 // Profile for a radius on an orthorhombic grid with spacings dr (the rule
 // that regenerates the reference's fixture sphere_bounds).
-static void profile_from_radius(double radius, const double dr[3], Profile &P, int lb[3],
+void gpw::profile_from_radius(double radius, const double dr[3], Profile &P, int lb[3],
                                 int ub[3]) {
     double drmin = std::min(dr[0], std::min(dr[1], dr[2]));
     int imr = (int)std::ceil(radius / drmin);
@@ -139,7 +80,7 @@ static void profile_from_radius(double radius, const double dr[3], Profile &P, i
 }
 
 // Profile from a caller's sphere_bounds (grid_base_ref_c.cc:413-470 walk).
-static gpw_status profile_from_bounds(const int lb[3], const int ub[3], int nspheres,
+gpw_status gpw::profile_from_bounds(const int lb[3], const int ub[3], int nspheres,
                                       const int *sb, Profile &P) {
     if (nspheres < 1 || !sb) return fail(GPW_ERR_SPHERE, "sphere_bounds empty");
     for (int d = 0; d < 3; d++) P.n[d] = std::max(std::max(-lb[d], ub[d] - 1), 0);
@@ -166,14 +107,6 @@ static gpw_status profile_from_bounds(const int lb[3], const int ub[3], int nsph
 
 // ---------------------------------------------------------------------------
 
-struct PairGeom {
-    int cc[3];
-    double roff[3];  // general cells: lattice centre gp
-    int lb[3], ub[3];
-    int prof;        // index into profile list, -1 none
-    double R2 = 0.0; // general cells: radius^2
-};
-
 // work-item classes by polynomial length L = lp + 1 (one kernel instantiation
 // each, so registers are sized for the class, not for the largest L)
 constexpr int NCLS = 4;
@@ -192,11 +125,6 @@ struct gpw_tasklist {
     gpw_grid_desc grid{};
     int compute_tau = 0;
     int npairs = 0;
-    int force_general = 0;  // single tasks with use_ortho = 0 on any cell
-    int force_ortho = 0;    // single tasks with use_ortho = 1: the orthorhombic
-                            // path on dh[i][i] even when dh has off-diagonal
-                            // terms (grid_base_ref_c.cc:981 takes it on the
-                            // caller's word, :546 reads only dh[i][i])
     int lp_max = 0;
     int la_max = 0, lb_max = 0;  // largest given la_max / lb_max
     TileGrid tg{};
@@ -298,14 +226,14 @@ static gpw_status upload(T **dst, const std::vector<T> &src) {
     return GPW_OK;
 }
 
-static bool is_ortho(const gpw_grid_desc &g) {
+bool gpw::is_ortho(const gpw_grid_desc &g) {
     for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++)
             if (i != j && g.dh[i][j] != 0.0) return false;
     return true;
 }
 
-static gpw_status check_pair(const gpw_pair_desc &p, int tau) {
+gpw_status gpw::check_pair(const gpw_pair_desc &p, int tau) {
     if (p.la_min < 0 || p.lb_min < 0 || p.la_max < p.la_min || p.lb_max < p.lb_min)
         return fail(GPW_ERR_ARG, "bad angular momentum range");
     // +1 headroom for the force extension
@@ -316,7 +244,7 @@ static gpw_status check_pair(const gpw_pair_desc &p, int tau) {
 }
 
 // FP64 pair set-up exactly as grid_collocate_internal (grid_base_ref_c.cc:878-886)
-static PairDev make_pair_dev(const gpw_pair_desc &p, int tau) {
+PairDev gpw::make_pair_dev(const gpw_pair_desc &p, int tau) {
     PairDev d{};
     d.zeta = p.zeta;
     d.zetb = p.zetb;
@@ -340,7 +268,7 @@ static PairDev make_pair_dev(const gpw_pair_desc &p, int tau) {
 }
 
 // cube centre and offset (grid_base_ref_c.cc:534-547)
-static void cube_centre(const gpw_grid_desc &g, const double rp[3], int cc[3], double roff[3]) {
+void gpw::cube_centre(const gpw_grid_desc &g, const double rp[3], int cc[3], double roff[3]) {
     for (int i = 0; i < 3; i++) {
         double v = 0.0;
         for (int j = 0; j < 3; j++) v += g.dh_inv[j][i] * rp[j];
@@ -352,7 +280,7 @@ static void cube_centre(const gpw_grid_desc &g, const double rp[3], int cc[3], d
 // General cell (grid_base_ref_c.cc:636-644, 700-721): lattice centre gp and
 // the lattice box around the sphere of the pair's radius, from the 27 points
 // rp + {-1,0,1}^3 * radius.
-static void general_geometry(const gpw_grid_desc &g, const double rp[3], double radius,
+void gpw::general_geometry(const gpw_grid_desc &g, const double rp[3], double radius,
                              PairGeom &geo) {
     for (int i = 0; i < 3; i++) {
         geo.cc[i] = 0;
@@ -399,10 +327,9 @@ static bool general_row_host(const gpw_grid_desc &g, double gp0, double dj, doub
 }
 
 // Split the pair's cube into sub-boxes that fit the period, append SubDevs.
-static gpw_status make_subs(const gpw_tasklist &tl, const PairDev &pd, int pair_index,
-                            const PairGeom &geo, int prof_off, int prof_stride,
-                            std::vector<SubDev> &out) {
-    const gpw_grid_desc &g = tl.grid;
+gpw_status gpw::make_subs(const gpw_grid_desc &g, const PairDev &pd, int pair_index,
+                         const PairGeom &geo, int prof_off, int prof_stride,
+                         std::vector<SubDev> &out) {
     std::vector<std::pair<int, int>> ranges[3];
     int base_of[3][64];
     for (int d = 0; d < 3; d++) {
@@ -529,7 +456,7 @@ static double contraction_flops(const SubWork &w, int lp) {
 // (x - xp)^lx (y - yp)^ly (z - zp)^lz expanded in lattice displacements
 // (i - ip)^il (j - jp)^jl (k - kp)^kl, T[ijk][xyz] with both indices packed
 // as (z-like, y-like, x-like) triples of total degree <= lp.
-static void multinomial_matrix(const double dh[3][3], int lp, double *T) {
+void gpw::multinomial_matrix(const double dh[3][3], int lp, double *T) {
     const int n = ncoset(lp);
     std::vector<double> hp((size_t)(lp + 1) * 9);  // hp[k][j][i] = dh[j][i]^k
     for (int i = 0; i < 3; i++)
@@ -567,6 +494,54 @@ static void multinomial_matrix(const double dh[3][3], int lp, double *T) {
                                     }
                         }
             }
+}
+
+// Lattice transforms T(lp) of a general cell for lp = 0 .. lpm as CSR, in
+// both directions (dir 0: T, dir 1: T^T), exact zeros dropped; row pointers
+// of (lp, dir) start at rpoff[2 * lp + dir].
+static gpw_status lattice_csr_of(const double dh[3][3], const std::vector<int> &lps, LatticeHost &out) {
+    out.rp.clear();
+    out.col.clear();
+    out.val.clear();
+    out.rpoff.assign(2 * (LPMAX + 1), 0);
+    std::vector<double> T;
+    for (int lp : lps) {
+        const int n = ncoset(lp);
+        T.assign((size_t)n * n, 0.0);
+        multinomial_matrix(dh, lp, T.data());
+        const double *Tl = T.data();
+        for (int dir = 0; dir < 2; dir++) {
+            out.rpoff[2 * lp + dir] = (long long)out.rp.size();
+            for (int r = 0; r < n; r++) {
+                out.rp.push_back((int)out.val.size());
+                for (int c = 0; c < n; c++) {
+                    const double v = dir == 0 ? Tl[(size_t)r * n + c] : Tl[(size_t)c * n + r];
+                    if (v != 0.0) {
+                        out.col.push_back(c);
+                        out.val.push_back(v);
+                    }
+                }
+            }
+            out.rp.push_back((int)out.val.size());
+        }
+    }
+    if (out.val.size() > 0x7fffffffULL) return fail(GPW_ERR_ARG, "lattice transform too large");
+    return GPW_OK;
+}
+
+gpw_status gpw::lattice_csr(const double dh[3][3], int lpm, LatticeHost &out) {
+    std::vector<int> lps;
+    for (int lp = 0; lp <= lpm; lp++) lps.push_back(lp);
+    return lattice_csr_of(dh, lps, out);
+}
+
+// only the degrees lps[0 .. n) (rows of other degrees are absent)
+gpw_status gpw::lattice_csr_some(const double dh[3][3], const int *lps, int n, LatticeHost &out) {
+    std::vector<int> v;
+    for (int i = 0; i < n; i++)
+        if (lps[i] >= 0 && lps[i] <= LPMAX && std::find(v.begin(), v.end(), lps[i]) == v.end())
+            v.push_back(lps[i]);
+    return lattice_csr_of(dh, v, out);
 }
 
 // Host part of a task list: profiles, pairs, sub-boxes, tile work lists.
@@ -615,7 +590,7 @@ static gpw_status plan_tasklist(gpw_tasklist *tl, const std::vector<gpw_pair_des
     for (size_t i = 0; i < pairs.size(); i++) {
         tl->pair_sub_start[i] = (int)tl->subs_h.size();
         const int pi = geos[i].prof;
-        gpw_status st = make_subs(*tl, tl->pairs_h[i], (int)i, geos[i], pi >= 0 ? prof_off[pi] : -1,
+        gpw_status st = make_subs(tl->grid, tl->pairs_h[i], (int)i, geos[i], pi >= 0 ? prof_off[pi] : -1,
                                   pi >= 0 ? prof_stride[pi] : 0, tl->subs_h);
         if (st) return st;
     }
@@ -628,7 +603,7 @@ static gpw_status plan_tasklist(gpw_tasklist *tl, const std::vector<gpw_pair_des
     tg.ntx = (tg.nx + TX - 1) / TX;
     tg.nty = (tg.ny + TY - 1) / TY;
     tg.ntz = (tg.nz + TZ - 1) / TZ;
-    tg.general = tl->force_general ? 1 : tl->force_ortho ? 0 : !is_ortho(g) ? 1 : 0;
+    tg.general = !is_ortho(g) ? 1 : 0;
     for (int d = 0; d < 3; d++) {
         tg.dr[d] = g.dh[d][d];
         tg.period[d] = g.periodic[d] ? g.npts[d] : BIG;
@@ -866,42 +841,12 @@ static gpw_status upload_tasklist(gpw_tasklist *tl, const HostPlan &hp) {
         }
     }
     if (tl->tg.general) {
-        const int lpm = std::min(tl->lp_max + 2, LPMAX);
-        std::vector<long long> off(LPMAX + 1, 0);
-        long long total = 0;
-        for (int lp = 0; lp <= lpm; lp++) {
-            off[lp] = total;
-            total += (long long)ncoset(lp) * ncoset(lp);
-        }
-        std::vector<double> T((size_t)total);
-        for (int lp = 0; lp <= lpm; lp++) multinomial_matrix(tl->grid.dh, lp, T.data() + off[lp]);
-        // CSR of T (dir 0) and of its transpose (dir 1), exact zeros dropped
-        std::vector<int> rp, col;
-        std::vector<double> val;
-        std::vector<long long> rpoff(2 * (LPMAX + 1), 0);
-        for (int lp = 0; lp <= lpm; lp++) {
-            const int n = ncoset(lp);
-            const double *Tl = T.data() + off[lp];
-            for (int dir = 0; dir < 2; dir++) {
-                rpoff[2 * lp + dir] = (long long)rp.size();
-                for (int r = 0; r < n; r++) {
-                    rp.push_back((int)val.size());
-                    for (int c = 0; c < n; c++) {
-                        const double v = dir == 0 ? Tl[(size_t)r * n + c] : Tl[(size_t)c * n + r];
-                        if (v != 0.0) {
-                            col.push_back(c);
-                            val.push_back(v);
-                        }
-                    }
-                }
-                rp.push_back((int)val.size());
-            }
-        }
-        if (val.size() > 0x7fffffffULL) return fail(GPW_ERR_ARG, "lattice transform too large");
-        if ((st = upload(&tl->d_mt_rp, rp))) return st;
-        if ((st = upload(&tl->d_mt_col, col))) return st;
-        if ((st = upload(&tl->d_mt_val, val))) return st;
-        if ((st = upload(&tl->d_mt_rpoff, rpoff))) return st;
+        LatticeHost lh;
+        if ((st = lattice_csr(tl->grid.dh, std::min(tl->lp_max + 2, LPMAX), lh))) return st;
+        if ((st = upload(&tl->d_mt_rp, lh.rp))) return st;
+        if ((st = upload(&tl->d_mt_col, lh.col))) return st;
+        if ((st = upload(&tl->d_mt_val, lh.val))) return st;
+        if ((st = upload(&tl->d_mt_rpoff, lh.rpoff))) return st;
     }
     return GPW_OK;
 }
@@ -915,7 +860,7 @@ static gpw_status finish_tasklist(gpw_tasklist *tl, const std::vector<gpw_pair_d
     return upload_tasklist(tl, hp);
 }
 
-static gpw_status check_grid(const gpw_grid_desc *g) {
+gpw_status gpw::check_grid(const gpw_grid_desc *g) {
     if (!g) return fail(GPW_ERR_ARG, "grid descriptor is NULL");
     for (int d = 0; d < 3; d++) {
         if (g->npts[d] <= 0 || g->ngrid[d] <= 0) return fail(GPW_ERR_ARG, "grid sizes must be > 0");
@@ -1503,195 +1448,6 @@ gpw_status gpw_collocate_core_host(const int *length, int L, const double *co, c
 gpw_status gpw_integrate_core_host(const int *length, int L, const double *P, int ldp,
                                    const double *V, double *Velems) {
     return core_host(false, length, L, nullptr, P, ldp, V, Velems);
-}
-
-// ---- single task (drop-in) ------------------------------------------------
-
-struct SingleTask {
-    gpw_grid_desc g;
-    gpw_pair_desc p;
-    int tau;
-};
-
-// Validates the reference arguments and builds a one-pair task list whose
-// geometry is the caller's lb_cube / ub_cube / sphere_bounds.
-static gpw_status single_setup(int compute_tau, int use_ortho, int la_max, int la_min, int lb_max,
-                               int lb_min, double zeta, double zetb, double rscale, const double *dh,
-                               const double *dh_inv, const double *ra, const double *rab,
-                               const int *npts, const int *ngrid, const int *lb_grid,
-                               const int *periodic, int lmax, double radius, const int *lb_cube,
-                               const int *ub_cube, int nspheres, const int *sphere_bounds,
-                               int maxco, int o1, int o2, gpw_tasklist **out) {
-    *out = nullptr;
-    if (!dh || !dh_inv || !ra || !rab || !npts || !ngrid || !lb_grid || !periodic)
-        return fail(GPW_ERR_ARG, "NULL argument");
-    const int tau = compute_tau ? 1 : 0;
-    if (la_min < 0 || lb_min < 0 || la_max < la_min || lb_max < lb_min)
-        return fail(GPW_ERR_ARG, "bad angular momentum range");
-    // reference preconditions (alpha sized by lmax, grid_base_ref_c.cc:946;
-    // coef_xyt sized by 2*lmax, :246-248)
-    if (lmax < la_max + tau || lmax < lb_max + tau || 2 * lmax < la_max + lb_max + 2 * tau)
-        return fail(GPW_ERR_LMAX, "lmax below the local angular momentum");
-    if (o1 < 0 || o2 < 0 || o1 + ncoset(la_max) > maxco || o2 + ncoset(lb_max) > maxco)
-        return fail(GPW_ERR_ARG, "pab block outside maxco");
-    if (use_ortho && (!lb_cube || !ub_cube)) return fail(GPW_ERR_ARG, "ortho path needs lb_cube/ub_cube");
-    if (!use_ortho && !(radius > 0.0)) return fail(GPW_ERR_ARG, "general path needs radius > 0");
-    gpw_grid_desc g{};
-    for (int i = 0; i < 3; i++) {
-        g.npts[i] = npts[i];
-        g.ngrid[i] = ngrid[i];
-        g.lb_grid[i] = lb_grid[i];
-        g.periodic[i] = periodic[i] ? 1 : 0;
-        for (int j = 0; j < 3; j++) {
-            g.dh[i][j] = dh[3 * i + j];
-            g.dh_inv[i][j] = dh_inv[3 * i + j];
-        }
-    }
-    for (int d = 0; d < 3; d++) {
-        if (g.npts[d] <= 0 || g.ngrid[d] <= 0) return fail(GPW_ERR_ARG, "grid sizes must be > 0");
-        if (g.periodic[d] && g.ngrid[d] != g.npts[d])
-            return fail(GPW_ERR_ARG, "periodic axes need ngrid == npts");
-    }
-    gpw_pair_desc p{};
-    p.la_max = la_max;
-    p.la_min = la_min;
-    p.lb_max = lb_max;
-    p.lb_min = lb_min;
-    p.zeta = zeta;
-    p.zetb = zetb;
-    p.rscale = rscale;
-    for (int i = 0; i < 3; i++) {
-        p.ra[i] = ra[i];
-        p.rab[i] = rab[i];
-    }
-    p.radius = radius;
-    // the device block is re-packed: ncoset(lb_max) x ncoset(la_max)
-    p.block_offset = 0;
-    gpw_status st = check_pair(p, tau);
-    if (st) return st;
-    auto *tl = new (std::nothrow) gpw_tasklist();
-    if (!tl) return fail(GPW_ERR_ALLOC, "host allocation failed");
-    tl->grid = g;
-    tl->compute_tau = tau;
-    tl->npairs = 1;
-    tl->force_general = use_ortho ? 0 : 1;
-    tl->force_ortho = use_ortho ? 1 : 0;
-    std::vector<Profile> profiles;
-    std::vector<PairGeom> geos(1);
-    const PairDev pd = make_pair_dev(p, tau);
-    if (use_ortho) {
-        profiles.resize(1);
-        if ((st = profile_from_bounds(lb_cube, ub_cube, nspheres, sphere_bounds, profiles[0]))) {
-            delete tl;
-            return st;
-        }
-        cube_centre(g, pd.rp, geos[0].cc, geos[0].roff);
-        for (int d = 0; d < 3; d++) {
-            geos[0].lb[d] = lb_cube[d];
-            geos[0].ub[d] = ub_cube[d];
-        }
-        geos[0].prof = 0;
-    } else {
-        general_geometry(g, pd.rp, radius, geos[0]);
-    }
-    std::vector<gpw_pair_desc> pv(1, p);
-    if ((st = finish_tasklist(tl, pv, geos, profiles, 1))) {
-        delete tl;
-        return st;
-    }
-    *out = tl;
-    return GPW_OK;
-}
-
-gpw_status gpw_collocate_pgf_product_rspace(
-    int compute_tau, int use_ortho, int la_max, int la_min, int lb_max, int lb_min, double zeta,
-    double zetb, double rscale, const double *dh, const double *dh_inv, const double *ra,
-    const double *rab, const int *npts, const int *ngrid, const int *lb_grid, const int *periodic,
-    int lmax, double radius, const int *lb_cube, const int *ub_cube, int nspheres,
-    const int *sphere_bounds, int maxco, int o1, int o2, const double *pab, double *grid) {
-    if (!pab || !grid) return fail(GPW_ERR_ARG, "NULL argument");
-    gpw_tasklist *tl = nullptr;
-    gpw_status st = single_setup(compute_tau, use_ortho, la_max, la_min, lb_max, lb_min, zeta, zetb,
-                                 rscale, dh, dh_inv, ra, rab, npts, ngrid, lb_grid, periodic, lmax,
-                                 radius, lb_cube, ub_cube, nspheres, sphere_bounds, maxco, o1, o2, &tl);
-    if (st) return st;
-    const int na = ncoset(la_max), nb = ncoset(lb_max);
-    std::vector<double> blk((size_t)na * nb);
-    for (int j = 0; j < nb; j++)
-        for (int i = 0; i < na; i++) blk[(size_t)j * na + i] = pab[(size_t)(o2 + j) * maxco + o1 + i];
-    const size_t ng = (size_t)ngrid[0] * ngrid[1] * ngrid[2];
-    double *d_pab = nullptr, *d_grid = nullptr;
-    gpw_status rc = GPW_OK;
-    hipError_t e;
-    if ((e = hipMalloc((void **)&d_pab, blk.size() * sizeof(double))) != hipSuccess ||
-        (e = hipMalloc((void **)&d_grid, ng * sizeof(double))) != hipSuccess) {
-        rc = fail(GPW_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
-    } else if ((e = hipMemcpy(d_pab, blk.data(), blk.size() * sizeof(double), hipMemcpyHostToDevice)) !=
-                   hipSuccess ||
-               (e = hipMemcpy(d_grid, grid, ng * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess) {
-        rc = fail(GPW_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
-    } else if ((rc = gpw_tasklist_collocate_plain(tl, d_pab, d_grid, nullptr)) == GPW_OK) {
-        if ((e = hipMemcpy(grid, d_grid, ng * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess)
-            rc = fail(GPW_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
-    }
-    if (d_pab) (void)hipFree(d_pab);
-    if (d_grid) (void)hipFree(d_grid);
-    delete tl;
-    return rc;
-}
-
-gpw_status gpw_integrate_pgf_product_rspace(
-    int compute_tau, int use_ortho, int la_max, int la_min, int lb_max, int lb_min, double zeta,
-    double zetb, double rscale, const double *dh, const double *dh_inv, const double *ra,
-    const double *rab, const int *npts, const int *ngrid, const int *lb_grid, const int *periodic,
-    int lmax, double radius, const int *lb_cube, const int *ub_cube, int nspheres,
-    const int *sphere_bounds, int maxco, int o1, int o2, const double *grid, const double *pab,
-    double *hab, double *forces) {
-    if (!grid || !hab || (forces && !pab)) return fail(GPW_ERR_ARG, "NULL argument");
-    gpw_tasklist *tl = nullptr;
-    gpw_status st = single_setup(compute_tau, use_ortho, la_max, la_min, lb_max, lb_min, zeta, zetb,
-                                 rscale, dh, dh_inv, ra, rab, npts, ngrid, lb_grid, periodic, lmax,
-                                 radius, lb_cube, ub_cube, nspheres, sphere_bounds, maxco, o1, o2, &tl);
-    if (st) return st;
-    const int na = ncoset(la_max), nb = ncoset(lb_max);
-    std::vector<double> blk((size_t)na * nb, 0.0);
-    if (pab)
-        for (int j = 0; j < nb; j++)
-            for (int i = 0; i < na; i++) blk[(size_t)j * na + i] = pab[(size_t)(o2 + j) * maxco + o1 + i];
-    const size_t ng = (size_t)ngrid[0] * ngrid[1] * ngrid[2];
-    double *d_pab = nullptr, *d_grid = nullptr, *d_hab = nullptr, *d_f = nullptr;
-    gpw_status rc = GPW_OK;
-    hipError_t e = hipSuccess;
-    std::vector<double> hblk(blk.size(), 0.0), f(6, 0.0);
-    if ((e = hipMalloc((void **)&d_pab, blk.size() * sizeof(double))) != hipSuccess ||
-        (e = hipMalloc((void **)&d_hab, blk.size() * sizeof(double))) != hipSuccess ||
-        (e = hipMalloc((void **)&d_f, 6 * sizeof(double))) != hipSuccess ||
-        (e = hipMalloc((void **)&d_grid, ng * sizeof(double))) != hipSuccess) {
-        rc = fail(GPW_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
-    } else if ((e = hipMemcpy(d_pab, blk.data(), blk.size() * sizeof(double), hipMemcpyHostToDevice)) !=
-                   hipSuccess ||
-               (e = hipMemcpy(d_hab, hblk.data(), hblk.size() * sizeof(double), hipMemcpyHostToDevice)) !=
-                   hipSuccess ||
-               (e = hipMemcpy(d_f, f.data(), 6 * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess ||
-               (e = hipMemcpy(d_grid, grid, ng * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess) {
-        rc = fail(GPW_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
-    } else if ((rc = gpw_tasklist_integrate_plain(tl, d_grid, d_pab, d_hab, forces ? d_f : nullptr,
-                                                  nullptr)) == GPW_OK) {
-        if ((e = hipMemcpy(hblk.data(), d_hab, hblk.size() * sizeof(double), hipMemcpyDeviceToHost)) !=
-                hipSuccess ||
-            (e = hipMemcpy(f.data(), d_f, 6 * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess)
-            rc = fail(GPW_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
-    }
-    if (rc == GPW_OK) {
-        for (int j = 0; j < nb; j++)
-            for (int i = 0; i < na; i++) hab[(size_t)(o2 + j) * maxco + o1 + i] += hblk[(size_t)j * na + i];
-        if (forces)
-            for (int k = 0; k < 6; k++) forces[k] += f[k];
-    }
-    for (double *p : {d_pab, d_grid, d_hab, d_f})
-        if (p) (void)hipFree(p);
-    delete tl;
-    return rc;
 }
 
 }  // extern "C"

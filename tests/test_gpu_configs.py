@@ -244,3 +244,66 @@ def test_tile_index_sets_per_pair(gpw, oracle, torch_cuda, shear):
         assert rel_err(got["grid"], ref["grid"]) < TOL, i
         assert rel_err(got["hab"], ref["hab"]) < TOL, i
     assert not bad, bad
+
+
+@pytest.mark.parametrize("case", ["window", "wide", "nonperiodic", "general-window", "general-wide"])
+@pytest.mark.parametrize("tau", [False, True])
+def test_dropin_host_and_device_arrays(gpw, oracle, torch_cuda, case, tau):
+    """The single-task calls on host arrays (only the cube's points travel;
+    the whole grid when the cube holds more points than the grid: "wide")
+    and on device arrays (the _dev entry points, pab / hab read and written
+    in place as maxco x maxco), against the oracle, with forces; the same
+    call twice accumulates twice."""
+    torch = torch_cuda
+    say("drop-in host/device arrays", case, tau)
+    rng = np.random.default_rng(900 + 7 * tau + ["window", "wide", "nonperiodic", "general-window",
+                                                   "general-wide"].index(case))
+    kw = dict(la=(0, 2), lb=(1, 3), tau=tau, ortho=not case.startswith("general"))
+    if case in ("window", "general-window"):
+        kw.update(npts=(48, 50, 52), cell=(12.0, 12.5, 13.0), zeta=1.5, zetb=2.5)
+    elif case in ("wide", "general-wide"):
+        kw.update(npts=(16, 18, 20), cell=(4.0, 4.5, 5.0), zeta=0.3, zetb=0.4)
+    else:
+        kw.update(npts=(40, 36, 44), cell=(10.0, 9.0, 11.0), periodic=(0, 1, 0),
+                  window=(30, 36, 32), zeta=2.0, zetb=2.0)
+    if case.startswith("general"):
+        kw["shear"] = 0.2
+    t, pab = random_task(rng, oracle, **kw)
+    if case == "nonperiodic":
+        t["ra"] = np.array([5.0, 4.0, 5.5]); t["rab"] = np.array([0.3, -0.2, 0.1])
+        t["lb_grid"] = [-5, 0, -6]
+    ref = oracle.collocate(t, pab)
+    assert np.count_nonzero(ref) > 100
+    W = rng.uniform(-1, 1, ref.shape)
+    h_ref, f_ref = oracle.integrate(t, W, pab=pab, forces=True)
+    # host arrays, twice: accumulates
+    base = rng.uniform(-1, 1, ref.shape)
+    grid = base.copy()
+    gpw.collocate_pgf_product_rspace(t, pab, grid)
+    assert rel_err(grid - base, ref) < TOL
+    gpw.collocate_pgf_product_rspace(t, pab, grid)
+    assert rel_err(grid - base, 2 * ref) < TOL
+    g0 = np.zeros_like(ref)
+    gpw.collocate_pgf_product_rspace(t, pab, g0)
+    assert same_support(g0, ref)
+    h, f = gpw.integrate_pgf_product_rspace(t, W, pab=pab, forces=True)
+    assert rel_err(h, h_ref) < TOL and rel_err(f, f_ref) < TOL
+    # device arrays
+    gd = dev(torch, base)
+    pd = dev(torch, pab)
+    gpw.collocate_pgf_product_rspace_dev(t, pd, gd)
+    gpw.collocate_pgf_product_rspace_dev(t, pd, gd)
+    torch.cuda.synchronize()
+    assert rel_err(gd.cpu().numpy() - base, 2 * ref) < TOL
+    hd = torch.zeros((t["maxco"], t["maxco"]), dtype=torch.float64, device="cuda")
+    fd = torch.zeros(6, dtype=torch.float64, device="cuda")
+    gpw.integrate_pgf_product_rspace_dev(t, dev(torch, W), hd, pab=pd, forces=fd)
+    torch.cuda.synchronize()
+    assert rel_err(hd.cpu().numpy(), h_ref) < TOL
+    assert rel_err(fd.cpu().numpy().reshape(2, 3), f_ref) < TOL
+    # only the block (o2.., o1..) of hab is written
+    mask = np.zeros_like(h_ref, dtype=bool)
+    na, nb = (t["la_max"] + 1) * (t["la_max"] + 2) * (t["la_max"] + 3) // 6, \
+        (t["lb_max"] + 1) * (t["lb_max"] + 2) * (t["lb_max"] + 3) // 6
+    mask[t["o2"]:t["o2"] + nb, t["o1"]:t["o1"] + na] = True
+    assert not hd.cpu().numpy()[~mask].any()
