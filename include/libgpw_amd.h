@@ -141,6 +141,27 @@ gpw_status gpw_integrate_pgf_product_rspace_dev(
     const double *grid_dev, const double *pab_dev, double *hab_dev, double *forces_dev,
     void *stream);
 
+/* Allocation-free variant of gpw_integrate_pgf_product_rspace_dev (no
+ * counterpart in the reference): for callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc. Same arguments and results; it calls no hipMalloc,
+ * hipFree or other device-allocating routine. Its device scratch (the pair's
+ * geometry, moments and transform workspace) is a fixed array of
+ * GPW_NOALLOC_SCRATCH_BYTES per GPU, held by the library's code object until
+ * the process ends; a task needing more returns GPW_ERR_ARG before any
+ * device work. Every call on one GPU uses that GPU's one array: calls that
+ * overlap on a GPU (other host threads, or other streams not ordered with
+ * this one) race, so order them on one stream. */
+#define GPW_NOALLOC_SCRATCH_BYTES (8u << 20)
+gpw_status gpw_integrate_pgf_product_rspace_noalloc(
+    int compute_tau, int use_ortho, int la_max, int la_min, int lb_max, int lb_min,
+    double zeta, double zetb, double rscale, const double *dh, const double *dh_inv,
+    const double *ra, const double *rab, const int *npts, const int *ngrid,
+    const int *lb_grid, const int *periodic, int lmax, double radius, const int *lb_cube,
+    const int *ub_cube, int nspheres, const int *sphere_bounds, int maxco, int o1, int o2,
+    const double *grid_dev, const double *pab_dev, double *hab_dev, double *forces_dev,
+    void *stream);
+
 /* ---- batched task lists (device memory, stream ordered) ------------------ */
 
 /* Builds the per-pair geometry (cube centre, cube and sphere bounds from each

@@ -24,6 +24,7 @@ STATUS = {0: "GPW_OK", 1: "GPW_ERR_ARG", 2: "GPW_ERR_LMAX", 3: "GPW_ERR_BOUNDS",
 EXPORTS = (
     "gpw_collocate_pgf_product_rspace", "gpw_integrate_pgf_product_rspace",
     "gpw_collocate_pgf_product_rspace_dev", "gpw_integrate_pgf_product_rspace_dev",
+    "gpw_integrate_pgf_product_rspace_noalloc",
     "gpw_tasklist_create", "gpw_tasklist_destroy", "gpw_tasklist_stats_get",
     "gpw_tasklist_plan_stats", "gpw_pair_geometry",
     "gpw_tasklist_collocate", "gpw_tasklist_integrate",
@@ -86,6 +87,7 @@ def lib():
         L.gpw_integrate_pgf_product_rspace.argtypes = single + [dp, dp, dp, dp]
         L.gpw_collocate_pgf_product_rspace_dev.argtypes = single + [vp, vp, vp]
         L.gpw_integrate_pgf_product_rspace_dev.argtypes = single + [vp, vp, vp, vp, vp]
+        L.gpw_integrate_pgf_product_rspace_noalloc.argtypes = single + [vp, vp, vp, vp, vp]
         L.gpw_tasklist_create.argtypes = [C.POINTER(GridDesc), C.c_int, C.POINTER(PairDesc),
                                           C.c_int, ip, C.c_int, C.POINTER(vp)]
         L.gpw_tasklist_destroy.argtypes = [vp]

@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "gpw_host.hpp"
@@ -380,6 +381,33 @@ gpw_status cached_region(size_t need, hipStream_t st, char **out) {
     return GPW_OK;
 }
 
+// Scratch of the allocation-free entry point: one array per GPU, loaded
+// with the library's code object. Host code never passes `scratch` itself to
+// a kernel or a copy (on the host it is only a stand-in symbol): it uses the
+// device address that hipGetSymbolAddress returns for the current GPU.
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) unsigned char scratch[GPW_NOALLOC_SCRATCH_BYTES];
+std::mutex g_scratch_mu;
+char *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// device address of the current GPU's scratch, looked up on the first call
+// on that GPU (with it current) and kept
+gpw_status scratch_region(char **out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(GPW_ERR_UNSUPPORTED, "device ordinal too large");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(GPW_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<char *>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return GPW_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -510,6 +538,35 @@ gpw_status gpw_integrate_pgf_product_rspace_dev(
     hipStream_t hs = (hipStream_t)stream;
     char *R = nullptr;
     if ((st = cached_region(P.total, hs, &R))) return st;
+    stage_head(P, nullptr, t_stage);
+    HIP_TRY(hipMemcpyAsync(R, t_stage.data(), P.head, hipMemcpyHostToDevice, hs));
+    return run_integrate(P, R, grid_dev, pab_dev, hab_dev, forces_dev, hs);
+}
+
+gpw_status gpw_integrate_pgf_product_rspace_noalloc(
+    int compute_tau, int use_ortho, int la_max, int la_min, int lb_max, int lb_min, double zeta,
+    double zetb, double rscale, const double *dh, const double *dh_inv, const double *ra,
+    const double *rab, const int *npts, const int *ngrid, const int *lb_grid, const int *periodic,
+    int lmax, double radius, const int *lb_cube, const int *ub_cube, int nspheres,
+    const int *sphere_bounds, int maxco, int o1, int o2, const double *grid_dev, const double *pab_dev,
+    double *hab_dev, double *forces_dev, void *stream) {
+    if (!grid_dev || !hab_dev || (forces_dev && !pab_dev)) return fail(GPW_ERR_ARG, "NULL argument");
+    Task t;
+    gpw_status st = task_args(compute_tau, use_ortho, la_max, la_min, lb_max, lb_min, zeta, zetb, rscale,
+                              dh, dh_inv, ra, rab, npts, ngrid, lb_grid, periodic, lmax, radius,
+                              lb_cube, ub_cube, maxco, o1, o2, t);
+    if (st) return st;
+    Plan P;
+    if ((st = make_plan(t, nspheres, sphere_bounds, lb_cube, ub_cube, false, P))) return st;
+    if (P.total > (size_t)GPW_NOALLOC_SCRATCH_BYTES)
+        return fail(GPW_ERR_ARG, "task needs " + std::to_string(P.total) +
+                                     " bytes of device scratch, more than GPW_NOALLOC_SCRATCH_BYTES (" +
+                                     std::to_string((size_t)GPW_NOALLOC_SCRATCH_BYTES) + ")");
+    char *R = nullptr;
+    if ((st = scratch_region(&R))) return st;
+    hipStream_t hs = (hipStream_t)stream;
+    // the whole region is rewritten or cleared before it is read: the head
+    // by this upload, the moments by a fill kernel, the workspace by k_hab
     stage_head(P, nullptr, t_stage);
     HIP_TRY(hipMemcpyAsync(R, t_stage.data(), P.head, hipMemcpyHostToDevice, hs));
     return run_integrate(P, R, grid_dev, pab_dev, hab_dev, forces_dev, hs);
