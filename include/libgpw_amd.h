@@ -20,6 +20,13 @@
  *   gpw_collocate_core / gpw_integrate_core
  *                                     <- collocate_core<T> / integrate_core<T>,
  *                                         src/lib/collocate.cc:543 / :438
+ *   gpw_collocate_folded / gpw_integrate_folded
+ *                                     <- collocate<T> / integrate<T>,
+ *                                         src/lib/collocate.cc:681-745 / :626-679
+ *   gpw_fold_tables / gpw_nonzero_values
+ *                                     <- compute_folded_polynomials_all_powers /
+ *                                         compute_nonzero_values,
+ *                                         src/lib/collocate.cc:308-331 / :32-57
  */
 #ifndef LIBGPW_AMD_H
 #define LIBGPW_AMD_H
@@ -253,6 +260,52 @@ gpw_status gpw_collocate_core_host(const int *length, int L, const double *co, c
                                    int ldp, double *V);
 gpw_status gpw_integrate_core_host(const int *length, int L, const double *P, int ldp,
                                    const double *V, double *Velems);
+
+/* ---- libgpw folded-table path --------------------------------------------- */
+/* Periodic folding of 1D tables, compute_folded_polynomials_all_powers
+ * (src/lib/collocate.cc:308-331) with the support masks of
+ * compute_nonzero_values (:32-57), for the three directions d = 0 (z), 1 (y),
+ * 2 (x): entry i of an unfolded table is grid coordinate
+ * center[d] - (length[d] - 1) / 2 + i, and
+ *   folded[d][a][g] = sum of the entries whose coordinate is g mod period[d]
+ *   mask[d][g]      = 1 where some entry lands (every g < period[d] when
+ *                     length[d] >= period[d]), else 0
+ * for g < ldp (ldm); both are overwritten. unfolded: (3, lu, ldu), rows
+ * a < L read; folded: (3, L, ldp); mask: (3, ldm) ints or NULL. Device
+ * memory, stream ordered. */
+gpw_status gpw_fold_tables(int L, const int *length, const int *center, const int *period,
+                           const double *unfolded, int lu, int ldu, double *folded, int ldp,
+                           int *mask, int ldm, void *stream);
+/* collocate<T> (src/lib/collocate.cc:681-745) on a local grid window of
+ * grid_size = (nz, ny, nx) points whose (0, 0, 0) is global coordinate
+ * lower_corner (z, y, x): for every point whose three masks are set,
+ *   density[z][y][x] += sum_{a,g,b} co[a][g][b] P[0][g][c0 + z] P[1][b][c1 + y]
+ *                       P[2][a][c2 + x]
+ * (co: L^3, a the x power, g the z power, b the y power; P: (3, L, ldp)
+ * folded tables; mask: (3, ldm)). Points with a clear mask are untouched. */
+gpw_status gpw_collocate_folded(int L, const double *co, const double *P, int ldp,
+                                const int *mask, int ldm, const int *grid_size,
+                                const int *lower_corner, double *density, void *stream);
+/* integrate<T> (src/lib/collocate.cc:626-679), the adjoint on the same set:
+ *   co[g][b][a] = sum V[z][y][x] P[0][g][c0 + z] P[1][b][c1 + y] P[2][a][c2 + x]
+ * (co overwritten; zero when no point is set). */
+gpw_status gpw_integrate_folded(int L, const double *P, int ldp, const int *mask, int ldm,
+                                const int *grid_size, const int *lower_corner, const double *V,
+                                double *co, void *stream);
+/* The same three on host arrays (staged through the calling thread's device
+ * workspace; drop-ins for the reference's CPU templates). */
+gpw_status gpw_fold_tables_host(int L, const int *length, const int *center, const int *period,
+                                const double *unfolded, int lu, int ldu, double *folded, int ldp,
+                                int *mask, int ldm);
+gpw_status gpw_collocate_folded_host(int L, const double *co, const double *P, int ldp,
+                                     const int *mask, int ldm, const int *grid_size,
+                                     const int *lower_corner, double *density);
+gpw_status gpw_integrate_folded_host(int L, const double *P, int ldp, const int *mask, int ldm,
+                                     const int *grid_size, const int *lower_corner,
+                                     const double *V, double *co);
+/* compute_nonzero_values (src/lib/collocate.cc:32-57) for one direction, on
+ * the host: mask[0 .. ldm) as described for gpw_fold_tables. */
+gpw_status gpw_nonzero_values(int center, int period, int length, int *mask, int ldm);
 
 /* last HIP error string recorded by the library (thread-local) */
 const char *gpw_last_error(void);

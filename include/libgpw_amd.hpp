@@ -10,6 +10,9 @@
 //   grid_collocate_pgf_product_rspace  src/lib/grid_base_ref_c.h:40-67
 //   collocate_core<T>                  src/lib/collocate.cc:543 (decl. src/lib/collocate.hpp:4-7)
 //   integrate_core<T>                  src/lib/collocate.cc:438
+//   collocate<T> / integrate<T>        src/lib/collocate.cc:681-745 / :626-679
+//   compute_folded_polynomials_all_powers, compute_nonzero_values
+//                                      src/lib/collocate.cc:308-331 / :32-57
 // and adds grid_integrate_pgf_product_rspace, the adjoint with the same
 // argument list (no counterpart in the reference).
 //
@@ -88,6 +91,59 @@ void integrate_core(const int *length, const Poly &polynomials, const V &Vgrid, 
     gpw::check(gpw_integrate_core_host(length, L, &polynomials(0, 0, 0),
                                        (int)polynomials.size(2), &Vgrid(0, 0, 0),
                                        &Velems(0, 0, 0)));
+}
+
+// compute_folded_polynomials_all_powers (src/lib/collocate.cc:308-331):
+// p_alpha (3, L + 1, length) unfolded tables (rows a < L are folded; the
+// reference keeps x - x_i in the last row), p_alpha_beta_folded (3, L, ldp)
+// overwritten with the periodic folds.
+template <class Pa, class Pf>
+void compute_folded_polynomials_all_powers(const int center[3], const int period[3],
+                                           const int length[3], const Pa &p_alpha,
+                                           Pf &p_alpha_beta_folded) {
+    gpw::check(gpw_fold_tables_host((int)p_alpha_beta_folded.size(1), length, center, period,
+                                    &p_alpha(0, 0, 0), (int)p_alpha.size(1), (int)p_alpha.size(2),
+                                    &p_alpha_beta_folded(0, 0, 0), (int)p_alpha_beta_folded.size(2),
+                                    nullptr, 0));
+}
+
+// compute_nonzero_values (src/lib/collocate.cc:32-57): row `dir` of the
+// support masks non_zeroes_ (3, ldm)
+template <class Mask>
+void compute_nonzero_values(const int center, const int period, const int length,
+                            const int lower_corner, const int length_dir, const int dir,
+                            Mask &non_zeroes_) {
+    (void)lower_corner;
+    (void)length_dir;
+    gpw::check(gpw_nonzero_values(center, period, length, &non_zeroes_(dir, 0),
+                                  (int)non_zeroes_.size(1)));
+}
+
+// collocate<T> (src/lib/collocate.cc:681-745): density (grid_size[0] x
+// grid_size[1] x grid_size[2], the local window at grid_lower_corner) += the
+// coefficient cube co (L^3) contracted with the folded tables on the points
+// whose masks (3, ldm) are set. Vtmp, the reference's packed scratch, is not
+// used.
+template <class Co, class Pf, class Mask, class Vt, class Dens>
+void collocate(const Co &co, const Pf &p_alpha_beta_folded, const Mask &yz_non_vanishing_elements,
+               const int grid_size[3], const int grid_lower_corner[3], Vt &Vtmp, Dens &density) {
+    (void)Vtmp;
+    gpw::check(gpw_collocate_folded_host(
+        (int)co.size(0), &co(0, 0, 0), &p_alpha_beta_folded(0, 0, 0),
+        (int)p_alpha_beta_folded.size(2), &yz_non_vanishing_elements(0, 0),
+        (int)yz_non_vanishing_elements.size(1), grid_size, grid_lower_corner, &density(0, 0, 0)));
+}
+
+// integrate<T> (src/lib/collocate.cc:626-679): co(g, b, a) = the potential V
+// (the local window) contracted with the folded tables on the same points
+template <class Pf, class Mask, class V, class Co>
+void integrate(const Pf &p_alpha_beta_folded, const Mask &yz_non_vanishing_elements,
+               const int *grid_size, const int grid_lower_corner[3], const V &Vgrid, Co &co) {
+    gpw::check(gpw_integrate_folded_host(
+        (int)p_alpha_beta_folded.size(1), &p_alpha_beta_folded(0, 0, 0),
+        (int)p_alpha_beta_folded.size(2), &yz_non_vanishing_elements(0, 0),
+        (int)yz_non_vanishing_elements.size(1), grid_size, grid_lower_corner, &Vgrid(0, 0, 0),
+        &co(0, 0, 0)));
 }
 
 #endif

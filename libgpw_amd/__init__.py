@@ -32,6 +32,8 @@ EXPORTS = (
     "gpw_tasklist_set_timing", "gpw_tasklist_timings",
     "gpw_collocate_core", "gpw_integrate_core", "gpw_collocate_core_host",
     "gpw_integrate_core_host", "gpw_tasklist_autotune", "gpw_last_error",
+    "gpw_fold_tables", "gpw_collocate_folded", "gpw_integrate_folded", "gpw_fold_tables_host",
+    "gpw_collocate_folded_host", "gpw_integrate_folded_host", "gpw_nonzero_values",
 )
 PHASES = ("coef", "collocate", "reduce", "integrate", "moments", "hab")
 
@@ -107,6 +109,15 @@ def lib():
         L.gpw_collocate_core_host.argtypes = [ip, C.c_int, dp, dp, C.c_int, dp]
         L.gpw_integrate_core_host.argtypes = [ip, C.c_int, dp, C.c_int, dp, dp]
         L.gpw_tasklist_autotune.argtypes = [vp, C.c_int, C.c_int, vp, dp, dp]
+        L.gpw_fold_tables.argtypes = [C.c_int, ip, ip, ip, vp, C.c_int, C.c_int, vp, C.c_int, vp,
+                                      C.c_int, vp]
+        L.gpw_collocate_folded.argtypes = [C.c_int, vp, vp, C.c_int, vp, C.c_int, ip, ip, vp, vp]
+        L.gpw_integrate_folded.argtypes = [C.c_int, vp, C.c_int, vp, C.c_int, ip, ip, vp, vp, vp]
+        L.gpw_fold_tables_host.argtypes = [C.c_int, ip, ip, ip, dp, C.c_int, C.c_int, dp, C.c_int,
+                                           ip, C.c_int]
+        L.gpw_collocate_folded_host.argtypes = [C.c_int, dp, dp, C.c_int, ip, C.c_int, ip, ip, dp]
+        L.gpw_integrate_folded_host.argtypes = [C.c_int, dp, C.c_int, ip, C.c_int, ip, ip, dp, dp]
+        L.gpw_nonzero_values.argtypes = [C.c_int, C.c_int, C.c_int, ip, C.c_int]
         L.gpw_last_error.argtypes = []
         L.gpw_last_error.restype = C.c_char_p
         for n in EXPORTS:
@@ -411,3 +422,55 @@ def integrate_core(length, P, V, stream=None):
     _check(lib().gpw_integrate_core(_ci(length), L, _ptr(P), int(P.shape[2]),
                                     _ptr(V, nv, "V", exact=True), _ptr(out), _stream(stream)))
     return out
+
+
+# ---------------------------------------------------------------------------
+# libgpw folded-table path (device tensors)
+
+def _iptr(t, numel=None, name="mask"):
+    import torch
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.int32 and t.is_contiguous()):
+        raise ValueError(f"{name}: need a contiguous int32 device tensor")
+    if numel is not None and t.numel() < numel:
+        raise ValueError(f"{name}: {t.numel()} elements, need at least {numel}")
+    return C.c_void_p(t.data_ptr())
+
+
+def fold_tables(unfolded, length, center, period, ldp, with_mask=True, stream=None):
+    """Periodic folds (3, L, ldp) and masks (3, ldp) of device tables
+    unfolded (3, L, ldu) (compute_folded_polynomials_all_powers +
+    compute_nonzero_values)."""
+    import torch
+    L, ldu = int(unfolded.shape[1]), int(unfolded.shape[2])
+    folded = torch.empty((3, L, ldp), dtype=torch.float64, device=unfolded.device)
+    mask = torch.empty((3, ldp), dtype=torch.int32, device=unfolded.device) if with_mask else None
+    _check(lib().gpw_fold_tables(L, _ci(length), _ci(center), _ci(period),
+                                 _ptr(unfolded, 3 * L * ldu, "unfolded"), L, ldu, _ptr(folded), ldp,
+                                 None if mask is None else C.c_void_p(mask.data_ptr()), ldp,
+                                 _stream(stream)))
+    return folded, mask
+
+
+def collocate_folded(co, P, mask, grid_size, lower_corner, density, stream=None):
+    """collocate<T>: density (grid_size, the local window) += the masked
+    contraction of co (L^3) with the folded tables P (3, L, ldp)."""
+    L, ldp = int(P.shape[1]), int(P.shape[2])
+    n = int(grid_size[0]) * int(grid_size[1]) * int(grid_size[2])
+    _check(lib().gpw_collocate_folded(L, _ptr(co, L ** 3, "co"), _ptr(P, 3 * L * ldp, "P"), ldp,
+                                      _iptr(mask, 3 * int(mask.shape[1])), int(mask.shape[1]),
+                                      _ci(grid_size), _ci(lower_corner),
+                                      _ptr(density, n, "density", exact=True), _stream(stream)))
+    return density
+
+
+def integrate_folded(P, mask, grid_size, lower_corner, V, stream=None):
+    """integrate<T>: co[g][b][a] of the potential V on the masked window."""
+    import torch
+    L, ldp = int(P.shape[1]), int(P.shape[2])
+    n = int(grid_size[0]) * int(grid_size[1]) * int(grid_size[2])
+    co = torch.empty((L, L, L), dtype=torch.float64, device=P.device)
+    _check(lib().gpw_integrate_folded(L, _ptr(P, 3 * L * ldp, "P"), ldp,
+                                      _iptr(mask, 3 * int(mask.shape[1])), int(mask.shape[1]),
+                                      _ci(grid_size), _ci(lower_corner), _ptr(V, n, "V", exact=True),
+                                      _ptr(co), _stream(stream)))
+    return co

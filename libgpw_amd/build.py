@@ -31,6 +31,7 @@ SOURCES = [
     ("gpw_kernels.hip", []),
     ("gpw_plan.cpp", ["-ffp-contract=off"]),
     ("gpw_single.cpp", ["-ffp-contract=off"]),
+    ("gpw_folded.cpp", []),
 ]
 
 

@@ -111,5 +111,18 @@ hipError_t launch_collocate_core(const int *length, int L, const double *co, con
                                  int ldp, double *V, hipStream_t st);
 hipError_t launch_integrate_core(const int *length, int L, const double *P, int ldp,
                                  const double *V, double *Velems, hipStream_t st);
+hipError_t launch_fold_tables(int L, const int *length, const int *center, const int *period,
+                              const double *unfolded, int lu, int ldu, double *folded, int ldp,
+                              int *mask, int ldm, hipStream_t st);
+hipError_t launch_collocate_folded(int L, const double *co, const double *P, int ldp, const int *mask,
+                                   int ldm, const int *gs, const int *lc, double *density,
+                                   hipStream_t st);
+hipError_t launch_integrate_folded(int L, const double *P, int ldp, const int *mask, int ldm,
+                                   const int *gs, const int *lc, const double *V, double *co,
+                                   hipStream_t st);
+
+// the calling thread's device workspace on the current device (grow only,
+// kept until the process ends; see gpw_single.cpp)
+gpw_status thread_workspace(size_t need, hipStream_t st, char **out);
 
 }  // namespace gpw

@@ -1844,6 +1844,111 @@ __global__ __launch_bounds__(256) void k_integrate_core(int n0, int n1, int n2, 
 }
 
 // ---------------------------------------------------------------------------
+// libgpw folded-table path (reference collocate.cc:32-57, 276-331, 626-745)
+
+__device__ __forceinline__ int pmod_dev(long long a, int m) {
+    const long long r = a % m;
+    return (int)(r < 0 ? r + m : r);
+}
+
+// Periodic fold of the 1D tables: block (a, d) folds power a of direction d
+// (a == L: the support mask of direction d). Entry i of an unfolded table is
+// grid coordinate p0 + i, p0 = center - (length - 1) / 2; folded[g] gathers
+// the entries i = g - p0 (mod period) in increasing i, so the sums are
+// deterministic.
+__global__ void k_fold_tables(int L, int3 length, int3 center, int3 period,
+                              const double *__restrict__ unfolded, int lu, int ldu,
+                              double *__restrict__ folded, int ldp, int *__restrict__ mask, int ldm) {
+    const int a = blockIdx.x, d = blockIdx.y;
+    const int len = d == 0 ? length.x : d == 1 ? length.y : length.z;
+    const int per = d == 0 ? period.x : d == 1 ? period.y : period.z;
+    const long long p0 = (long long)(d == 0 ? center.x : d == 1 ? center.y : center.z) - (len - 1) / 2;
+    if (a == L) {
+        if (!mask) return;
+        for (int g = threadIdx.x; g < ldm; g += blockDim.x) {
+            int m = 0;
+            if (g < per) m = len >= per ? 1 : pmod_dev((long long)g - p0, per) < len;
+            mask[(long long)d * ldm + g] = m;
+        }
+        return;
+    }
+    const double *u = unfolded + ((long long)d * lu + a) * ldu;
+    double *f = folded + ((long long)d * L + a) * ldp;
+    for (int g = threadIdx.x; g < ldp; g += blockDim.x) {
+        double acc = 0.0;
+        if (g < per)
+            for (int i = pmod_dev((long long)g - p0, per); i < len; i += per) acc += u[i];
+        f[g] = acc;
+    }
+}
+
+// collocate<T>: one thread per (z, y) row of the local window whose z and y
+// masks are set; xyz[a] = sum_{g,b} co[a][g][b] Pz[g][c0+z] Py[b][c1+y], then
+// density[z][y][x] += sum_a xyz[a] Px[a][c2+x] on the x points whose mask is set
+__global__ void k_collocate_folded(int L, const double *__restrict__ co, const double *__restrict__ P,
+                                   int ldp, const int *__restrict__ mask, int ldm, int3 gs, int3 lc,
+                                   double *__restrict__ density) {
+    const long long rowid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (rowid >= (long long)gs.x * gs.y) return;
+    const int z = rowid / gs.y, y = rowid % gs.y;
+    if (!mask[lc.x + z] || !mask[ldm + lc.y + y]) return;
+    const double *Pz = P, *Py = P + (long long)L * ldp, *Px = P + 2LL * L * ldp;
+    const int *mx = mask + 2LL * ldm + lc.z;
+    double xyz[LPMAX + 1];
+    for (int a = 0; a < L; a++) {
+        double s = 0.0;
+        for (int g = 0; g < L; g++) {
+            const double pz = Pz[(long long)g * ldp + lc.x + z];
+            for (int b = 0; b < L; b++) s += co[((long long)a * L + g) * L + b] * pz * Py[(long long)b * ldp + lc.y + y];
+        }
+        xyz[a] = s;
+    }
+    double *row = density + rowid * gs.z;
+    for (int x = 0; x < gs.z; x++) {
+        if (!mx[x]) continue;
+        double v = 0.0;
+        for (int a = 0; a < L; a++) v += xyz[a] * Px[(long long)a * ldp + lc.z + x];
+        row[x] += v;
+    }
+}
+
+// integrate<T>: one workgroup per z plane of the window (mask set):
+// co[g][b][a] += Pz[g][c0+z] sum_y Py[b][c1+y] sum_x V[z][y][x] Px[a][c2+x]
+// over the set masks (co cleared beforehand)
+__global__ __launch_bounds__(256) void k_integrate_folded(int L, const double *__restrict__ P, int ldp,
+                                                          const int *__restrict__ mask, int ldm, int3 gs,
+                                                          int3 lc, const double *__restrict__ V,
+                                                          double *__restrict__ co) {
+    extern __shared__ double sm_if[];  // [L][L] partial (b, a)
+    const int z = blockIdx.x;
+    if (!mask[lc.x + z]) return;
+    const double *Pz = P, *Py = P + (long long)L * ldp, *Px = P + 2LL * L * ldp;
+    const int *my = mask + ldm + lc.y, *mx = mask + 2LL * ldm + lc.z;
+    for (int w = threadIdx.x; w < L * L; w += blockDim.x) sm_if[w] = 0.0;
+    __syncthreads();
+    for (int y = threadIdx.x; y < gs.y; y += blockDim.x) {
+        if (!my[y]) continue;
+        const double *row = V + ((long long)z * gs.y + y) * gs.z;
+        double t[LPMAX + 1];
+        for (int a = 0; a < L; a++) t[a] = 0.0;
+        for (int x = 0; x < gs.z; x++) {
+            if (!mx[x]) continue;
+            const double v = row[x];
+            for (int a = 0; a < L; a++) t[a] += v * Px[(long long)a * ldp + lc.z + x];
+        }
+        for (int b = 0; b < L; b++) {
+            const double py = Py[(long long)b * ldp + lc.y + y];
+            for (int a = 0; a < L; a++) atomicAdd(&sm_if[b * L + a], t[a] * py);
+        }
+    }
+    __syncthreads();
+    for (int w = threadIdx.x; w < L * L * L; w += blockDim.x) {
+        const int g = w / (L * L), ba = w % (L * L);
+        atomicAdd(&co[w], sm_if[ba] * Pz[(long long)g * ldp + lc.x + z]);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // launchers (host side)
 
 template <int LM>
@@ -2053,6 +2158,39 @@ hipError_t launch_integrate_core(const int *length, int L, const double *P, int 
     if (e != hipSuccess || length[0] == 0) return e;
     hipLaunchKernelGGL(k_integrate_core, dim3(length[0]), dim3(256), sizeof(double) * L * L, st,
                        length[0], length[1], length[2], L, P, ldp, V, Velems);
+    return hipGetLastError();
+}
+
+hipError_t launch_fold_tables(int L, const int *length, const int *center, const int *period,
+                              const double *unfolded, int lu, int ldu, double *folded, int ldp,
+                              int *mask, int ldm, hipStream_t st) {
+    hipLaunchKernelGGL(k_fold_tables, dim3(L + 1, 3), dim3(256), 0, st, L,
+                       make_int3(length[0], length[1], length[2]),
+                       make_int3(center[0], center[1], center[2]),
+                       make_int3(period[0], period[1], period[2]), unfolded, lu, ldu, folded, ldp,
+                       mask, ldm);
+    return hipGetLastError();
+}
+
+hipError_t launch_collocate_folded(int L, const double *co, const double *P, int ldp, const int *mask,
+                                   int ldm, const int *gs, const int *lc, double *density,
+                                   hipStream_t st) {
+    const long long rows = (long long)gs[0] * gs[1];
+    const int blocks = (int)((rows + 127) / 128);
+    if (blocks == 0 || gs[2] == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_collocate_folded, dim3(blocks), dim3(128), 0, st, L, co, P, ldp, mask, ldm,
+                       make_int3(gs[0], gs[1], gs[2]), make_int3(lc[0], lc[1], lc[2]), density);
+    return hipGetLastError();
+}
+
+hipError_t launch_integrate_folded(int L, const double *P, int ldp, const int *mask, int ldm,
+                                   const int *gs, const int *lc, const double *V, double *co,
+                                   hipStream_t st) {
+    hipError_t e = launch_fill(co, (long long)L * L * L, 0.0, st);
+    if (e != hipSuccess || gs[0] == 0 || gs[1] == 0 || gs[2] == 0) return e;
+    hipLaunchKernelGGL(k_integrate_folded, dim3(gs[0]), dim3(256), sizeof(double) * L * L, st, L, P,
+                       ldp, mask, ldm, make_int3(gs[0], gs[1], gs[2]), make_int3(lc[0], lc[1], lc[2]),
+                       V, co);
     return hipGetLastError();
 }
 

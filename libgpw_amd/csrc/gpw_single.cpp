@@ -355,7 +355,9 @@ thread_local std::vector<CachedRegion> t_regions;
 thread_local std::vector<char> t_stage;
 thread_local std::vector<double> t_host;
 
-gpw_status cached_region(size_t need, hipStream_t st, char **out) {
+}  // namespace
+
+gpw_status gpw::thread_workspace(size_t need, hipStream_t st, char **out) {
     int dev = 0;
     HIP_TRY(hipGetDevice(&dev));
     CachedRegion *r = nullptr;
@@ -367,8 +369,9 @@ gpw_status cached_region(size_t need, hipStream_t st, char **out) {
     }
     if (r->cap < need) {
         if (r->ptr) {
-            // earlier calls of this thread may still read the old region
-            HIP_TRY(hipStreamSynchronize(st));
+            // earlier calls of this thread (on any stream) may still read it
+            (void)st;
+            HIP_TRY(hipDeviceSynchronize());
             HIP_TRY(hipFree(r->ptr));
             r->ptr = nullptr;
             r->cap = 0;
@@ -380,6 +383,8 @@ gpw_status cached_region(size_t need, hipStream_t st, char **out) {
     *out = r->ptr;
     return GPW_OK;
 }
+
+namespace {
 
 // Scratch of the allocation-free entry point: one array per GPU, loaded
 // with the library's code object. Host code never passes `scratch` itself to
@@ -427,7 +432,7 @@ gpw_status gpw_collocate_pgf_product_rspace(
     Plan P;
     if ((st = make_plan(t, nspheres, sphere_bounds, lb_cube, ub_cube, true, P))) return st;
     char *R = nullptr;
-    if ((st = cached_region(P.total, nullptr, &R))) return st;
+    if ((st = thread_workspace(P.total, nullptr, &R))) return st;
     stage_head(P, pab, t_stage);
     HIP_TRY(hipMemcpy(R, t_stage.data(), P.head, hipMemcpyHostToDevice));
     const long long gpts = P.mode == HOST_WINDOW ? P.nbox : P.ngrid_pts;
@@ -464,7 +469,7 @@ gpw_status gpw_integrate_pgf_product_rspace(
     Plan P;
     if ((st = make_plan(t, nspheres, sphere_bounds, lb_cube, ub_cube, true, P))) return st;
     char *R = nullptr;
-    if ((st = cached_region(P.total, nullptr, &R))) return st;
+    if ((st = thread_workspace(P.total, nullptr, &R))) return st;
     stage_head(P, pab, t_stage);
     if (P.mode == HOST_WINDOW) {
         // head and the gathered boxes in one upload (o_grid follows the head)
@@ -514,7 +519,7 @@ gpw_status gpw_collocate_pgf_product_rspace_dev(
     if ((st = make_plan(t, nspheres, sphere_bounds, lb_cube, ub_cube, false, P))) return st;
     hipStream_t hs = (hipStream_t)stream;
     char *R = nullptr;
-    if ((st = cached_region(P.total, hs, &R))) return st;
+    if ((st = thread_workspace(P.total, hs, &R))) return st;
     stage_head(P, nullptr, t_stage);
     HIP_TRY(hipMemcpyAsync(R, t_stage.data(), P.head, hipMemcpyHostToDevice, hs));
     return run_collocate(P, R, pab_dev, grid_dev, hs);
@@ -537,7 +542,7 @@ gpw_status gpw_integrate_pgf_product_rspace_dev(
     if ((st = make_plan(t, nspheres, sphere_bounds, lb_cube, ub_cube, false, P))) return st;
     hipStream_t hs = (hipStream_t)stream;
     char *R = nullptr;
-    if ((st = cached_region(P.total, hs, &R))) return st;
+    if ((st = thread_workspace(P.total, hs, &R))) return st;
     stage_head(P, nullptr, t_stage);
     HIP_TRY(hipMemcpyAsync(R, t_stage.data(), P.head, hipMemcpyHostToDevice, hs));
     return run_integrate(P, R, grid_dev, pab_dev, hab_dev, forces_dev, hs);

@@ -19,6 +19,11 @@
  *   general (non-ortho) path .. src/lib/grid_base_ref_c.cc:593-846
  *   pair setup ................ src/lib/grid_base_ref_c.cc:878-986
  *   libgpw tensor contraction . src/lib/collocate_test.cc:11-33, 77-99
+ *   libgpw fold / mask ........ src/lib/collocate.cc:32-57, 276-331
+ *   libgpw collocate<T> /
+ *   integrate<T> (pack,
+ *   unpack) ................... src/lib/collocate.cc:626-745,
+ *                               src/lib/polynomials.hpp:5-86, src/lib/utils.hpp:20-181
  * Integrate and forces have no runnable reference: they are defined here as
  * the exact adjoint of collocate (math in src/lib/doc.tex:363-415) and as the
  * analytic derivative with respect to the two centres at a fixed index set.
@@ -26,7 +31,9 @@
  * Parity pins (see tests/test_oracle.py): the four CP2K .task fixtures, a
  * direct brute-force evaluator (orc_collocate_brute), ortho == general on an
  * orthorhombic cell, adjointness <collocate(P), W> == <P, integrate(W)>,
- * central finite differences for forces, and the libgpw naive contractions.
+ * central finite differences for forces, and the libgpw naive contractions;
+ * the folded-table path by a direct periodic image sum of separable Gaussian
+ * tables, adjointness, and windows that are slices of the full period.
  * The cube/sphere-bounds generator reproduces both ortho fixtures exactly.
  *
  * Build with -ffp-contract=off so index arithmetic is reproducible.
@@ -1000,6 +1007,121 @@ void orc_integrate_core(const int length[3], int L, const double *P, int ldp, co
                     }
                 Velems[((size_t)g * L + b) * L + a] = res;
             }
+}
+
+/* ------------------------------------------------------------------------ */
+/* libgpw folded-table path                                                  */
+/*
+ * Periodic folding of 1D tables, compute_folded_polynomials (5-argument form,
+ * collocate.cc:276-301) called per direction and power by
+ * compute_folded_polynomials_all_powers (collocate.cc:308-331): entry i of an
+ * unfolded table of `length` points is the grid coordinate
+ * center - (length - 1) / 2 + i, and the folded table is the image sum
+ * F[g] = sum over i with (center - (length - 1) / 2 + i) mod period == g.
+ * The support mask is compute_nonzero_values (collocate.cc:32-57): all ones
+ * when length >= period, else ones on those grid coordinates. Defects of the
+ * reference not carried over: the (length + 1) / 2 centring of the other
+ * overload (collocate.cc:140,162), the negative remainder of
+ * (i + p0 + period) % period when p0 < -period, the integer `16 * period` /
+ * `32 * period` offsets that bound how far a table may reach.
+ */
+static int pmod(long long a, int m) {
+    long long r = a % m;
+    return (int)(r < 0 ? r + m : r);
+}
+
+void orc_fold_tables(int L, const int length[3], const int center[3], const int period[3],
+                     const double *unfolded, int ldu, double *folded, int ldp, int *mask,
+                     int ldm) {
+    for (int d = 0; d < 3; d++) {
+        const long long p0 = (long long)center[d] - (length[d] - 1) / 2;
+        for (int a = 0; a < L; a++) {
+            double *f = folded + ((size_t)d * L + a) * ldp;
+            const double *u = unfolded + ((size_t)d * L + a) * ldu;
+            for (int g = 0; g < ldp; g++) f[g] = 0.0;
+            for (int i = 0; i < length[d]; i++) f[pmod(p0 + i, period[d])] += u[i];
+        }
+        if (mask) {
+            int *m = mask + (size_t)d * ldm;
+            for (int g = 0; g < ldm; g++) m[g] = 0;
+            if (length[d] >= period[d]) {
+                for (int g = 0; g < period[d] && g < ldm; g++) m[g] = 1;
+            } else {
+                for (int i = 0; i < length[d]; i++) m[pmod(p0 + i, period[d])] = 1;
+            }
+        }
+    }
+}
+
+/*
+ * collocate<T> (collocate.cc:681-745): the coefficient cube contracted with
+ * the folded tables over the points of the local grid window whose three
+ * masks are set -- pack the support (compute_polynomials_non_zero_elems,
+ * polynomials.hpp:5-86), collocate_core on it, add it back
+ * (unpack_elems_and_add, utils.hpp:102-181):
+ *   density[z][y][x] += sum co[a][g][b] P[0][g][c0+z] P[1][b][c1+y] P[2][a][c2+x]
+ * for 0 <= z < grid_size[0] etc. and mask[d][c_d + .] != 0, c = lower_corner.
+ * Defects not carried over: the lower corner counted twice
+ * (polynomials.hpp:21-22), the loop bounds of pack / unpack on the wrong axes
+ * (utils.hpp:60,81,127,140,161). Vtmp (the packed scratch) is not needed.
+ */
+void orc_collocate_folded(int L, const double *co, const double *P, int ldp, const int *mask,
+                          int ldm, const int grid_size[3], const int lower_corner[3],
+                          double *density) {
+    const double *Pz = P, *Py = P + (size_t)L * ldp, *Px = P + (size_t)2 * L * ldp;
+    const int *mz = mask, *my = mask + ldm, *mx = mask + 2 * (size_t)ldm;
+    const int c0 = lower_corner[0], c1 = lower_corner[1], c2 = lower_corner[2];
+    for (int z = 0; z < grid_size[0]; z++) {
+        if (!mz[c0 + z]) continue;
+        for (int y = 0; y < grid_size[1]; y++) {
+            if (!my[c1 + y]) continue;
+            double *row = density + ((size_t)z * grid_size[1] + y) * grid_size[2];
+            for (int x = 0; x < grid_size[2]; x++) {
+                if (!mx[c2 + x]) continue;
+                double v = 0.0;
+                for (int a = 0; a < L; a++)
+                    for (int g = 0; g < L; g++)
+                        for (int b = 0; b < L; b++)
+                            v += co[((size_t)a * L + g) * L + b] * Pz[(size_t)g * ldp + c0 + z] *
+                                 Py[(size_t)b * ldp + c1 + y] * Px[(size_t)a * ldp + c2 + x];
+                row[x] += v;
+            }
+        }
+    }
+}
+
+/*
+ * integrate<T> (collocate.cc:626-679; its call of pack_elems does not
+ * compile, collocate.cc:674): the potential on the masked support of the
+ * local window packed (pack_elems, utils.hpp:20-99) and contracted by
+ * integrate_core:
+ *   co[g][b][a] = sum V[z][y][x] P[0][g][c0+z] P[1][b][c1+y] P[2][a][c2+x]
+ * over the same points as orc_collocate_folded; co is overwritten (zero when
+ * no point is set, where the reference would leave it untouched).
+ */
+void orc_integrate_folded(int L, const double *P, int ldp, const int *mask, int ldm,
+                          const int grid_size[3], const int lower_corner[3], const double *V,
+                          double *co) {
+    const double *Pz = P, *Py = P + (size_t)L * ldp, *Px = P + (size_t)2 * L * ldp;
+    const int *mz = mask, *my = mask + ldm, *mx = mask + 2 * (size_t)ldm;
+    const int c0 = lower_corner[0], c1 = lower_corner[1], c2 = lower_corner[2];
+    for (size_t i = 0; i < (size_t)L * L * L; i++) co[i] = 0.0;
+    for (int z = 0; z < grid_size[0]; z++) {
+        if (!mz[c0 + z]) continue;
+        for (int y = 0; y < grid_size[1]; y++) {
+            if (!my[c1 + y]) continue;
+            const double *row = V + ((size_t)z * grid_size[1] + y) * grid_size[2];
+            for (int x = 0; x < grid_size[2]; x++) {
+                if (!mx[c2 + x]) continue;
+                for (int g = 0; g < L; g++)
+                    for (int b = 0; b < L; b++)
+                        for (int a = 0; a < L; a++)
+                            co[((size_t)g * L + b) * L + a] += row[x] * Pz[(size_t)g * ldp + c0 + z] *
+                                                              Py[(size_t)b * ldp + c1 + y] *
+                                                              Px[(size_t)a * ldp + c2 + x];
+            }
+        }
+    }
 }
 
 /* ------------------------------------------------------------------------ */

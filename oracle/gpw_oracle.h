@@ -86,6 +86,26 @@ void orc_collocate_core(const int length[3], int L, const double *co,
 void orc_integrate_core(const int length[3], int L, const double *P, int ldp,
                         const double *V, double *Velems);
 
+/* libgpw folded-table path (reference collocate.cc:32-57, 276-331, 626-745,
+ * polynomials.hpp:5-86, utils.hpp:20-181), defects fixed:
+ *   fold:      folded[d][a][g] = sum_i unfolded[d][a][i] over the i whose grid
+ *              coordinate center[d] - (length[d]-1)/2 + i is g mod period[d];
+ *              mask[d][g] = 1 on those g (all g < period when length >= period).
+ *              unfolded (3, L, ldu), folded (3, L, ldp), mask (3, ldm) or NULL.
+ *   collocate: density[z][y][x] += sum co[a][g][b] P[0][g][c0+z] P[1][b][c1+y]
+ *              P[2][a][c2+x] on the local window grid_size (z, y, x) at
+ *              lower_corner c where all three masks are set;
+ *   integrate: co[g][b][a] = sum V[z][y][x] P[0][g][c0+z] ... on the same set. */
+void orc_fold_tables(int L, const int length[3], const int center[3], const int period[3],
+                     const double *unfolded, int ldu, double *folded, int ldp, int *mask,
+                     int ldm);
+void orc_collocate_folded(int L, const double *co, const double *P, int ldp, const int *mask,
+                          int ldm, const int grid_size[3], const int lower_corner[3],
+                          double *density);
+void orc_integrate_folded(int L, const double *P, int ldp, const int *mask, int ldm,
+                          const int grid_size[3], const int lower_corner[3], const double *V,
+                          double *co);
+
 /* ---- batches of primitive pairs on one grid (CPU baseline and test driver) */
 typedef struct orc_pair {
     int la_max, la_min, lb_max, lb_min;

@@ -82,6 +82,9 @@ def lib():
                                           C.c_int, dp, dp, C.c_int]
         L.orc_batch_integrate.argtypes = [C.POINTER(OrcGrid), C.c_int, C.POINTER(OrcPair),
                                           C.c_int, dp, dp, dp, dp, C.c_int]
+        L.orc_fold_tables.argtypes = [C.c_int, ip, ip, ip, dp, C.c_int, dp, C.c_int, ip, C.c_int]
+        L.orc_collocate_folded.argtypes = [C.c_int, dp, dp, C.c_int, ip, C.c_int, ip, ip, dp]
+        L.orc_integrate_folded.argtypes = [C.c_int, dp, C.c_int, ip, C.c_int, ip, ip, dp, dp]
         for f in ("orc_collocate", "orc_integrate", "orc_collocate_brute", "orc_coef_xyz",
                   "orc_sphere_bounds", "orc_batch_collocate", "orc_batch_integrate",
                   "orc_ncoset", "orc_coset"):
@@ -267,3 +270,43 @@ def batch_integrate(grid_desc: OrcGrid, pairs, grid: np.ndarray, nblock: int,
                                      _dp(None if pab is None else np.ascontiguousarray(pab)),
                                      _dp(hab), _dp(f), int(nthreads)))
     return hab, (None if f is None else f.reshape(len(pairs), 2, 3))
+
+
+# ---------------------------------------------------------------------------
+# libgpw folded-table path
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def fold_tables(unfolded: np.ndarray, length, center, period, ldp: int):
+    """(folded (3, L, ldp), mask (3, ldp)) of unfolded tables (3, L, ldu)."""
+    u = np.ascontiguousarray(unfolded, dtype=np.float64)
+    L = u.shape[1]
+    folded = np.zeros((3, L, ldp))
+    mask = np.zeros((3, ldp), dtype=np.int32)
+    lib().orc_fold_tables(L, (C.c_int * 3)(*length), (C.c_int * 3)(*center), (C.c_int * 3)(*period),
+                          _dp(u), u.shape[2], _dp(folded), ldp, _ip(mask), ldp)
+    return folded, mask
+
+
+def collocate_folded(co, P, mask, grid_size, lower_corner, density=None):
+    co = np.ascontiguousarray(co, dtype=np.float64)
+    P = np.ascontiguousarray(P, dtype=np.float64)
+    mask = np.ascontiguousarray(mask, dtype=np.int32)
+    if density is None:
+        density = np.zeros(tuple(int(v) for v in grid_size))
+    lib().orc_collocate_folded(co.shape[0], _dp(co), _dp(P), P.shape[2], _ip(mask), mask.shape[1],
+                               (C.c_int * 3)(*grid_size), (C.c_int * 3)(*lower_corner), _dp(density))
+    return density
+
+
+def integrate_folded(P, mask, grid_size, lower_corner, V):
+    P = np.ascontiguousarray(P, dtype=np.float64)
+    mask = np.ascontiguousarray(mask, dtype=np.int32)
+    L = P.shape[1]
+    co = np.zeros((L, L, L))
+    lib().orc_integrate_folded(L, _dp(P), P.shape[2], _ip(mask), mask.shape[1],
+                               (C.c_int * 3)(*grid_size), (C.c_int * 3)(*lower_corner),
+                               _dp(np.ascontiguousarray(V, dtype=np.float64)), _dp(co))
+    return co

@@ -295,3 +295,70 @@ int main() {
     subprocess.run(["g++", "-std=c++17", "-I" + inc, str(src), "-o", str(exe), "-L" + libdir,
                     "-lgpw_amd", "-Wl,-rpath," + libdir], check=True)
     assert subprocess.run([str(exe)]).returncode == 0
+
+
+def test_nonzero_values_matches_oracle_mask(gpw, oracle):
+    """gpw_nonzero_values (host-only) is the oracle's support mask for
+    windows shorter and longer than the period, centres far outside it."""
+    import ctypes as C
+    rng = np.random.default_rng(3)
+    for _ in range(60):
+        period = int(rng.integers(1, 40))
+        length = int(rng.integers(0, 90))
+        center = int(rng.integers(-200, 200))
+        ldm = period + int(rng.integers(0, 5))
+        m = np.full(ldm, 7, dtype=np.int32)
+        assert gpw.lib().gpw_nonzero_values(center, period, length,
+                                            m.ctypes.data_as(C.POINTER(C.c_int)), ldm) == 0
+        U = np.ones((3, 1, max(length, 1)))
+        _, ref = oracle.fold_tables(U, [length] * 3, [center] * 3, [period] * 3, ldm)
+        assert np.array_equal(m, ref[0]), (period, length, center)
+    m = np.zeros(4, dtype=np.int32)
+    assert gpw.lib().gpw_nonzero_values(0, 5, 3, m.ctypes.data_as(C.POINTER(C.c_int)), 4) == 1
+
+
+def test_cpp_templates_compile_and_link(gpw, tmp_path):
+    """The C++ drop-in header instantiates every template with an mdarray-like
+    row-major type (operator(), size(i) in declaration order) and links
+    against the library (no GPU needed: nothing is run)."""
+    src = tmp_path / "use_hpp.cpp"
+    src.write_text(r'''
+#include <vector>
+#include "libgpw_amd.hpp"
+template <class T, int N> struct arr {
+    std::vector<T> v; int d[N];
+    template <class... I> arr(I... n) : d{int(n)...} { size_t s = 1; for (int x : d) s *= x; v.resize(s); }
+    size_t size(int i) const { return d[i]; }
+    template <class... I> T &operator()(I... i) { return v[off(i...)]; }
+    template <class... I> const T &operator()(I... i) const { return v[off(i...)]; }
+    template <class... I> size_t off(I... i) const { int ix[N] = {int(i)...}; size_t o = 0;
+        for (int k = 0; k < N; k++) o = o * d[k] + ix[k]; return o; }
+};
+int main() {
+    const int c[3] = {1, 2, 3}, p[3] = {8, 8, 8}, len[3] = {12, 12, 12}, gs[3] = {8, 8, 8}, lc[3] = {0, 0, 0};
+    arr<double, 3> pa(3, 4, 12), pf(3, 3, 8), co(3, 3, 3), V(8, 8, 8), d(8, 8, 8), vt(8, 8, 8), P(3, 3, 8);
+    arr<int, 2> m(3, 8);
+    arr<double, 2> pab(4, 4);
+    compute_folded_polynomials_all_powers(c, p, len, pa, pf);
+    for (int dir = 0; dir < 3; dir++) compute_nonzero_values(c[dir], p[dir], len[dir], 0, 8, dir, m);
+    collocate(co, pf, m, gs, lc, vt, d);
+    integrate(pf, m, gs, lc, V, co);
+    collocate_core(gs, co, P, d);
+    integrate_core(gs, P, V, co);
+    double dh[3][3] = {}, dhi[3][3] = {}, ra[3] = {}, rab[3] = {};
+    const int npts[3] = {8, 8, 8}, lbg[3] = {0, 0, 0}, lb[3] = {-1, -1, -1}, ub[3] = {2, 2, 2};
+    const bool per[3] = {true, true, true};
+    std::vector<int> sb;
+    grid_collocate_pgf_product_rspace(false, true, 0, 0, 0, 0, 1.0, 1.0, 1.0, dh, dhi, ra, rab, npts,
+                                      npts, lbg, per, 1, 1.0, lb, ub, 0, sb, 4, 0, 0, pab, d);
+    grid_integrate_pgf_product_rspace(false, true, 0, 0, 0, 0, 1.0, 1.0, 1.0, dh, dhi, ra, rab, npts,
+                                      npts, lbg, per, 1, 1.0, lb, ub, 0, sb, 4, 0, 0, d, pab, pab);
+    return 0;
+}
+''')
+    exe = tmp_path / "use_hpp"
+    lib_dir = os.path.dirname(gpw.LIB_PATH)
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"), str(src),
+                        "-o", str(exe), "-L" + lib_dir, "-lgpw_amd", "-Wl,-rpath," + lib_dir],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
