@@ -57,11 +57,15 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp])
     os.replace(tmp, LIB)
-    # task-file replay driver: plain C++ against the drop-in header
+    # task-file replay driver: plain C++ against the drop-in header and the
+    # C ABI (the batch mode allocates its device buffers with the HIP runtime)
     exe = os.path.join(LIBDIR, "gpw_replay")
-    _run([os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall",
-          "-I" + os.path.join(ROOT, "include"), os.path.join(CSRC, "gpw_replay_main.cpp"),
-          "-o", exe + ".tmp", "-L" + LIBDIR, "-lgpw_amd", "-Wl,-rpath,$ORIGIN"])
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    _run([os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__",
+          "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(rocm, "include"),
+          os.path.join(CSRC, "gpw_replay_main.cpp"), "-o", exe + ".tmp", "-L" + LIBDIR, "-lgpw_amd",
+          "-L" + os.path.join(rocm, "lib"), "-lamdhip64", "-Wl,-rpath,$ORIGIN",
+          "-Wl,-rpath," + os.path.join(rocm, "lib")])
     os.replace(exe + ".tmp", exe)
     return LIB
 

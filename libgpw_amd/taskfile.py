@@ -136,3 +136,53 @@ def write_task(path: str, t: dict, grid: np.ndarray) -> None:
         for z, y, x in idx:
             w(f"grid {z} {y} {x} {grid[z, y, x]:.{D}e}\n")
         w(FOOTER + "\n")
+
+
+def ncoset(l: int) -> int:
+    return 0 if l < 0 else (l + 1) * (l + 2) * (l + 3) // 6
+
+
+def batch(tasks: list) -> dict:
+    """Inputs of one task list for tasks that share a grid and compute_tau:
+    grid (npts, ngrid, lb_grid, periodic, dh), pairs (arrays, one per task)
+    and the packed pab blocks (each task's ncoset(lb_max) x ncoset(la_max)
+    block at row o2, column o1 of its maxco x maxco pab). The task list
+    derives each pair's cube from its radius; for orthorhombic tasks that is
+    the file's lb_cube / ub_cube / sphere_bounds when they follow the CP2K
+    cube rule (libgpw_amd.pair_geometry tells)."""
+    if not tasks:
+        raise ValueError("no tasks")
+    t0 = tasks[0]
+    keys = ("npts", "ngrid", "lb_grid", "periodic")
+    for t in tasks[1:]:
+        if any(list(t[k]) != list(t0[k]) for k in keys) or not np.array_equal(t["dh"], t0["dh"]) \
+                or bool(t["compute_tau"]) != bool(t0["compute_tau"]):
+            raise ValueError("tasks of one batch need one grid and one compute_tau")
+    pairs = {k: [] for k in ("la_max", "la_min", "lb_max", "lb_min", "zeta", "zetb", "rscale", "ra",
+                             "rab", "radius", "block_offset")}
+    blocks, off = [], 0
+    for t in tasks:
+        na, nb = ncoset(t["la_max"]), ncoset(t["lb_max"])
+        for k in ("la_max", "la_min", "lb_max", "lb_min", "zeta", "zetb", "rscale", "radius"):
+            pairs[k].append(t[k])
+        pairs["ra"].append(np.asarray(t["ra"]))
+        pairs["rab"].append(np.asarray(t["rab"]))
+        pairs["block_offset"].append(off)
+        blocks.append(np.asarray(t["pab"])[t["o2"]:t["o2"] + nb, t["o1"]:t["o1"] + na].ravel())
+        off += na * nb
+    out = {k: np.asarray(v) for k, v in pairs.items()}
+    for k in ("la_max", "la_min", "lb_max", "lb_min"):
+        out[k] = out[k].astype(np.int32)
+    return {"npts": list(t0["npts"]), "ngrid": list(t0["ngrid"]), "lb_grid": list(t0["lb_grid"]),
+            "periodic": [int(bool(v)) for v in t0["periodic"]], "dh": np.asarray(t0["dh"]),
+            "compute_tau": bool(t0["compute_tau"]), "pairs": out,
+            "pab": np.concatenate(blocks), "nblock": off}
+
+
+def expected_grid(t: dict) -> np.ndarray:
+    """The task's listed grid values as a dense (z, y, x) array."""
+    n = t["ngrid"]
+    g = np.zeros((n[2], n[1], n[0]))
+    for z, y, x, v in t["grid_entries"]:
+        g[z, y, x] = v
+    return g

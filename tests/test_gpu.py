@@ -318,6 +318,68 @@ def test_replay_driver_on_fixture(gpw, torch_cuda, name):
     assert r.returncode == 0, r.stdout + r.stderr
 
 
+@pytest.mark.parametrize("name", FIXTURES)
+def test_replay_driver_full_goldens(gpw, torch_cuda, name):
+    """The replay driver on the full-3D goldens: every grid point compared
+    (--complete: every point the file does not list must come out zero)."""
+    import os
+    import subprocess
+    say("replay driver full golden", name)
+    exe = os.path.join(os.path.dirname(gpw.LIB_PATH), "gpw_replay")
+    here = os.path.dirname(os.path.abspath(__file__))
+    r = subprocess.run([exe, os.path.join(here, "golden", name + ".full.task"), "2", "--complete"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "and every other point" in r.stdout
+
+
+def test_replay_driver_batch(gpw, torch_cuda):
+    """--batch: files on one grid become one device task list, collocated
+    `cycles` times; repeated files are repeated pairs. Every fixture is
+    reproduced by the task list's own cube rule, so nothing falls back to
+    the drop-in."""
+    import os
+    import subprocess
+    say("replay driver batch")
+    exe = os.path.join(os.path.dirname(gpw.LIB_PATH), "gpw_replay")
+    g = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    files = [os.path.join(g, n + ".full.task") for n in
+             ["collocate_ortho_density"] * 3 + ["collocate_ortho_tau", "collocate_general_density"] +
+             ["collocate_general_tau"] * 2]
+    r = subprocess.run([exe, "--batch", "3", "--complete"] + files, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = [ln for ln in r.stdout.splitlines() if ln.startswith("batch of")]
+    assert len(lines) == 4, r.stdout
+    assert "batch of 3 tasks" in lines[0] and "(3 in the task list, 0 drop-in)" in lines[0]
+    assert "batch of 2 tasks" in lines[3] and "(2 in the task list, 0 drop-in)" in lines[3]
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_tasklist_full_goldens(gpw, torch_cuda, name):
+    """Each fixture as a pair of a task list (three copies of it): every
+    point of the grid equals three times the full-3D golden, and the points
+    holding a value are the golden's."""
+    from libgpw_amd.taskfile import batch, expected_grid, read_task
+    import os
+    torch = torch_cuda
+    say("task list on full golden", name)
+    here = os.path.dirname(os.path.abspath(__file__))
+    t = read_task(os.path.join(here, "golden", name + ".full.task"))
+    b = batch([t, t, t])
+    g = gpw.grid_desc(b["npts"], b["dh"], ngrid=b["ngrid"], lb_grid=b["lb_grid"], periodic=b["periodic"])
+    tl = gpw.TaskList(g, b["pairs"], compute_tau=b["compute_tau"])
+    n = b["ngrid"]
+    grid = torch.zeros((n[2], n[1], n[0]), dtype=torch.float64, device="cuda")
+    tl.collocate(dev(torch, b["pab"]), grid)
+    torch.cuda.synchronize()
+    ref = 3.0 * expected_grid(t)
+    got = grid.cpu().numpy()
+    assert same_support(got, ref)
+    assert rel_err(got, ref) < 1e-13
+    tl.close()
+
+
 def test_libgpw_core_host_arrays(gpw, oracle, torch_cuda):
     import ctypes as C
     say("libgpw core host arrays")

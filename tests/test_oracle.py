@@ -230,3 +230,26 @@ def test_oracle_error_codes(oracle):
         oracle.collocate(dict(t, sphere_bounds=bad), pab)
     with pytest.raises(oracle.OracleError):
         oracle.collocate(dict(t, la_min=3), pab)
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_full_goldens_agree_with_fixtures_and_oracle(oracle, name):
+    """The full-3D goldens (tests/golden/make_full_tasks.py) hold the
+    fixture's own task and, on the x = 0 plane, exactly the fixture's dumped
+    values; every other plane is the oracle's collocation."""
+    import os
+    from libgpw_amd.taskfile import expected_grid, read_task
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    t = fixture(name)
+    f = read_task(os.path.join(here, name + ".full.task"))
+    for k in ("la_max", "la_min", "lb_max", "lb_min", "zeta", "zetb", "rscale", "radius", "maxco"):
+        assert f[k] == t[k], k
+    assert np.array_equal(f["pab"], t["pab"]) and np.array_equal(f["ra"], t["ra"])
+    full = expected_grid(f)
+    plane = fixture_plane(t)
+    assert np.count_nonzero(full[:, :, 0]) == np.count_nonzero(plane)
+    if plane.any():
+        assert rel_err(full[:, :, 0], plane) < 1e-13
+    ref = oracle.collocate(t, t["pab"])
+    assert np.array_equal(full != 0.0, ref != 0.0)
+    assert np.array_equal(full, ref)
