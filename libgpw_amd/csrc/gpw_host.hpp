@@ -121,6 +121,19 @@ hipError_t launch_integrate_folded(int L, const double *P, int ldp, const int *m
                                    const int *gs, const int *lc, const double *V, double *co,
                                    hipStream_t st);
 
+// device planner of the tile work lists (gpw_kernels.hip)
+hipError_t launch_plan_count(const SubDev *subs, int nsub, const int *prof, const TileGrid &tg, int *counts,
+                             unsigned long long *work, hipStream_t st);
+hipError_t launch_plan_entries(const SubDev *subs, int nsub, const long long *ent_start, const long long *base0,
+                               const long long *base2, const TileGrid &tg, int ntiles,
+                               const unsigned char *small_mask, int npairs, int ne, int *ent_sub,
+                               long long *off0, long long *off2, unsigned *keys, int *vals, int *hist,
+                               hipStream_t st);
+hipError_t launch_plan_records(const SubDev *subs, const int *ent_sub, const long long *off, const int *sorted_e,
+                               int n, EntryRec *recs, hipStream_t st);
+hipError_t sort_pairs_u32(void *temp, size_t *temp_bytes, const unsigned *keys_in, unsigned *keys_out,
+                          const int *vals_in, int *vals_out, int n, int end_bit, hipStream_t st);
+
 // the calling thread's device workspace on the current device (grow only,
 // kept until the process ends; see gpw_single.cpp)
 gpw_status thread_workspace(size_t need, hipStream_t st, char **out);

@@ -29,6 +29,15 @@ constexpr int TR_NT = 256;
 // workgroup per pair) instead of the tiles; limits chosen by measurement
 constexpr int SMALL_NX = 64, SMALL_NY = 48, SMALL_NZ = 48, SMALL_NYZ_ORTHO = 32, SMALL_LMAX = 12;
 
+// work-item classes by polynomial length L = lp + 1 (one tile-kernel
+// instantiation each, so registers are sized for the class, not for the
+// largest L), and the three tile-list sets: 0 collocate, 1 integrate with
+// the force extension (lp + 2), 2 integrate without
+constexpr int NCLS = 4, NSET = 3;
+__host__ __device__ constexpr int class_of(int lp) {
+    return lp + 1 <= 4 ? 0 : lp + 1 <= 8 ? 1 : lp + 1 <= 12 ? 2 : 3;
+}
+
 __host__ __device__ constexpr int ncoset(int l) {
     return l < 0 ? 0 : ((l + 1) * (l + 2) * (l + 3)) / 6;
 }

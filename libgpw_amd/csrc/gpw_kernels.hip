@@ -18,6 +18,7 @@
 //     buffer): no atomics, results are deterministic.
 //   * plain kernels: one launch per sub-box of one pair (the baseline form).
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 
@@ -1844,6 +1845,180 @@ __global__ __launch_bounds__(256) void k_integrate_core(int n0, int n1, int n2, 
 }
 
 // ---------------------------------------------------------------------------
+// device planner of the tile work lists (see plan_device in gpw_plan.cpp)
+
+// tiles of axis d a sub-box touches, ascending: r0..r1, then q0..q1 (empty
+// when q0 > q1) -- a run that wraps around a periodic axis touches the
+// tiles from 0 and those up to the last one (the host's axis_tiles)
+__device__ __forceinline__ void plan_axis_ranges(const SubDev &s, int d, int n, int T, int ntile, int &r0,
+                                                 int &r1, int &q0, int &q1) {
+    const int len = s.ub[d] - s.lb[d] + 1;
+    const int start = s.base[d];
+    q0 = 1;
+    q1 = 0;
+    if (s.period[d] != BIG) {
+        const int stop = start + len - 1;
+        if (stop < n) {
+            r0 = start / T;
+            r1 = stop / T;
+        } else {
+            const int w1 = (stop - n) / T, s0 = start / T;
+            if (w1 + 1 >= s0) {
+                r0 = 0;
+                r1 = ntile - 1;
+            } else {
+                r0 = 0;
+                r1 = w1;
+                q0 = s0;
+                q1 = ntile - 1;
+            }
+        }
+    } else {
+        r0 = start / T;
+        r1 = (start + len - 1) / T;
+    }
+}
+
+__device__ __forceinline__ int plan_axis_count(const SubDev &s, int d, const TileGrid &tg) {
+    int r0, r1, q0, q1;
+    const int n = d == 0 ? tg.nx : d == 1 ? tg.ny : tg.nz;
+    const int T = d == 0 ? TX : d == 1 ? TY : TZ;
+    const int nt = d == 0 ? tg.ntx : d == 1 ? tg.nty : tg.ntz;
+    plan_axis_ranges(s, d, n, T, nt, r0, r1, q0, q1);
+    return (r1 - r0 + 1) + (q1 >= q0 ? q1 - q0 + 1 : 0);
+}
+
+__device__ __forceinline__ int plan_axis_tile(const SubDev &s, int d, const TileGrid &tg, int k) {
+    int r0, r1, q0, q1;
+    const int n = d == 0 ? tg.nx : d == 1 ? tg.ny : tg.nz;
+    const int T = d == 0 ? TX : d == 1 ? TY : TZ;
+    const int nt = d == 0 ? tg.ntx : d == 1 ? tg.nty : tg.ntz;
+    plan_axis_ranges(s, d, n, T, nt, r0, r1, q0, q1);
+    const int n1 = r1 - r0 + 1;
+    return k < n1 ? r0 + k : q0 + (k - n1);
+}
+
+// One wave per sub-box: its number of (sub-box, tile) items, and its sphere
+// points, rows and z-planes added to the per-lp counters work[3 lp + 0..2]
+// (the algorithmic work of the stats; integer sums, so order-independent).
+__global__ __launch_bounds__(256) void k_plan_count(const SubDev *__restrict__ subs, int nsub,
+                                                    const int *__restrict__ prof, TileGrid tg,
+                                                    int *__restrict__ counts,
+                                                    unsigned long long *__restrict__ work) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const long long si = (long long)blockIdx.x * 4 + wave;
+    if (si >= nsub) return;
+    const SubDev s = subs[si];
+    if (lane == 0)
+        counts[si] = plan_axis_count(s, 0, tg) * plan_axis_count(s, 1, tg) * plan_axis_count(s, 2, tg);
+    unsigned long long pts = 0, rows = 0, planes = 0;
+    for (int k = s.lb[2]; k <= s.ub[2]; k++) {
+        int rows_k = 0;
+        for (int j = s.lb[1] + lane; j <= s.ub[1]; j += 64) {
+            int lo, hi;
+            if (tg.general) {
+                double qb, qc;
+                general_row_dev(tg, s.roff[0], (double)j - s.roff[1], (double)k - s.roff[2], s.R2, s.zetp,
+                                lo, hi, qb, qc);
+                lo = max(lo, s.lb[0]);
+                hi = min(hi, s.ub[0]);
+            } else {
+                const int T = s.prof_off < 0 ? BIG : prof[s.prof_off + fold_off(k) * s.prof_stride + fold_off(j)];
+                if (T < 0) continue;
+                lo = max(s.lb[0], -T);
+                hi = min(s.ub[0], T + 1);
+            }
+            if (hi >= lo) {
+                pts += (unsigned long long)(hi - lo + 1);
+                rows_k++;
+            }
+        }
+        rows += (unsigned long long)rows_k;
+        if (__any(rows_k > 0)) planes++;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        pts += __shfl_xor(pts, o);
+        rows += __shfl_xor(rows, o);
+    }
+    if (lane == 0) {
+        atomicAdd(&work[3 * s.lp + 0], pts);
+        atomicAdd(&work[3 * s.lp + 1], rows);
+        atomicAdd(&work[3 * s.lp + 2], planes);
+    }
+}
+
+// One thread per item e (items numbered sub-box-major, sub-boxes pair-major):
+// its sub-box (binary search in ent_start), tile, partial-moment offsets,
+// and its sort key per tile-list set, class * ntiles + tile, or NCLS *
+// ntiles when the set leaves it out (pairs of the small-sphere integrate
+// kernel); hist counts the items per set and key.
+__global__ void k_plan_entries(const SubDev *__restrict__ subs, int nsub,
+                               const long long *__restrict__ ent_start, const long long *__restrict__ base0,
+                               const long long *__restrict__ base2, TileGrid tg, int ntiles,
+                               const unsigned char *__restrict__ small_mask, int npairs, int ne,
+                               int *__restrict__ ent_sub, long long *__restrict__ off0,
+                               long long *__restrict__ off2, unsigned *__restrict__ keys,
+                               int *__restrict__ vals, int *__restrict__ hist) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    int lo = 0, hi = nsub - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (ent_start[mid] <= e)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const SubDev sd = subs[lo];
+    const int k = (int)(e - ent_start[lo]);
+    const int nxs = plan_axis_count(sd, 0, tg), nys = plan_axis_count(sd, 1, tg);
+    const int tx = plan_axis_tile(sd, 0, tg, k % nxs);
+    const int ty = plan_axis_tile(sd, 1, tg, (k / nxs) % nys);
+    const int tz = plan_axis_tile(sd, 2, tg, k / (nxs * nys));
+    const int tile = (tz * tg.nty + ty) * tg.ntx + tx;
+    ent_sub[e] = lo;
+    off0[e] = base0[lo] + (long long)k * ncoset(sd.lp);
+    off2[e] = base2[lo] + (long long)k * ncoset(sd.lp + 2);
+    vals[e] = e;
+    const unsigned sent = (unsigned)(NCLS * ntiles);
+    const unsigned k0 = (unsigned)(class_of(sd.lp) * ntiles + tile);
+    const unsigned k1 = small_mask[npairs + sd.pair] ? sent : (unsigned)(class_of(sd.lp + 2) * ntiles + tile);
+    const unsigned k2 = small_mask[sd.pair] ? sent : k0;
+    const unsigned kk[3] = {k0, k1, k2};
+#pragma unroll
+    for (int a = 0; a < NSET; a++) {
+        keys[(long long)a * ne + e] = kk[a];
+        if (kk[a] != sent) atomicAdd(&hist[(long long)a * NCLS * ntiles + kk[a]], 1);
+    }
+}
+
+// records of one set in sorted order: position p holds item sorted_e[p]
+__global__ void k_plan_records(const SubDev *__restrict__ subs, const int *__restrict__ ent_sub,
+                               const long long *__restrict__ off, const int *__restrict__ sorted_e, int n,
+                               EntryRec *__restrict__ recs) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int e = sorted_e[p];
+    const SubDev sd = subs[ent_sub[e]];
+    EntryRec r;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        r.roff[d] = sd.roff[d];
+        r.base[d] = sd.base[d];
+        r.lb[d] = sd.lb[d];
+        r.ub[d] = sd.ub[d];
+    }
+    r.zetp = sd.zetp;
+    r.R2 = sd.R2;
+    r.ent_off = off[e];
+    r.lp = sd.lp;
+    r.coef_off = sd.coef_off;
+    r.pad = 0;
+    recs[p] = r;
+}
+
+// ---------------------------------------------------------------------------
 // libgpw folded-table path (reference collocate.cc:32-57, 276-331, 626-745)
 
 __device__ __forceinline__ int pmod_dev(long long a, int m) {
@@ -2159,6 +2334,40 @@ hipError_t launch_integrate_core(const int *length, int L, const double *P, int 
     hipLaunchKernelGGL(k_integrate_core, dim3(length[0]), dim3(256), sizeof(double) * L * L, st,
                        length[0], length[1], length[2], L, P, ldp, V, Velems);
     return hipGetLastError();
+}
+
+hipError_t launch_plan_count(const SubDev *subs, int nsub, const int *prof, const TileGrid &tg, int *counts,
+                             unsigned long long *work, hipStream_t st) {
+    if (nsub <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_plan_count, dim3((nsub + 3) / 4), dim3(256), 0, st, subs, nsub, prof, tg, counts, work);
+    return hipGetLastError();
+}
+
+hipError_t launch_plan_entries(const SubDev *subs, int nsub, const long long *ent_start, const long long *base0,
+                               const long long *base2, const TileGrid &tg, int ntiles,
+                               const unsigned char *small_mask, int npairs, int ne, int *ent_sub,
+                               long long *off0, long long *off2, unsigned *keys, int *vals, int *hist,
+                               hipStream_t st) {
+    if (ne <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_plan_entries, dim3((ne + 255) / 256), dim3(256), 0, st, subs, nsub, ent_start, base0,
+                       base2, tg, ntiles, small_mask, npairs, ne, ent_sub, off0, off2, keys, vals, hist);
+    return hipGetLastError();
+}
+
+hipError_t launch_plan_records(const SubDev *subs, const int *ent_sub, const long long *off, const int *sorted_e,
+                               int n, EntryRec *recs, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_plan_records, dim3((n + 255) / 256), dim3(256), 0, st, subs, ent_sub, off, sorted_e, n,
+                       recs);
+    return hipGetLastError();
+}
+
+// stable sort of (key, value) pairs on the low end_bit bits of the keys
+// (temp NULL: *temp_bytes returns the scratch size)
+hipError_t sort_pairs_u32(void *temp, size_t *temp_bytes, const unsigned *keys_in, unsigned *keys_out,
+                          const int *vals_in, int *vals_out, int n, int end_bit, hipStream_t st) {
+    return hipcub::DeviceRadixSort::SortPairs(temp, *temp_bytes, keys_in, keys_out, vals_in, vals_out, n, 0,
+                                              end_bit, st);
 }
 
 hipError_t launch_fold_tables(int L, const int *length, const int *center, const int *period,

@@ -109,14 +109,9 @@ gpw_status gpw::profile_from_bounds(const int lb[3], const int ub[3], int nspher
 
 // work-item classes by polynomial length L = lp + 1 (one kernel instantiation
 // each, so registers are sized for the class, not for the largest L)
-constexpr int NCLS = 4;
-static int class_of(int lp) {
-    const int L = lp + 1;
-    return L <= 4 ? 0 : L <= 8 ? 1 : L <= 12 ? 2 : 3;
-}
 
 enum { PH_COEF, PH_CTILES, PH_CREDUCE, PH_ITILES, PH_ISUM, PH_HAB, NPH };
-constexpr int NSET = 3, NSCLS = 3;
+constexpr int NSCLS = 3;
 // tile-list set of integrate without (wf = 0) / with (wf = 1) forces
 static inline int iset(int wf) { return wf ? 1 : 2; }
 constexpr int NEVENT = 2048;
@@ -559,9 +554,12 @@ struct HostPlan {
     long long coef_total = 0, partial_total = 0;
 };
 
-static gpw_status plan_tasklist(gpw_tasklist *tl, const std::vector<gpw_pair_desc> &pairs,
-                                const std::vector<PairGeom> &geos,
-                                const std::vector<Profile> &profiles, int splits, HostPlan &hp) {
+// Host part shared by both planners: profiles, pairs, sub-boxes (with the
+// squared sphere radius of their pair), the tile grid, and the pairs of the
+// small-sphere integrate kernel. O(pairs + sub-boxes).
+static gpw_status plan_common(gpw_tasklist *tl, const std::vector<gpw_pair_desc> &pairs,
+                              const std::vector<PairGeom> &geos, const std::vector<Profile> &profiles,
+                              int splits, HostPlan &hp) {
     const gpw_grid_desc &g = tl->grid;
     std::vector<int> &prof_flat = hp.prof_flat;
     std::vector<int> prof_off(profiles.size()), prof_stride(profiles.size());
@@ -584,17 +582,22 @@ static gpw_status plan_tasklist(gpw_tasklist *tl, const std::vector<gpw_pair_des
         tl->lb_max = std::max(tl->lb_max, pairs[i].lb_max);
     }
     hp.coef_total = coef_total;
-    // sub-boxes (contiguous per pair)
+    // sub-boxes (contiguous per pair); orthorhombic sub-boxes carry the
+    // squared radius of their profile (general ones their pair's radius^2)
     tl->subs_h.clear();
+    tl->subs_h.reserve(pairs.size() + pairs.size() / 4);
     tl->pair_sub_start.assign(pairs.size() + 1, 0);
     for (size_t i = 0; i < pairs.size(); i++) {
         tl->pair_sub_start[i] = (int)tl->subs_h.size();
         const int pi = geos[i].prof;
-        gpw_status st = make_subs(tl->grid, tl->pairs_h[i], (int)i, geos[i], pi >= 0 ? prof_off[pi] : -1,
+        PairGeom geo = geos[i];
+        if (pi >= 0) geo.R2 = profiles[pi].R2;
+        gpw_status st = make_subs(tl->grid, tl->pairs_h[i], (int)i, geo, pi >= 0 ? prof_off[pi] : -1,
                                   pi >= 0 ? prof_stride[pi] : 0, tl->subs_h);
         if (st) return st;
     }
     tl->pair_sub_start[pairs.size()] = (int)tl->subs_h.size();
+    if (tl->subs_h.size() > 0x3fffffffULL) return fail(GPW_ERR_ARG, "too many sub-boxes");
     // tiles
     TileGrid &tg = tl->tg;
     tg.nx = g.ngrid[0];
@@ -615,16 +618,46 @@ static gpw_status plan_tasklist(gpw_tasklist *tl, const std::vector<gpw_pair_des
         tl->csplit[c] = tl->splits;
         tl->isplit[0][c] = tl->isplit[1][c] = tl->splits;
     }
-    // squared sphere radius of every sub-box (from its pair's profile)
-    std::vector<double> sub_R2(tl->subs_h.size(), 0.0);
-    for (size_t si = 0; si < tl->subs_h.size(); si++) {
-        const int pi = geos[tl->subs_h[si].pair].prof;
-        sub_R2[si] = tl->tg.general ? tl->subs_h[si].R2 : pi >= 0 ? profiles[pi].R2 : 0.0;
+    // pairs for the small-sphere integrate kernel, per force extension
+    hp.small.clear();
+    hp.small_mask.assign(2 * pairs.size(), 0);
+    for (int w = 0; w < 2; w++) {
+        std::vector<int> by_cls[NSCLS];
+        for (size_t p = 0; p < pairs.size(); p++) {
+            if (tl->pair_sub_start[p + 1] - tl->pair_sub_start[p] != 1) continue;
+            const int si = tl->pair_sub_start[p];
+            const SubDev &sd = tl->subs_h[si];
+            const int L = sd.lp + 2 * w + 1;
+            const int nyz = tl->tg.general ? SMALL_NY : SMALL_NYZ_ORTHO;
+            if (L > SMALL_LMAX || sd.ub[0] - sd.lb[0] + 1 > SMALL_NX ||
+                sd.ub[1] - sd.lb[1] + 1 > std::min(nyz, SMALL_NY) ||
+                sd.ub[2] - sd.lb[2] + 1 > std::min(nyz, SMALL_NZ))
+                continue;
+            hp.small_mask[w * pairs.size() + p] = 1;
+            by_cls[(L - 1) / 4].push_back(si);
+        }
+        for (int c = 0; c < NSCLS; c++) {
+            tl->small_off[w][c] = (long long)hp.small.size();
+            tl->small_cnt[w][c] = (long long)by_cls[c].size();
+            hp.small.insert(hp.small.end(), by_cls[c].begin(), by_cls[c].end());
+        }
     }
-    // entries, pair-major
+    tl->mstride = ncoset(tl->lp_max + 2);
+    return GPW_OK;
+}
+
+// Host planner of the work items (gpw_tasklist_plan_stats; the task lists
+// themselves are planned on the device, gpw_devplan). Items ("entries") are
+// (sub-box, tile) pairs numbered pair-major, so a pair's partial moments are
+// contiguous; per (set, class) a CSR over tiles lists the item records in
+// the order the tile kernels sweep them (sub-box order within a tile).
+static gpw_status plan_entries_host(gpw_tasklist *tl, HostPlan &hp) {
+    const gpw_grid_desc &g = tl->grid;
+    const TileGrid &tg = tl->tg;
+    const size_t npairs = tl->pairs_h.size();
     std::vector<int> ent_tile;
     std::vector<int> tx, ty, tz;
-    hp.pe_start.assign(pairs.size() + 1, 0);
+    hp.pe_start.assign(npairs + 1, 0);
     tl->cube_points = 0;
     tl->flops_collocate = tl->flops_integrate[0] = tl->flops_integrate[1] = 0.0;
     for (size_t si = 0; si < tl->subs_h.size(); si++) {
@@ -639,13 +672,13 @@ static gpw_status plan_tasklist(gpw_tasklist *tl, const std::vector<gpw_pair_des
                     hp.ent_sub.push_back((int)si);
                     hp.pe_start[s.pair + 1]++;
                 }
-        const SubWork w = tl->tg.general ? sub_points_general(g, s) : sub_points(s, prof_flat);
+        const SubWork w = tg.general ? sub_points_general(g, s) : sub_points(s, hp.prof_flat);
         tl->cube_points += w.pts;
         tl->flops_collocate += contraction_flops(w, s.lp);
         tl->flops_integrate[0] += contraction_flops(w, s.lp);
         tl->flops_integrate[1] += contraction_flops(w, s.lp + 2);
     }
-    for (size_t p = 0; p < pairs.size(); p++) hp.pe_start[p + 1] += hp.pe_start[p];
+    for (size_t p = 0; p < npairs; p++) hp.pe_start[p + 1] += hp.pe_start[p];
     const size_t ne = ent_tile.size();
     if (ne > 0x7fffffffULL) return fail(GPW_ERR_ARG, "too many work items");
     tl->nentries = (long long)ne;
@@ -662,33 +695,6 @@ static gpw_status plan_tasklist(gpw_tasklist *tl, const std::vector<gpw_pair_des
         acc2 += ncoset(lp + 2);
     }
     hp.partial_total = acc2;
-    // pairs for the small-sphere integrate kernel, per force extension
-    std::vector<char> small[2];
-    for (int w = 0; w < 2; w++) small[w].assign(pairs.size(), 0);
-    hp.small.clear();
-    hp.small_mask.assign(2 * pairs.size(), 0);
-    for (int w = 0; w < 2; w++) {
-        std::vector<int> by_cls[NSCLS];
-        for (size_t p = 0; p < pairs.size(); p++) {
-            if (tl->pair_sub_start[p + 1] - tl->pair_sub_start[p] != 1) continue;
-            const int si = tl->pair_sub_start[p];
-            const SubDev &sd = tl->subs_h[si];
-            const int L = sd.lp + 2 * w + 1;
-            const int nyz = tl->tg.general ? SMALL_NY : SMALL_NYZ_ORTHO;
-            if (L > SMALL_LMAX || sd.ub[0] - sd.lb[0] + 1 > SMALL_NX ||
-                sd.ub[1] - sd.lb[1] + 1 > std::min(nyz, SMALL_NY) ||
-                sd.ub[2] - sd.lb[2] + 1 > std::min(nyz, SMALL_NZ))
-                continue;
-            small[w][p] = 1;
-            hp.small_mask[w * pairs.size() + p] = 1;
-            by_cls[(L - 1) / 4].push_back(si);
-        }
-        for (int c = 0; c < NSCLS; c++) {
-            tl->small_off[w][c] = (long long)hp.small.size();
-            tl->small_cnt[w][c] = (long long)by_cls[c].size();
-            hp.small.insert(hp.small.end(), by_cls[c].begin(), by_cls[c].end());
-        }
-    }
     // per (set, class): CSR over tiles of the item records
     const int nt1 = tl->ntiles + 1;
     hp.tile_start.assign((size_t)NSET * NCLS * nt1, 0);
@@ -699,7 +705,8 @@ static gpw_status plan_tasklist(gpw_tasklist *tl, const std::vector<gpw_pair_des
         std::vector<int> cls(ne);
         for (size_t e = 0; e < ne; e++) {
             const SubDev &sd = tl->subs_h[hp.ent_sub[e]];
-            const bool skip = a == 1 ? small[1][sd.pair] : a == 2 ? small[0][sd.pair] : false;
+            const bool skip = a == 1 ? hp.small_mask[npairs + sd.pair] != 0
+                                     : a == 2 ? hp.small_mask[sd.pair] != 0 : false;
             cls[e] = skip ? -1 : class_of(sd.lp + lp_add);
         }
         for (int c = 0; c < NCLS; c++) {
@@ -723,7 +730,7 @@ static gpw_status plan_tasklist(gpw_tasklist *tl, const std::vector<gpw_pair_des
                     r.ub[d] = sd.ub[d];
                 }
                 r.zetp = sd.zetp;
-                r.R2 = sub_R2[hp.ent_sub[e]];
+                r.R2 = sd.R2;
                 r.ent_off = a == 1 ? hp.off2[e] : hp.off0[e];
                 r.lp = sd.lp;
                 r.coef_off = sd.coef_off;
@@ -731,8 +738,15 @@ static gpw_status plan_tasklist(gpw_tasklist *tl, const std::vector<gpw_pair_des
             }
         }
     }
-    tl->mstride = ncoset(tl->lp_max + 2);
     return GPW_OK;
+}
+
+static gpw_status plan_tasklist(gpw_tasklist *tl, const std::vector<gpw_pair_desc> &pairs,
+                                const std::vector<PairGeom> &geos,
+                                const std::vector<Profile> &profiles, int splits, HostPlan &hp) {
+    gpw_status st = plan_common(tl, pairs, geos, profiles, splits, hp);
+    if (st) return st;
+    return plan_entries_host(tl, hp);
 }
 
 // Groups of pairs by the LDS size class of their transform (powers of two
@@ -799,7 +813,149 @@ static gpw_status group_pairs(gpw_tasklist *tl, std::vector<int> &order, std::ve
     return GPW_OK;
 }
 
-static gpw_status upload_tasklist(gpw_tasklist *tl, const HostPlan &hp) {
+// device memory freed when the scope ends (the planner's temporaries)
+struct DevTemp {
+    std::vector<void *> ptrs;
+    ~DevTemp() {
+        for (void *p : ptrs) (void)hipFree(p);
+    }
+    template <class T>
+    hipError_t alloc(T **p, size_t n) {
+        *p = nullptr;
+        hipError_t e = hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T));
+        if (e == hipSuccess) ptrs.push_back(*p);
+        return e;
+    }
+};
+
+// Device planner of the work items, from the uploaded sub-boxes: the same
+// items, partial-moment offsets, per-(set, class) tile CSR and records as
+// plan_entries_host, with the same order of items within each tile.
+//  1. k_plan_count: items per sub-box and the algorithmic work (stats);
+//  2. host: prefix sums over sub-boxes (item starts, partial offsets);
+//  3. k_plan_entries: each item's sub-box, tile, offsets and sort keys
+//     (class * ntiles + tile per set), with per-key counts -> tile CSR;
+//  4. per set, a stable radix sort of the keys (items of one key keep their
+//     sub-box order) and k_plan_records writes the records in that order.
+static gpw_status plan_device(gpw_tasklist *tl, HostPlan &hp, hipStream_t st) {
+    const int nsub = (int)tl->subs_h.size();
+    const int npairs = tl->npairs;
+    const int ntiles = tl->ntiles;
+    DevTemp tmp;
+    int *d_counts = nullptr;
+    unsigned long long *d_work = nullptr;
+    const int nwork = 3 * (LPMAX + 1);
+    HIP_TRY(tmp.alloc(&d_counts, nsub));
+    HIP_TRY(tmp.alloc(&d_work, nwork));
+    HIP_TRY(hipMemsetAsync(d_work, 0, nwork * sizeof(unsigned long long), st));
+    HIP_TRY(launch_plan_count(tl->d_subs, nsub, tl->d_prof, tl->tg, d_counts, d_work, st));
+    std::vector<int> counts(nsub);
+    std::vector<unsigned long long> work(nwork);
+    if (nsub) HIP_TRY(hipMemcpyAsync(counts.data(), d_counts, nsub * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(work.data(), d_work, nwork * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    // stats
+    tl->cube_points = 0;
+    tl->flops_collocate = tl->flops_integrate[0] = tl->flops_integrate[1] = 0.0;
+    for (int lp = 0; lp <= LPMAX; lp++) {
+        SubWork w;
+        w.pts = (long long)work[3 * lp];
+        w.rows = (long long)work[3 * lp + 1];
+        w.planes = (long long)work[3 * lp + 2];
+        tl->cube_points += w.pts;
+        tl->flops_collocate += contraction_flops(w, lp);
+        tl->flops_integrate[0] += contraction_flops(w, lp);
+        tl->flops_integrate[1] += contraction_flops(w, lp + 2);
+    }
+    // item starts (sub-box-major; sub-boxes are pair-major) and partial offsets
+    std::vector<long long> ent_start(nsub + 1), base0(nsub), base2(nsub);
+    long long ne = 0, acc0 = 0, acc2 = 0;
+    hp.pe_start.assign(npairs + 1, 0);
+    for (int s = 0; s < nsub; s++) {
+        const SubDev &sd = tl->subs_h[s];
+        ent_start[s] = ne;
+        base0[s] = acc0;
+        base2[s] = acc2;
+        ne += counts[s];
+        acc0 += (long long)counts[s] * ncoset(sd.lp);
+        acc2 += (long long)counts[s] * ncoset(sd.lp + 2);
+    }
+    ent_start[nsub] = ne;
+    for (int p = 0; p < npairs; p++) hp.pe_start[p] = (int)ent_start[tl->pair_sub_start[p]];
+    hp.pe_start[npairs] = (int)ne;
+    if (ne > 0x7fffffffLL) return fail(GPW_ERR_ARG, "too many work items");
+    tl->nentries = ne;
+    tl->tile_points = ne * (long long)(TX * TY * TZ);
+    hp.partial_total = acc2;
+    gpw_status stt;
+    if ((stt = upload(&tl->d_pe_start, hp.pe_start))) return stt;
+    // items, keys, per-key counts
+    const int n = (int)ne;
+    const size_t nkey = (size_t)NCLS * ntiles;
+    long long *d_es = nullptr, *d_b0 = nullptr, *d_b2 = nullptr;
+    int *d_ent_sub = nullptr, *d_vals = nullptr, *d_sorted = nullptr, *d_hist = nullptr;
+    unsigned *d_keys = nullptr, *d_keys_out = nullptr;
+    HIP_TRY(tmp.alloc(&d_es, nsub + 1));
+    HIP_TRY(tmp.alloc(&d_b0, nsub));
+    HIP_TRY(tmp.alloc(&d_b2, nsub));
+    HIP_TRY(tmp.alloc(&d_ent_sub, n));
+    HIP_TRY(tmp.alloc(&d_vals, n));
+    HIP_TRY(tmp.alloc(&d_sorted, n));
+    HIP_TRY(tmp.alloc(&d_keys, (size_t)NSET * n));
+    HIP_TRY(tmp.alloc(&d_keys_out, n));
+    HIP_TRY(tmp.alloc(&d_hist, NSET * nkey));
+    HIP_TRY(hipMalloc((void **)&tl->d_ent_off0, sizeof(long long) * std::max(n, 1)));
+    HIP_TRY(hipMalloc((void **)&tl->d_ent_off2, sizeof(long long) * std::max(n, 1)));
+    HIP_TRY(hipMemcpyAsync(d_es, ent_start.data(), (nsub + 1) * sizeof(long long), hipMemcpyHostToDevice, st));
+    if (nsub) {
+        HIP_TRY(hipMemcpyAsync(d_b0, base0.data(), nsub * sizeof(long long), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(d_b2, base2.data(), nsub * sizeof(long long), hipMemcpyHostToDevice, st));
+    }
+    HIP_TRY(hipMemsetAsync(d_hist, 0, NSET * nkey * sizeof(int), st));
+    HIP_TRY(launch_plan_entries(tl->d_subs, nsub, d_es, d_b0, d_b2, tl->tg, ntiles, tl->d_small_mask, npairs, n,
+                                d_ent_sub, tl->d_ent_off0, tl->d_ent_off2, d_keys, d_vals, d_hist, st));
+    std::vector<int> hist(NSET * nkey);
+    HIP_TRY(hipMemcpyAsync(hist.data(), d_hist, NSET * nkey * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    // tile CSR per (set, class), record segments in set-major, class-major order
+    const int nt1 = ntiles + 1;
+    hp.tile_start.assign((size_t)NSET * NCLS * nt1, 0);
+    long long nrec = 0, set_base[NSET];
+    for (int a = 0; a < NSET; a++) {
+        set_base[a] = nrec;
+        for (int c = 0; c < NCLS; c++) {
+            int *start = hp.tile_start.data() + (size_t)(a * NCLS + c) * nt1;
+            for (int t = 0; t < ntiles; t++) start[t + 1] = start[t] + hist[(size_t)a * nkey + (size_t)c * ntiles + t];
+            tl->list_off[a][c] = nrec;
+            tl->class_entries[a][c] = start[ntiles];
+            nrec += start[ntiles];
+        }
+    }
+    if ((stt = upload(&tl->d_tile_start, hp.tile_start))) return stt;
+    HIP_TRY(hipMalloc((void **)&tl->d_recs, sizeof(EntryRec) * std::max<long long>(nrec, 1)));
+    // sort each set's keys (values: item ids) and write its records
+    int end_bit = 1;
+    while ((1ULL << end_bit) <= (unsigned long long)nkey) end_bit++;
+    size_t tbytes = 0;
+    HIP_TRY(sort_pairs_u32(nullptr, &tbytes, d_keys, d_keys_out, d_vals, d_sorted, n, end_bit, st));
+    void *d_temp = nullptr;
+    HIP_TRY(tmp.alloc((char **)&d_temp, tbytes));
+    for (int a = 0; a < NSET; a++) {
+        const long long nvalid = (a + 1 < NSET ? set_base[a + 1] : nrec) - set_base[a];
+        if (nvalid == 0) continue;
+        size_t tb = tbytes;
+        HIP_TRY(sort_pairs_u32(d_temp, &tb, d_keys + (size_t)a * n, d_keys_out, d_vals, d_sorted, n, end_bit, st));
+        HIP_TRY(launch_plan_records(tl->d_subs, d_ent_sub, a == 1 ? tl->d_ent_off2 : tl->d_ent_off0, d_sorted,
+                                    (int)nvalid, tl->d_recs + set_base[a], st));
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    return GPW_OK;
+}
+
+// Uploads the host plan and, unless the host planned the work items
+// (hp.recs filled), plans them on the device; then allocates the work
+// buffers.
+static gpw_status upload_tasklist(gpw_tasklist *tl, HostPlan &hp, bool device_items) {
     gpw_status st;
     std::vector<int> order, segs;
     if ((st = group_pairs(tl, order, segs))) return st;  // argument checks before device work
@@ -808,13 +964,17 @@ static gpw_status upload_tasklist(gpw_tasklist *tl, const HostPlan &hp) {
     if ((st = upload(&tl->d_segs, segs))) return st;
     if ((st = upload(&tl->d_subs, tl->subs_h))) return st;
     if ((st = upload(&tl->d_prof, hp.prof_flat))) return st;
-    if ((st = upload(&tl->d_tile_start, hp.tile_start))) return st;
     if ((st = upload(&tl->d_small, hp.small))) return st;
     if ((st = upload(&tl->d_small_mask, hp.small_mask))) return st;
-    if ((st = upload(&tl->d_recs, hp.recs))) return st;
-    if ((st = upload(&tl->d_pe_start, hp.pe_start))) return st;
-    if ((st = upload(&tl->d_ent_off0, hp.off0))) return st;
-    if ((st = upload(&tl->d_ent_off2, hp.off2))) return st;
+    if (device_items) {
+        if ((st = plan_device(tl, hp, nullptr))) return st;
+    } else {
+        if ((st = upload(&tl->d_tile_start, hp.tile_start))) return st;
+        if ((st = upload(&tl->d_recs, hp.recs))) return st;
+        if ((st = upload(&tl->d_pe_start, hp.pe_start))) return st;
+        if ((st = upload(&tl->d_ent_off0, hp.off0))) return st;
+        if ((st = upload(&tl->d_ent_off2, hp.off2))) return st;
+    }
     HIP_TRY(hipMalloc((void **)&tl->d_coef, sizeof(double) * std::max<long long>(hp.coef_total, 1)));
     tl->coef_total = hp.coef_total;
     HIP_TRY(hipMalloc((void **)&tl->d_partial, sizeof(double) * std::max<long long>(hp.partial_total, 1)));
@@ -855,9 +1015,9 @@ static gpw_status finish_tasklist(gpw_tasklist *tl, const std::vector<gpw_pair_d
                                   const std::vector<PairGeom> &geos,
                                   const std::vector<Profile> &profiles, int splits) {
     HostPlan hp;
-    gpw_status st = plan_tasklist(tl, pairs, geos, profiles, splits, hp);
+    gpw_status st = plan_common(tl, pairs, geos, profiles, splits, hp);
     if (st) return st;
-    return upload_tasklist(tl, hp);
+    return upload_tasklist(tl, hp, true);
 }
 
 gpw_status gpw::check_grid(const gpw_grid_desc *g) {

@@ -307,3 +307,50 @@ def test_dropin_host_and_device_arrays(gpw, oracle, torch_cuda, case, tau):
         (t["lb_max"] + 1) * (t["lb_max"] + 2) * (t["lb_max"] + 3) // 6
     mask[t["o2"]:t["o2"] + nb, t["o1"]:t["o1"] + na] = True
     assert not hd.cpu().numpy()[~mask].any()
+
+
+@pytest.mark.parametrize("which", ["config2", "config3-tau", "config4-subset", "config5-level0",
+                                   "windowed-general"])
+def test_device_planner_matches_host_planner(gpw, torch_cuda, which):
+    """Task lists are planned on the device (items, offsets, tile lists,
+    work statistics); gpw_tasklist_plan_stats runs the host planner. Both
+    give the same item count, sub-boxes, sphere points and algorithmic
+    flops, for orthorhombic and general cells, windows and tau."""
+    from libgpw_amd import workloads as Wk
+    say("device vs host planner", which)
+    tau = which.endswith("tau")
+    if which == "config2":
+        cfg = Wk.config(2)
+    elif which == "config3-tau":
+        cfg = Wk.config(3)
+    elif which == "config4-subset":
+        cfg = Wk.subset(Wk.config(4), np.arange(0, 100000, 10))
+    elif which == "config5-level0":
+        cfg = Wk.level(Wk.config(5, npairs=40000), 0)
+    else:
+        rng = np.random.default_rng(5)
+        npts, ngrid, lb_grid, per = (40, 36, 44), (34, 36, 32), (-3, 0, -6), (0, 1, 0)
+        dh = np.diag(np.array([10.0, 9.0, 11.0]) / np.asarray(npts))
+        dh[1, 0] = 0.2 * dh[0, 0]
+        pairs, nblock = _ranged_pairs(rng, 30, dh * np.asarray(npts)[:, None])
+        pairs["zeta"] = rng.uniform(2.5, 5.0, 30)
+        pairs["zetb"] = rng.uniform(2.5, 5.0, 30)
+        pairs["radius"] = np.sqrt(np.log(1e10) / (pairs["zeta"] + pairs["zetb"]))
+        # centres inside the window, away from its non-periodic edges
+        lo = (12 - np.asarray(lb_grid)) * np.diag(dh)
+        hi = (np.asarray(ngrid) - 12 - np.asarray(lb_grid)) * np.diag(dh)
+        pairs["ra"] = rng.uniform(lo, hi, (30, 3))
+        pairs["rab"] = rng.normal(0.0, 0.3, (30, 3))
+        g = gpw.grid_desc(npts, dh, ngrid=ngrid, lb_grid=lb_grid, periodic=per)
+        cfg = None
+    if cfg is not None:
+        g = gpw.grid_desc(cfg["npts"], cfg["dh"])
+        pairs = cfg["pairs"]
+    host = gpw.plan_stats(g, pairs, compute_tau=tau, splits=3)
+    tl = gpw.TaskList(g, pairs, compute_tau=tau, splits=3)
+    devs = tl.stats()
+    tl.close()
+    for k in ("npairs", "nsub", "ntiles", "nentries", "cube_points", "tile_points", "lp_max",
+              "flops_collocate"):
+        assert devs[k] == host[k], (k, devs[k], host[k])
+    assert devs["flops_integrate"] == host["flops_integrate"]
