@@ -90,6 +90,10 @@ typedef struct gpw_tasklist_stats {
      * point of each pair's sphere; integrate with forces runs at lp + 2       */
     double flops_collocate;
     double flops_integrate[2];  /* without / with forces                          */
+    /* wall time of gpw_tasklist_create (ms): host geometry and plan of pairs and
+     * sub-boxes, their upload, the work items planned on the device, the
+     * work buffers (0 for gpw_tasklist_plan_stats)                           */
+    double plan_ms[4];
 } gpw_tasklist_stats;
 
 /* ---- single task, host memory (drop-in for the reference's collocate) ---- */
@@ -181,6 +185,12 @@ gpw_status gpw_tasklist_create(const gpw_grid_desc *grid, int npairs,
                                const gpw_pair_desc *pairs, int compute_tau,
                                const int *tile, int splits, gpw_tasklist **out);
 gpw_status gpw_tasklist_destroy(gpw_tasklist *tl);
+/* Re-plans a task list in place for new pair data (same number of pairs,
+ * typically moved centres): geometry and work items are rebuilt, the
+ * autotuned splits and the collocate partial grids are kept. Waits for the
+ * device first; allocates, so not inside captured or timed regions. On
+ * failure the task list has no pairs until a successful update. */
+gpw_status gpw_tasklist_update(gpw_tasklist *tl, int npairs, const gpw_pair_desc *pairs);
 gpw_status gpw_tasklist_stats_get(const gpw_tasklist *tl, gpw_tasklist_stats *out);
 
 /* Host-only planning (no device needed): the statistics gpw_tasklist_create

@@ -25,7 +25,7 @@ EXPORTS = (
     "gpw_collocate_pgf_product_rspace", "gpw_integrate_pgf_product_rspace",
     "gpw_collocate_pgf_product_rspace_dev", "gpw_integrate_pgf_product_rspace_dev",
     "gpw_integrate_pgf_product_rspace_noalloc",
-    "gpw_tasklist_create", "gpw_tasklist_destroy", "gpw_tasklist_stats_get",
+    "gpw_tasklist_create", "gpw_tasklist_destroy", "gpw_tasklist_update", "gpw_tasklist_stats_get",
     "gpw_tasklist_plan_stats", "gpw_pair_geometry",
     "gpw_tasklist_collocate", "gpw_tasklist_integrate",
     "gpw_tasklist_collocate_plain", "gpw_tasklist_integrate_plain",
@@ -62,7 +62,8 @@ class TaskListStats(C.Structure):
                 ("nentries", C.c_longlong), ("cube_points", C.c_longlong),
                 ("tile_points", C.c_longlong), ("lp_max", C.c_int), ("tile", C.c_int * 3),
                 ("splits", C.c_int), ("csplits", C.c_int * 4), ("isplits", (C.c_int * 4) * 2),
-                ("flops_collocate", C.c_double), ("flops_integrate", C.c_double * 2)]
+                ("flops_collocate", C.c_double), ("flops_integrate", C.c_double * 2),
+                ("plan_ms", C.c_double * 4)]
 
 
 _lib = None
@@ -93,6 +94,7 @@ def lib():
         L.gpw_tasklist_create.argtypes = [C.POINTER(GridDesc), C.c_int, C.POINTER(PairDesc),
                                           C.c_int, ip, C.c_int, C.POINTER(vp)]
         L.gpw_tasklist_destroy.argtypes = [vp]
+        L.gpw_tasklist_update.argtypes = [vp, C.c_int, C.POINTER(PairDesc)]
         L.gpw_tasklist_stats_get.argtypes = [vp, C.POINTER(TaskListStats)]
         L.gpw_tasklist_plan_stats.argtypes = [C.POINTER(GridDesc), C.c_int, C.POINTER(PairDesc),
                                               C.c_int, C.c_int, C.POINTER(TaskListStats)]
@@ -159,19 +161,23 @@ _PAIR_DTYPE = np.dtype({
     "itemsize": C.sizeof(PairDesc)})
 
 
-def pair_array(p: dict):
-    """gpw_pair_desc array of a dict of per-pair arrays (filled column-wise)."""
+def pair_array(p: dict) -> np.ndarray:
+    """gpw_pair_desc array (numpy, the C layout) of a dict of per-pair
+    arrays, filled column-wise; pass it to the C ABI with _pairs_ptr."""
     n = len(p["la_max"])
-    arr = (PairDesc * n)()
+    v = np.empty(max(n, 1), dtype=_PAIR_DTYPE)
     if n == 0:
-        return arr
-    v = np.frombuffer(arr, dtype=_PAIR_DTYPE, count=n)
+        return v[:0]
     for k in ("la_max", "la_min", "lb_max", "lb_min", "zeta", "zetb", "rscale", "radius",
               "block_offset"):
         v[k] = np.asarray(p[k])
     v["ra"] = np.asarray(p["ra"], dtype=np.float64).reshape(n, 3)
     v["rab"] = np.asarray(p["rab"], dtype=np.float64).reshape(n, 3)
-    return arr
+    return v
+
+
+def _pairs_ptr(arr: np.ndarray):
+    return arr.ctypes.data_as(C.POINTER(PairDesc))
 
 
 def _stats_dict(s: TaskListStats) -> dict:
@@ -180,6 +186,7 @@ def _stats_dict(s: TaskListStats) -> dict:
     out["csplits"] = list(s.csplits)
     out["isplits"] = [list(r) for r in s.isplits]
     out["flops_integrate"] = list(s.flops_integrate)
+    out["plan_ms"] = dict(zip(("host", "upload", "device_items", "buffers"), s.plan_ms))
     return out
 
 
@@ -187,7 +194,7 @@ def plan_stats(grid: GridDesc, pairs: dict, compute_tau: bool = False, splits: i
     """Host-only planning statistics (no device needed)."""
     arr = pair_array(pairs)
     s = TaskListStats()
-    _check(lib().gpw_tasklist_plan_stats(C.byref(grid), len(arr), arr, int(compute_tau),
+    _check(lib().gpw_tasklist_plan_stats(C.byref(grid), len(arr), _pairs_ptr(arr), int(compute_tau),
                                          int(splits), C.byref(s)))
     return _stats_dict(s)
 
@@ -200,7 +207,7 @@ def pair_geometry(grid: GridDesc, pairs: dict, i: int, compute_tau: bool = False
     cap = 1 << 18
     sb = np.zeros(cap, dtype=np.int32)
     n = C.c_int(0)
-    _check(lib().gpw_pair_geometry(C.byref(grid), arr, int(compute_tau), cc, roff, lb, ub,
+    _check(lib().gpw_pair_geometry(C.byref(grid), _pairs_ptr(arr), int(compute_tau), cc, roff, lb, ub,
                                    sb.ctypes.data_as(C.POINTER(C.c_int)), cap, C.byref(n)))
     return {"cube_centre": list(cc), "roffset": list(roff), "lb_cube": list(lb),
             "ub_cube": list(ub), "sphere_bounds": sb[:n.value].copy()}
@@ -248,11 +255,13 @@ class TaskList:
         self.ngrid_points = int(grid.ngrid[0]) * int(grid.ngrid[1]) * int(grid.ngrid[2])
         self.nblock = 0
         if self.npairs:
-            ends = [int(bo) + ncoset(int(b)) * ncoset(int(a)) for bo, a, b in
-                    zip(pairs["block_offset"], pairs["la_max"], pairs["lb_max"])]
-            self.nblock = max(ends)
+            la = np.asarray(pairs["la_max"], dtype=np.int64)
+            lb = np.asarray(pairs["lb_max"], dtype=np.int64)
+            nc = lambda l: (l + 1) * (l + 2) * (l + 3) // 6
+            ends = np.asarray(pairs["block_offset"], dtype=np.int64) + nc(la) * nc(lb)
+            self.nblock = int(ends.max())
         h = C.c_void_p()
-        _check(lib().gpw_tasklist_create(C.byref(grid), self.npairs, self._pairs,
+        _check(lib().gpw_tasklist_create(C.byref(grid), self.npairs, _pairs_ptr(self._pairs),
                                          int(compute_tau), None, int(splits), C.byref(h)))
         self._h = h
 
@@ -266,6 +275,21 @@ class TaskList:
             self.close()
         except Exception:
             pass
+
+    def update(self, pairs: dict):
+        """Re-plan in place for new pair data (same number of pairs, e.g.
+        moved centres), keeping the autotuned splits."""
+        arr = pair_array(pairs)
+        if len(arr) != self.npairs:
+            raise ValueError(f"update needs {self.npairs} pairs, got {len(arr)}")
+        _check(lib().gpw_tasklist_update(self._h, self.npairs, _pairs_ptr(arr)))
+        self._pairs = arr
+        la = np.asarray(pairs["la_max"], dtype=np.int64)
+        lb = np.asarray(pairs["lb_max"], dtype=np.int64)
+        nc = lambda l: (l + 1) * (l + 2) * (l + 3) // 6
+        if self.npairs:
+            self.nblock = int((np.asarray(pairs["block_offset"], dtype=np.int64) + nc(la) * nc(lb)).max())
+        return self
 
     def stats(self) -> dict:
         s = TaskListStats()

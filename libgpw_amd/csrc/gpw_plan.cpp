@@ -15,11 +15,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "gpw_host.hpp"
@@ -28,6 +30,31 @@
 using namespace gpw;
 
 static thread_local std::string g_last_error;
+
+static double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+// f(begin, end) over [0, n) in contiguous chunks on up to 16 host threads
+// (one chunk below 4,096 items); chunks are numbered in order, so results
+// gathered per chunk concatenate in item order
+template <class F>
+static void parallel_chunks(size_t n, F &&f) {
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    nt = (unsigned)std::min<size_t>(nt, std::max<size_t>(1, n / 2048));
+    if (nt <= 1) {
+        f(0, n, 0);
+        return;
+    }
+    const size_t chunk = (n + nt - 1) / nt;
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; t++) {
+        const size_t b = t * chunk, e = std::min(n, b + chunk);
+        if (b < e) th.emplace_back([&f, b, e, t] { f(b, e, t); });
+    }
+    for (auto &x : th) x.join();
+}
 
 gpw_status gpw::fail(gpw_status code, const std::string &msg) {
     g_last_error = msg;
@@ -50,6 +77,17 @@ static int sphere_extent(double base, double h, double R2) {
     }
 }
 
+// sphere_extent(base, h, R2) when it is known to be at most `upper`
+static int sphere_extent_below(double base, double h, double R2, int upper) {
+    int n = upper;
+    while (n > 0) {
+        const double x = (double)n * h;
+        if (base + x * x <= R2) return n;
+        n--;
+    }
+    return 0;
+}
+
 // Profile for a radius on an orthorhombic grid with spacings dr (the rule
 // that regenerates the reference's fixture sphere_bounds).
 void gpw::profile_from_radius(double radius, const double dr[3], Profile &P, int lb[3],
@@ -67,14 +105,19 @@ void gpw::profile_from_radius(double radius, const double dr[3], Profile &P, int
         ub[d] = 1 + n;
     }
     P.T.assign((size_t)(P.n[2] + 1) * (P.n[1] + 1), -1);
+    // the predicate base + (n h)^2 <= R2 is monotone in n and in base, so
+    // each extent is the previous row's (or plane's) searched downwards
+    int jn = P.n[1];
     for (int kk = 0; kk <= P.n[2]; kk++) {
         const double zk = -kk * dr[2];
         const double base = zk * zk;
-        const int jn = sphere_extent(base, dr[1], R2);
+        jn = sphere_extent_below(base, dr[1], R2, jn);
+        int ext = P.n[0];
         for (int jj = 0; jj <= jn; jj++) {
             const double yj = -jj * dr[1];
             const double b2 = base + yj * yj;
-            P.T[(size_t)kk * (P.n[1] + 1) + jj] = sphere_extent(b2, dr[0], R2);
+            ext = sphere_extent_below(b2, dr[0], R2, ext);
+            P.T[(size_t)kk * (P.n[1] + 1) + jj] = ext;
         }
     }
 }
@@ -132,6 +175,7 @@ struct gpw_tasklist {
     long long nentries = 0;
     long long cube_points = 0, tile_points = 0;
     double flops_collocate = 0.0, flops_integrate[2] = {0.0, 0.0};
+    double plan_ms[4] = {0.0, 0.0, 0.0, 0.0};  // see gpw_tasklist_stats
     // tile lists: set 0 = collocate (every item), set 1 = integrate with the
     // force extension (lp_add 2), set 2 = integrate without; the integrate
     // sets leave out the pairs of the small-sphere kernel (small_*)
@@ -186,12 +230,22 @@ struct gpw_tasklist {
     std::vector<int> event_phase;
     int events_used = 0;
 
+    // frees the planned buffers (not the collocate partial grids or the
+    // timing events, which a re-plan of the same grid keeps)
+    void release_plan() {
+        void **ptrs[] = {(void **)&d_pairs, (void **)&d_subs, (void **)&d_prof, (void **)&d_tile_start,
+                         (void **)&d_small, (void **)&d_small_mask, (void **)&d_recs, (void **)&d_pe_start,
+                         (void **)&d_ent_off0, (void **)&d_ent_off2, (void **)&d_coef, (void **)&d_partial,
+                         (void **)&d_Mbuf, (void **)&d_mt_rp, (void **)&d_mt_col, (void **)&d_mt_val,
+                         (void **)&d_mt_rpoff, (void **)&d_hab_ws, (void **)&d_order, (void **)&d_segs};
+        for (void **p : ptrs) {
+            if (*p) (void)hipFree(*p);
+            *p = nullptr;
+        }
+    }
     ~gpw_tasklist() {
-        void *ptrs[] = {d_pairs, d_subs, d_prof, d_tile_start, d_small, d_small_mask, d_recs, d_pe_start,
-                        d_ent_off0, d_ent_off2, d_coef, d_partial, d_cpartial, d_Mbuf,
-                        d_mt_rp, d_mt_col, d_mt_val, d_mt_rpoff, d_hab_ws, d_order, d_segs};
-        for (void *p : ptrs)
-            if (p) (void)hipFree(p);
+        release_plan();
+        if (d_cpartial) (void)hipFree(d_cpartial);
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
     }
 };
@@ -325,35 +379,37 @@ static bool general_row_host(const gpw_grid_desc &g, double gp0, double dj, doub
 gpw_status gpw::make_subs(const gpw_grid_desc &g, const PairDev &pd, int pair_index,
                          const PairGeom &geo, int prof_off, int prof_stride,
                          std::vector<SubDev> &out) {
-    std::vector<std::pair<int, int>> ranges[3];
-    int base_of[3][64];
+    int rlo[3][64], rhi[3][64], base_of[3][64], nr[3];
     for (int d = 0; d < 3; d++) {
         const int lb = geo.lb[d], ub = geo.ub[d];
+        nr[d] = 0;
         if (g.periodic[d]) {
             const int N = g.npts[d];
-            int k = 0;
-            for (int s = lb; s <= ub; s += N) {
-                if (k >= 64) return fail(GPW_ERR_ARG, "cube spans more than 64 periods");
-                ranges[d].push_back({s, std::min(ub, s + N - 1)});
-                base_of[d][k++] = imod_h(geo.cc[d] + s, N);
+            for (long long s = lb; s <= ub; s += N) {
+                if (nr[d] >= 64) return fail(GPW_ERR_ARG, "cube spans more than 64 periods");
+                rlo[d][nr[d]] = (int)s;
+                rhi[d][nr[d]] = (int)std::min<long long>(ub, s + N - 1);
+                base_of[d][nr[d]++] = imod_h(geo.cc[d] + (int)s, N);
             }
         } else {
             const int gshift = imod_h(geo.cc[d] + lb + g.lb_grid[d], g.npts[d]) - lb;
             if (lb + gshift < 0 || ub + gshift > g.ngrid[d] - 1)
                 return fail(GPW_ERR_BOUNDS, "non-periodic cube outside the local grid");
-            ranges[d].push_back({lb, ub});
+            rlo[d][0] = lb;
+            rhi[d][0] = ub;
             base_of[d][0] = lb + gshift;
+            nr[d] = 1;
         }
     }
-    for (size_t iz = 0; iz < ranges[2].size(); iz++)
-        for (size_t iy = 0; iy < ranges[1].size(); iy++)
-            for (size_t ix = 0; ix < ranges[0].size(); ix++) {
+    for (int iz = 0; iz < nr[2]; iz++)
+        for (int iy = 0; iy < nr[1]; iy++)
+            for (int ix = 0; ix < nr[0]; ix++) {
                 SubDev s{};
-                const size_t idx[3] = {ix, iy, iz};
+                const int idx[3] = {ix, iy, iz};
                 for (int d = 0; d < 3; d++) {
                     s.roff[d] = geo.roff[d];
-                    s.lb[d] = ranges[d][idx[d]].first;
-                    s.ub[d] = ranges[d][idx[d]].second;
+                    s.lb[d] = rlo[d][idx[d]];
+                    s.ub[d] = rhi[d][idx[d]];
                     s.base[d] = base_of[d][idx[d]];
                     s.period[d] = g.periodic[d] ? g.npts[d] : BIG;
                 }
@@ -569,35 +625,63 @@ static gpw_status plan_common(gpw_tasklist *tl, const std::vector<gpw_pair_desc>
         prof_flat.insert(prof_flat.end(), profiles[i].T.begin(), profiles[i].T.end());
     }
     // pairs + coefficient offsets
-    tl->pairs_h.resize(pairs.size());
+    const size_t np = pairs.size();
+    tl->pairs_h.resize(np);
+    parallel_chunks(np, [&](size_t b, size_t e, unsigned) {
+        for (size_t i = b; i < e; i++) tl->pairs_h[i] = make_pair_dev(pairs[i], tl->compute_tau);
+    });
     long long coef_total = 0;
-    for (size_t i = 0; i < pairs.size(); i++) {
-        PairDev pd = make_pair_dev(pairs[i], tl->compute_tau);
+    for (size_t i = 0; i < np; i++) {
+        PairDev &pd = tl->pairs_h[i];
         pd.coef_off = (int)coef_total;
         coef_total += ncoset(pd.lp);
         if (coef_total > 0x7fffffffLL) return fail(GPW_ERR_ARG, "too many coefficients");
-        tl->pairs_h[i] = pd;
         tl->lp_max = std::max(tl->lp_max, pd.lp);
         tl->la_max = std::max(tl->la_max, pairs[i].la_max);
         tl->lb_max = std::max(tl->lb_max, pairs[i].lb_max);
     }
     hp.coef_total = coef_total;
-    // sub-boxes (contiguous per pair); orthorhombic sub-boxes carry the
+    // sub-boxes (contiguous per pair), built per chunk of pairs on threads
+    // and concatenated in pair order; orthorhombic sub-boxes carry the
     // squared radius of their profile (general ones their pair's radius^2)
-    tl->subs_h.clear();
-    tl->subs_h.reserve(pairs.size() + pairs.size() / 4);
-    tl->pair_sub_start.assign(pairs.size() + 1, 0);
-    for (size_t i = 0; i < pairs.size(); i++) {
-        tl->pair_sub_start[i] = (int)tl->subs_h.size();
-        const int pi = geos[i].prof;
-        PairGeom geo = geos[i];
-        if (pi >= 0) geo.R2 = profiles[pi].R2;
-        gpw_status st = make_subs(tl->grid, tl->pairs_h[i], (int)i, geo, pi >= 0 ? prof_off[pi] : -1,
-                                  pi >= 0 ? prof_stride[pi] : 0, tl->subs_h);
-        if (st) return st;
+    std::vector<std::vector<SubDev>> part(16);
+    std::vector<std::vector<int>> part_start(16);
+    std::vector<size_t> part_begin(16, 0);
+    std::vector<int> bad(16, 0);
+    std::vector<std::string> why(16);
+    parallel_chunks(np, [&](size_t b, size_t e, unsigned t) {
+        part_begin[t] = b;
+        part[t].reserve((e - b) + (e - b) / 4);
+        for (size_t i = b; i < e; i++) {
+            part_start[t].push_back((int)part[t].size());
+            const int pi = geos[i].prof;
+            PairGeom geo = geos[i];
+            if (pi >= 0) geo.R2 = profiles[pi].R2;
+            gpw_status st = make_subs(tl->grid, tl->pairs_h[i], (int)i, geo, pi >= 0 ? prof_off[pi] : -1,
+                                      pi >= 0 ? prof_stride[pi] : 0, part[t]);
+            if (st) {
+                bad[t] = st;
+                why[t] = gpw_last_error();
+                return;
+            }
+        }
+    });
+    for (int t = 0; t < 16; t++)
+        if (bad[t]) return fail(bad[t], why[t]);
+    size_t nsub = 0;
+    for (int t = 0; t < 16; t++) nsub += part[t].size();
+    if (nsub > 0x3fffffffULL) return fail(GPW_ERR_ARG, "too many sub-boxes");
+    tl->subs_h.resize(nsub);
+    tl->pair_sub_start.assign(np + 1, 0);
+    size_t at = 0;
+    for (int t = 0; t < 16; t++) {
+        if (part[t].empty()) continue;
+        std::copy(part[t].begin(), part[t].end(), tl->subs_h.begin() + at);
+        for (size_t k = 0; k < part_start[t].size(); k++)
+            tl->pair_sub_start[part_begin[t] + k] = (int)at + part_start[t][k];
+        at += part[t].size();
     }
-    tl->pair_sub_start[pairs.size()] = (int)tl->subs_h.size();
-    if (tl->subs_h.size() > 0x3fffffffULL) return fail(GPW_ERR_ARG, "too many sub-boxes");
+    tl->pair_sub_start[np] = (int)nsub;
     // tiles
     TileGrid &tg = tl->tg;
     tg.nx = g.ngrid[0];
@@ -756,13 +840,18 @@ static gpw_status plan_tasklist(gpw_tasklist *tl, const std::vector<gpw_pair_des
 // same la_max and lb_max); any other overlap is an argument error.
 static gpw_status group_pairs(gpw_tasklist *tl, std::vector<int> &order, std::vector<int> &segs) {
     const int t = tl->compute_tau;
-    // overlap check over all blocks
+    // overlap check over all blocks (pairs usually come in block order:
+    // then no sort is needed)
+    bool sorted = true;
+    for (int i = 1; i < tl->npairs && sorted; i++)
+        sorted = tl->pairs_h[i - 1].block_off <= tl->pairs_h[i].block_off;
     {
         std::vector<int> idx(tl->npairs);
         for (int i = 0; i < tl->npairs; i++) idx[i] = i;
-        std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
-            return tl->pairs_h[a].block_off < tl->pairs_h[b].block_off;
-        });
+        if (!sorted)
+            std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
+                return tl->pairs_h[a].block_off < tl->pairs_h[b].block_off;
+            });
         long long end = -1;
         const PairDev *prev = nullptr;
         for (int i : idx) {
@@ -792,9 +881,10 @@ static gpw_status group_pairs(gpw_tasklist *tl, std::vector<int> &order, std::ve
     tl->groups.clear();
     for (auto &v : byc) {
         if (v.empty()) continue;
-        std::stable_sort(v.begin(), v.end(), [&](int a, int b) {
-            return tl->pairs_h[a].block_off < tl->pairs_h[b].block_off;
-        });
+        if (!sorted)
+            std::stable_sort(v.begin(), v.end(), [&](int a, int b) {
+                return tl->pairs_h[a].block_off < tl->pairs_h[b].block_off;
+            });
         gpw_tasklist::PairGroup g{(int)order.size(), (int)v.size(), 0, 0, (int)segs.size(), 0};
         long long last = -1;
         for (int i : v) {
@@ -967,7 +1057,11 @@ static gpw_status upload_tasklist(gpw_tasklist *tl, HostPlan &hp, bool device_it
     if ((st = upload(&tl->d_small, hp.small))) return st;
     if ((st = upload(&tl->d_small_mask, hp.small_mask))) return st;
     if (device_items) {
+        const double t0 = now_ms();
+        tl->plan_ms[1] += t0 - tl->plan_ms[3];  // uploads since the host plan
         if ((st = plan_device(tl, hp, nullptr))) return st;
+        tl->plan_ms[2] = now_ms() - t0;
+        tl->plan_ms[3] = now_ms();
     } else {
         if ((st = upload(&tl->d_tile_start, hp.tile_start))) return st;
         if ((st = upload(&tl->d_recs, hp.recs))) return st;
@@ -980,7 +1074,7 @@ static gpw_status upload_tasklist(gpw_tasklist *tl, HostPlan &hp, bool device_it
     HIP_TRY(hipMalloc((void **)&tl->d_partial, sizeof(double) * std::max<long long>(hp.partial_total, 1)));
     HIP_TRY(hipMalloc((void **)&tl->d_Mbuf,
                       sizeof(double) * std::max<long long>((long long)tl->npairs * tl->mstride, 1)));
-    if (tl->splits > 1) {
+    if (tl->splits > 1 && !tl->d_cpartial) {  // (a re-plan keeps its partial grids)
         const long long ng = (long long)tl->tg.nx * tl->tg.ny * tl->tg.nz;
         HIP_TRY(hipMalloc((void **)&tl->d_cpartial, sizeof(double) * ng * tl->splits));
         tl->cpart_cap = tl->splits;
@@ -1015,9 +1109,14 @@ static gpw_status finish_tasklist(gpw_tasklist *tl, const std::vector<gpw_pair_d
                                   const std::vector<PairGeom> &geos,
                                   const std::vector<Profile> &profiles, int splits) {
     HostPlan hp;
+    const double t0 = now_ms();
     gpw_status st = plan_common(tl, pairs, geos, profiles, splits, hp);
     if (st) return st;
-    return upload_tasklist(tl, hp, true);
+    tl->plan_ms[0] += now_ms() - t0;
+    tl->plan_ms[3] = now_ms();  // start of the uploads (upload_tasklist books them)
+    if ((st = upload_tasklist(tl, hp, true))) return st;
+    tl->plan_ms[3] = now_ms() - tl->plan_ms[3];  // buffers after the device items
+    return GPW_OK;
 }
 
 gpw_status gpw::check_grid(const gpw_grid_desc *g) {
@@ -1042,36 +1141,62 @@ static gpw_status radius_geometry(const gpw_grid_desc *grid, int npairs, const g
                                   int tau, std::vector<PairGeom> &geos,
                                   std::vector<Profile> &profiles) {
     geos.assign(npairs, PairGeom{});
-    std::map<long long, int> prof_cache;  // imr -> profile index
     const double dr[3] = {grid->dh[0][0], grid->dh[1][1], grid->dh[2][2]};
     const double drmin = std::min(dr[0], std::min(dr[1], dr[2]));
-    gpw_status st;
-    for (int i = 0; i < npairs; i++) {
-        if ((st = check_pair(pairs[i], tau))) return st;
-        if (!(pairs[i].radius > 0.0)) return fail(GPW_ERR_ARG, "radius must be > 0");
-        const PairDev pd = make_pair_dev(pairs[i], tau);
-        PairGeom &geo = geos[i];
-        if (!is_ortho(*grid)) {
-            general_geometry(*grid, pd.rp, pairs[i].radius, geo);
-            continue;
+    const bool ortho = is_ortho(*grid);
+    // per pair (threads): checks, cube centre / lattice box, profile key imr
+    std::vector<long long> imr(ortho ? npairs : 0);
+    std::vector<int> bad(16, -1);
+    std::vector<std::string> why(16);
+    std::vector<gpw_status> code(16, GPW_OK);
+    parallel_chunks((size_t)npairs, [&](size_t b, size_t e, unsigned t) {
+        for (size_t i = b; i < e; i++) {
+            gpw_status st = check_pair(pairs[i], tau);
+            if (!st && !(pairs[i].radius > 0.0)) st = fail(GPW_ERR_ARG, "radius must be > 0");
+            if (st) {
+                bad[t] = (int)i;
+                code[t] = st;
+                why[t] = gpw_last_error();
+                return;
+            }
+            const PairDev pd = make_pair_dev(pairs[i], tau);
+            PairGeom &geo = geos[i];
+            if (!ortho) {
+                general_geometry(*grid, pd.rp, pairs[i].radius, geo);
+                continue;
+            }
+            cube_centre(*grid, pd.rp, geo.cc, geo.roff);
+            long long m = (long long)std::ceil(pairs[i].radius / drmin);
+            imr[i] = m < 1 ? 1 : m;
         }
-        cube_centre(*grid, pd.rp, geo.cc, geo.roff);
-        long long imr = (long long)std::ceil(pairs[i].radius / drmin);
-        if (imr < 1) imr = 1;
-        auto it = prof_cache.find(imr);
+    });
+    for (int t = 0; t < 16; t++)  // the first failing chunk in pair order
+        if (bad[t] >= 0) return fail(code[t], why[t]);
+    if (!ortho) return GPW_OK;
+    // one profile per imr, numbered in order of first appearance
+    std::map<long long, int> prof_cache;
+    std::vector<int> prof_of(npairs);
+    for (int i = 0; i < npairs; i++) {
+        auto it = prof_cache.find(imr[i]);
         if (it == prof_cache.end()) {
             Profile P;
             int lb[3], ub[3];
             profile_from_radius(pairs[i].radius, dr, P, lb, ub);
             profiles.push_back(std::move(P));
-            it = prof_cache.emplace(imr, (int)profiles.size() - 1).first;
+            it = prof_cache.emplace(imr[i], (int)profiles.size() - 1).first;
         }
-        geo.prof = it->second;
-        for (int d = 0; d < 3; d++) {
-            geo.lb[d] = -profiles[geo.prof].n[d];
-            geo.ub[d] = 1 + profiles[geo.prof].n[d];
-        }
+        prof_of[i] = it->second;
     }
+    parallel_chunks((size_t)npairs, [&](size_t b, size_t e, unsigned) {
+        for (size_t i = b; i < e; i++) {
+            PairGeom &geo = geos[i];
+            geo.prof = prof_of[i];
+            for (int d = 0; d < 3; d++) {
+                geo.lb[d] = -profiles[geo.prof].n[d];
+                geo.ub[d] = 1 + profiles[geo.prof].n[d];
+            }
+        }
+    });
     return GPW_OK;
 }
 
@@ -1091,12 +1216,46 @@ gpw_status gpw_tasklist_create(const gpw_grid_desc *grid, int npairs, const gpw_
     std::vector<gpw_pair_desc> pv(pairs, pairs + npairs);
     std::vector<PairGeom> geos;
     std::vector<Profile> profiles;
-    if ((st = radius_geometry(grid, npairs, pairs, tl->compute_tau, geos, profiles)) ||
-        (st = finish_tasklist(tl, pv, geos, profiles, splits))) {
+    const double t0 = now_ms();
+    st = radius_geometry(grid, npairs, pairs, tl->compute_tau, geos, profiles);
+    tl->plan_ms[0] = now_ms() - t0;
+    if (st || (st = finish_tasklist(tl, pv, geos, profiles, splits))) {
         delete tl;
         return st;
     }
     *out = tl;
+    return GPW_OK;
+}
+
+gpw_status gpw_tasklist_update(gpw_tasklist *tl, int npairs, const gpw_pair_desc *pairs) {
+    if (!tl || npairs != tl->npairs || (npairs > 0 && !pairs))
+        return fail(GPW_ERR_ARG, "update needs the task list's number of pairs");
+    // the new plan is built on the host first; device buffers are replaced
+    // only once it is valid
+    std::vector<PairGeom> geos;
+    std::vector<Profile> profiles;
+    const double t0 = now_ms();
+    gpw_status st = radius_geometry(&tl->grid, npairs, pairs, tl->compute_tau, geos, profiles);
+    if (st) return st;
+    int csplit[NCLS], isplit[2][NCLS];
+    std::copy(tl->csplit, tl->csplit + NCLS, csplit);
+    std::copy(&tl->isplit[0][0], &tl->isplit[0][0] + 2 * NCLS, &isplit[0][0]);
+    HIP_TRY(hipDeviceSynchronize());  // earlier calls may still read the old plan
+    tl->release_plan();
+    tl->lp_max = tl->la_max = tl->lb_max = 0;
+    for (int a = 0; a < NSET; a++)
+        for (int c = 0; c < NCLS; c++) tl->class_entries[a][c] = tl->list_off[a][c] = 0;
+    tl->plan_ms[0] = now_ms() - t0;
+    tl->plan_ms[1] = tl->plan_ms[2] = tl->plan_ms[3] = 0.0;
+    std::vector<gpw_pair_desc> pv(pairs, pairs + npairs);
+    if ((st = finish_tasklist(tl, pv, geos, profiles, tl->splits))) {
+        tl->release_plan();
+        tl->npairs = 0;  // unusable until a successful update
+        return st;
+    }
+    // keep the measured splits (the partial grids were kept for them)
+    std::copy(csplit, csplit + NCLS, tl->csplit);
+    std::copy(&isplit[0][0], &isplit[0][0] + 2 * NCLS, &tl->isplit[0][0]);
     return GPW_OK;
 }
 
@@ -1181,6 +1340,7 @@ gpw_status gpw_tasklist_stats_get(const gpw_tasklist *tl, gpw_tasklist_stats *o)
     o->flops_collocate = tl->flops_collocate;
     o->flops_integrate[0] = tl->flops_integrate[0];
     o->flops_integrate[1] = tl->flops_integrate[1];
+    for (int i = 0; i < 4; i++) o->plan_ms[i] = tl->plan_ms[i];
     for (int c = 0; c < NCLS; c++) {
         o->csplits[c] = tl->csplit[c];
         o->isplits[0][c] = tl->isplit[0][c];

@@ -354,3 +354,43 @@ def test_device_planner_matches_host_planner(gpw, torch_cuda, which):
               "flops_collocate"):
         assert devs[k] == host[k], (k, devs[k], host[k])
     assert devs["flops_integrate"] == host["flops_integrate"]
+
+
+def test_collocator_replans_in_place(gpw, oracle, torch_cuda):
+    """A Collocator keeps one task list (tuned splits included) while the
+    centres move: each call matches the oracle at its own centres (grid,
+    dL/dpab, forces), a call at unchanged centres does not re-plan."""
+    from libgpw_amd import workloads as Wk
+    from libgpw_amd.autograd import Collocator
+    torch = torch_cuda
+    cfg = Wk.subset(Wk.config(3), np.arange(600))
+    p = cfg["pairs"]
+    W = Wk.potential(cfg)
+    Wd = dev(torch, W)
+    col = Collocator(p, cfg["npts"], cfg["dh"], splits=2, autotune=True)
+    rng = np.random.default_rng(3)
+    ra0, rb0 = p["ra"].copy(), p["ra"] + p["rab"]
+    splits0 = None
+    for step in range(3):
+        shift = rng.normal(0.0, 0.05, ra0.shape) if step else 0.0
+        ra_h, rb_h = ra0 + shift, rb0 + shift
+        say("collocator step", step)
+        pab = dev(torch, cfg["pab"]).requires_grad_(True)
+        ra = torch.tensor(ra_h, dtype=torch.float64, requires_grad=True)
+        rb = torch.tensor(rb_h, dtype=torch.float64, requires_grad=True)
+        grid = col(pab, ra, rb)
+        gpab, gra, grb = torch.autograd.grad((grid * Wd).sum(), (pab, ra, rb))
+        moved = dict(cfg, pairs=dict(p, ra=ra_h, rab=rb_h - ra_h))
+        ref = oracle_batch(oracle, moved, W=W, forces=True)
+        assert rel_err(grid.detach().cpu().numpy(), ref["grid"]) < TOL, step
+        assert rel_err(gpab.cpu().numpy(), ref["hab"]) < TOL, step
+        f = np.concatenate([gra.numpy(), grb.numpy()], axis=1).reshape(-1, 2, 3)
+        assert rel_err(f, ref["forces"]) < TOL, step
+        st = col._tl.stats()
+        if splits0 is None:
+            splits0 = (st["csplits"], st["isplits"])
+        assert (st["csplits"], st["isplits"]) == splits0
+    assert col.replans == 2
+    grid = col(dev(torch, cfg["pab"]), torch.tensor(ra_h), torch.tensor(rb_h))
+    assert col.replans == 2
+    col.close()
