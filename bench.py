@@ -147,6 +147,57 @@ def cpu_leg(O, legs, nthreads, budget_s=10.0):
     return (time.perf_counter() - t0) / reps, outs, reps
 
 
+def dropin_per_call(G, torch, cfg, V_h, n):
+    """Per-call time of the single-task entry points (the reference's
+    grid_collocate_pgf_product_rspace argument list) on the first n pairs:
+    host arrays (only the cube's points cross PCIe) and device arrays, each
+    a collocate then an integrate with forces per pair."""
+    import math
+    p = cfg["pairs"]
+    g = G.grid_desc(cfg["npts"], cfg["dh"])
+    npts = list(cfg["npts"])
+    tasks = []
+    for i in range(n):
+        geo = G.pair_geometry(g, p, i)
+        la, lb = int(p["la_max"][i]), int(p["lb_max"][i])
+        maxco = G.ncoset(max(la, lb))
+        t = {"compute_tau": False, "use_ortho": True, "la_max": la, "la_min": int(p["la_min"][i]),
+             "lb_max": lb, "lb_min": int(p["lb_min"][i]), "zeta": float(p["zeta"][i]),
+             "zetb": float(p["zetb"][i]), "rscale": float(p["rscale"][i]), "dh": cfg["dh"],
+             "dh_inv": np.linalg.inv(cfg["dh"]), "ra": p["ra"][i], "rab": p["rab"][i], "npts": npts,
+             "ngrid": npts, "lb_grid": [0, 0, 0], "periodic": [1, 1, 1], "lmax": max(la, lb),
+             "radius": float(p["radius"][i]), "lb_cube": geo["lb_cube"], "ub_cube": geo["ub_cube"],
+             "sphere_bounds": geo["sphere_bounds"], "nspheres": len(geo["sphere_bounds"]),
+             "maxco": maxco, "o1": 0, "o2": 0}
+        tasks.append(t)
+    rng = np.random.default_rng(3)
+    pabs = [rng.uniform(-1, 1, (t["maxco"], t["maxco"])) for t in tasks]
+    n3 = (npts[2], npts[1], npts[0])
+    grid_h = np.zeros(n3)
+    out = {"pairs": n, "sample": f"first {n} pairs of the workload, collocate + integrate with forces"}
+    for mode in ("host", "device"):
+        if mode == "device":
+            gd = torch.zeros(n3, dtype=torch.float64, device="cuda")
+            Vd = torch.from_numpy(V_h).cuda()
+            pds = [torch.from_numpy(q).cuda() for q in pabs]
+            hds = [torch.zeros_like(q) for q in pds]
+            fd = torch.zeros(6, dtype=torch.float64, device="cuda")
+        for rep in range(2):  # the first pass grows the library's workspace
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for t, q, k in zip(tasks, pabs, range(n)):
+                if mode == "host":
+                    G.collocate_pgf_product_rspace(t, q, grid_h)
+                    G.integrate_pgf_product_rspace(t, V_h, pab=q, forces=True)
+                else:
+                    G.collocate_pgf_product_rspace_dev(t, pds[k], gd)
+                    G.integrate_pgf_product_rspace_dev(t, Vd, hds[k], pab=pds[k], forces=fd)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+        out[f"{mode}_arrays_us_per_pair"] = round(dt / n * 1e6, 1)
+    return out
+
+
 def rel(a, b):
     s = np.abs(b).max()
     return float(np.abs(a - b).max() / s) if s > 0 else float(np.abs(a).max())
@@ -214,7 +265,8 @@ def run_single(args, torch, G, Wk, dev, cfg):
                       "collocate_splits": st["csplits"],
                       "integrate_splits": st["isplits"][1 if with_forces else 0],
                       "tile": st["tile"], "sub_boxes": st["nsub"],
-                      "work_items": st["nentries"], "sphere_points": pts, "plan_s": round(plan_s, 3),
+                      "work_items": st["nentries"], "sphere_points": pts, "plan_s": round(plan_s, 4),
+                      "plan_ms": {k: round(v, 2) for k, v in st["plan_ms"].items()},
                       "phase_ms": {k: round(v, 4) for k, v in phase_ms.items()}},
            "roofline": roof,
            # one pass including the host + device planning of the task list
@@ -245,6 +297,10 @@ def run_single(args, torch, G, Wk, dev, cfg):
                                       "median of 3 steps",
                             "speedup_batched": round(value / plain_v, 2)}
         tlp.close()
+
+    # ---- the reference's one-call-per-pair interface on the same pairs
+    if not args.no_plain and cfg["index"] != 5:
+        out["single_task"] = dropin_per_call(G, torch, cfg, V_h, min(200, npairs))
 
     # ---- CPU oracle baseline on a bounded sample + error of the GPU result
     if not args.no_cpu:
