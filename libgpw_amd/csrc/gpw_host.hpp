@@ -125,7 +125,7 @@ hipError_t launch_integrate_folded(int L, const double *P, int ldp, const int *m
 hipError_t launch_plan_count(const SubDev *subs, int nsub, const int *prof, const TileGrid &tg, int *counts,
                              unsigned long long *work, hipStream_t st);
 hipError_t launch_plan_entries(const SubDev *subs, int nsub, const long long *ent_start, const long long *base0,
-                               const long long *base2, const TileGrid &tg, int ntiles,
+                               const long long *base2, const int *prof, const TileGrid &tg, int ntiles,
                                const unsigned char *small_mask, int npairs, int ne, int *ent_sub,
                                long long *off0, long long *off2, unsigned *keys, int *vals, int *hist,
                                hipStream_t st);

@@ -132,4 +132,54 @@ struct TileGrid {
     double dh[3][3];         // step vectors (rows), used by the general path
 };
 
+// a mod m in [0, m) (m > 0)
+__host__ __device__ inline int imod_hd(int a, int m) {
+    const int r = a % m;
+    return r < 0 ? r + m : r;
+}
+
+// Offsets of sub-box s along axis d covered by the grid indices g0..g1 of one
+// tile: at most two ascending runs a[r]..b[r] (empty when a[r] > b[r]), the
+// second one after a periodic wrap, each clipped to [lb, ub].
+__host__ __device__ inline void tile_offset_runs(const SubDev &s, int d, int g0, int g1, int (&a)[2],
+                                                 int (&b)[2]) {
+    const int P = s.period[d];
+    const int s0 = imod_hd(g0 - s.base[d], P);
+    const int last = s0 + (g1 - g0);  // < P + TX: no overflow (P <= BIG)
+    a[0] = s.lb[d] + s0;
+    b[0] = s.lb[d] + (last < P ? last : P - 1);
+    a[1] = s.lb[d];
+    b[1] = last >= P ? s.lb[d] + (last - P) : s.lb[d] - 1;
+    for (int r = 0; r < 2; r++) b[r] = b[r] < s.ub[d] ? b[r] : s.ub[d];
+}
+
+// Whether the sphere of sub-box s has a point in tile (tx, ty, tz): a (sub-box,
+// tile) pair without one is no work item. row(j, k, lo, hi) gives the x
+// offsets lo..hi of row (j, k) inside the sphere (false: none) by the same
+// rule the tile kernels apply point by point, so host and device planners
+// and the kernels agree on every item.
+template <class Row>
+__host__ __device__ inline bool item_has_points(const SubDev &s, const TileGrid &tg, int tx, int ty, int tz,
+                                                Row row) {
+    int xa[2], xb[2];
+    const int gx1 = (tx + 1) * TX < tg.nx ? (tx + 1) * TX : tg.nx;
+    tile_offset_runs(s, 0, tx * TX, gx1 - 1, xa, xb);
+    if (xa[0] > xb[0] && xa[1] > xb[1]) return false;
+    const int gz1 = (tz + 1) * TZ < tg.nz ? (tz + 1) * TZ : tg.nz;
+    const int gy1 = (ty + 1) * TY < tg.ny ? (ty + 1) * TY : tg.ny;
+    for (int gz = tz * TZ; gz < gz1; gz++) {
+        const int k = s.lb[2] + imod_hd(gz - s.base[2], s.period[2]);
+        if (k > s.ub[2]) continue;
+        for (int gy = ty * TY; gy < gy1; gy++) {
+            const int j = s.lb[1] + imod_hd(gy - s.base[1], s.period[1]);
+            if (j > s.ub[1]) continue;
+            int lo, hi;
+            if (!row(j, k, lo, hi)) continue;
+            for (int r = 0; r < 2; r++)
+                if ((lo > xa[r] ? lo : xa[r]) <= (hi < xb[r] ? hi : xb[r])) return true;
+        }
+    }
+    return false;
+}
+
 }  // namespace gpw

@@ -1898,9 +1898,34 @@ __device__ __forceinline__ int plan_axis_tile(const SubDev &s, int d, const Tile
     return k < n1 ? r0 + k : q0 + (k - n1);
 }
 
-// One wave per sub-box: its number of (sub-box, tile) items, and its sphere
-// points, rows and z-planes added to the per-lp counters work[3 lp + 0..2]
-// (the algorithmic work of the stats; integer sums, so order-independent).
+// Candidate c of sub-box s (tiles its index box touches, x fastest): its
+// tile coordinates and whether the sphere has a point in it (item_has_points
+// with the rows of the tile kernels: the general quadratic bounds, or the
+// orthorhombic profile threshold, fold offsets <= T <=> -T <= i <= T + 1).
+__device__ __forceinline__ bool plan_candidate(const SubDev &s, const TileGrid &tg, const int *__restrict__ prof,
+                                               int c, int nxs, int nys, int &tile) {
+    const int tx = plan_axis_tile(s, 0, tg, c % nxs);
+    const int ty = plan_axis_tile(s, 1, tg, (c / nxs) % nys);
+    const int tz = plan_axis_tile(s, 2, tg, c / (nxs * nys));
+    tile = (tz * tg.nty + ty) * tg.ntx + tx;
+    return item_has_points(s, tg, tx, ty, tz, [&](int j, int k, int &lo, int &hi) {
+        if (tg.general) {
+            double qb, qc;
+            general_row_dev(tg, s.roff[0], (double)j - s.roff[1], (double)k - s.roff[2], s.R2, s.zetp, lo, hi,
+                            qb, qc);
+            return lo <= hi;
+        }
+        const int T = row_threshold(prof, s.prof_off, s.prof_stride, fold_off(k), fold_off(j));
+        lo = -T;
+        hi = T + 1;
+        return T >= 0;
+    });
+}
+
+// One wave per sub-box: its number of (sub-box, tile) items -- the tiles
+// that hold a point of its sphere -- and its sphere points, rows and
+// z-planes added to the per-lp counters work[3 lp + 0..2] (the algorithmic
+// work of the stats; integer sums, so order-independent).
 __global__ __launch_bounds__(256) void k_plan_count(const SubDev *__restrict__ subs, int nsub,
                                                     const int *__restrict__ prof, TileGrid tg,
                                                     int *__restrict__ counts,
@@ -1909,8 +1934,15 @@ __global__ __launch_bounds__(256) void k_plan_count(const SubDev *__restrict__ s
     const long long si = (long long)blockIdx.x * 4 + wave;
     if (si >= nsub) return;
     const SubDev s = subs[si];
-    if (lane == 0)
-        counts[si] = plan_axis_count(s, 0, tg) * plan_axis_count(s, 1, tg) * plan_axis_count(s, 2, tg);
+    {
+        const int nxs = plan_axis_count(s, 0, tg), nys = plan_axis_count(s, 1, tg);
+        const int ncand = nxs * nys * plan_axis_count(s, 2, tg);
+        int cnt = 0, tile;
+        for (int c = lane; c < ncand; c += 64) cnt += plan_candidate(s, tg, prof, c, nxs, nys, tile) ? 1 : 0;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+        if (lane == 0) counts[si] = cnt;
+    }
     unsigned long long pts = 0, rows = 0, planes = 0;
     for (int k = s.lb[2]; k <= s.ub[2]; k++) {
         int rows_k = 0;
@@ -1948,48 +1980,51 @@ __global__ __launch_bounds__(256) void k_plan_count(const SubDev *__restrict__ s
     }
 }
 
-// One thread per item e (items numbered sub-box-major, sub-boxes pair-major):
-// its sub-box (binary search in ent_start), tile, partial-moment offsets,
-// and its sort key per tile-list set, class * ntiles + tile, or NCLS *
-// ntiles when the set leaves it out (pairs of the small-sphere integrate
-// kernel); hist counts the items per set and key.
-__global__ void k_plan_entries(const SubDev *__restrict__ subs, int nsub,
+// One wave per sub-box: its items e = ent_start[sub] + k (k-th non-empty
+// candidate in candidate order; sub-boxes are pair-major), each with its
+// sub-box, partial-moment offsets and sort key per tile-list set, class *
+// ntiles + tile, or NCLS * ntiles when the set leaves it out (pairs of the
+// small-sphere integrate kernel); hist counts the items per set and key.
+__global__ __launch_bounds__(256) void k_plan_entries(const SubDev *__restrict__ subs, int nsub,
                                const long long *__restrict__ ent_start, const long long *__restrict__ base0,
-                               const long long *__restrict__ base2, TileGrid tg, int ntiles,
-                               const unsigned char *__restrict__ small_mask, int npairs, int ne,
+                               const long long *__restrict__ base2, const int *__restrict__ prof, TileGrid tg,
+                               int ntiles, const unsigned char *__restrict__ small_mask, int npairs, int ne,
                                int *__restrict__ ent_sub, long long *__restrict__ off0,
                                long long *__restrict__ off2, unsigned *__restrict__ keys,
                                int *__restrict__ vals, int *__restrict__ hist) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= ne) return;
-    int lo = 0, hi = nsub - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (ent_start[mid] <= e)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    const SubDev sd = subs[lo];
-    const int k = (int)(e - ent_start[lo]);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int si = blockIdx.x * 4 + wave;
+    if (si >= nsub) return;
+    const SubDev sd = subs[si];
     const int nxs = plan_axis_count(sd, 0, tg), nys = plan_axis_count(sd, 1, tg);
-    const int tx = plan_axis_tile(sd, 0, tg, k % nxs);
-    const int ty = plan_axis_tile(sd, 1, tg, (k / nxs) % nys);
-    const int tz = plan_axis_tile(sd, 2, tg, k / (nxs * nys));
-    const int tile = (tz * tg.nty + ty) * tg.ntx + tx;
-    ent_sub[e] = lo;
-    off0[e] = base0[lo] + (long long)k * ncoset(sd.lp);
-    off2[e] = base2[lo] + (long long)k * ncoset(sd.lp + 2);
-    vals[e] = e;
+    const int ncand = nxs * nys * plan_axis_count(sd, 2, tg);
+    const long long e0 = ent_start[si];
     const unsigned sent = (unsigned)(NCLS * ntiles);
-    const unsigned k0 = (unsigned)(class_of(sd.lp) * ntiles + tile);
-    const unsigned k1 = small_mask[npairs + sd.pair] ? sent : (unsigned)(class_of(sd.lp + 2) * ntiles + tile);
-    const unsigned k2 = small_mask[sd.pair] ? sent : k0;
-    const unsigned kk[3] = {k0, k1, k2};
+    const bool skip1 = small_mask[npairs + sd.pair] != 0, skip2 = small_mask[sd.pair] != 0;
+    int done = 0;
+    for (int c0 = 0; c0 < ncand; c0 += 64) {
+        const int c = c0 + lane;
+        int tile = 0;
+        const bool pass = c < ncand && plan_candidate(sd, tg, prof, c, nxs, nys, tile);
+        const unsigned long long m = __ballot(pass);
+        if (pass) {
+            const int k = done + __popcll(m & ((1ULL << lane) - 1ULL));
+            const long long e = e0 + k;
+            ent_sub[e] = si;
+            off0[e] = base0[si] + (long long)k * ncoset(sd.lp);
+            off2[e] = base2[si] + (long long)k * ncoset(sd.lp + 2);
+            vals[e] = (int)e;
+            const unsigned k0 = (unsigned)(class_of(sd.lp) * ntiles + tile);
+            const unsigned k1 = skip1 ? sent : (unsigned)(class_of(sd.lp + 2) * ntiles + tile);
+            const unsigned k2 = skip2 ? sent : k0;
+            const unsigned kk[3] = {k0, k1, k2};
 #pragma unroll
-    for (int a = 0; a < NSET; a++) {
-        keys[(long long)a * ne + e] = kk[a];
-        if (kk[a] != sent) atomicAdd(&hist[(long long)a * NCLS * ntiles + kk[a]], 1);
+            for (int a = 0; a < NSET; a++) {
+                keys[(long long)a * ne + e] = kk[a];
+                if (kk[a] != sent) atomicAdd(&hist[(long long)a * NCLS * ntiles + kk[a]], 1);
+            }
+        }
+        done += __popcll(m);
     }
 }
 
@@ -2344,13 +2379,13 @@ hipError_t launch_plan_count(const SubDev *subs, int nsub, const int *prof, cons
 }
 
 hipError_t launch_plan_entries(const SubDev *subs, int nsub, const long long *ent_start, const long long *base0,
-                               const long long *base2, const TileGrid &tg, int ntiles,
+                               const long long *base2, const int *prof, const TileGrid &tg, int ntiles,
                                const unsigned char *small_mask, int npairs, int ne, int *ent_sub,
                                long long *off0, long long *off2, unsigned *keys, int *vals, int *hist,
                                hipStream_t st) {
-    if (ne <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_plan_entries, dim3((ne + 255) / 256), dim3(256), 0, st, subs, nsub, ent_start, base0,
-                       base2, tg, ntiles, small_mask, npairs, ne, ent_sub, off0, off2, keys, vals, hist);
+    if (ne <= 0 || nsub <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_plan_entries, dim3((nsub + 3) / 4), dim3(256), 0, st, subs, nsub, ent_start, base0,
+                       base2, prof, tg, ntiles, small_mask, npairs, ne, ent_sub, off0, off2, keys, vals, hist);
     return hipGetLastError();
 }
 

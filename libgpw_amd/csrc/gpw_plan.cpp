@@ -473,8 +473,11 @@ static SubWork sub_points_general(const gpw_grid_desc &g, const SubDev &s) {
     return w;
 }
 
+// fold offset of the sphere profiles: |i| for i <= 0, i - 1 above
+static int fold_off_h(int i) { return i <= 0 ? -i : i - 1; }
+
 static SubWork sub_points(const SubDev &s, const std::vector<int> &prof) {
-    auto fo = [](int i) { return i <= 0 ? -i : i - 1; };
+    auto fo = fold_off_h;
     SubWork w;
     for (int k = s.lb[2]; k <= s.ub[2]; k++) {
         const long long rows0 = w.rows;
@@ -749,9 +752,19 @@ static gpw_status plan_entries_host(gpw_tasklist *tl, HostPlan &hp) {
         axis_tiles(s, 0, tg.nx, TX, tg.ntx, tx);
         axis_tiles(s, 1, tg.ny, TY, tg.nty, ty);
         axis_tiles(s, 2, tg.nz, TZ, tg.ntz, tz);
+        // the tiles that hold a point of the sub-box's sphere (the device
+        // planner's test, with the host's row bounds)
+        auto row = [&](int j, int k, int &lo, int &hi) {
+            if (tg.general) return general_row_host(g, s.roff[0], j - s.roff[1], k - s.roff[2], s.R2, lo, hi) && lo <= hi;
+            const int T = s.prof_off < 0 ? BIG : hp.prof_flat[s.prof_off + fold_off_h(k) * s.prof_stride + fold_off_h(j)];
+            lo = -T;
+            hi = T + 1;
+            return T >= 0;
+        };
         for (int a : tz)
             for (int b : ty)
                 for (int c : tx) {
+                    if (!item_has_points(s, tg, c, b, a, row)) continue;
                     ent_tile.push_back((a * tg.nty + b) * tg.ntx + c);
                     hp.ent_sub.push_back((int)si);
                     hp.pe_start[s.pair + 1]++;
@@ -921,7 +934,8 @@ struct DevTemp {
 // Device planner of the work items, from the uploaded sub-boxes: the same
 // items, partial-moment offsets, per-(set, class) tile CSR and records as
 // plan_entries_host, with the same order of items within each tile.
-//  1. k_plan_count: items per sub-box and the algorithmic work (stats);
+//  1. k_plan_count: items per sub-box (tiles holding a point of its sphere)
+//     and the algorithmic work (stats);
 //  2. host: prefix sums over sub-boxes (item starts, partial offsets);
 //  3. k_plan_entries: each item's sub-box, tile, offsets and sort keys
 //     (class * ntiles + tile per set), with per-key counts -> tile CSR;
@@ -1002,7 +1016,7 @@ static gpw_status plan_device(gpw_tasklist *tl, HostPlan &hp, hipStream_t st) {
         HIP_TRY(hipMemcpyAsync(d_b2, base2.data(), nsub * sizeof(long long), hipMemcpyHostToDevice, st));
     }
     HIP_TRY(hipMemsetAsync(d_hist, 0, NSET * nkey * sizeof(int), st));
-    HIP_TRY(launch_plan_entries(tl->d_subs, nsub, d_es, d_b0, d_b2, tl->tg, ntiles, tl->d_small_mask, npairs, n,
+    HIP_TRY(launch_plan_entries(tl->d_subs, nsub, d_es, d_b0, d_b2, tl->d_prof, tl->tg, ntiles, tl->d_small_mask, npairs, n,
                                 d_ent_sub, tl->d_ent_off0, tl->d_ent_off2, d_keys, d_vals, d_hist, st));
     std::vector<int> hist(NSET * nkey);
     HIP_TRY(hipMemcpyAsync(hist.data(), d_hist, NSET * nkey * sizeof(int), hipMemcpyDeviceToHost, st));
