@@ -118,9 +118,11 @@ struct EntryRec {
     int lb[3], ub[3];
     int lp;              // product l (without the force extension)
     int coef_off;
+    int prof_off;        // orthorhombic: the sub-box's sphere profile (see SubDev)
+    int prof_stride;
     int pad;
 };
-static_assert(sizeof(EntryRec) == 96, "EntryRec layout");
+static_assert(sizeof(EntryRec) == 104, "EntryRec layout");
 
 // Tile-kernel launch description.
 struct TileGrid {
@@ -130,6 +132,7 @@ struct TileGrid {
     int general;             // non-orthorhombic cell (lattice-coordinate path)
     double dr[3];
     double dh[3][3];         // step vectors (rows), used by the general path
+    const int *prof;         // sphere profiles of the sub-boxes (device; orthorhombic)
 };
 
 // a mod m in [0, m) (m > 0)

@@ -325,42 +325,6 @@ struct TileCfg {
 // are fused into FMAs after inlining, which moves points that lie exactly on
 // a sphere or row bound in or out).
 
-// Largest n >= 0 with b + (n h)^2 <= R2, evaluated with the same IEEE
-// operations (no contraction) as the host rule that generates the cube.
-__device__ __forceinline__ int sphere_extent_dev(double b, double h, double R2) {
-#pragma clang fp contract(off)
-    const double rem = R2 - b;
-    // initial guess only; the exact predicate below decides
-    int n = rem > 0.0 ? (int)(sqrtf((float)rem) / (float)h) : 0;
-    while (n > 0) {
-        const double x = (double)n * h;
-        if (b + x * x <= R2) break;
-        n--;
-    }
-    for (;;) {
-        const double x = (double)(n + 1) * h;
-        if (b + x * x <= R2)
-            n++;
-        else
-            return n;
-    }
-}
-
-// Sphere threshold of row (kk, jj) (fold offsets): x offsets with
-// fold <= T are inside; -1 if the row is empty. Matches the profile the
-// host planner builds for the reference's sphere_bounds layout.
-__device__ __forceinline__ int row_T(int kk, int jj, const TileGrid &tg, double R2) {
-#pragma clang fp contract(off)
-    if (kk < 0 || jj < 0) return -1;
-    const double zk = (double)kk * tg.dr[2];
-    const double base = zk * zk;
-    if (kk > 0 && base > R2) return -1;
-    const double yj = (double)jj * tg.dr[1];
-    const double b2 = base + yj * yj;
-    if (jj > 0 && b2 > R2) return -1;
-    return sphere_extent_dev(b2, tg.dr[0], R2);
-}
-
 // General cells: i-range of lattice row (j, k) inside the sphere, with the
 // operation order of grid_base_ref_c.cc:790-811 and no contraction, so the
 // index set equals the host's and the oracle's; qb, qc: the row's quadratic
@@ -498,7 +462,9 @@ __device__ __forceinline__ void tile_phase1(const EntryRec &R, int q, int lane, 
             const int row = lane + 64 * rr;
             const int kk = __shfl(code, TX + TY + row / TY);
             const int jj = __shfl(code, TX + row % TY);
-            T[row] = row_T(kk, jj, tg, R.R2);
+            T[row] = (kk < 0 || jj < 0) ? -1
+                     : R.prof_off < 0   ? BIG
+                                        : tg.prof[R.prof_off + kk * R.prof_stride + jj];
         }
     } else {
         // x blocks: min / max lattice index of the valid points, all valid?
@@ -671,15 +637,22 @@ __device__ __forceinline__ void colloc_block(const CollocSmem<LM, GEN> &sm, int 
     double cx[RB][L];
     const double *cxy = sm.cxy[q][z];
 #pragma unroll
-    for (int r = 0; r < RB; r++) {
-        const double *Yr = sm.Y[q][yb * RB + r];
+    for (int r = 0; r < RB; r++)
 #pragma unroll
         for (int l = 0; l < L; l++) cx[r][l] = 0.0;
+    // y stage, rows innermost: each cxy value is loaded once and used by its
+    // RB FMAs at once (a row-outer order keeps every cxy value live across
+    // the rows and spills registers)
 #pragma unroll
-        for (int lyp = 0; lyp <= lp; lyp++) {
-            const double y = Yr[lyp];
+    for (int lyp = 0; lyp <= lp; lyp++) {
+        double y[RB];
 #pragma unroll
-            for (int lxp = 0; lxp <= lp - lyp; lxp++) cx[r][lxp] += cxy[c2idx(lp, lyp, lxp)] * y;
+        for (int r = 0; r < RB; r++) y[r] = sm.Y[q][yb * RB + r][lyp];
+#pragma unroll
+        for (int lxp = 0; lxp <= lp - lyp; lxp++) {
+            const double c = cxy[c2idx(lp, lyp, lxp)];
+#pragma unroll
+            for (int r = 0; r < RB; r++) cx[r][lxp] += c * y[r];
         }
     }
     const double *Xg = sm.X[q] + xg * TileCfg<LM>::GS;
@@ -2049,6 +2022,8 @@ __global__ void k_plan_records(const SubDev *__restrict__ subs, const int *__res
     r.ent_off = off[e];
     r.lp = sd.lp;
     r.coef_off = sd.coef_off;
+    r.prof_off = sd.prof_off;
+    r.prof_stride = sd.prof_stride;
     r.pad = 0;
     recs[p] = r;
 }

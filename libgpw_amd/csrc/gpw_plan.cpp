@@ -831,6 +831,8 @@ static gpw_status plan_entries_host(gpw_tasklist *tl, HostPlan &hp) {
                 r.ent_off = a == 1 ? hp.off2[e] : hp.off0[e];
                 r.lp = sd.lp;
                 r.coef_off = sd.coef_off;
+                r.prof_off = sd.prof_off;
+                r.prof_stride = sd.prof_stride;
                 list[fillp[ent_tile[e]]++] = r;
             }
         }
@@ -1068,6 +1070,7 @@ static gpw_status upload_tasklist(gpw_tasklist *tl, HostPlan &hp, bool device_it
     if ((st = upload(&tl->d_segs, segs))) return st;
     if ((st = upload(&tl->d_subs, tl->subs_h))) return st;
     if ((st = upload(&tl->d_prof, hp.prof_flat))) return st;
+    tl->tg.prof = tl->d_prof;
     if ((st = upload(&tl->d_small, hp.small))) return st;
     if ((st = upload(&tl->d_small_mask, hp.small_mask))) return st;
     if (device_items) {
