@@ -79,15 +79,18 @@ hipError_t launch_integrate_tiles_84(const TileGrid &tg, const EntryRec *recs8, 
                                      int splits8, const EntryRec *recs4, const int *start4,
                                      int splits4, const double *grid, int lp_add, int ntiles,
                                      double *partial, hipStream_t st);
+// nab: the most (a, b) entries of the block shells la_min..la_max x
+// lb_min..lb_max among the launched pairs (LDS of the one-pair-per-wave path)
+int block_shell_entries(int la_min, int la_max, int lb_min, int lb_max);
 hipError_t launch_coef(const PairDev *pairs, const int *order, int pair0, int npairs, int tau,
-                       int lam_max, int lbm_max, const double *pab, double *coef,
+                       int lam_max, int lbm_max, int nab, const double *pab, double *coef,
                        const LatticeCSR &lt, double *ws,
                        long long ws_stride, int ws_blocks, hipStream_t st);
 size_t coef_smem_bytes(int lam, int lbm, bool lds_blocks);
 size_t coef_ws_doubles(int lam, int lbm);
 bool coef_needs_workspace(int lam, int lbm);
 hipError_t launch_hab(const PairDev *pairs, const int *order, const int *segs, int seg0, int nseg,
-                      int tau, int with_forces, int la_max, int lb_max, const double *Mbuf,
+                      int tau, int with_forces, int la_max, int lb_max, int nab, const double *Mbuf,
                       int mstride, const double *pab, double *hab, double *forces,
                       const LatticeCSR &lt, double *ws,
                       long long ws_stride, int ws_blocks, hipStream_t st);

@@ -23,6 +23,8 @@ constexpr size_t COEF_LDS_MAX = 24 * 1024;
 constexpr size_t HAB_LDS_MAX = 64 * 1024;
 constexpr int HAB_WS_BLOCKS = 4096;
 constexpr int TR_NT = 256;
+// the one-pair-per-wave transforms (tau = 0) take blocks up to this LDS size
+constexpr size_t WAVE_LDS_MAX = 64 * 1024;
 // pairs whose cube is one sub-box of at most SMALL_NX x SMALL_NY x SMALL_NZ
 // points (general cells; SMALL_NX x SMALL_NYZ_ORTHO^2 on orthorhombic ones,
 // where the tile kernels are faster) are integrated by k_moments_small (one

@@ -55,6 +55,23 @@ struct AlphaView {
     }
 };
 
+// binomial coefficients C(n, k), n < 20 (Pascal's triangle at compile time)
+struct BinomTab {
+    int c[20][20];
+};
+__host__ __device__ constexpr BinomTab make_binom() {
+    BinomTab t{};
+    for (int n = 0; n < 20; n++) {
+        t.c[n][0] = 1;
+        for (int k = 1; k <= n; k++) t.c[n][k] = t.c[n - 1][k - 1] + (k < n ? t.c[n - 1][k] : 0);
+    }
+    return t;
+}
+__constant__ BinomTab kBinom = make_binom();
+
+// alpha_d(lb, la, k): coefficient of (x - xp)^k in (x - xa)^la (x - xb)^lb,
+// from the binomial expansions about xp (grid_base_ref_c.cc:195-230, with
+// exact integer binomials).
 // pp points to the pair in global memory: its per-axis fields are read with a
 // run-time axis index, which on a register copy would force it to scratch
 __device__ void fill_alpha(const PairDev *__restrict__ pp, AlphaView A, int tid, int nthr) {
@@ -66,15 +83,16 @@ __device__ void fill_alpha(const PairDev *__restrict__ pp, AlphaView A, int tid,
         for (int k = 0; k <= A.lpm; k++) A.at(d, lb, la, k) = 0.0;
         const double pa = pp->rp[d] - pp->ra[d];
         const double pb = pp->rp[d] - pp->rb[d];
-        double ca = 1.0, powa = 1.0;
+        double powa = 1.0;
         for (int i = 0; i <= la; i++) {
-            double cb = 1.0, powb = 1.0;
+            // C(la, la - i) = C(la, i): the term (x - xp)^(la - i) (xp - xa)^i
+            const double ca = (double)kBinom.c[la][i];
+            double powb = 1.0;
             for (int j = 0; j <= lb; j++) {
+                const double cb = (double)kBinom.c[lb][j];
                 A.at(d, lb, la, (la - i) + (lb - j)) += ca * cb * powa * powb;
-                cb *= (double)(lb - j) / (double)(j + 1);
                 powb *= pb;
             }
-            ca *= (double)(la - i) / (double)(i + 1);
             powa *= pa;
         }
     }
@@ -275,6 +293,120 @@ __global__ __launch_bounds__(TR_NT) void k_coef(const PairDev *__restrict__ pair
     if (!lt.rp) continue;
     __syncthreads();
     lattice_transform(lt, lp, 0, cxyz, coef + p.coef_off, threadIdx.x, blockDim.x);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// cab -> cxyz of one pair per wave (tau = 0), in slices of fixed (az, bz), so
+// that the per-pair LDS is the alpha table, the pair's block and one slice,
+// and many pairs are resident per CU (one 64-lane workgroup each; workgroup
+// barriers of a single wave are cheap). Global memory is touched once per
+// pair (block in, coefficients out); the slice loops read only LDS. The
+// factorisation is the three-step contraction of grid_base_ref_c.cc:234-315,
+// one axis per step.
+
+constexpr int TW = 64;
+
+// shells la_min..la_max of a block: first coset index and count
+__device__ __forceinline__ void shell_range(int lmin, int lmax, int &lo, int &n) {
+    lo = ncoset(lmin - 1);
+    n = ncoset(lmax) - lo;
+}
+
+// LDS doubles of k_coef_wave for a block of up to (la, lb) whose shells hold
+// at most nab (a, b) entries: alpha, the block, the slice S1[by][ay][lx] and
+// (general cells) the xyz coefficients before the lattice transform
+__host__ __device__ constexpr int coefw_doubles(int la, int lb, int nab) {
+    return 3 * (lb + 1) * (la + 1) * (la + lb + 1) + nab + (la + 1) * (lb + 1) * (la + lb + 1) + ncoset(la + lb);
+}
+
+// cab -> cxyz of one pair per wave: for each slice (az, bz)
+//   S1[by][ay][lx] = sum_{ax, bx} pab[b][a] alpha_x(bx, ax, lx)
+//   t(ly, lx)      = sum_{ay, by} alpha_y(by, ay, ly) S1[by][ay][lx]
+//   c(lz, ly, lx) += t(ly, lx) alpha_z(bz, az, lz)
+// with (ax, bx) over the shells la_min..la_max, lb_min..lb_max; lane s owns
+// the columns (ly, lx) = c2 = lane + 64 s of c in registers. LPM bounds
+// lp + 1 (register arrays).
+template <int LPM>
+__global__ __launch_bounds__(TW) void k_coef_wave(const PairDev *__restrict__ pairs,
+                                                  const int *__restrict__ order, int pair0, int npairs,
+                                                  const double *__restrict__ pab, double *__restrict__ coef,
+                                                  LatticeCSR lt) {
+    constexpr int NCOL = (ncoef2(LPM - 1) + TW - 1) / TW;
+    extern __shared__ double smw[];
+    const int lane = threadIdx.x;
+    for (int ip = pair0 + blockIdx.x; ip < pair0 + npairs; ip += gridDim.x) {
+        __syncthreads();  // LDS reuse across pairs
+        const int pi = order ? order[ip] : ip;
+        const PairDev p = pairs[pi];
+        const int la = p.la_max, lb = p.lb_max, lp = la + lb;
+        int na_lo, nA, nb_lo, nB;
+        shell_range(p.la_min, la, na_lo, nA);
+        shell_range(p.lb_min, lb, nb_lo, nB);
+        AlphaView A{smw, la, lb, lp};
+        double *P = smw + 3 * (lb + 1) * (la + 1) * (lp + 1);  // P[jb][ia] = pab[nb_lo + jb][na_lo + ia]
+        double *S1 = P + nA * nB;
+        double *cx = S1 + (la + 1) * (lb + 1) * (lp + 1);
+        fill_alpha(&pairs[pi], A, lane, TW);
+        for (int w = lane; w < nA * nB; w += TW)
+            P[w] = pab[p.block_off + (long long)(nb_lo + w / nA) * p.ld + na_lo + w % nA];
+        const int n2 = ncoef2(lp);
+        int ly[NCOL], lx[NCOL];
+        double c[NCOL][LPM];
+#pragma unroll
+        for (int s = 0; s < NCOL; s++) {
+            ly[s] = lx[s] = 0;
+            if (lane + TW * s < n2) decode_c2(lp, lane + TW * s, ly[s], lx[s]);
+#pragma unroll
+            for (int k = 0; k < LPM; k++) c[s][k] = 0.0;
+        }
+        __syncthreads();
+        for (int az = 0; az <= la; az++) {
+            for (int bz = 0; bz <= lb; bz++) {
+                const int ma = la - az, mb = lb - bz, X = ma + mb + 1;
+                const int nyy = (ma + 1) * (mb + 1);
+                // w -> (l, ay, by) by float reciprocals: exact quotients for w < 2^12
+                const float rX = 1.0f / (float)X, rA = 1.0f / (float)(ma + 1);
+                for (int w = lane; w < nyy * X; w += TW) {
+                    const int yy = (int)(((float)w + 0.5f) * rX), l = w - yy * X;
+                    const int by = (int)(((float)yy + 0.5f) * rA), ay = yy - by * (ma + 1);
+                    const int ax_lo = max(0, p.la_min - ay - az), ax_hi = ma - ay;
+                    const int bx_lo = max(0, p.lb_min - by - bz), bx_hi = mb - by;
+                    double s = 0.0;
+                    for (int ax = ax_lo; ax <= ax_hi; ax++) {
+                        const double *col = P + coset(ax, ay, az) - na_lo;
+                        for (int bx = max(bx_lo, l - ax); bx <= bx_hi; bx++)
+                            s += col[(coset(bx, by, bz) - nb_lo) * nA] * A.at(0, bx, ax, l);
+                    }
+                    S1[w] = s;
+                }
+                __syncthreads();
+#pragma unroll
+                for (int s = 0; s < NCOL; s++) {
+                    if (lane + TW * s >= n2 || ly[s] + lx[s] > ma + mb) continue;
+                    double t = 0.0;
+                    for (int ay = 0; ay <= ma; ay++)
+                        for (int by = max(0, ly[s] - ay); by <= mb; by++)
+                            t += A.at(1, by, ay, ly[s]) * S1[(by * (ma + 1) + ay) * X + lx[s]];
+#pragma unroll
+                    for (int lz = 0; lz < LPM; lz++)
+                        if (lz <= az + bz) c[s][lz] += t * A.at(2, bz, az, lz);
+                }
+                __syncthreads();
+            }
+        }
+        double *dst = lt.rp ? cx : coef + p.coef_off;
+#pragma unroll
+        for (int s = 0; s < NCOL; s++) {
+            if (lane + TW * s >= n2) continue;
+#pragma unroll
+            for (int lz = 0; lz < LPM; lz++)
+                if (lz <= lp - ly[s] - lx[s]) dst[cidx(lp, lz, ly[s], lx[s])] = p.prefactor * c[s][lz];
+        }
+        if (lt.rp) {
+            __syncthreads();
+            lattice_transform(lt, lp, 0, cx, coef + p.coef_off, lane, TW);
+        }
     }
 }
 
@@ -2225,13 +2357,28 @@ size_t coef_smem_bytes(int lam, int lbm, bool lds_blocks) {
 
 bool coef_needs_workspace(int lam, int lbm) { return coef_smem_bytes(lam, lbm, true) > COEF_LDS_MAX; }
 
+int block_shell_entries(int la_min, int la_max, int lb_min, int lb_max) {
+    return (ncoset(la_max) - ncoset(la_min - 1)) * (ncoset(lb_max) - ncoset(lb_min - 1));
+}
+
 // pairs order[pair0 .. pair0 + npairs) (or pair0 .. when order is NULL), whose
 // la_max / lb_max are at most la_max / lb_max (LDS sizing)
 hipError_t launch_coef(const PairDev *pairs, const int *order, int pair0, int npairs, int tau,
-                       int lam_max, int lbm_max, const double *pab, double *coef,
+                       int lam_max, int lbm_max, int nab, const double *pab, double *coef,
                        const LatticeCSR &lt, double *ws,
                        long long ws_stride, int ws_blocks, hipStream_t st) {
     if (npairs <= 0) return hipSuccess;
+    if (!tau && sizeof(double) * coefw_doubles(lam_max, lbm_max, nab) <= WAVE_LDS_MAX) {
+        // one pair per wave
+        const size_t smem = sizeof(double) * coefw_doubles(lam_max, lbm_max, nab);
+        switch (class_of(lam_max + lbm_max)) {
+        case 0: hipLaunchKernelGGL(k_coef_wave<4>, dim3(npairs), dim3(TW), smem, st, pairs, order, pair0, npairs, pab, coef, lt); break;
+        case 1: hipLaunchKernelGGL(k_coef_wave<8>, dim3(npairs), dim3(TW), smem, st, pairs, order, pair0, npairs, pab, coef, lt); break;
+        case 2: hipLaunchKernelGGL(k_coef_wave<12>, dim3(npairs), dim3(TW), smem, st, pairs, order, pair0, npairs, pab, coef, lt); break;
+        default: hipLaunchKernelGGL(k_coef_wave<17>, dim3(npairs), dim3(TW), smem, st, pairs, order, pair0, npairs, pab, coef, lt); break;
+        }
+        return hipGetLastError();
+    }
     const int lam = lam_max + tau, lbm = lbm_max + tau;
     const bool big = coef_needs_workspace(lam, lbm);
     if (big && (!ws || ws_stride < (long long)coef_ws_doubles(lam, lbm) || ws_blocks < 1))
@@ -2261,7 +2408,7 @@ bool hab_needs_workspace(int lamx, int lbmx, int lam, int lbm) {
 }
 
 hipError_t launch_hab(const PairDev *pairs, const int *order, const int *segs, int seg0, int nseg,
-                      int tau, int with_forces, int la_max, int lb_max, const double *Mbuf,
+                      int tau, int with_forces, int la_max, int lb_max, int nab, const double *Mbuf,
                       int mstride, const double *pab, double *hab, double *forces,
                       const LatticeCSR &lt, double *ws,
                       long long ws_stride, int ws_blocks, hipStream_t st) {
@@ -2269,6 +2416,7 @@ hipError_t launch_hab(const PairDev *pairs, const int *order, const int *segs, i
     const int npairs = nseg;
     const int fx = with_forces ? 1 : 0;
     const int lamx = la_max + tau + fx, lbmx = lb_max + tau + fx;
+    (void)nab;
     const bool big = hab_needs_workspace(lamx, lbmx, la_max + tau, lb_max + tau);
     if (big && (!ws || ws_stride < (long long)hab_ws_doubles(lamx, lbmx, la_max + tau, lb_max + tau) ||
                 ws_blocks < 1))

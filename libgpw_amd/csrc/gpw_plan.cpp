@@ -215,8 +215,9 @@ struct gpw_tasklist {
     // pairs: one block, one primitive pair per exponent pair) are adjacent and
     // form one segment, walked by one workgroup of k_hab
     struct PairGroup {
-        int start, count, la_max, lb_max, seg0, nseg;
+        int start, count, la_max, lb_max, seg0, nseg, nab;
     };
+    int nab_max = 0;  // most block-shell entries of a pair (block_shell_entries)
     std::vector<PairGroup> groups;
     int *d_order = nullptr;
     int *d_segs = nullptr;
@@ -848,9 +849,9 @@ static gpw_status plan_tasklist(gpw_tasklist *tl, const std::vector<gpw_pair_des
     return plan_entries_host(tl, hp);
 }
 
-// Groups of pairs by the LDS size class of their transform (powers of two
-// from 8 KiB), so that small blocks do not run at the occupancy of the
-// largest one; inside a group pairs are sorted by block offset and pairs of
+// Groups of pairs by la_max + lb_max (+ 2 tau), which sizes the LDS and the
+// registers of their transforms, so that small blocks do not run at the
+// occupancy of the largest one; inside a group pairs are sorted by block offset and pairs of
 // one block form a segment. Blocks may be shared only whole (same offset,
 // same la_max and lb_max); any other overlap is an argument error.
 static gpw_status group_pairs(gpw_tasklist *tl, std::vector<int> &order, std::vector<int> &segs) {
@@ -885,22 +886,21 @@ static gpw_status group_pairs(gpw_tasklist *tl, std::vector<int> &order, std::ve
     std::vector<std::vector<int>> byc;
     for (int i = 0; i < tl->npairs; i++) {
         const PairDev &p = tl->pairs_h[i];
-        size_t b = coef_smem_bytes(p.la_max + t, p.lb_max + t, true);
-        int c = 0;
-        while (c < 12 && (size_t(8192) << c) < b) c++;
+        const int c = p.la_max + p.lb_max + 2 * t;
         if ((int)byc.size() <= c) byc.resize(c + 1);
         byc[c].push_back(i);
     }
     order.clear();
     segs.clear();
     tl->groups.clear();
+    tl->nab_max = 0;
     for (auto &v : byc) {
         if (v.empty()) continue;
         if (!sorted)
             std::stable_sort(v.begin(), v.end(), [&](int a, int b) {
                 return tl->pairs_h[a].block_off < tl->pairs_h[b].block_off;
             });
-        gpw_tasklist::PairGroup g{(int)order.size(), (int)v.size(), 0, 0, (int)segs.size(), 0};
+        gpw_tasklist::PairGroup g{(int)order.size(), (int)v.size(), 0, 0, (int)segs.size(), 0, 0};
         long long last = -1;
         for (int i : v) {
             if (tl->pairs_h[i].block_off != last) {
@@ -909,10 +909,13 @@ static gpw_status group_pairs(gpw_tasklist *tl, std::vector<int> &order, std::ve
                 last = tl->pairs_h[i].block_off;
             }
             order.push_back(i);
-            g.la_max = std::max(g.la_max, tl->pairs_h[i].la_max);
-            g.lb_max = std::max(g.lb_max, tl->pairs_h[i].lb_max);
+            const PairDev &pd = tl->pairs_h[i];
+            g.la_max = std::max(g.la_max, pd.la_max);
+            g.lb_max = std::max(g.lb_max, pd.lb_max);
+            g.nab = std::max(g.nab, block_shell_entries(pd.la_min, pd.la_max, pd.lb_min, pd.lb_max));
         }
         tl->groups.push_back(g);
+        tl->nab_max = std::max(tl->nab_max, g.nab);
     }
     segs.push_back((int)order.size());
     return GPW_OK;
@@ -1577,11 +1580,11 @@ static gpw_status run_coef(gpw_tasklist *tl, const double *pab, int pair0, int n
     if (pair0 == 0 && n == tl->npairs) {  // the whole list: one launch per size group
         for (const auto &g : tl->groups)
             HIP_TRY(launch_coef(tl->d_pairs, tl->d_order, g.start, g.count, tl->compute_tau, g.la_max,
-                                g.lb_max, pab, tl->d_coef, tl->lat(), tl->d_hab_ws,
+                                g.lb_max, g.nab, pab, tl->d_coef, tl->lat(), tl->d_hab_ws,
                                 tl->hab_ws_stride, tl->hab_ws_blocks, st));
         return GPW_OK;
     }
-    HIP_TRY(launch_coef(tl->d_pairs, nullptr, pair0, n, tl->compute_tau, tl->la_max, tl->lb_max, pab,
+    HIP_TRY(launch_coef(tl->d_pairs, nullptr, pair0, n, tl->compute_tau, tl->la_max, tl->lb_max, tl->nab_max, pab,
                         tl->d_coef, tl->lat(), tl->d_hab_ws, tl->hab_ws_stride,
                         tl->hab_ws_blocks, st));
     return GPW_OK;
@@ -1654,7 +1657,7 @@ gpw_status gpw_tasklist_integrate(gpw_tasklist *tl, const double *grid, const do
         PhaseTimer ph(tl, st, PH_HAB);
         for (const auto &g : tl->groups)
             HIP_TRY(launch_hab(tl->d_pairs, tl->d_order, tl->d_segs, g.seg0, g.nseg, tl->compute_tau,
-                               wf, g.la_max, g.lb_max, tl->d_Mbuf, tl->mstride, pab, hab, forces,
+                               wf, g.la_max, g.lb_max, g.nab, tl->d_Mbuf, tl->mstride, pab, hab, forces,
                                tl->lat(), tl->d_hab_ws, tl->hab_ws_stride,
                                tl->hab_ws_blocks, st));
     }
@@ -1719,8 +1722,10 @@ gpw_status gpw_tasklist_integrate_plain(gpw_tasklist *tl, const double *grid, co
                                          sd.ub[1] - sd.lb[1] + 1, 2 * wf, tl->d_prof, grid,
                                          tl->mstride, tl->d_Mbuf, st));
         }
-        HIP_TRY(launch_hab(tl->d_pairs, nullptr, nullptr, p, 1, tl->compute_tau, wf, tl->pairs_h[p].la_max,
-                           tl->pairs_h[p].lb_max, tl->d_Mbuf, tl->mstride, pab, hab, forces,
+        const PairDev &pd = tl->pairs_h[p];
+        HIP_TRY(launch_hab(tl->d_pairs, nullptr, nullptr, p, 1, tl->compute_tau, wf, pd.la_max, pd.lb_max,
+                           block_shell_entries(pd.la_min, pd.la_max, pd.lb_min, pd.lb_max), tl->d_Mbuf,
+                           tl->mstride, pab, hab, forces,
                            tl->lat(), tl->d_hab_ws, tl->hab_ws_stride,
                            tl->hab_ws_blocks, st));
     }

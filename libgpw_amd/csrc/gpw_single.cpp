@@ -311,7 +311,8 @@ gpw_status run_collocate(const Plan &P, char *R, const double *pab_dev, double *
     double *coef = reinterpret_cast<double *>(R + P.o_coef);
     double *ws = P.ws_doubles ? reinterpret_cast<double *>(R + P.o_ws) : nullptr;
     const double *pab = P.mode == DEV ? pab_dev : reinterpret_cast<const double *>(R + P.o_pab);
-    HIP_TRY(launch_coef(dpair, nullptr, 0, 1, P.t.tau, P.t.p.la_max, P.t.p.lb_max, pab, coef,
+    HIP_TRY(launch_coef(dpair, nullptr, 0, 1, P.t.tau, P.t.p.la_max, P.t.p.lb_max,
+                        block_shell_entries(P.t.p.la_min, P.t.p.la_max, P.t.p.lb_min, P.t.p.lb_max), pab, coef,
                         lattice_view(P, R), ws, P.ws_doubles, 1, st));
     const SubDev *subs = reinterpret_cast<const SubDev *>(R + (P.mode == HOST_WINDOW ? P.o_bsubs : P.o_subs));
     const int *prof = reinterpret_cast<const int *>(R + P.o_prof);
@@ -340,7 +341,8 @@ gpw_status run_integrate(const Plan &P, char *R, const double *grid, const doubl
         HIP_TRY(launch_moments_plain(launch_grid(P, si), subs, (int)si, s.ub[2] - s.lb[2] + 1,
                                      s.ub[1] - s.lb[1] + 1, 2 * wf, prof, src, P.mstride, Mbuf, st));
     }
-    HIP_TRY(launch_hab(dpair, nullptr, nullptr, 0, 1, P.t.tau, wf, P.t.p.la_max, P.t.p.lb_max, Mbuf,
+    HIP_TRY(launch_hab(dpair, nullptr, nullptr, 0, 1, P.t.tau, wf, P.t.p.la_max, P.t.p.lb_max,
+                       block_shell_entries(P.t.p.la_min, P.t.p.la_max, P.t.p.lb_min, P.t.p.lb_max), Mbuf,
                        P.mstride, pab, hab, forces, lattice_view(P, R), ws, P.ws_doubles, 1, st));
     return GPW_OK;
 }
