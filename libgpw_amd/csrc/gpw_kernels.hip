@@ -2003,28 +2003,49 @@ __device__ __forceinline__ int plan_axis_tile(const SubDev &s, int d, const Tile
     return k < n1 ? r0 + k : q0 + (k - n1);
 }
 
-// Candidate c of sub-box s (tiles its index box touches, x fastest): its
-// tile coordinates and whether the sphere has a point in it (item_has_points
-// with the rows of the tile kernels: the general quadratic bounds, or the
-// orthorhombic profile threshold, fold offsets <= T <=> -T <= i <= T + 1).
-__device__ __forceinline__ bool plan_candidate(const SubDev &s, const TileGrid &tg, const int *__restrict__ prof,
-                                               int c, int nxs, int nys, int &tile) {
-    const int tx = plan_axis_tile(s, 0, tg, c % nxs);
-    const int ty = plan_axis_tile(s, 1, tg, (c / nxs) % nys);
-    const int tz = plan_axis_tile(s, 2, tg, c / (nxs * nys));
-    tile = (tz * tg.nty + ty) * tg.ntx + tx;
-    return item_has_points(s, tg, tx, ty, tz, [&](int j, int k, int &lo, int &hi) {
-        if (tg.general) {
-            double qb, qc;
-            general_row_dev(tg, s.roff[0], (double)j - s.roff[1], (double)k - s.roff[2], s.R2, s.zetp, lo, hi,
-                            qb, qc);
-            return lo <= hi;
+// x offsets lo..hi of row (j, k) of sub-box s inside its sphere, as the tile
+// kernels decide point by point: the general quadratic bounds, or the
+// orthorhombic profile threshold (fold offsets <= T <=> -T <= i <= T + 1)
+__device__ __forceinline__ bool plan_row(const SubDev &s, const TileGrid &tg, const int *__restrict__ prof, int j,
+                                         int k, int &lo, int &hi) {
+    if (tg.general) {
+        double qb, qc;
+        general_row_dev(tg, s.roff[0], (double)j - s.roff[1], (double)k - s.roff[2], s.R2, s.zetp, lo, hi, qb, qc);
+        return lo <= hi;
+    }
+    const int T = row_threshold(prof, s.prof_off, s.prof_stride, fold_off(k), fold_off(j));
+    lo = -T;
+    hi = T + 1;
+    return T >= 0;
+}
+
+// Work items of sub-box s in the tiles of candidate row (iy, iz) (the sub-box's
+// iy-th y tile and iz-th z tile): bit ix - ix0 is set when the sphere has a
+// point in the tile of the ix-th x tile, ix0 <= ix < ix0 + 64 (the test of
+// item_has_points). The wave's lanes take the tile rows, each row its x
+// offset range against every candidate x tile; OR over the wave.
+__device__ unsigned long long plan_row_mask(const SubDev &s, const TileGrid &tg, const int *__restrict__ prof,
+                                            int iy, int iz, int ix0, int nxs, int lane) {
+    const int ty = plan_axis_tile(s, 1, tg, iy), tz = plan_axis_tile(s, 2, tg, iz);
+    const int ix1 = min(nxs, ix0 + 64);
+    unsigned long long m = 0;
+    for (int r = lane; r < TY * TZ; r += 64) {
+        const int gz = tz * TZ + r / TY, gy = ty * TY + r % TY;
+        if (gz >= tg.nz || gy >= tg.ny) continue;
+        const int k = s.lb[2] + imod_hd(gz - s.base[2], s.period[2]);
+        const int j = s.lb[1] + imod_hd(gy - s.base[1], s.period[1]);
+        int lo, hi;
+        if (k > s.ub[2] || j > s.ub[1] || !plan_row(s, tg, prof, j, k, lo, hi)) continue;
+        for (int ix = ix0; ix < ix1; ix++) {
+            const int tx = plan_axis_tile(s, 0, tg, ix);
+            int xa[2], xb[2];
+            tile_offset_runs(s, 0, tx * TX, min((tx + 1) * TX, tg.nx) - 1, xa, xb);
+            if (max(lo, xa[0]) <= min(hi, xb[0]) || max(lo, xa[1]) <= min(hi, xb[1])) m |= 1ULL << (ix - ix0);
         }
-        const int T = row_threshold(prof, s.prof_off, s.prof_stride, fold_off(k), fold_off(j));
-        lo = -T;
-        hi = T + 1;
-        return T >= 0;
-    });
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m |= __shfl_xor(m, o);
+    return m;
 }
 
 // One wave per sub-box: its number of (sub-box, tile) items -- the tiles
@@ -2040,12 +2061,11 @@ __global__ __launch_bounds__(256) void k_plan_count(const SubDev *__restrict__ s
     if (si >= nsub) return;
     const SubDev s = subs[si];
     {
-        const int nxs = plan_axis_count(s, 0, tg), nys = plan_axis_count(s, 1, tg);
-        const int ncand = nxs * nys * plan_axis_count(s, 2, tg);
-        int cnt = 0, tile;
-        for (int c = lane; c < ncand; c += 64) cnt += plan_candidate(s, tg, prof, c, nxs, nys, tile) ? 1 : 0;
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+        const int nxs = plan_axis_count(s, 0, tg), nys = plan_axis_count(s, 1, tg), nzs = plan_axis_count(s, 2, tg);
+        int cnt = 0;
+        for (int iz = 0; iz < nzs; iz++)
+            for (int iy = 0; iy < nys; iy++)
+                for (int ix0 = 0; ix0 < nxs; ix0 += 64) cnt += __popcll(plan_row_mask(s, tg, prof, iy, iz, ix0, nxs, lane));
         if (lane == 0) counts[si] = cnt;
     }
     unsigned long long pts = 0, rows = 0, planes = 0;
@@ -2101,18 +2121,20 @@ __global__ __launch_bounds__(256) void k_plan_entries(const SubDev *__restrict__
     const int si = blockIdx.x * 4 + wave;
     if (si >= nsub) return;
     const SubDev sd = subs[si];
-    const int nxs = plan_axis_count(sd, 0, tg), nys = plan_axis_count(sd, 1, tg);
-    const int ncand = nxs * nys * plan_axis_count(sd, 2, tg);
+    const int nxs = plan_axis_count(sd, 0, tg), nys = plan_axis_count(sd, 1, tg), nzs = plan_axis_count(sd, 2, tg);
     const long long e0 = ent_start[si];
     const unsigned sent = (unsigned)(NCLS * ntiles);
     const bool skip1 = small_mask[npairs + sd.pair] != 0, skip2 = small_mask[sd.pair] != 0;
     int done = 0;
-    for (int c0 = 0; c0 < ncand; c0 += 64) {
-        const int c = c0 + lane;
-        int tile = 0;
-        const bool pass = c < ncand && plan_candidate(sd, tg, prof, c, nxs, nys, tile);
-        const unsigned long long m = __ballot(pass);
+    for (int iz = 0; iz < nzs; iz++)
+    for (int iy = 0; iy < nys; iy++)
+    for (int ix0 = 0; ix0 < nxs; ix0 += 64) {
+        const unsigned long long m = plan_row_mask(sd, tg, prof, iy, iz, ix0, nxs, lane);
+        // lane b takes candidate x tile ix0 + b
+        const bool pass = (m >> lane) & 1ULL;
         if (pass) {
+            const int tile = (plan_axis_tile(sd, 2, tg, iz) * tg.nty + plan_axis_tile(sd, 1, tg, iy)) * tg.ntx +
+                             plan_axis_tile(sd, 0, tg, ix0 + lane);
             const int k = done + __popcll(m & ((1ULL << lane) - 1ULL));
             const long long e = e0 + k;
             ent_sub[e] = si;
