@@ -94,6 +94,12 @@ typedef struct gpw_tasklist_stats {
      * sub-boxes, their upload, the work items planned on the device, the
      * work buffers (0 for gpw_tasklist_plan_stats)                           */
     double plan_ms[4];
+    /* pairs collocated by the small-sphere path (one box per pair, added to
+     * the grid tile by tile in a fixed order) and the points of their boxes;
+     * the collocate tile kernels sweep (nentries of the other pairs) tiles */
+    long long csmall_pairs;
+    long long csmall_points;
+    long long collocate_tile_points;  /* points swept by the collocate tile kernels */
 } gpw_tasklist_stats;
 
 /* ---- single task, host memory (drop-in for the reference's collocate) ---- */

@@ -63,7 +63,8 @@ class TaskListStats(C.Structure):
                 ("tile_points", C.c_longlong), ("lp_max", C.c_int), ("tile", C.c_int * 3),
                 ("splits", C.c_int), ("csplits", C.c_int * 4), ("isplits", (C.c_int * 4) * 2),
                 ("flops_collocate", C.c_double), ("flops_integrate", C.c_double * 2),
-                ("plan_ms", C.c_double * 4)]
+                ("plan_ms", C.c_double * 4), ("csmall_pairs", C.c_longlong),
+                ("csmall_points", C.c_longlong), ("collocate_tile_points", C.c_longlong)]
 
 
 _lib = None

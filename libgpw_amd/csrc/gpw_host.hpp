@@ -132,8 +132,14 @@ hipError_t launch_plan_entries(const SubDev *subs, int nsub, const long long *en
                                const unsigned char *small_mask, int npairs, int ne, int *ent_sub,
                                long long *off0, long long *off2, unsigned *keys, int *vals, int *hist,
                                hipStream_t st);
-hipError_t launch_plan_records(const SubDev *subs, const int *ent_sub, const long long *off, const int *sorted_e,
-                               int n, EntryRec *recs, hipStream_t st);
+hipError_t launch_plan_records(const SubDev *subs, const int *ent_sub, const long long *off,
+                               const long long *sub_off, const int *sorted_e, int n, EntryRec *recs,
+                               hipStream_t st);
+hipError_t launch_collocate_small(int cls, const TileGrid &tg, const SubDev *subs, const int *list, int n,
+                                  const double *coef, const int *prof, const long long *box_off, double *sval,
+                                  hipStream_t st);
+hipError_t launch_gather_small(const TileGrid &tg, const EntryRec *recs, const int *tile_start, int ntiles,
+                               const double *sval, double *grid, hipStream_t st);
 hipError_t sort_pairs_u32(void *temp, size_t *temp_bytes, const unsigned *keys_in, unsigned *keys_out,
                           const int *vals_in, int *vals_out, int n, int end_bit, hipStream_t st);
 

@@ -31,11 +31,19 @@ constexpr size_t WAVE_LDS_MAX = 64 * 1024;
 // workgroup per pair) instead of the tiles; limits chosen by measurement
 constexpr int SMALL_NX = 64, SMALL_NY = 48, SMALL_NZ = 48, SMALL_NYZ_ORTHO = 32, SMALL_LMAX = 12;
 
+// collocate of small spheres: pairs whose cube is one sub-box of at most
+// CSMALL_N points per axis (general cells; CSMALL_N_ORTHO on orthorhombic
+// ones, 0: none) with lp + 1 <= SMALL_LMAX are computed per pair into a
+// scratch box and added to the grid tile by tile (k_collocate_small,
+// k_gather_small) instead of the collocate tile kernels; limits measured
+constexpr int CSMALL_N = 16, CSMALL_N_ORTHO = 0;
+
 // work-item classes by polynomial length L = lp + 1 (one tile-kernel
 // instantiation each, so registers are sized for the class, not for the
-// largest L), and the three tile-list sets: 0 collocate, 1 integrate with
-// the force extension (lp + 2), 2 integrate without
-constexpr int NCLS = 4, NSET = 3;
+// largest L), and the four tile-list sets: 0 collocate, 1 integrate with
+// the force extension (lp + 2), 2 integrate without, 3 the small-sphere
+// collocate boxes per tile (class 0 only)
+constexpr int NCLS = 4, NSET = 4;
 __host__ __device__ constexpr int class_of(int lp) {
     return lp + 1 <= 4 ? 0 : lp + 1 <= 8 ? 1 : lp + 1 <= 12 ? 2 : 3;
 }

@@ -1743,6 +1743,199 @@ __global__ __launch_bounds__(NROW) void k_moments_small(TileGrid tg, const SubDe
 }
 
 // ---------------------------------------------------------------------------
+// Collocate of small spheres, in two passes so that every grid point is still
+// summed in one fixed order (no atomics):
+//  1. k_collocate_small: one workgroup per pair whose cube is a single
+//     sub-box of at most CSMALL_N points per axis; its values on the whole
+//     box (zero outside the sphere) go to the pair's slice of a scratch
+//     buffer, box-major (z, y, x). Same stages as k_moments_small in the
+//     collocate direction: z stage cxy = sum_lz coef Z^lz in LDS, then one
+//     thread per row: cx = sum_ly cxy Y^ly, and the points of the row's
+//     sphere range by Horner in the displacement times the Gaussian
+//     recursion started at the range's first point.
+//  2. k_gather_small: one workgroup per tile adds the boxes of the tile's
+//     small pairs (its tile list, in item order) to the tile's grid points.
+// Tile kernels leave these pairs out of their lists.
+
+template <int LM>
+struct CSmallSmem {
+    double coef[LM * (LM + 1) * (LM + 2) / 6];
+    double Yt[CSMALL_N][LM], Zt[CSMALL_N][LM];
+    double cxy[CSMALL_N][LM * (LM + 1) / 2];
+};
+
+template <int LM>
+__global__ __launch_bounds__(NROW) void k_collocate_small(TileGrid tg, const SubDev *__restrict__ subs,
+                                                          const int *__restrict__ list,
+                                                          const double *__restrict__ coef,
+                                                          const int *__restrict__ prof,
+                                                          const long long *__restrict__ box_off,
+                                                          double *__restrict__ sval) {
+    __shared__ CSmallSmem<LM> sm;
+    const int si = list[blockIdx.x];
+    const SubDev s = subs[si];
+    const bool gen = tg.general != 0;
+    const int lp = s.lp, L = lp + 1;  // L <= LM (host classification)
+    const int nx = s.ub[0] - s.lb[0] + 1, ny = s.ub[1] - s.lb[1] + 1, nz = s.ub[2] - s.lb[2] + 1;
+    const int t = threadIdx.x;
+    const int n2 = ncoef2(lp), n3 = ncoset(lp);
+    for (int c = t; c < n3; c += NROW) sm.coef[c] = coef[s.coef_off + c];
+    for (int w = t; w < ny + nz; w += NROW) {
+        const int d = w < ny ? 1 : 2, idx = w < ny ? w : w - ny;
+        const int o = s.lb[d] + idx;
+        const double r = gen ? __dsub_rn((double)o, s.roff[d]) : o * tg.dr[d] - s.roff[d];
+        double *dst = d == 1 ? sm.Yt[idx] : sm.Zt[idx];
+        double v = 1.0;
+        for (int l = 0; l < L; l++) {
+            dst[l] = v;
+            v *= r;
+        }
+    }
+    __syncthreads();
+    // z stage: cxy[z][(ly, lx)] = sum_lz coef[(lz, ly, lx)] Z[z]^lz
+    for (int w = t; w < nz * n2; w += NROW) {
+        const int z = w / n2, c2 = w % n2;
+        int ly, lx;
+        decode_c2(lp, c2, ly, lx);
+        double a = 0.0;
+        for (int lz = 0; lz <= lp - ly - lx; lz++) a += sm.coef[cidx(lp, lz, ly, lx)] * sm.Zt[z][lz];
+        sm.cxy[z][c2] = a;
+    }
+    __syncthreads();
+    double qa;
+    if (gen) {
+        double a = 0.0;
+        for (int x = 0; x < 3; x++) a += tg.dh[0][x] * tg.dh[0][x];
+        qa = -s.zetp * a;
+    } else {
+        qa = -s.zetp * tg.dr[0] * tg.dr[0];
+    }
+    const double kap = exp(2.0 * qa);
+    double *box = sval + box_off[si];
+    for (int row = t; row < ny * nz; row += NROW) {
+        const int y = row % ny, z = row / ny;
+        const int j = s.lb[1] + y, k = s.lb[2] + z;
+        // y stage of the row
+        double cx[LM];
+#pragma unroll
+        for (int l = 0; l < LM; l++) cx[l] = 0.0;
+        for (int ly = 0; ly <= lp; ly++) {
+            const double yv = sm.Yt[y][ly];
+            const double *c = sm.cxy[z] + c2idx(lp, ly, 0);
+#pragma unroll
+            for (int lx = 0; lx < LM; lx++)
+                if (lx <= lp - ly) cx[lx] += c[lx] * yv;
+        }
+        // the row's points inside the sphere (the tile kernels' set)
+        int ilo = s.lb[0], ihi = s.ub[0];
+        double ex0 = 0.0, d0 = 0.0;
+        if (gen) {
+            const double dj = __dsub_rn((double)j, s.roff[1]), dk = __dsub_rn((double)k, s.roff[2]);
+            int lo, hi;
+            double qb_, qc_;
+            general_row_dev(tg, s.roff[0], dj, dk, s.R2, s.zetp, lo, hi, qb_, qc_);
+            ilo = max(ilo, lo);
+            ihi = min(ihi, hi);
+            double b = 0.0, c = 0.0;
+            for (int x = 0; x < 3; x++) {
+                const double w = tg.dh[1][x] * dj + tg.dh[2][x] * dk;
+                b += w * tg.dh[0][x];
+                c += w * w;
+            }
+            const double qb = -2.0 * s.zetp * b, qc = -s.zetp * c;
+            const double u0 = (double)ilo - s.roff[0];
+            ex0 = (qa * u0 + qb) * u0 + qc;
+            d0 = qa * (2.0 * u0 + 1.0) + qb;
+        } else {
+            const int T = row_threshold(prof, s.prof_off, s.prof_stride, fold_off(k), fold_off(j));
+            if (T < 0) {
+                ihi = ilo - 1;
+            } else {
+                ilo = max(ilo, -T);
+                ihi = min(ihi, T + 1);
+            }
+            const double rx = ilo * tg.dr[0] - s.roff[0];
+            const double ry = j * tg.dr[1] - s.roff[1], rz = k * tg.dr[2] - s.roff[2];
+            ex0 = -s.zetp * (rx * rx + ry * ry + rz * rz);
+            d0 = -s.zetp * (2.0 * rx * tg.dr[0] + tg.dr[0] * tg.dr[0]);
+        }
+        double *out = box + ((long long)z * ny + y) * nx - s.lb[0];
+        double e = ilo <= ihi ? exp(ex0) : 0.0, rho = ilo <= ihi ? exp(d0) : 0.0;
+        for (int i = s.lb[0]; i <= s.ub[0]; i++) {
+            double v = 0.0;
+            if (i >= ilo && i <= ihi) {
+                const double u = gen ? (double)i - s.roff[0] : i * tg.dr[0] - s.roff[0];
+                double p = cx[LM - 1];
+#pragma unroll
+                for (int l = LM - 2; l >= 0; l--) p = p * u + cx[l];
+                v = p * e;
+                e *= rho;
+                rho *= kap;
+            }
+            out[i] = v;
+        }
+    }
+}
+
+// one workgroup per tile, one thread per tile row (y, z) holding its TX
+// points in registers: the boxes listed for the tile (recs: base, lb, ub of
+// the box; ent_off its offset in the scratch buffer) are added in list order,
+// each thread reading the part of the box row that lies on its row; the
+// records pass through LDS in chunks
+constexpr int GQ = 64;
+
+__global__ __launch_bounds__(NROW) void k_gather_small(TileGrid tg, const EntryRec *__restrict__ recs,
+                                                       const int *__restrict__ tile_start,
+                                                       const double *__restrict__ sval,
+                                                       double *__restrict__ grid) {
+    __shared__ EntryRec rec[GQ];
+    const int tile = blockIdx.x;
+    const int beg = tile_start[tile], end = tile_start[tile + 1];
+    if (beg == end) return;
+    const int tix = tile % tg.ntx, tiy = (tile / tg.ntx) % tg.nty, tiz = tile / (tg.ntx * tg.nty);
+    const int t = threadIdx.x;
+    const int gx0 = tix * TX, gy = tiy * TY + (t % TY), gz = tiz * TZ + t / TY;
+    const bool live = gy < tg.ny && gz < tg.nz;
+    // grid index g -> box offset (g - base) mod period; g and base lie in
+    // [0, n), so the difference is in (-period, period)
+    auto boff = [](int g, int base, int period) {
+        const int d = g - base;
+        return d < 0 ? d + period : d;
+    };
+    double acc[TX];
+#pragma unroll
+    for (int j = 0; j < TX; j++) acc[j] = 0.0;
+    const int *rec_words = reinterpret_cast<const int *>(recs);
+    for (int c0 = beg; c0 < end; c0 += GQ) {
+        const int ng = min(GQ, end - c0);
+        __syncthreads();
+        for (int w = t; w < ng * RECW; w += NROW)
+            reinterpret_cast<int *>(rec)[w] = rec_words[(long long)c0 * RECW + w];
+        __syncthreads();
+        if (!live) continue;
+        for (int g = 0; g < ng; g++) {
+            const EntryRec &R = rec[g];
+            const int nx = R.ub[0] - R.lb[0] + 1, ny = R.ub[1] - R.lb[1] + 1, nz = R.ub[2] - R.lb[2] + 1;
+            const int kz = boff(gz, R.base[2], tg.period[2]);
+            const int jy = boff(gy, R.base[1], tg.period[1]);
+            if (kz >= nz || jy >= ny) continue;
+            const double *row = sval + R.ent_off + ((long long)kz * ny + jy) * nx;
+            int ix = boff(gx0, R.base[0], tg.period[0]);
+#pragma unroll
+            for (int j = 0; j < TX; j++, ix++) {
+                if (ix == tg.period[0]) ix = 0;
+                if (ix < nx && gx0 + j < tg.nx) acc[j] += row[ix];
+            }
+        }
+    }
+    if (!live) return;
+    double *out = grid + ((long long)gz * tg.ny + gy) * tg.nx + gx0;
+#pragma unroll
+    for (int j = 0; j < TX; j++)
+        if (gx0 + j < tg.nx) out[j] += acc[j];
+}
+
+// ---------------------------------------------------------------------------
 // plain kernels: one launch per sub-box. Workgroup = 8 rows x 32 lanes over
 // one z-plane of the sub-box; blockIdx.x -> z offset, blockIdx.y -> row chunk.
 
@@ -2109,7 +2302,9 @@ __global__ __launch_bounds__(256) void k_plan_count(const SubDev *__restrict__ s
 // candidate in candidate order; sub-boxes are pair-major), each with its
 // sub-box, partial-moment offsets and sort key per tile-list set, class *
 // ntiles + tile, or NCLS * ntiles when the set leaves it out (pairs of the
-// small-sphere integrate kernel); hist counts the items per set and key.
+// small-sphere kernels; set 3 holds only the small-sphere collocate pairs);
+// hist counts the items per set and key. small_mask: [3][npairs] (integrate
+// without / with forces, collocate).
 __global__ __launch_bounds__(256) void k_plan_entries(const SubDev *__restrict__ subs, int nsub,
                                const long long *__restrict__ ent_start, const long long *__restrict__ base0,
                                const long long *__restrict__ base2, const int *__restrict__ prof, TileGrid tg,
@@ -2125,6 +2320,7 @@ __global__ __launch_bounds__(256) void k_plan_entries(const SubDev *__restrict__
     const long long e0 = ent_start[si];
     const unsigned sent = (unsigned)(NCLS * ntiles);
     const bool skip1 = small_mask[npairs + sd.pair] != 0, skip2 = small_mask[sd.pair] != 0;
+    const bool csmall = small_mask[2 * (long long)npairs + sd.pair] != 0;
     int done = 0;
     for (int iz = 0; iz < nzs; iz++)
     for (int iy = 0; iy < nys; iy++)
@@ -2141,10 +2337,12 @@ __global__ __launch_bounds__(256) void k_plan_entries(const SubDev *__restrict__
             off0[e] = base0[si] + (long long)k * ncoset(sd.lp);
             off2[e] = base2[si] + (long long)k * ncoset(sd.lp + 2);
             vals[e] = (int)e;
-            const unsigned k0 = (unsigned)(class_of(sd.lp) * ntiles + tile);
+            const unsigned kc = (unsigned)(class_of(sd.lp) * ntiles + tile);
+            const unsigned k0 = csmall ? sent : kc;
             const unsigned k1 = skip1 ? sent : (unsigned)(class_of(sd.lp + 2) * ntiles + tile);
-            const unsigned k2 = skip2 ? sent : k0;
-            const unsigned kk[3] = {k0, k1, k2};
+            const unsigned k2 = skip2 ? sent : kc;
+            const unsigned k3 = csmall ? (unsigned)tile : sent;
+            const unsigned kk[NSET] = {k0, k1, k2, k3};
 #pragma unroll
             for (int a = 0; a < NSET; a++) {
                 keys[(long long)a * ne + e] = kk[a];
@@ -2156,9 +2354,10 @@ __global__ __launch_bounds__(256) void k_plan_entries(const SubDev *__restrict__
 }
 
 // records of one set in sorted order: position p holds item sorted_e[p]
+// (sub_off: ent_off is the sub-box's offset sub_off[sub] instead of off[item])
 __global__ void k_plan_records(const SubDev *__restrict__ subs, const int *__restrict__ ent_sub,
-                               const long long *__restrict__ off, const int *__restrict__ sorted_e, int n,
-                               EntryRec *__restrict__ recs) {
+                               const long long *__restrict__ off, const long long *__restrict__ sub_off,
+                               const int *__restrict__ sorted_e, int n, EntryRec *__restrict__ recs) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int e = sorted_e[p];
@@ -2173,7 +2372,7 @@ __global__ void k_plan_records(const SubDev *__restrict__ subs, const int *__res
     }
     r.zetp = sd.zetp;
     r.R2 = sd.R2;
-    r.ent_off = off[e];
+    r.ent_off = sub_off ? sub_off[ent_sub[e]] : off[e];
     r.lp = sd.lp;
     r.coef_off = sd.coef_off;
     r.prof_off = sd.prof_off;
@@ -2534,11 +2733,33 @@ hipError_t launch_plan_entries(const SubDev *subs, int nsub, const long long *en
     return hipGetLastError();
 }
 
-hipError_t launch_plan_records(const SubDev *subs, const int *ent_sub, const long long *off, const int *sorted_e,
-                               int n, EntryRec *recs, hipStream_t st) {
+hipError_t launch_plan_records(const SubDev *subs, const int *ent_sub, const long long *off,
+                               const long long *sub_off, const int *sorted_e, int n, EntryRec *recs,
+                               hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_plan_records, dim3((n + 255) / 256), dim3(256), 0, st, subs, ent_sub, off, sorted_e, n,
-                       recs);
+    hipLaunchKernelGGL(k_plan_records, dim3((n + 255) / 256), dim3(256), 0, st, subs, ent_sub, off, sub_off,
+                       sorted_e, n, recs);
+    return hipGetLastError();
+}
+
+// class c: L = lp + 1 in (4c, 4c + 4], c <= 2
+hipError_t launch_collocate_small(int cls, const TileGrid &tg, const SubDev *subs, const int *list, int n,
+                                  const double *coef, const int *prof, const long long *box_off, double *sval,
+                                  hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    switch (cls) {
+    case 0: hipLaunchKernelGGL(k_collocate_small<4>, dim3(n), dim3(NROW), 0, st, tg, subs, list, coef, prof, box_off, sval); break;
+    case 1: hipLaunchKernelGGL(k_collocate_small<8>, dim3(n), dim3(NROW), 0, st, tg, subs, list, coef, prof, box_off, sval); break;
+    case 2: hipLaunchKernelGGL(k_collocate_small<12>, dim3(n), dim3(NROW), 0, st, tg, subs, list, coef, prof, box_off, sval); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_small(const TileGrid &tg, const EntryRec *recs, const int *tile_start, int ntiles,
+                               const double *sval, double *grid, hipStream_t st) {
+    if (ntiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_small, dim3(ntiles), dim3(NROW), 0, st, tg, recs, tile_start, sval, grid);
     return hipGetLastError();
 }
 

@@ -184,6 +184,11 @@ struct gpw_tasklist {
     // sub-box ids of the pairs integrated by k_moments_small, per [force
     // extension][class of L]: d_small[small_off[w][c] .. + small_cnt[w][c])
     long long small_off[2][NSCLS] = {}, small_cnt[2][NSCLS] = {};
+    // sub-box ids of the pairs collocated by k_collocate_small, per class of
+    // L: d_csmall[csmall_off[c] .. + csmall_cnt[c]); their boxes in d_sval at
+    // d_box_off[sub] (box-major doubles, sval_total in all)
+    long long csmall_off[NSCLS] = {}, csmall_cnt[NSCLS] = {};
+    long long sval_total = 0;
 
     std::vector<PairDev> pairs_h;
     std::vector<SubDev> subs_h;
@@ -194,7 +199,10 @@ struct gpw_tasklist {
     int *d_prof = nullptr;
     int *d_tile_start = nullptr;  // [NSET][NCLS][ntiles + 1]
     int *d_small = nullptr;
-    unsigned char *d_small_mask = nullptr;  // [2][npairs]: pair integrated by k_moments_small
+    unsigned char *d_small_mask = nullptr;  // [3][npairs]: integrated (w/o, with forces) / collocated per pair
+    int *d_csmall = nullptr;
+    long long *d_box_off = nullptr;          // per sub-box: its small-collocate box in d_sval (-1: none)
+    double *d_sval = nullptr;
     EntryRec *d_recs = nullptr;   // item records in tile-list order, per (lp_add, class)
     int *d_pe_start = nullptr;
     long long *d_ent_off0 = nullptr, *d_ent_off2 = nullptr;
@@ -235,7 +243,8 @@ struct gpw_tasklist {
     // timing events, which a re-plan of the same grid keeps)
     void release_plan() {
         void **ptrs[] = {(void **)&d_pairs, (void **)&d_subs, (void **)&d_prof, (void **)&d_tile_start,
-                         (void **)&d_small, (void **)&d_small_mask, (void **)&d_recs, (void **)&d_pe_start,
+                         (void **)&d_small, (void **)&d_small_mask, (void **)&d_csmall, (void **)&d_box_off,
+                         (void **)&d_sval, (void **)&d_recs, (void **)&d_pe_start,
                          (void **)&d_ent_off0, (void **)&d_ent_off2, (void **)&d_coef, (void **)&d_partial,
                          (void **)&d_Mbuf, (void **)&d_mt_rp, (void **)&d_mt_col, (void **)&d_mt_val,
                          (void **)&d_mt_rpoff, (void **)&d_hab_ws, (void **)&d_order, (void **)&d_segs};
@@ -610,7 +619,9 @@ struct HostPlan {
     std::vector<int> tile_start;  // [NSET][NCLS][ntiles + 1]
     std::vector<EntryRec> recs;   // tile-list order, segments per (set, class)
     std::vector<int> small;       // sub-box ids for k_moments_small
-    std::vector<unsigned char> small_mask;  // [2][npairs]
+    std::vector<unsigned char> small_mask;  // [3][npairs]: integrate w/o, with forces; collocate
+    std::vector<int> csmall;      // sub-box ids for k_collocate_small
+    std::vector<long long> box_off;  // per sub-box: offset of its small-collocate box, -1 none
     long long coef_total = 0, partial_total = 0;
 };
 
@@ -708,7 +719,7 @@ static gpw_status plan_common(gpw_tasklist *tl, const std::vector<gpw_pair_desc>
     }
     // pairs for the small-sphere integrate kernel, per force extension
     hp.small.clear();
-    hp.small_mask.assign(2 * pairs.size(), 0);
+    hp.small_mask.assign(3 * pairs.size(), 0);
     for (int w = 0; w < 2; w++) {
         std::vector<int> by_cls[NSCLS];
         for (size_t p = 0; p < pairs.size(); p++) {
@@ -729,6 +740,37 @@ static gpw_status plan_common(gpw_tasklist *tl, const std::vector<gpw_pair_desc>
             tl->small_cnt[w][c] = (long long)by_cls[c].size();
             hp.small.insert(hp.small.end(), by_cls[c].begin(), by_cls[c].end());
         }
+    }
+    // pairs for the small-sphere collocate, their boxes in sub-box order
+    hp.csmall.clear();
+    hp.box_off.assign(tl->subs_h.size(), -1);
+    {
+        const int lim = tl->tg.general ? CSMALL_N : CSMALL_N_ORTHO;
+        std::vector<int> by_cls[NSCLS];
+        long long tot = 0;
+        for (size_t p = 0; p < pairs.size() && lim > 0; p++) {
+            if (tl->pair_sub_start[p + 1] - tl->pair_sub_start[p] != 1) continue;
+            const int si = tl->pair_sub_start[p];
+            const SubDev &sd = tl->subs_h[si];
+            const int L = sd.lp + 1;
+            long long vol = 1;
+            bool fits = L <= SMALL_LMAX;
+            for (int d = 0; d < 3; d++) {
+                fits = fits && sd.ub[d] - sd.lb[d] + 1 <= lim;
+                vol *= sd.ub[d] - sd.lb[d] + 1;
+            }
+            if (!fits) continue;
+            hp.small_mask[2 * pairs.size() + p] = 1;
+            hp.box_off[si] = tot;
+            tot += vol;
+            by_cls[(L - 1) / 4].push_back(si);
+        }
+        for (int c = 0; c < NSCLS; c++) {
+            tl->csmall_off[c] = (long long)hp.csmall.size();
+            tl->csmall_cnt[c] = (long long)by_cls[c].size();
+            hp.csmall.insert(hp.csmall.end(), by_cls[c].begin(), by_cls[c].end());
+        }
+        tl->sval_total = tot;
     }
     tl->mstride = ncoset(tl->lp_max + 2);
     return GPW_OK;
@@ -803,9 +845,11 @@ static gpw_status plan_entries_host(gpw_tasklist *tl, HostPlan &hp) {
         std::vector<int> cls(ne);
         for (size_t e = 0; e < ne; e++) {
             const SubDev &sd = tl->subs_h[hp.ent_sub[e]];
-            const bool skip = a == 1 ? hp.small_mask[npairs + sd.pair] != 0
-                                     : a == 2 ? hp.small_mask[sd.pair] != 0 : false;
-            cls[e] = skip ? -1 : class_of(sd.lp + lp_add);
+            const bool csm = hp.small_mask[2 * npairs + sd.pair] != 0;
+            const bool skip = a == 0 ? csm
+                            : a == 1 ? hp.small_mask[npairs + sd.pair] != 0
+                            : a == 2 ? hp.small_mask[sd.pair] != 0 : !csm;
+            cls[e] = skip ? -1 : a == 3 ? 0 : class_of(sd.lp + lp_add);
         }
         for (int c = 0; c < NCLS; c++) {
             int *start = hp.tile_start.data() + (size_t)(a * NCLS + c) * nt1;
@@ -829,7 +873,7 @@ static gpw_status plan_entries_host(gpw_tasklist *tl, HostPlan &hp) {
                 }
                 r.zetp = sd.zetp;
                 r.R2 = sd.R2;
-                r.ent_off = a == 1 ? hp.off2[e] : hp.off0[e];
+                r.ent_off = a == 3 ? hp.box_off[hp.ent_sub[e]] : a == 1 ? hp.off2[e] : hp.off0[e];
                 r.lp = sd.lp;
                 r.coef_off = sd.coef_off;
                 r.prof_off = sd.prof_off;
@@ -1054,7 +1098,8 @@ static gpw_status plan_device(gpw_tasklist *tl, HostPlan &hp, hipStream_t st) {
         if (nvalid == 0) continue;
         size_t tb = tbytes;
         HIP_TRY(sort_pairs_u32(d_temp, &tb, d_keys + (size_t)a * n, d_keys_out, d_vals, d_sorted, n, end_bit, st));
-        HIP_TRY(launch_plan_records(tl->d_subs, d_ent_sub, a == 1 ? tl->d_ent_off2 : tl->d_ent_off0, d_sorted,
+        HIP_TRY(launch_plan_records(tl->d_subs, d_ent_sub, a == 1 ? tl->d_ent_off2 : tl->d_ent_off0,
+                                    a == 3 ? tl->d_box_off : nullptr, d_sorted,
                                     (int)nvalid, tl->d_recs + set_base[a], st));
     }
     HIP_TRY(hipStreamSynchronize(st));
@@ -1076,6 +1121,9 @@ static gpw_status upload_tasklist(gpw_tasklist *tl, HostPlan &hp, bool device_it
     tl->tg.prof = tl->d_prof;
     if ((st = upload(&tl->d_small, hp.small))) return st;
     if ((st = upload(&tl->d_small_mask, hp.small_mask))) return st;
+    if ((st = upload(&tl->d_csmall, hp.csmall))) return st;
+    if ((st = upload(&tl->d_box_off, hp.box_off))) return st;
+    if (tl->sval_total > 0) HIP_TRY(hipMalloc((void **)&tl->d_sval, sizeof(double) * tl->sval_total));
     if (device_items) {
         const double t0 = now_ms();
         tl->plan_ms[1] += t0 - tl->plan_ms[3];  // uploads since the host plan
@@ -1361,6 +1409,11 @@ gpw_status gpw_tasklist_stats_get(const gpw_tasklist *tl, gpw_tasklist_stats *o)
     o->flops_integrate[0] = tl->flops_integrate[0];
     o->flops_integrate[1] = tl->flops_integrate[1];
     for (int i = 0; i < 4; i++) o->plan_ms[i] = tl->plan_ms[i];
+    o->csmall_pairs = tl->csmall_cnt[0] + tl->csmall_cnt[1] + tl->csmall_cnt[2];
+    o->csmall_points = tl->sval_total;
+    long long ce = 0;
+    for (int c = 0; c < NCLS; c++) ce += tl->class_entries[0][c];
+    o->collocate_tile_points = ce * (long long)(TX * TY * TZ);
     for (int c = 0; c < NCLS; c++) {
         o->csplits[c] = tl->csplit[c];
         o->isplits[0][c] = tl->isplit[0][c];
@@ -1603,11 +1656,22 @@ gpw_status gpw_tasklist_collocate(gpw_tasklist *tl, const double *pab, double *g
     {
         PhaseTimer ph(tl, st, PH_CTILES);
         HIP_TRY(collocate_classes(tl, grid, smax, st));
+        // small spheres: each pair's box into the scratch buffer
+        for (int c = 0; c < NSCLS; c++)
+            HIP_TRY(launch_collocate_small(c, tl->tg, tl->d_subs, tl->d_csmall + tl->csmall_off[c],
+                                           (int)tl->csmall_cnt[c], tl->d_coef, tl->d_prof, tl->d_box_off,
+                                           tl->d_sval, st));
     }
-    if (smax > 1) {
+    const bool gather = tl->class_entries[3][0] > 0;
+    if (smax > 1 || gather) {
         PhaseTimer ph(tl, st, PH_CREDUCE);
         const long long ng = (long long)tl->tg.nx * tl->tg.ny * tl->tg.nz;
-        HIP_TRY(launch_reduce_partials(tl->d_cpartial, smax, ng, grid, st));
+        if (smax > 1) HIP_TRY(launch_reduce_partials(tl->d_cpartial, smax, ng, grid, st));
+        // then the small boxes, tile by tile in list order
+        if (gather)
+            HIP_TRY(launch_gather_small(tl->tg, tl->d_recs + tl->list_off[3][0],
+                                        tl->d_tile_start + (size_t)3 * NCLS * (tl->ntiles + 1), tl->ntiles,
+                                        tl->d_sval, grid, st));
     }
     return GPW_OK;
 }

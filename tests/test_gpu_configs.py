@@ -394,3 +394,39 @@ def test_collocator_replans_in_place(gpw, oracle, torch_cuda):
     grid = col(dev(torch, cfg["pab"]), torch.tensor(ra_h), torch.tensor(rb_h))
     assert col.replans == 2
     col.close()
+
+
+def test_small_sphere_collocate_general_cell(gpw, oracle, torch_cuda):
+    """General cell with steep and soft pairs: the steep ones (cube one
+    sub-box of at most 16 points per axis) are collocated into per-pair boxes
+    and added tile by tile in list order, the soft ones by the tile kernels.
+    Both against the oracle (same support, 1e-10), with one and three
+    workgroups per tile; repeated calls give bitwise equal grids (every grid
+    point is summed in one fixed order)."""
+    from libgpw_amd import workloads as Wk
+    torch = torch_cuda
+    rng = np.random.default_rng(123)
+    npts = (48, 40, 44)
+    cell = Wk.triclinic_cell(9.6, 8.0, 8.8, 90.0, 90.0, 75.0)
+    dh = cell / np.asarray(npts, dtype=np.float64)[:, None]
+    steep, _, _ = Wk.make_pairs(rng, 300, 3, 3, 4.0, 40.0, cell)
+    soft, _, _ = Wk.make_pairs(rng, 60, 2, 2, 0.2, 1.0, cell)
+    pairs = {k: np.concatenate([np.asarray(steep[k]), np.asarray(soft[k])]) for k in steep}
+    off, nblock = Wk._blocks(pairs["la_max"], pairs["lb_max"])
+    pairs["block_offset"] = off
+    cfg = {"npts": npts, "dh": dh, "pairs": pairs, "pab": rng.uniform(-1.0, 1.0, nblock), "nblock": nblock}
+    say("small-sphere collocate, general cell", len(pairs["la_max"]), "pairs")
+    st = gpw.plan_stats(gpw.grid_desc(npts, dh), pairs)
+    assert st["csmall_pairs"] >= 200 and st["csmall_pairs"] < len(pairs["la_max"]), st["csmall_pairs"]
+    assert st["csmall_points"] > 0
+    ref = oracle_batch(oracle, cfg)
+    first = None
+    for splits in (1, 1, 3):
+        got = run_tasklist(gpw, torch, cfg, splits=splits)
+        assert got["stats"]["csmall_pairs"] == st["csmall_pairs"]
+        assert same_support(got["grid"], ref["grid"]), splits
+        assert rel_err(got["grid"], ref["grid"]) < TOL, splits
+        if first is None:
+            first = got["grid"]
+        elif splits == 1:
+            assert np.array_equal(got["grid"], first)
