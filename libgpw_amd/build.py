@@ -28,6 +28,10 @@ COMMON = [
 ]
 
 SOURCES = [
+    ("gpw_tiles_colloc_ortho.hip", []),
+    ("gpw_tiles_colloc_general.hip", []),
+    ("gpw_tiles_integ_ortho.hip", []),
+    ("gpw_tiles_integ_general.hip", []),
     ("gpw_kernels.hip", []),
     ("gpw_plan.cpp", ["-ffp-contract=off"]),
     ("gpw_single.cpp", ["-ffp-contract=off"]),
@@ -46,14 +50,19 @@ def _run(cmd):
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
-    objs = []
+    objs, cmds = [], []
     for src, extra in SOURCES:
         obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-        cmd = [HIPCC, "-x", "hip", *COMMON, *extra, "-c", os.path.join(CSRC, src), "-o", obj]
-        if verbose:
-            print(" ".join(cmd))
-        _run(cmd)
+        cmds.append([HIPCC, "-x", "hip", *COMMON, *extra, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+    # the translation units compile independently: run them side by side
+    from concurrent.futures import ThreadPoolExecutor
+    if verbose:
+        for cmd in cmds:
+            print(" ".join(cmd))
+    with ThreadPoolExecutor(max_workers=min(len(cmds), max(1, os.cpu_count() or 1))) as ex:
+        for f in [ex.submit(_run, cmd) for cmd in cmds]:
+            f.result()
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp])
     os.replace(tmp, LIB)

@@ -1,0 +1,273 @@
+// gpw_device.hpp -- device helpers shared by the HIP translation units:
+// index helpers, the binomial table and alpha (cab -> cxyz coefficients),
+// coset decoding, the effective (tau-shifted) block, the lattice transform of
+// general cells, the register-block layout of the tile kernels and the
+// exact row bounds of general cells.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "gpw_internal.hpp"
+
+namespace gpw {
+
+__device__ __forceinline__ int imod(int a, int m) {
+    int r = a % m;
+    return r < 0 ? r + m : r;
+}
+
+// mirror index of an offset: the reference pairs offsets ig <= 0 with 1 - ig
+// (grid_base_ref_c.cc:418-474); ii' = -ig for both members of the pair.
+__device__ __forceinline__ int fold_off(int i) { return i <= 0 ? -i : i - 1; }
+
+// row threshold of the sphere: offsets along x with fold_off(i) <= T are inside
+__device__ __forceinline__ int row_threshold(const int *__restrict__ prof, int prof_off,
+                                             int stride, int kk, int jj) {
+    if (kk < 0 || jj < 0) return -1;
+    if (prof_off < 0) return BIG;
+    return prof[prof_off + kk * stride + jj];
+}
+
+// ---------------------------------------------------------------------------
+// coefficient transform: pab block -> coef_xyz (cab -> cxyz)
+// (grid_base_ref_c.cc:65-315 as the direct triple product)
+
+struct AlphaView {
+    double *a;  // [3][lbm+1][lam+1][lpm+1]
+    int lam, lbm, lpm;
+    __device__ double &at(int d, int lb, int la, int k) const {
+        return a[((d * (lbm + 1) + lb) * (lam + 1) + la) * (lpm + 1) + k];
+    }
+};
+
+// binomial coefficients C(n, k), n < 20 (Pascal's triangle at compile time)
+struct BinomTab {
+    int c[20][20];
+};
+__host__ __device__ constexpr BinomTab make_binom() {
+    BinomTab t{};
+    for (int n = 0; n < 20; n++) {
+        t.c[n][0] = 1;
+        for (int k = 1; k <= n; k++) t.c[n][k] = t.c[n - 1][k - 1] + (k < n ? t.c[n - 1][k] : 0);
+    }
+    return t;
+}
+static __constant__ BinomTab kBinom = make_binom();
+
+// alpha_d(lb, la, k): coefficient of (x - xp)^k in (x - xa)^la (x - xb)^lb,
+// from the binomial expansions about xp (grid_base_ref_c.cc:195-230, with
+// exact integer binomials).
+// pp points to the pair in global memory: its per-axis fields are read with a
+// run-time axis index, which on a register copy would force it to scratch
+__device__ inline void fill_alpha(const PairDev *__restrict__ pp, AlphaView A, int tid, int nthr) {
+    const int n = 3 * (A.lbm + 1) * (A.lam + 1);
+    for (int w = tid; w < n; w += nthr) {
+        const int la = w % (A.lam + 1);
+        const int lb = (w / (A.lam + 1)) % (A.lbm + 1);
+        const int d = w / ((A.lam + 1) * (A.lbm + 1));
+        for (int k = 0; k <= A.lpm; k++) A.at(d, lb, la, k) = 0.0;
+        const double pa = pp->rp[d] - pp->ra[d];
+        const double pb = pp->rp[d] - pp->rb[d];
+        double powa = 1.0;
+        for (int i = 0; i <= la; i++) {
+            // C(la, la - i) = C(la, i): the term (x - xp)^(la - i) (xp - xa)^i
+            const double ca = (double)kBinom.c[la][i];
+            double powb = 1.0;
+            for (int j = 0; j <= lb; j++) {
+                const double cb = (double)kBinom.c[lb][j];
+                A.at(d, lb, la, (la - i) + (lb - j)) += ca * cb * powa * powb;
+                powb *= pb;
+            }
+            powa *= pa;
+        }
+    }
+}
+
+__device__ __forceinline__ void decode_coset(int l, int c, int &lx, int &ly, int &lz) {
+    // inverse of coset() within shell l: c = ((l-lx)(l-lx+1))/2 + lz
+    int m = 0;
+    while (((m + 1) * (m + 2)) / 2 <= c) m++;
+    lx = l - m;
+    lz = c - (m * (m + 1)) / 2;
+    ly = l - lx - lz;
+}
+
+__device__ __forceinline__ void decode_c3(int lp, int c, int &lzp, int &lyp, int &lxp) {
+    lzp = 0;
+    while (c >= ncoef2(lp - lzp)) {
+        c -= ncoef2(lp - lzp);
+        lzp++;
+    }
+    const int m = lp - lzp;
+    lyp = 0;
+    while (c > m - lyp) {
+        c -= m - lyp + 1;
+        lyp++;
+    }
+    lxp = c;
+}
+
+__device__ __forceinline__ void decode_c2(int lp, int c, int &lyp, int &lxp) {
+    lyp = 0;
+    while (c > lp - lyp) {
+        c -= lp - lyp + 1;
+        lyp++;
+    }
+    lxp = c;
+}
+
+// Cartesian exponents of every coset index c < NCOSET_LMAX, packed as
+// lx | ly << 4 | lz << 8 (a table in LDS instead of the shell search and
+// decode per lookup)
+__device__ inline void fill_decode(int *tab, int tid, int nthr) {
+    for (int c = tid; c < NCOSET_LMAX; c += nthr) {
+        int l = 0;
+        while (ncoset(l) <= c) l++;
+        int x, y, z;
+        decode_coset(l, c - ncoset(l - 1), x, y, z);
+        tab[c] = x | (y << 4) | (z << 8);
+    }
+}
+
+// (lyp, lxp) of every packed index c < C2(lp), as lyp | lxp << 8
+constexpr int NC2MAX = ncoef2(LPMAX + 2);
+__device__ inline void fill_decode2(int *tab, int lp, int tid, int nthr) {
+    for (int c = tid; c < ncoef2(lp); c += nthr) {
+        int y, x;
+        decode_c2(lp, c, y, x);
+        tab[c] = y | (x << 8);
+    }
+}
+
+// effective block of the local (tau-shifted) ranges, eff[jco*ldE + ico]
+__device__ inline void build_eff(const PairDev &p, int tau, const double *__restrict__ pab, double *eff,
+                          int ldE, int nlb, int nla, int tid, int nthr) {
+    for (int w = tid; w < nla * nlb; w += nthr) eff[(w / nla) * ldE + (w % nla)] = 0.0;
+    __syncthreads();
+    // one thread per original (a, b) entry; tau scatters to neighbours, which
+    // can collide, so the tau branch is serialised over b inside one thread.
+    const int na0 = ncoset(p.la_max), na_lo = ncoset(p.la_min - 1);
+    const int nb0 = ncoset(p.lb_max), nb_lo = ncoset(p.lb_min - 1);
+    if (!tau) {
+        for (int w = tid; w < (na0 - na_lo) * (nb0 - nb_lo); w += nthr) {
+            const int ico = na_lo + w % (na0 - na_lo);
+            const int jco = nb_lo + w / (na0 - na_lo);
+            eff[jco * ldE + ico] = pab[p.block_off + (long long)jco * p.ld + ico];
+        }
+        __syncthreads();
+        return;
+    }
+    if (tid == 0) {
+        const double za = p.zeta, zb = p.zetb;
+        for (int jco = nb_lo; jco < nb0; jco++) {
+            int lb = 0;
+            while (ncoset(lb) <= jco) lb++;
+            int bx, by, bz;
+            decode_coset(lb, jco - ncoset(lb - 1), bx, by, bz);
+            const int b[3] = {bx, by, bz};
+            for (int ico = na_lo; ico < na0; ico++) {
+                int la = 0;
+                while (ncoset(la) <= ico) la++;
+                int ax, ay, az;
+                decode_coset(la, ico - ncoset(la - 1), ax, ay, az);
+                const int a[3] = {ax, ay, az};
+                const double v = pab[p.block_off + (long long)jco * p.ld + ico];
+                for (int d = 0; d < 3; d++) {
+                    int am[3] = {a[0], a[1], a[2]}, ap[3] = {a[0], a[1], a[2]};
+                    int bm[3] = {b[0], b[1], b[2]}, bp[3] = {b[0], b[1], b[2]};
+                    am[d] = a[d] > 0 ? a[d] - 1 : 0;
+                    bm[d] = b[d] > 0 ? b[d] - 1 : 0;
+                    ap[d] = a[d] + 1;
+                    bp[d] = b[d] + 1;
+                    const int iam = coset(am[0], am[1], am[2]), iap = coset(ap[0], ap[1], ap[2]);
+                    const int jbm = coset(bm[0], bm[1], bm[2]), jbp = coset(bp[0], bp[1], bp[2]);
+                    eff[jbm * ldE + iam] += 0.5 * a[d] * b[d] * v;
+                    eff[jbp * ldE + iam] += 0.5 * (-2.0 * a[d] * zb) * v;
+                    eff[jbm * ldE + iap] += 0.5 * (-2.0 * za * b[d]) * v;
+                    eff[jbp * ldE + iap] += 0.5 * (4.0 * za * zb) * v;
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// General cells: out = T in (dir 0) or T^T in (dir 1) for the monomials of
+// total degree <= lp, from the CSR rows of (lp, dir).
+__device__ inline void lattice_transform(const LatticeCSR &lt, int lp, int dir, const double *in,
+                                  double *out, int tid, int nthr) {
+    const int *rp = lt.rp + lt.rp_off[2 * lp + dir];
+    for (int r = tid; r < ncoset(lp); r += nthr) {
+        double a = 0.0;
+        for (int k = rp[r]; k < rp[r + 1]; k++) a += lt.val[k] * in[lt.col[k]];
+        out[r] = a;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// tile kernels
+//
+// Workgroup = one TX x TY x TZ tile (32 x 16 x 16 points), 256 threads. Thread
+// t owns a register block of RB = 4 consecutive rows (same z) x XB = 8 points:
+//   xg = t & 3 (x block), yb = (t >> 2) & 3 (row block), z = t >> 4.
+// The 16 threads of one z plane are one DPP row (16 consecutive lanes).
+// Every LDS table value read in the x stage feeds RB FMAs.
+//
+// A workgroup walks its work items in chunks of Q pairs. Table phase: wave q
+// builds everything of pair q of the chunk (1D tables, x codes and block
+// classes, the z-stage, the sphere threshold of every tile row) using only
+// LDS and in-wave shuffles. The item records of the chunk after next and the
+// coefficients of the next chunk are loaded into registers at the start of a
+// chunk and stored to LDS rings at its end, so global latency stays off the
+// critical path.
+
+constexpr int XB = 8, RB = 4;
+constexpr int NXG = TX / XB;  // x blocks per tile
+constexpr int RECW = (int)(sizeof(EntryRec) / 4);
+static_assert(TX + TY + TZ == 64, "table phase maps one wave per pair");
+static_assert(NXG * (TY / RB) * TZ == NROW, "one register block per thread");
+static_assert(NXG * (TY / RB) == 16, "one z plane per DPP row");
+static_assert(TX == 4 * XB && TY == 16 && TZ == 16, "lane layout of the table phase");
+
+// Index-set predicates. They must round exactly as the host planner (built
+// with -ffp-contract=off) does, so every product is kept apart from the sum
+// that consumes it: the functions below are written with plain operators
+// under "#pragma clang fp contract(off)" (the __dmul_rn / __dadd_rn helpers
+// of the HIP headers are plain operators compiled in the header's scope and
+// are fused into FMAs after inlining, which moves points that lie exactly on
+// a sphere or row bound in or out).
+
+// General cells: i-range of lattice row (j, k) inside the sphere, with the
+// operation order of grid_base_ref_c.cc:790-811 and no contraction, so the
+// index set equals the host's and the oracle's; qb, qc: the row's quadratic
+// scaled by -zetp (exponent of point i: (qa*i + qb)*i + qc).
+__device__ __forceinline__ void general_row_dev(const TileGrid &tg, double gp0, double dj, double dk,
+                                                double R2, double zetp, int &lo, int &hi,
+                                                double &qb, double &qc) {
+#pragma clang fp contract(off)
+    double a = 0.0, b = 0.0, c = 0.0;
+#pragma unroll
+    for (int x = 0; x < 3; x++) {
+        const double pointk = tg.dh[2][x] * dk;
+        const double pointj = pointk + tg.dh[1][x] * dj;
+        const double v = pointj - gp0 * tg.dh[0][x];
+        a = a + tg.dh[0][x] * tg.dh[0][x];
+        b = b + (2.0 * v) * tg.dh[0][x];
+        c = c + v * v;
+    }
+    double d = b * b - (4.0 * a) * (c - R2);
+    if (d < 0.0) {
+        lo = 1;
+        hi = 0;
+        qb = qc = 0.0;
+        return;
+    }
+    d = __dsqrt_rn(d);
+    const double a2 = 2.0 * a;
+    lo = (int)ceil((-b - d) / a2);
+    hi = (int)floor((-b + d) / a2);
+    qb = b * -zetp;
+    qc = c * -zetp;
+}
+
+}  // namespace gpw

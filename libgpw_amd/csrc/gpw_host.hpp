@@ -66,7 +66,19 @@ void multinomial_matrix(const double dh[3][3], int lp, double *T);
 gpw_status lattice_csr(const double dh[3][3], int lpm, LatticeHost &out);
 gpw_status lattice_csr_some(const double dh[3][3], const int *lps, int n, LatticeHost &out);
 
-// kernel launchers (gpw_kernels.hip)
+// tile-kernel launchers per cell kind (GEN: general cell), defined in
+// gpw_tiles_colloc.hpp / gpw_tiles_integ.hpp and instantiated in the
+// gpw_tiles_{colloc,integ}_{ortho,general}.hip translation units
+template <bool GEN>
+hipError_t launch_collocate_tiles_t(int cls, const TileGrid &tg, const EntryRec *recs,
+                                    const int *tile_start, const double *coef, int ntiles, int splits,
+                                    double *out, int out_mode, hipStream_t st);
+template <bool GEN>
+hipError_t launch_integrate_tiles_t(int cls, const TileGrid &tg, const EntryRec *recs,
+                                    const int *tile_start, const double *grid, int lp_add,
+                                    int ntiles, int splits, double *partial, hipStream_t st);
+
+// kernel launchers (gpw_kernels.hip, gpw_tiles_integ_ortho.hip)
 hipError_t launch_collocate_tiles(int cls, const TileGrid &tg, const EntryRec *recs,
                                   const int *tile_start, const double *coef, int ntiles, int splits,
                                   double *out, int out_mode, hipStream_t st);

@@ -17,207 +17,14 @@
 //     each. Each grid point is written by exactly one workgroup (or one split
 //     buffer): no atomics, results are deterministic.
 //   * plain kernels: one launch per sub-box of one pair (the baseline form).
-#include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 
-#include "gpw_internal.hpp"
+#include "gpw_device.hpp"
+#include "gpw_host.hpp"
 
 namespace gpw {
-
-__device__ __forceinline__ int imod(int a, int m) {
-    int r = a % m;
-    return r < 0 ? r + m : r;
-}
-
-// mirror index of an offset: the reference pairs offsets ig <= 0 with 1 - ig
-// (grid_base_ref_c.cc:418-474); ii' = -ig for both members of the pair.
-__device__ __forceinline__ int fold_off(int i) { return i <= 0 ? -i : i - 1; }
-
-// row threshold of the sphere: offsets along x with fold_off(i) <= T are inside
-__device__ __forceinline__ int row_threshold(const int *__restrict__ prof, int prof_off,
-                                             int stride, int kk, int jj) {
-    if (kk < 0 || jj < 0) return -1;
-    if (prof_off < 0) return BIG;
-    return prof[prof_off + kk * stride + jj];
-}
-
-// ---------------------------------------------------------------------------
-// coefficient transform: pab block -> coef_xyz (cab -> cxyz)
-// (grid_base_ref_c.cc:65-315 as the direct triple product)
-
-struct AlphaView {
-    double *a;  // [3][lbm+1][lam+1][lpm+1]
-    int lam, lbm, lpm;
-    __device__ double &at(int d, int lb, int la, int k) const {
-        return a[((d * (lbm + 1) + lb) * (lam + 1) + la) * (lpm + 1) + k];
-    }
-};
-
-// binomial coefficients C(n, k), n < 20 (Pascal's triangle at compile time)
-struct BinomTab {
-    int c[20][20];
-};
-__host__ __device__ constexpr BinomTab make_binom() {
-    BinomTab t{};
-    for (int n = 0; n < 20; n++) {
-        t.c[n][0] = 1;
-        for (int k = 1; k <= n; k++) t.c[n][k] = t.c[n - 1][k - 1] + (k < n ? t.c[n - 1][k] : 0);
-    }
-    return t;
-}
-__constant__ BinomTab kBinom = make_binom();
-
-// alpha_d(lb, la, k): coefficient of (x - xp)^k in (x - xa)^la (x - xb)^lb,
-// from the binomial expansions about xp (grid_base_ref_c.cc:195-230, with
-// exact integer binomials).
-// pp points to the pair in global memory: its per-axis fields are read with a
-// run-time axis index, which on a register copy would force it to scratch
-__device__ void fill_alpha(const PairDev *__restrict__ pp, AlphaView A, int tid, int nthr) {
-    const int n = 3 * (A.lbm + 1) * (A.lam + 1);
-    for (int w = tid; w < n; w += nthr) {
-        const int la = w % (A.lam + 1);
-        const int lb = (w / (A.lam + 1)) % (A.lbm + 1);
-        const int d = w / ((A.lam + 1) * (A.lbm + 1));
-        for (int k = 0; k <= A.lpm; k++) A.at(d, lb, la, k) = 0.0;
-        const double pa = pp->rp[d] - pp->ra[d];
-        const double pb = pp->rp[d] - pp->rb[d];
-        double powa = 1.0;
-        for (int i = 0; i <= la; i++) {
-            // C(la, la - i) = C(la, i): the term (x - xp)^(la - i) (xp - xa)^i
-            const double ca = (double)kBinom.c[la][i];
-            double powb = 1.0;
-            for (int j = 0; j <= lb; j++) {
-                const double cb = (double)kBinom.c[lb][j];
-                A.at(d, lb, la, (la - i) + (lb - j)) += ca * cb * powa * powb;
-                powb *= pb;
-            }
-            powa *= pa;
-        }
-    }
-}
-
-__device__ __forceinline__ void decode_coset(int l, int c, int &lx, int &ly, int &lz) {
-    // inverse of coset() within shell l: c = ((l-lx)(l-lx+1))/2 + lz
-    int m = 0;
-    while (((m + 1) * (m + 2)) / 2 <= c) m++;
-    lx = l - m;
-    lz = c - (m * (m + 1)) / 2;
-    ly = l - lx - lz;
-}
-
-__device__ __forceinline__ void decode_c3(int lp, int c, int &lzp, int &lyp, int &lxp) {
-    lzp = 0;
-    while (c >= ncoef2(lp - lzp)) {
-        c -= ncoef2(lp - lzp);
-        lzp++;
-    }
-    const int m = lp - lzp;
-    lyp = 0;
-    while (c > m - lyp) {
-        c -= m - lyp + 1;
-        lyp++;
-    }
-    lxp = c;
-}
-
-__device__ __forceinline__ void decode_c2(int lp, int c, int &lyp, int &lxp) {
-    lyp = 0;
-    while (c > lp - lyp) {
-        c -= lp - lyp + 1;
-        lyp++;
-    }
-    lxp = c;
-}
-
-// Cartesian exponents of every coset index c < NCOSET_LMAX, packed as
-// lx | ly << 4 | lz << 8 (a table in LDS instead of the shell search and
-// decode per lookup)
-__device__ void fill_decode(int *tab, int tid, int nthr) {
-    for (int c = tid; c < NCOSET_LMAX; c += nthr) {
-        int l = 0;
-        while (ncoset(l) <= c) l++;
-        int x, y, z;
-        decode_coset(l, c - ncoset(l - 1), x, y, z);
-        tab[c] = x | (y << 4) | (z << 8);
-    }
-}
-
-// (lyp, lxp) of every packed index c < C2(lp), as lyp | lxp << 8
-constexpr int NC2MAX = ncoef2(LPMAX + 2);
-__device__ void fill_decode2(int *tab, int lp, int tid, int nthr) {
-    for (int c = tid; c < ncoef2(lp); c += nthr) {
-        int y, x;
-        decode_c2(lp, c, y, x);
-        tab[c] = y | (x << 8);
-    }
-}
-
-// effective block of the local (tau-shifted) ranges, eff[jco*ldE + ico]
-__device__ void build_eff(const PairDev &p, int tau, const double *__restrict__ pab, double *eff,
-                          int ldE, int nlb, int nla, int tid, int nthr) {
-    for (int w = tid; w < nla * nlb; w += nthr) eff[(w / nla) * ldE + (w % nla)] = 0.0;
-    __syncthreads();
-    // one thread per original (a, b) entry; tau scatters to neighbours, which
-    // can collide, so the tau branch is serialised over b inside one thread.
-    const int na0 = ncoset(p.la_max), na_lo = ncoset(p.la_min - 1);
-    const int nb0 = ncoset(p.lb_max), nb_lo = ncoset(p.lb_min - 1);
-    if (!tau) {
-        for (int w = tid; w < (na0 - na_lo) * (nb0 - nb_lo); w += nthr) {
-            const int ico = na_lo + w % (na0 - na_lo);
-            const int jco = nb_lo + w / (na0 - na_lo);
-            eff[jco * ldE + ico] = pab[p.block_off + (long long)jco * p.ld + ico];
-        }
-        __syncthreads();
-        return;
-    }
-    if (tid == 0) {
-        const double za = p.zeta, zb = p.zetb;
-        for (int jco = nb_lo; jco < nb0; jco++) {
-            int lb = 0;
-            while (ncoset(lb) <= jco) lb++;
-            int bx, by, bz;
-            decode_coset(lb, jco - ncoset(lb - 1), bx, by, bz);
-            const int b[3] = {bx, by, bz};
-            for (int ico = na_lo; ico < na0; ico++) {
-                int la = 0;
-                while (ncoset(la) <= ico) la++;
-                int ax, ay, az;
-                decode_coset(la, ico - ncoset(la - 1), ax, ay, az);
-                const int a[3] = {ax, ay, az};
-                const double v = pab[p.block_off + (long long)jco * p.ld + ico];
-                for (int d = 0; d < 3; d++) {
-                    int am[3] = {a[0], a[1], a[2]}, ap[3] = {a[0], a[1], a[2]};
-                    int bm[3] = {b[0], b[1], b[2]}, bp[3] = {b[0], b[1], b[2]};
-                    am[d] = a[d] > 0 ? a[d] - 1 : 0;
-                    bm[d] = b[d] > 0 ? b[d] - 1 : 0;
-                    ap[d] = a[d] + 1;
-                    bp[d] = b[d] + 1;
-                    const int iam = coset(am[0], am[1], am[2]), iap = coset(ap[0], ap[1], ap[2]);
-                    const int jbm = coset(bm[0], bm[1], bm[2]), jbp = coset(bp[0], bp[1], bp[2]);
-                    eff[jbm * ldE + iam] += 0.5 * a[d] * b[d] * v;
-                    eff[jbp * ldE + iam] += 0.5 * (-2.0 * a[d] * zb) * v;
-                    eff[jbm * ldE + iap] += 0.5 * (-2.0 * za * b[d]) * v;
-                    eff[jbp * ldE + iap] += 0.5 * (4.0 * za * zb) * v;
-                }
-            }
-        }
-    }
-    __syncthreads();
-}
-
-// General cells: out = T in (dir 0) or T^T in (dir 1) for the monomials of
-// total degree <= lp, from the CSR rows of (lp, dir).
-__device__ void lattice_transform(const LatticeCSR &lt, int lp, int dir, const double *in,
-                                  double *out, int tid, int nthr) {
-    const int *rp = lt.rp + lt.rp_off[2 * lp + dir];
-    for (int r = tid; r < ncoset(lp); r += nthr) {
-        double a = 0.0;
-        for (int k = rp[r]; k < rp[r + 1]; k++) a += lt.val[k] * in[lt.col[k]];
-        out[r] = a;
-    }
-}
 
 // cab -> cxyz of one pair per workgroup, factorised over the Cartesian axes
 // (the three-step contraction of grid_base_ref_c.cc:234-315, in two passes):
@@ -410,618 +217,6 @@ __global__ __launch_bounds__(TW) void k_coef_wave(const PairDev *__restrict__ pa
     }
 }
 
-// ---------------------------------------------------------------------------
-// tile kernels
-//
-// Workgroup = one TX x TY x TZ tile (32 x 16 x 16 points), 256 threads. Thread
-// t owns a register block of RB = 4 consecutive rows (same z) x XB = 8 points:
-//   xg = t & 3 (x block), yb = (t >> 2) & 3 (row block), z = t >> 4.
-// The 16 threads of one z plane are one DPP row (16 consecutive lanes).
-// Every LDS table value read in the x stage feeds RB FMAs.
-//
-// A workgroup walks its work items in chunks of Q pairs. Table phase: wave q
-// builds everything of pair q of the chunk (1D tables, x codes and block
-// classes, the z-stage, the sphere threshold of every tile row) using only
-// LDS and in-wave shuffles. The item records of the chunk after next and the
-// coefficients of the next chunk are loaded into registers at the start of a
-// chunk and stored to LDS rings at its end, so global latency stays off the
-// critical path.
-
-constexpr int XB = 8, RB = 4;
-constexpr int NXG = TX / XB;  // x blocks per tile
-constexpr int RECW = (int)(sizeof(EntryRec) / 4);
-static_assert(TX + TY + TZ == 64, "table phase maps one wave per pair");
-static_assert(NXG * (TY / RB) * TZ == NROW, "one register block per thread");
-static_assert(NXG * (TY / RB) == 16, "one z plane per DPP row");
-static_assert(TX == 4 * XB && TY == 16 && TZ == 16, "lane layout of the table phase");
-
-template <int LM>
-struct TileCfg {
-    static constexpr int Q = LM <= 8 ? 4 : 2;           // pairs per chunk
-    static constexpr int C2M = LM * (LM + 1) / 2;       // (lyp, lxp) terms
-    static constexpr int C3M = LM * (LM + 1) * (LM + 2) / 6;
-    static constexpr int GS = XB * LM + 4;              // doubles per x block (+32 B spreads banks)
-    // y-table row stride, odd: the 4 rows 4 * yb + r a wave reads at once
-    // (yb = 0..3) fall into distinct banks
-    static constexpr int YS = LM | 1;
-    static constexpr int XS = NXG * GS;
-    static constexpr int NPF = (Q * C3M + NROW - 1) / NROW;  // coef prefetch registers
-    __device__ static int xat(int x) { return (x / XB) * GS + (x % XB) * LM; }
-};
-
-// Index-set predicates. They must round exactly as the host planner (built
-// with -ffp-contract=off) does, so every product is kept apart from the sum
-// that consumes it: the functions below are written with plain operators
-// under "#pragma clang fp contract(off)" (the __dmul_rn / __dadd_rn helpers
-// of the HIP headers are plain operators compiled in the header's scope and
-// are fused into FMAs after inlining, which moves points that lie exactly on
-// a sphere or row bound in or out).
-
-// General cells: i-range of lattice row (j, k) inside the sphere, with the
-// operation order of grid_base_ref_c.cc:790-811 and no contraction, so the
-// index set equals the host's and the oracle's; qb, qc: the row's quadratic
-// scaled by -zetp (exponent of point i: (qa*i + qb)*i + qc).
-__device__ __forceinline__ void general_row_dev(const TileGrid &tg, double gp0, double dj, double dk,
-                                                double R2, double zetp, int &lo, int &hi,
-                                                double &qb, double &qc) {
-#pragma clang fp contract(off)
-    double a = 0.0, b = 0.0, c = 0.0;
-#pragma unroll
-    for (int x = 0; x < 3; x++) {
-        const double pointk = tg.dh[2][x] * dk;
-        const double pointj = pointk + tg.dh[1][x] * dj;
-        const double v = pointj - gp0 * tg.dh[0][x];
-        a = a + tg.dh[0][x] * tg.dh[0][x];
-        b = b + (2.0 * v) * tg.dh[0][x];
-        c = c + v * v;
-    }
-    double d = b * b - (4.0 * a) * (c - R2);
-    if (d < 0.0) {
-        lo = 1;
-        hi = 0;
-        qb = qc = 0.0;
-        return;
-    }
-    d = __dsqrt_rn(d);
-    const double a2 = 2.0 * a;
-    lo = (int)ceil((-b - d) / a2);
-    hi = (int)floor((-b + d) / a2);
-    qb = b * -zetp;
-    qc = c * -zetp;
-}
-
-// per-row data of the general path (unused storage on orthorhombic grids).
-// Row (y, z) is the lattice row at displacements dj = j - gp1, dk = k - gp2;
-// lo..hi its points inside the sphere. With u = i - gp0 and
-// w = dj h1 + dk h2, the exponent -zetp |u h0 + w|^2 of point i is
-// qa u^2 + qb u + qc, qb = c01 dj + c02 dk, qc = (c11 dj + c12 dk) dj + c22 dk^2.
-// kap = exp(2 qa) is the step ratio of the exponential recursion along a row.
-template <int Q, bool GEN>
-struct GenRows {
-    int lo[Q][NROW], hi[Q][NROW];
-    double dj[Q][TY], dk[Q][TZ];
-    double qa[Q], kap[Q], gp0[Q], c01[Q], c02[Q], c11[Q], c12[Q], c22[Q];
-    int ball[Q][NXG];  // 1: every point of the x block is in the sub-box
-    // c0: when the valid points j of the x block have the lattice indices
-    // c0 + j (no periodic wrap between them), else BIG; vmask: valid points
-    int c0[Q][NXG], vmask[Q][NXG];
-};
-template <int Q>
-struct GenRows<Q, false> {};
-
-// Table phase of pair slot q, run by one whole wave (lane = t % 64).
-// Lanes 0..31: x entries, 32..47: y entries, 48..63: z entries.
-// WITH_CXY: z lanes also contract coef with their Z powers (collocate).
-// GEN (general cells): tables are pure powers of the lattice displacement
-// i - gp; the Gaussian is applied per point from the row quadratic.
-template <int L, int LM, bool WITH_CXY, bool GEN, int Q>
-__device__ __forceinline__ void tile_phase1(const EntryRec &R, int q, int lane, int x0, int y0,
-                                            int z0, const TileGrid &tg,
-                                            const double *__restrict__ coef, double *X,
-                                            double (*Y)[TileCfg<LM>::YS], double (*Z)[LM],
-                                            double (*cxy)[TileCfg<LM>::C2M], int *xi, int *T,
-                                            int *bmax, int *bmin, GenRows<Q, GEN> &gr) {
-    constexpr int lp = L - 1;
-    const int d = lane < TX ? 0 : lane < TX + TY ? 1 : 2;
-    const int pos = d == 0 ? lane : d == 1 ? lane - TX : lane - TX - TY;
-    const int g = (d == 0 ? x0 : d == 1 ? y0 : z0) + pos;
-    const int nmax = d == 0 ? tg.nx : d == 1 ? tg.ny : tg.nz;
-    const int period = d == 0 ? tg.period[0] : d == 1 ? tg.period[1] : tg.period[2];
-    const double dr = d == 0 ? tg.dr[0] : d == 1 ? tg.dr[1] : tg.dr[2];
-    int i = BIG;
-    bool ok = g < nmax;
-    if (ok) {
-        i = R.lb[d] + imod(g - R.base[d], period);
-        ok = i <= R.ub[d];
-    }
-    double tv[L];
-    {
-        const double r = GEN ? (double)i - R.roff[d] : i * dr - R.roff[d];
-        double v = ok ? (GEN ? 1.0 : exp(-R.zetp * r * r)) : 0.0;
-#pragma unroll
-        for (int k = 0; k < L; k++) {
-            tv[k] = v;
-            v *= r;
-        }
-    }
-    int code;
-    if constexpr (GEN)
-        code = ok ? i : (d == 0 ? BIG : -BIG);
-    else
-        code = ok ? fold_off(i) : (d == 0 ? BIG : -1);
-    if (d == 0) {
-        double *dst = X + TileCfg<LM>::xat(pos);
-#pragma unroll
-        for (int k = 0; k < L; k++) dst[k] = tv[k];
-        xi[pos] = code;
-    } else if (d == 1) {
-#pragma unroll
-        for (int k = 0; k < L; k++) Y[pos][k] = tv[k];
-    } else {
-        if (Z) {
-#pragma unroll
-            for (int k = 0; k < L; k++) Z[pos][k] = tv[k];
-        }
-        if constexpr (WITH_CXY) {
-            // z-stage: cxy[z][(lyp,lxp)] = sum_lzp coef[(lzp,lyp,lxp)] Z^lzp
-#pragma unroll
-            for (int lyp = 0; lyp <= lp; lyp++)
-#pragma unroll
-                for (int lxp = 0; lxp <= lp - lyp; lxp++) {
-                    double a = 0.0;
-#pragma unroll
-                    for (int lzp = 0; lzp <= lp - lyp - lxp; lzp++)
-                        a += coef[cidx(lp, lzp, lyp, lxp)] * tv[lzp];
-                    cxy[pos][c2idx(lp, lyp, lxp)] = ok ? a : 0.0;
-                }
-        }
-    }
-    if constexpr (!GEN) {
-        // x blocks: max code (BIG if any point invalid) and min code over 8 lanes
-        int mx = code, mn = code;
-#pragma unroll
-        for (int o = 1; o < XB; o <<= 1) {
-            mx = max(mx, __shfl_xor(mx, o, XB));
-            mn = min(mn, __shfl_xor(mn, o, XB));
-        }
-        if (d == 0 && (lane % XB) == 0) {
-            bmax[lane / XB] = mx;
-            bmin[lane / XB] = mn;
-        }
-        // thresholds of the 256 tile rows (row = z * TY + y), 4 per lane
-#pragma unroll
-        for (int rr = 0; rr < NROW / 64; rr++) {
-            const int row = lane + 64 * rr;
-            const int kk = __shfl(code, TX + TY + row / TY);
-            const int jj = __shfl(code, TX + row % TY);
-            T[row] = (kk < 0 || jj < 0) ? -1
-                     : R.prof_off < 0   ? BIG
-                                        : tg.prof[R.prof_off + kk * R.prof_stride + jj];
-        }
-    } else {
-        // x blocks: min / max lattice index of the valid points, all valid?
-        // and whether the valid points share one offset i - j
-        int mn = ok ? i : BIG, mx = ok ? i : -BIG, all = ok ? 1 : 0;
-        int omn = ok ? i - (lane & (XB - 1)) : BIG, omx = ok ? omn : -BIG;
-        int vm = ok ? 1 << (lane & (XB - 1)) : 0;
-#pragma unroll
-        for (int o = 1; o < XB; o <<= 1) {
-            mx = max(mx, __shfl_xor(mx, o, XB));
-            mn = min(mn, __shfl_xor(mn, o, XB));
-            all = min(all, __shfl_xor(all, o, XB));
-            omx = max(omx, __shfl_xor(omx, o, XB));
-            omn = min(omn, __shfl_xor(omn, o, XB));
-            vm |= __shfl_xor(vm, o, XB);
-        }
-        if (d == 0 && (lane % XB) == 0) {
-            bmax[lane / XB] = mx;
-            bmin[lane / XB] = mn;
-            gr.ball[q][lane / XB] = all;
-            gr.c0[q][lane / XB] = omn == omx ? omn : BIG;
-            gr.vmask[q][lane / XB] = vm;
-        }
-#pragma unroll
-        for (int rr = 0; rr < NROW / 64; rr++) {
-            const int row = lane + 64 * rr;
-            const int kidx = __shfl(code, TX + TY + row / TY);
-            const int jidx = __shfl(code, TX + row % TY);
-            int lo = 1, hi = 0;
-            double qb, qc;
-            if (kidx != -BIG && jidx != -BIG)
-                general_row_dev(tg, R.roff[0], __dsub_rn((double)jidx, R.roff[1]),
-                                __dsub_rn((double)kidx, R.roff[2]), R.R2, R.zetp, lo, hi, qb, qc);
-            gr.lo[q][row] = lo;
-            gr.hi[q][row] = hi;
-        }
-        // displacements of the tile's y and z lattice rows (0 when invalid:
-        // such rows are empty)
-        if (d == 1) gr.dj[q][pos] = ok ? __dsub_rn((double)i, R.roff[1]) : 0.0;
-        if (d == 2) gr.dk[q][pos] = ok ? __dsub_rn((double)i, R.roff[2]) : 0.0;
-        if (lane == 0) {
-            double a = 0.0;
-#pragma unroll
-            for (int x = 0; x < 3; x++) a = __dadd_rn(a, __dmul_rn(tg.dh[0][x], tg.dh[0][x]));
-            gr.qa[q] = a * -R.zetp;
-            gr.kap[q] = exp(2.0 * (a * -R.zetp));
-            gr.gp0[q] = R.roff[0];
-            double h01 = 0.0, h02 = 0.0, h11 = 0.0, h12 = 0.0, h22 = 0.0;
-#pragma unroll
-            for (int x = 0; x < 3; x++) {
-                h01 += tg.dh[0][x] * tg.dh[1][x];
-                h02 += tg.dh[0][x] * tg.dh[2][x];
-                h11 += tg.dh[1][x] * tg.dh[1][x];
-                h12 += tg.dh[1][x] * tg.dh[2][x];
-                h22 += tg.dh[2][x] * tg.dh[2][x];
-            }
-            gr.c01[q] = -2.0 * R.zetp * h01;
-            gr.c02[q] = -2.0 * R.zetp * h02;
-            gr.c11[q] = -R.zetp * h11;
-            gr.c12[q] = -2.0 * R.zetp * h12;
-            gr.c22[q] = -R.zetp * h22;
-        }
-        (void)T;
-    }
-}
-
-// Phase-3 data of one thread's register block on the general path.
-// contig: the valid points j of the block have the lattice indices c0 + j;
-// then bit j of bits[r] says whether point j of row r is valid and inside
-// the sphere, and u0 = c0 - gp0.
-struct GenBlock {
-    int bits[RB];
-    double qa, kap, gp0, u0;
-    int bmn, bmx;
-    bool contig;
-};
-
-// exponent coefficients qb, qc of row yb * RB + r of plane z (see GenRows)
-template <int Q>
-__device__ __forceinline__ void gen_row_coef(const GenRows<Q, true> &gr, int q, int z, int y,
-                                             double &qb, double &qc) {
-    const double dk = gr.dk[q][z], dj = gr.dj[q][y];
-    qb = gr.c01[q] * dj + gr.c02[q] * dk;
-    qc = (gr.c11[q] * dj + gr.c12[q] * dk) * dj + gr.c22[q] * dk * dk;
-}
-
-// Gaussian factors of the block's rows by the recursion along the row:
-// e(u+1) = e(u) rho(u), rho(u+1) = rho(u) kap, started at u0 (two exp per
-// row instead of one per point). Returns false when the recursion cannot be
-// used: the block is not contiguous, or the start value of a row with points
-// inside underflows (later factors would be lost).
-template <int Q>
-__device__ __forceinline__ bool gen_row_start(const GenRows<Q, true> &gr, int q, int z, int yb,
-                                              const GenBlock &gb, double (&e)[RB],
-                                              double (&rho)[RB]) {
-    bool fast = gb.contig;
-#pragma unroll
-    for (int r = 0; r < RB; r++) {
-        double qb, qc;
-        gen_row_coef(gr, q, z, yb * RB + r, qb, qc);
-        e[r] = exp((gb.qa * gb.u0 + qb) * gb.u0 + qc);
-        rho[r] = exp(gb.qa * (2.0 * gb.u0 + 1.0) + qb);
-        fast = fast && (gb.bits[r] == 0 || e[r] >= 1e-290);
-    }
-    return fast;
-}
-
-#define GPW_LP_DISPATCH(LM, CALL)                                                   \
-    switch (lp) {                                                                   \
-    case 0: CALL(1); break;                                                         \
-    case 1: CALL(2); break;                                                         \
-    case 2: CALL(3); break;                                                         \
-    case 3: CALL(4); break;                                                         \
-    default:                                                                        \
-        if constexpr (LM > 4) {                                                     \
-            switch (lp) {                                                           \
-            case 4: CALL(5); break;                                                 \
-            case 5: CALL(6); break;                                                 \
-            case 6: CALL(7); break;                                                 \
-            case 7: CALL(8); break;                                                 \
-            default:                                                                \
-                if constexpr (LM > 8) {                                             \
-                    switch (lp) {                                                   \
-                    case 8: CALL(9); break;                                         \
-                    case 9: CALL(10); break;                                        \
-                    case 10: CALL(11); break;                                       \
-                    case 11: CALL(12); break;                                       \
-                    default:                                                        \
-                        if constexpr (LM > 12) {                                    \
-                            switch (lp) {                                           \
-                            case 12: CALL(13); break;                               \
-                            case 13: CALL(14); break;                               \
-                            case 14: CALL(15); break;                               \
-                            case 15: CALL(16); break;                               \
-                            default: CALL(17); break;                               \
-                            }                                                       \
-                        }                                                           \
-                        break;                                                      \
-                    }                                                               \
-                }                                                                   \
-                break;                                                              \
-            }                                                                       \
-        }                                                                           \
-        break;                                                                      \
-    }
-
-template <int LM, bool GEN>
-struct CollocSmem {
-    using C = TileCfg<LM>;
-    GenRows<C::Q, GEN> gr;
-    EntryRec rec[3][C::Q];
-    double coef[2][C::Q][C::C3M];
-    double X[C::Q][C::XS];
-    double Y[C::Q][TY][C::YS];
-    double cxy[C::Q][TZ][C::C2M];
-    int xi[C::Q][TX];
-    int T[C::Q][NROW];
-    int bmax[C::Q][NXG], bmin[C::Q][NXG];
-    int lp[C::Q];
-};
-
-// register block of one thread for one pair: rows y = 4*yb + r of plane z,
-// points x = 8*xg + j
-template <int L, int LM, bool GEN>
-__device__ __forceinline__ void colloc_block(const CollocSmem<LM, GEN> &sm, int q, int xg, int yb,
-                                             int z, const int (&Tr)[RB], bool full,
-                                             const GenBlock &gb, double (&acc)[RB][XB]) {
-    constexpr int lp = L - 1;
-    (void)full;
-    double cx[RB][L];
-    const double *cxy = sm.cxy[q][z];
-#pragma unroll
-    for (int r = 0; r < RB; r++)
-#pragma unroll
-        for (int l = 0; l < L; l++) cx[r][l] = 0.0;
-    // y stage, rows innermost: each cxy value is loaded once and used by its
-    // RB FMAs at once (a row-outer order keeps every cxy value live across
-    // the rows and spills registers)
-#pragma unroll
-    for (int lyp = 0; lyp <= lp; lyp++) {
-        double y[RB];
-#pragma unroll
-        for (int r = 0; r < RB; r++) y[r] = sm.Y[q][yb * RB + r][lyp];
-#pragma unroll
-        for (int lxp = 0; lxp <= lp - lyp; lxp++) {
-            const double c = cxy[c2idx(lp, lyp, lxp)];
-#pragma unroll
-            for (int r = 0; r < RB; r++) cx[r][lxp] += c * y[r];
-        }
-    }
-    const double *Xg = sm.X[q] + xg * TileCfg<LM>::GS;
-    if constexpr (GEN) {
-        double e[RB], rho[RB];
-        if (gen_row_start(sm.gr, q, z, yb, gb, e, rho)) {
-#pragma unroll
-            for (int j = 0; j < XB; j++) {
-                double xv[L];
-#pragma unroll
-                for (int l = 0; l < L; l++) xv[l] = Xg[j * LM + l];
-#pragma unroll
-                for (int r = 0; r < RB; r++) {
-                    double v = 0.0;
-#pragma unroll
-                    for (int l = 0; l < L; l++) v += cx[r][l] * xv[l];
-                    const double m = (gb.bits[r] & (1 << j)) ? e[r] : 0.0;
-                    acc[r][j] = __builtin_fma(v, m, acc[r][j]);
-                    if (j < XB - 1) {
-                        e[r] *= rho[r];
-                        rho[r] *= gb.kap;
-                    }
-                }
-            }
-        } else {
-            // per-point exponentials (a block across a periodic wrap, or far
-            // from a steep Gaussian): one point per iteration, so that only
-            // RB exponentials are live at a time
-#pragma unroll 1
-            for (int j = 0; j < XB; j++) {
-                double xv[L];
-#pragma unroll
-                for (int l = 0; l < L; l++) xv[l] = Xg[j * LM + l];
-                const int code = sm.xi[q][xg * XB + j];
-                const double u = (double)code - gb.gp0;
-                double d[RB];
-#pragma unroll
-                for (int r = 0; r < RB; r++) {
-                    double v = 0.0;
-#pragma unroll
-                    for (int l = 0; l < L; l++) v += cx[r][l] * xv[l];
-                    const int row = z * TY + yb * RB + r;
-                    double qb, qc;
-                    gen_row_coef(sm.gr, q, z, yb * RB + r, qb, qc);
-                    const bool in = code >= sm.gr.lo[q][row] && code <= sm.gr.hi[q][row];
-                    d[r] = in ? v * exp((gb.qa * u + qb) * u + qc) : 0.0;
-                }
-#pragma unroll
-                for (int jj = 0; jj < XB; jj++)
-#pragma unroll
-                    for (int r = 0; r < RB; r++) acc[r][jj] += jj == j ? d[r] : 0.0;
-            }
-        }
-    } else {
-#pragma unroll
-        for (int j = 0; j < XB; j++) {
-            double xv[L];
-#pragma unroll
-            for (int l = 0; l < L; l++) xv[l] = Xg[j * LM + l];
-            const int code = sm.xi[q][xg * XB + j];
-#pragma unroll
-            for (int r = 0; r < RB; r++) {
-                double v = 0.0;
-#pragma unroll
-                for (int l = 0; l < L; l++) v += cx[r][l] * xv[l];
-                // the sphere mask as a 0/1 factor of one FMA (cheaper than a
-                // 64-bit select and an add)
-                const double m = (full || code <= Tr[r]) ? 1.0 : 0.0;
-                acc[r][j] = __builtin_fma(v, m, acc[r][j]);
-            }
-        }
-    }
-}
-
-// classification of a thread's register block for pair q: returns false when
-// no point of the block is inside; sets full when every point is inside.
-template <bool GEN, class Smem>
-__device__ __forceinline__ bool classify_block(const Smem &sm, int q, int xg, int yb, int z,
-                                               int (&Tr)[RB], GenBlock &gb, bool &full) {
-    const int bmn = sm.bmin[q][xg], bmx = sm.bmax[q][xg];
-    if constexpr (!GEN) {
-        int tmin = BIG, tmax = -1;
-#pragma unroll
-        for (int r = 0; r < RB; r++) {
-            Tr[r] = sm.T[q][z * TY + yb * RB + r];
-            tmin = min(tmin, Tr[r]);
-            tmax = max(tmax, Tr[r]);
-        }
-        if (tmax < 0 || bmn > tmax) return false;
-        full = bmx <= tmin;
-        return true;
-    } else {
-        bool any = false;
-        full = sm.gr.ball[q][xg] != 0;
-        const int c0 = sm.gr.c0[q][xg], vm = sm.gr.vmask[q][xg];
-        gb.contig = c0 != BIG;
-#pragma unroll
-        for (int r = 0; r < RB; r++) {
-            const int row = z * TY + yb * RB + r;
-            const int lo = sm.gr.lo[q][row], hi = sm.gr.hi[q][row];
-            any = any || (lo <= hi && lo <= bmx && hi >= bmn);
-            full = full && lo <= bmn && bmx <= hi;
-            // points j = max(lo - c0, 0) .. min(hi - c0, 7) (contiguous blocks)
-            const int a = max(lo - c0, 0), b = min(hi - c0, XB - 1);
-            gb.bits[r] = (gb.contig && a <= b) ? ((0xFF >> (XB - 1 - b)) & (0xFF << a)) & vm : 0;
-            Tr[r] = 0;
-        }
-        gb.qa = sm.gr.qa[q];
-        gb.kap = sm.gr.kap[q];
-        gb.gp0 = sm.gr.gp0[q];
-        gb.bmn = bmn;
-        gb.bmx = bmx;
-        gb.u0 = (double)c0 - gb.gp0;
-        return any && bmn <= bmx;
-    }
-}
-
-// out_mode 0: grid += acc (splits == 1); 1: partial[split] = acc; 2: partial[split] += acc
-// General-cell collocate kernels run one workgroup per CU: the row data of
-// the general path would otherwise spill registers in the inner loops
-// (measured faster despite the lower occupancy). The general integrate
-// kernels keep two (their x stage runs in two halves of the powers above
-// L = 8, which measured faster than one workgroup per CU without spills).
-template <int LM, bool GEN>
-__global__ __launch_bounds__(NROW, GEN ? 1 : 2) void k_collocate_tiles(
-    TileGrid tg, const EntryRec *__restrict__ recs, const int *__restrict__ tile_start,
-    const double *__restrict__ coef, int splits, double *__restrict__ out, int out_mode) {
-    using C = TileCfg<LM>;
-    constexpr int Q = C::Q;
-    __shared__ CollocSmem<LM, GEN> sm;
-    const int tile = blockIdx.x / splits, split = blockIdx.x % splits;
-    const int tix = tile % tg.ntx, tiy = (tile / tg.ntx) % tg.nty, tiz = tile / (tg.ntx * tg.nty);
-    const int x0 = tix * TX, y0 = tiy * TY, z0 = tiz * TZ;
-    const int t = threadIdx.x, xg = t & 3, yb = (t >> 2) & 3, z = t >> 4;
-    const int beg = tile_start[tile], end = tile_start[tile + 1];
-    const int per = (end - beg + splits - 1) / splits;
-    const int mybeg = min(end, beg + split * per), myend = min(end, mybeg + per);
-    if (out_mode != 1 && mybeg >= myend) return;
-    const int *rec_words = reinterpret_cast<const int *>(recs);
-
-    // prologue: records of chunks 0 and 1, then coefficients of chunk 0
-    for (int w = t; w < 2 * Q * RECW; w += NROW) {
-        const int item = mybeg + w / RECW;
-        if (item < myend)
-            reinterpret_cast<int *>(&sm.rec[w / (Q * RECW)][0])[w % (Q * RECW)] =
-                rec_words[(long long)item * RECW + w % RECW];
-    }
-    __syncthreads();
-    for (int w = t; w < Q * C::C3M; w += NROW) {
-        const int q = w / C::C3M, idx = w % C::C3M;
-        if (mybeg + q < myend && idx < ncoset(sm.rec[0][q].lp))
-            sm.coef[0][q][idx] = coef[sm.rec[0][q].coef_off + idx];
-    }
-    __syncthreads();
-
-    double acc[RB][XB];
-#pragma unroll
-    for (int r = 0; r < RB; r++)
-#pragma unroll
-        for (int j = 0; j < XB; j++) acc[r][j] = 0.0;
-
-    int k = 0;
-    for (int c = mybeg; c < myend; c += Q, k++) {
-        const int nq = min(Q, myend - c);
-        const int rs = k % 3, cs = k & 1;
-        // prefetch: records of chunk k+2, coefficients of chunk k+1
-        int pf_rec = 0;
-        if (t < Q * RECW && c + 2 * Q + t / RECW < myend)
-            pf_rec = rec_words[(long long)(c + 2 * Q) * RECW + t];
-        double pf_coef[C::NPF];
-#pragma unroll
-        for (int j = 0; j < C::NPF; j++) {
-            pf_coef[j] = 0.0;
-            const int w = t + j * NROW;
-            if (w < Q * C::C3M) {
-                const int q = w / C::C3M, idx = w % C::C3M;
-                const EntryRec &Rn = sm.rec[(k + 1) % 3][q];
-                if (c + Q + q < myend && idx < ncoset(Rn.lp)) pf_coef[j] = coef[Rn.coef_off + idx];
-            }
-        }
-        // table phase: wave q -> pair q
-        {
-            const int q = t / 64, lane = t % 64;
-            if (q < nq) {
-                const EntryRec &R = sm.rec[rs][q];
-                const int lp = R.lp;
-                if (lane == 0) sm.lp[q] = lp;
-#define GPW_P1_CALL(L)                                                                        \
-    tile_phase1<L, LM, true, GEN, Q>(R, q, lane, x0, y0, z0, tg, sm.coef[cs][q], sm.X[q],      \
-                                     sm.Y[q], nullptr, sm.cxy[q], sm.xi[q], sm.T[q],          \
-                                     sm.bmax[q], sm.bmin[q], sm.gr)
-                GPW_LP_DISPATCH(LM, GPW_P1_CALL)
-#undef GPW_P1_CALL
-            }
-        }
-        __syncthreads();
-#pragma unroll 1
-        for (int q = 0; q < nq; q++) {
-            int Tr[RB];
-            GenBlock gb;
-            bool full;
-            if (!classify_block<GEN>(sm, q, xg, yb, z, Tr, gb, full)) continue;
-            const int lp = sm.lp[q];
-#define GPW_COLLOC_CALL(L) colloc_block<L, LM, GEN>(sm, q, xg, yb, z, Tr, full, gb, acc)
-            GPW_LP_DISPATCH(LM, GPW_COLLOC_CALL)
-#undef GPW_COLLOC_CALL
-        }
-        // commit the prefetched data to the LDS rings
-        if (t < Q * RECW) reinterpret_cast<int *>(&sm.rec[(k + 2) % 3][0])[t] = pf_rec;
-#pragma unroll
-        for (int j = 0; j < C::NPF; j++) {
-            const int w = t + j * NROW;
-            if (w < Q * C::C3M) sm.coef[cs ^ 1][w / C::C3M][w % C::C3M] = pf_coef[j];
-        }
-        __syncthreads();
-    }
-    const int gz = z0 + z;
-    if (gz >= tg.nz) return;
-    const long long ng = (long long)tg.nx * tg.ny * tg.nz;
-    double *dst = out_mode == 0 ? out : out + split * ng;
-#pragma unroll
-    for (int r = 0; r < RB; r++) {
-        const int gy = y0 + yb * RB + r;
-        if (gy >= tg.ny) continue;
-        double *row = dst + ((long long)gz * tg.ny + gy) * tg.nx + x0 + xg * XB;
-#pragma unroll
-        for (int j = 0; j < XB; j++) {
-            if (x0 + xg * XB + j < tg.nx) {
-                if (out_mode == 1)
-                    row[j] = acc[r][j];
-                else
-                    row[j] += acc[r][j];
-            }
-        }
-    }
-}
-
 __global__ void k_reduce_partials(const double *__restrict__ partial, int splits, long long ng,
                                   double *__restrict__ grid) {
     for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < ng;
@@ -1030,328 +225,6 @@ __global__ void k_reduce_partials(const double *__restrict__ partial, int splits
         for (int s = 0; s < splits; s++) a += partial[s * ng + g];
         grid[g] += a;
     }
-}
-
-// DPP move of a double (two 32-bit halves); CTRL is the dpp_ctrl code
-template <int CTRL>
-__device__ __forceinline__ double dpp_f64(double a) {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(a), CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(a), CTRL, 0xF, 0xF, false);
-    return __hiloint2double(hi, lo);
-}
-// sum over the 4 lanes of a quad (quad_perm [1,0,3,2] then [2,3,0,1])
-__device__ __forceinline__ double quad_sum(double v) {
-    v += dpp_f64<0xB1>(v);
-    return v + dpp_f64<0x4E>(v);
-}
-// sum over lanes l, l+4, l+8, l+12 of a DPP row (row_ror:4 then row_ror:8)
-__device__ __forceinline__ double stride4_sum(double v) {
-    v += dpp_f64<0x124>(v);
-    return v + dpp_f64<0x128>(v);
-}
-
-// Per-wave buffer of the integrate transposition: the x-stage results of the
-// wave's 64 rows (4 z-planes x 16 y), row (zl, y) at zl * ZS + y * RS. A store
-// writes rows 4 * yb + r (yb, zl = 0..3) of one r at once; RS = 1 (mod 4) and
-// ZS = 1 (mod 16) put those 16 rows on 16 distinct 8-byte bank pairs.
-template <int LM>
-struct RowT1 {
-    static constexpr int RS = LM + ((1 - LM) % 4 + 4) % 4;
-    static constexpr int ZS = TY * RS + 1;
-    static constexpr int SZ = 4 * ZS;
-};
-
-template <int LM, bool GEN>
-struct IntegSmem {
-    using C = TileCfg<LM>;
-    GenRows<C::Q, GEN> gr;
-    EntryRec rec[2][C::Q];
-    double X[C::Q][C::XS];
-    double Y[C::Q][TY][C::YS];
-    double Z[C::Q][TZ][LM];
-    double uz[C::Q][TZ][C::C2M];  // per-plane (lyp, lxp) moments
-    double t1[NROW / 64][RowT1<LM>::SZ];
-    int xi[C::Q][TX];
-    int T[C::Q][NROW];
-    int bmax[C::Q][NXG], bmin[C::Q][NXG];
-    int lp[C::Q];
-};
-
-// sum of a double over the 16 lanes of a DPP row; every lane gets the sum
-__device__ __forceinline__ double row_sum16(double v) { return stride4_sum(quad_sum(v)); }
-
-// x stage of integ_block for the powers L0 <= l < L1 of the x table, summed
-// over the lane quad and stored to the wave's transposition rows t1
-template <int L0, int L1, int LM, bool GEN>
-__device__ __forceinline__ void integ_x_part(IntegSmem<LM, GEN> &sm, int q, int xg, int yb, int z,
-                                             const int (&Tr)[RB], int mode, const GenBlock &gb,
-                                             const double (&v)[RB][XB], double *t1) {
-    constexpr int NL = L1 - L0;
-    double tx[RB][NL];
-#pragma unroll
-    for (int r = 0; r < RB; r++)
-#pragma unroll
-        for (int l = 0; l < NL; l++) tx[r][l] = 0.0;
-    if (mode != 0) {  // 1: masked, 2: whole block inside
-        const double *Xg = sm.X[q] + xg * TileCfg<LM>::GS;
-        const bool wfull = __all(mode == 2);  // wave-uniform among the active lanes
-        if constexpr (GEN) {
-            double e[RB], rho[RB];
-            if (gen_row_start(sm.gr, q, z, yb, gb, e, rho)) {
-#pragma unroll
-                for (int j = 0; j < XB; j++) {
-                    double xv[NL];
-#pragma unroll
-                    for (int l = 0; l < NL; l++) xv[l] = Xg[j * LM + L0 + l];
-#pragma unroll
-                    for (int r = 0; r < RB; r++) {
-                        const double w = (gb.bits[r] & (1 << j)) ? v[r][j] * e[r] : 0.0;
-#pragma unroll
-                        for (int l = 0; l < NL; l++) tx[r][l] += w * xv[l];
-                        if (j < XB - 1) {
-                            e[r] *= rho[r];
-                            rho[r] *= gb.kap;
-                        }
-                    }
-                }
-            } else {
-                // per-point exponentials (a block across a periodic wrap, or
-                // far from a steep Gaussian): one point per iteration, so that
-                // only RB exponentials are live at a time
-#pragma unroll 1
-                for (int j = 0; j < XB; j++) {
-                    double xv[NL];
-#pragma unroll
-                    for (int l = 0; l < NL; l++) xv[l] = Xg[j * LM + L0 + l];
-                    const int code = sm.xi[q][xg * XB + j];
-                    const double u = (double)code - gb.gp0;
-#pragma unroll
-                    for (int r = 0; r < RB; r++) {
-                        double vj = 0.0;
-#pragma unroll
-                        for (int jj = 0; jj < XB; jj++) vj = jj == j ? v[r][jj] : vj;
-                        const int row = z * TY + yb * RB + r;
-                        double qb, qc;
-                        gen_row_coef(sm.gr, q, z, yb * RB + r, qb, qc);
-                        const bool in = code >= sm.gr.lo[q][row] && code <= sm.gr.hi[q][row];
-                        const double w = in ? vj * exp((gb.qa * u + qb) * u + qc) : 0.0;
-#pragma unroll
-                        for (int l = 0; l < NL; l++) tx[r][l] += w * xv[l];
-                    }
-                }
-            }
-        } else {
-#pragma unroll
-            for (int j = 0; j < XB; j++) {
-                double xv[NL];
-#pragma unroll
-                for (int l = 0; l < NL; l++) xv[l] = Xg[j * LM + L0 + l];
-                const int code = sm.xi[q][xg * XB + j];
-#pragma unroll
-                for (int r = 0; r < RB; r++) {
-                    double w = v[r][j];
-                    if (!wfull) w = code <= Tr[r] ? w : 0.0;
-#pragma unroll
-                    for (int l = 0; l < NL; l++) tx[r][l] += w * xv[l];
-                }
-            }
-        }
-    }
-    // rows of the quad summed over its four x blocks; lane xg stores row r = xg
-#pragma unroll
-    for (int r = 0; r < RB; r++) {
-#pragma unroll
-        for (int l = 0; l < NL; l++) tx[r][l] = quad_sum(tx[r][l]);
-        if (xg == r) {
-#pragma unroll
-            for (int l = 0; l < NL; l++) t1[(yb * RB + r) * RowT1<LM>::RS + L0 + l] = tx[r][l];
-        }
-    }
-}
-
-// Register block (rows y = 4*yb + r of plane z, points x = 8*xg + j) of pair
-// q: x stage in registers, sum of the four x blocks of each row over the lane
-// quad (DPP), then a wave-local transposition through LDS for the y stage:
-// lane (zl, c) of the wave sums Y[y][lyp] * tx[zl][y][lxp] over the 16 rows
-// of its plane for the (lyp, lxp) terms c, c + 16, ...  A wave holds four
-// whole z-planes, so the transposition needs no workgroup barrier (LDS
-// operations of one wave complete in order).
-template <int L, int LM, bool GEN>
-__device__ __forceinline__ void integ_block(IntegSmem<LM, GEN> &sm, int q, int xg, int yb, int z,
-                                            const int (&Tr)[RB], int mode, const GenBlock &gb,
-                                            const double (&v)[RB][XB], int t) {
-    constexpr int lp = L - 1;
-    constexpr int N2 = L * (L + 1) / 2;
-    constexpr int ZS = RowT1<LM>::ZS;
-    const int lane = t & 63, wave = t >> 6;
-    const int zl = lane >> 4, cq = lane & 15;
-    double *uzp = sm.uz[q][wave * 4 + zl];
-    if (!__any(mode != 0)) {  // no point of the wave's four planes is inside
-        for (int c = cq; c < N2; c += 16) uzp[c] = 0.0;
-        return;
-    }
-    // general cells above L = 8: two passes over half the powers each (half
-    // the accumulator registers: no spills at two workgroups per CU)
-    constexpr int LA = (GEN && L > 8) ? (L + 1) / 2 : L;
-    double *t1 = sm.t1[wave] + zl * ZS;
-    integ_x_part<0, LA>(sm, q, xg, yb, z, Tr, mode, gb, v, t1);
-    if constexpr (LA < L) integ_x_part<LA, L>(sm, q, xg, yb, z, Tr, mode, gb, v, t1);
-    __builtin_amdgcn_wave_barrier();
-    const double *Yq = &sm.Y[q][0][0];
-    constexpr int YS = TileCfg<LM>::YS;
-    for (int c = cq; c < N2; c += 16) {
-        int lyp = 0, lxp = c;
-        while (lxp > lp - lyp) {
-            lxp -= lp - lyp + 1;
-            lyp++;
-        }
-        double a = 0.0;
-#pragma unroll
-        for (int y = 0; y < TY; y++) a += Yq[y * YS + lyp] * t1[y * RowT1<LM>::RS + lxp];
-        uzp[c] = a;
-    }
-    __builtin_amdgcn_wave_barrier();
-}
-
-// partial moments of every (sub-box, tile) item: partial[rec.ent_off + c3];
-// workgroup bid = tile * splits + split
-template <int LM, bool GEN>
-__device__ __forceinline__ void integrate_tile(IntegSmem<LM, GEN> &sm, int bid, const TileGrid &tg,
-                                               const EntryRec *__restrict__ recs,
-                                               const int *__restrict__ tile_start,
-                                               const double *__restrict__ grid, int lp_add,
-                                               int splits, double *__restrict__ partial) {
-    using C = TileCfg<LM>;
-    constexpr int Q = C::Q;
-    const int tile = bid / splits, split = bid % splits;
-    const int tix = tile % tg.ntx, tiy = (tile / tg.ntx) % tg.nty, tiz = tile / (tg.ntx * tg.nty);
-    const int x0 = tix * TX, y0 = tiy * TY, z0 = tiz * TZ;
-    const int t = threadIdx.x, xg = t & 3, yb = (t >> 2) & 3, z = t >> 4;
-    const int beg = tile_start[tile], end = tile_start[tile + 1];
-    const int per = (end - beg + splits - 1) / splits;
-    const int mybeg = min(end, beg + split * per), myend = min(end, mybeg + per);
-    if (mybeg >= myend) return;
-    const int *rec_words = reinterpret_cast<const int *>(recs);
-
-    for (int w = t; w < Q * RECW; w += NROW) {
-        const int item = mybeg + w / RECW;
-        if (item < myend)
-            reinterpret_cast<int *>(&sm.rec[0][0])[w] = rec_words[(long long)item * RECW + w % RECW];
-    }
-
-    double v[RB][XB];
-    {
-        const int gz = z0 + z;
-#pragma unroll
-        for (int r = 0; r < RB; r++) {
-            const int gy = y0 + yb * RB + r;
-            const bool rowok = gy < tg.ny && gz < tg.nz;
-            const long long row = ((long long)gz * tg.ny + gy) * tg.nx + x0 + xg * XB;
-#pragma unroll
-            for (int j = 0; j < XB; j++)
-                v[r][j] = (rowok && x0 + xg * XB + j < tg.nx) ? grid[row + j] : 0.0;
-        }
-    }
-    __syncthreads();
-
-    int k = 0;
-    for (int c = mybeg; c < myend; c += Q, k++) {
-        const int nq = min(Q, myend - c);
-        const int rs = k & 1;
-        int pf_rec = 0;
-        if (t < Q * RECW && c + Q + t / RECW < myend)
-            pf_rec = rec_words[(long long)(c + Q) * RECW + t];
-        {
-            const int q = t / 64, lane = t % 64;
-            if (q < nq) {
-                const EntryRec &R = sm.rec[rs][q];
-                const int lp = R.lp + lp_add;
-                if (lane == 0) sm.lp[q] = lp;
-#define GPW_P1_CALL(L)                                                                         \
-    tile_phase1<L, LM, false, GEN, Q>(R, q, lane, x0, y0, z0, tg, nullptr, sm.X[q], sm.Y[q],    \
-                                      sm.Z[q], nullptr, sm.xi[q], sm.T[q], sm.bmax[q],          \
-                                      sm.bmin[q], sm.gr)
-                GPW_LP_DISPATCH(LM, GPW_P1_CALL)
-#undef GPW_P1_CALL
-            }
-        }
-        __syncthreads();
-#pragma unroll 1
-        for (int q = 0; q < nq; q++) {
-            int Tr[RB];
-            GenBlock gb;
-            bool full = false;
-            const int mode = classify_block<GEN>(sm, q, xg, yb, z, Tr, gb, full) ? (full ? 2 : 1) : 0;
-            const int lp = sm.lp[q];
-#define GPW_INTEG_CALL(L) integ_block<L, LM, GEN>(sm, q, xg, yb, z, Tr, mode, gb, v, t)
-            GPW_LP_DISPATCH(LM, GPW_INTEG_CALL)
-#undef GPW_INTEG_CALL
-        }
-        __syncthreads();
-        // z-stage of all pairs of the chunk at once, thread -> (q, c3):
-        // M[(lzp,lyp,lxp)] = sum_z uz[q][z][(lyp,lxp)] Z[z][lzp]
-        {
-            int n3q[Q];
-            int tot = 0;
-#pragma unroll
-            for (int q = 0; q < Q; q++) {
-                n3q[q] = q < nq ? ncoset(sm.lp[q]) : 0;
-                tot += n3q[q];
-            }
-            for (int w = t; w < tot; w += NROW) {
-                int q = 0, c3 = w;
-#pragma unroll
-                for (int qq = 0; qq < Q - 1; qq++)
-                    if (q == qq && c3 >= n3q[qq]) {
-                        c3 -= n3q[qq];
-                        q++;
-                    }
-                const int lp = sm.lp[q];
-                int lzp, lyp, lxp;
-                decode_c3(lp, c3, lzp, lyp, lxp);
-                const double *uzq = &sm.uz[q][0][c2idx(lp, lyp, lxp)];
-                const double *zq = &sm.Z[q][0][lzp];
-                double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-#pragma unroll
-                for (int zz = 0; zz < TZ; zz += 4) {
-                    a0 += uzq[(zz + 0) * C::C2M] * zq[(zz + 0) * LM];
-                    a1 += uzq[(zz + 1) * C::C2M] * zq[(zz + 1) * LM];
-                    a2 += uzq[(zz + 2) * C::C2M] * zq[(zz + 2) * LM];
-                    a3 += uzq[(zz + 3) * C::C2M] * zq[(zz + 3) * LM];
-                }
-                partial[sm.rec[rs][q].ent_off + c3] = (a0 + a1) + (a2 + a3);
-            }
-        }
-        if (t < Q * RECW) reinterpret_cast<int *>(&sm.rec[rs ^ 1][0])[t] = pf_rec;
-        __syncthreads();
-    }
-}
-
-template <int LM, bool GEN>
-__global__ __launch_bounds__(NROW, (GEN && LM >= 17) ? 1 : 2) void k_integrate_tiles(
-    TileGrid tg, const EntryRec *__restrict__ recs, const int *__restrict__ tile_start,
-    const double *__restrict__ grid, int lp_add, int splits, double *__restrict__ partial) {
-    __shared__ IntegSmem<LM, GEN> sm;
-    integrate_tile<LM, GEN>(sm, blockIdx.x, tg, recs, tile_start, grid, lp_add, splits, partial);
-}
-
-// Two classes in one launch (workgroups of class A first, then class B), so
-// the tail of one class overlaps the other; LDS is the union of both.
-template <int LMA, int LMB, bool GEN>
-__global__ __launch_bounds__(NROW, 2) void k_integrate_tiles2(
-    TileGrid tg, const EntryRec *__restrict__ recsA, const int *__restrict__ startA, int splitsA,
-    const EntryRec *__restrict__ recsB, const int *__restrict__ startB, int splitsB,
-    const double *__restrict__ grid, int lp_add, int ntiles, double *__restrict__ partial) {
-    __shared__ union {
-        IntegSmem<LMA, GEN> a;
-        IntegSmem<LMB, GEN> b;
-    } sm;
-    const int na = ntiles * splitsA;
-    if ((int)blockIdx.x < na)
-        integrate_tile<LMA, GEN>(sm.a, blockIdx.x, tg, recsA, startA, grid, lp_add, splitsA, partial);
-    else
-        integrate_tile<LMB, GEN>(sm.b, blockIdx.x - na, tg, recsB, startB, grid, lp_add, splitsB,
-                                 partial);
 }
 
 // ---------------------------------------------------------------------------
@@ -2489,29 +1362,22 @@ __global__ __launch_bounds__(256) void k_integrate_folded(int L, const double *_
 // ---------------------------------------------------------------------------
 // launchers (host side)
 
-template <int LM>
-static hipError_t launch_collocate_tiles_lm(const TileGrid &tg, const EntryRec *recs,
-                                            const int *tile_start, const double *coef, int ntiles,
-                                            int splits, double *out, int out_mode, hipStream_t st) {
-    if (tg.general)
-        hipLaunchKernelGGL((k_collocate_tiles<LM, true>), dim3(ntiles * splits), dim3(NROW), 0, st,
-                           tg, recs, tile_start, coef, splits, out, out_mode);
-    else
-        hipLaunchKernelGGL((k_collocate_tiles<LM, false>), dim3(ntiles * splits), dim3(NROW), 0, st,
-                           tg, recs, tile_start, coef, splits, out, out_mode);
-    return hipGetLastError();
-}
-
-// class c handles L = lp + 1 in (4c, 4c + 4], the last class up to 17
 hipError_t launch_collocate_tiles(int cls, const TileGrid &tg, const EntryRec *recs,
                                   const int *tile_start, const double *coef, int ntiles, int splits,
                                   double *out, int out_mode, hipStream_t st) {
-    switch (cls) {
-    case 0: return launch_collocate_tiles_lm<4>(tg, recs, tile_start, coef, ntiles, splits, out, out_mode, st);
-    case 1: return launch_collocate_tiles_lm<8>(tg, recs, tile_start, coef, ntiles, splits, out, out_mode, st);
-    case 2: return launch_collocate_tiles_lm<12>(tg, recs, tile_start, coef, ntiles, splits, out, out_mode, st);
-    default: return launch_collocate_tiles_lm<17>(tg, recs, tile_start, coef, ntiles, splits, out, out_mode, st);
-    }
+    return tg.general ? launch_collocate_tiles_t<true>(cls, tg, recs, tile_start, coef, ntiles, splits,
+                                                       out, out_mode, st)
+                      : launch_collocate_tiles_t<false>(cls, tg, recs, tile_start, coef, ntiles, splits,
+                                                        out, out_mode, st);
+}
+
+hipError_t launch_integrate_tiles(int cls, const TileGrid &tg, const EntryRec *recs,
+                                  const int *tile_start, const double *grid, int lp_add,
+                                  int ntiles, int splits, double *partial, hipStream_t st) {
+    return tg.general ? launch_integrate_tiles_t<true>(cls, tg, recs, tile_start, grid, lp_add, ntiles,
+                                                       splits, partial, st)
+                      : launch_integrate_tiles_t<false>(cls, tg, recs, tile_start, grid, lp_add, ntiles,
+                                                        splits, partial, st);
 }
 
 hipError_t launch_reduce_partials(const double *partial, int splits, long long ng, double *grid,
@@ -2519,43 +1385,6 @@ hipError_t launch_reduce_partials(const double *partial, int splits, long long n
     const int blocks = (int)std::min<long long>((ng + 255) / 256, 8192);
     hipLaunchKernelGGL(k_reduce_partials, dim3(blocks), dim3(256), 0, st, partial, splits, ng, grid);
     return hipGetLastError();
-}
-
-template <int LM>
-static hipError_t launch_integrate_tiles_lm(const TileGrid &tg, const EntryRec *recs,
-                                            const int *tile_start, const double *grid, int lp_add,
-                                            int ntiles, int splits, double *partial,
-                                            hipStream_t st) {
-    if (tg.general)
-        hipLaunchKernelGGL((k_integrate_tiles<LM, true>), dim3(ntiles * splits), dim3(NROW), 0, st,
-                           tg, recs, tile_start, grid, lp_add, splits, partial);
-    else
-        hipLaunchKernelGGL((k_integrate_tiles<LM, false>), dim3(ntiles * splits), dim3(NROW), 0, st,
-                           tg, recs, tile_start, grid, lp_add, splits, partial);
-    return hipGetLastError();
-}
-
-// classes 1 (LM 8) and 0 (LM 4) of orthorhombic lists in one launch
-hipError_t launch_integrate_tiles_84(const TileGrid &tg, const EntryRec *recs8, const int *start8,
-                                     int splits8, const EntryRec *recs4, const int *start4,
-                                     int splits4, const double *grid, int lp_add, int ntiles,
-                                     double *partial, hipStream_t st) {
-    if (tg.general) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_integrate_tiles2<8, 4, false>), dim3(ntiles * (splits8 + splits4)),
-                       dim3(NROW), 0, st, tg, recs8, start8, splits8, recs4, start4, splits4, grid,
-                       lp_add, ntiles, partial);
-    return hipGetLastError();
-}
-
-hipError_t launch_integrate_tiles(int cls, const TileGrid &tg, const EntryRec *recs,
-                                  const int *tile_start, const double *grid, int lp_add,
-                                  int ntiles, int splits, double *partial, hipStream_t st) {
-    switch (cls) {
-    case 0: return launch_integrate_tiles_lm<4>(tg, recs, tile_start, grid, lp_add, ntiles, splits, partial, st);
-    case 1: return launch_integrate_tiles_lm<8>(tg, recs, tile_start, grid, lp_add, ntiles, splits, partial, st);
-    case 2: return launch_integrate_tiles_lm<12>(tg, recs, tile_start, grid, lp_add, ntiles, splits, partial, st);
-    default: return launch_integrate_tiles_lm<17>(tg, recs, tile_start, grid, lp_add, ntiles, splits, partial, st);
-    }
 }
 
 // workgroup size of k_coef / k_hab: one wave for small blocks, four waves

@@ -1,0 +1,279 @@
+// gpw_tiles_colloc.hpp -- collocate tile kernels k_collocate_tiles<LM, GEN>
+// and their launcher: grid (or a split buffer) += sum over the (sub-box, tile)
+// work items of each tile (z, y, x contraction of the reference's
+// src/lib/grid_base_ref_c.cc:403-508 and :593-846).
+#pragma once
+
+#include "gpw_tiles.hpp"
+
+namespace gpw {
+
+template <int LM, bool GEN>
+struct CollocSmem {
+    using C = TileCfg<LM>;
+    GenRows<C::Q, GEN> gr;
+    EntryRec rec[3][C::Q];
+    double coef[2][C::Q][C::C3M];
+    double X[C::Q][C::XS];
+    double Y[C::Q][TY][C::YS];
+    double cxy[C::Q][TZ][C::C2M];
+    int xi[C::Q][TX];
+    int T[C::Q][NROW];
+    int bmax[C::Q][NXG], bmin[C::Q][NXG];
+    int lp[C::Q];
+};
+
+// register block of one thread for one pair: rows y = 4*yb + r of plane z,
+// points x = 8*xg + j
+template <int L, int LM, bool GEN>
+__device__ __forceinline__ void colloc_block(const CollocSmem<LM, GEN> &sm, int q, int xg, int yb,
+                                             int z, const int (&Tr)[RB], bool full,
+                                             const GenBlock &gb, double (&acc)[RB][XB]) {
+    constexpr int lp = L - 1;
+    (void)full;
+    double cx[RB][L];
+    const double *cxy = sm.cxy[q][z];
+#pragma unroll
+    for (int r = 0; r < RB; r++)
+#pragma unroll
+        for (int l = 0; l < L; l++) cx[r][l] = 0.0;
+    // y stage, rows innermost: each cxy value is loaded once and used by its
+    // RB FMAs at once (a row-outer order keeps every cxy value live across
+    // the rows and spills registers)
+#pragma unroll
+    for (int lyp = 0; lyp <= lp; lyp++) {
+        double y[RB];
+#pragma unroll
+        for (int r = 0; r < RB; r++) y[r] = sm.Y[q][yb * RB + r][lyp];
+#pragma unroll
+        for (int lxp = 0; lxp <= lp - lyp; lxp++) {
+            const double c = cxy[c2idx(lp, lyp, lxp)];
+#pragma unroll
+            for (int r = 0; r < RB; r++) cx[r][lxp] += c * y[r];
+        }
+    }
+    const double *Xg = sm.X[q] + xg * TileCfg<LM>::GS;
+    if constexpr (GEN) {
+        double e[RB], rho[RB];
+        if (gen_row_start(sm.gr, q, z, yb, gb, e, rho)) {
+#pragma unroll
+            for (int j = 0; j < XB; j++) {
+                double xv[L];
+#pragma unroll
+                for (int l = 0; l < L; l++) xv[l] = Xg[j * LM + l];
+#pragma unroll
+                for (int r = 0; r < RB; r++) {
+                    double v = 0.0;
+#pragma unroll
+                    for (int l = 0; l < L; l++) v += cx[r][l] * xv[l];
+                    const double m = (gb.bits[r] & (1 << j)) ? e[r] : 0.0;
+                    acc[r][j] = __builtin_fma(v, m, acc[r][j]);
+                    if (j < XB - 1) {
+                        e[r] *= rho[r];
+                        rho[r] *= gb.kap;
+                    }
+                }
+            }
+        } else {
+            // per-point exponentials (a block across a periodic wrap, or far
+            // from a steep Gaussian): one point per iteration, so that only
+            // RB exponentials are live at a time
+#pragma unroll 1
+            for (int j = 0; j < XB; j++) {
+                double xv[L];
+#pragma unroll
+                for (int l = 0; l < L; l++) xv[l] = Xg[j * LM + l];
+                const int code = sm.xi[q][xg * XB + j];
+                const double u = (double)code - gb.gp0;
+                double d[RB];
+#pragma unroll
+                for (int r = 0; r < RB; r++) {
+                    double v = 0.0;
+#pragma unroll
+                    for (int l = 0; l < L; l++) v += cx[r][l] * xv[l];
+                    const int row = z * TY + yb * RB + r;
+                    double qb, qc;
+                    gen_row_coef(sm.gr, q, z, yb * RB + r, qb, qc);
+                    const bool in = code >= sm.gr.lo[q][row] && code <= sm.gr.hi[q][row];
+                    d[r] = in ? v * exp((gb.qa * u + qb) * u + qc) : 0.0;
+                }
+#pragma unroll
+                for (int jj = 0; jj < XB; jj++)
+#pragma unroll
+                    for (int r = 0; r < RB; r++) acc[r][jj] += jj == j ? d[r] : 0.0;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < XB; j++) {
+            double xv[L];
+#pragma unroll
+            for (int l = 0; l < L; l++) xv[l] = Xg[j * LM + l];
+            const int code = sm.xi[q][xg * XB + j];
+#pragma unroll
+            for (int r = 0; r < RB; r++) {
+                double v = 0.0;
+#pragma unroll
+                for (int l = 0; l < L; l++) v += cx[r][l] * xv[l];
+                // the sphere mask as a 0/1 factor of one FMA (cheaper than a
+                // 64-bit select and an add)
+                const double m = (full || code <= Tr[r]) ? 1.0 : 0.0;
+                acc[r][j] = __builtin_fma(v, m, acc[r][j]);
+            }
+        }
+    }
+}
+
+
+// out_mode 0: grid += acc (splits == 1); 1: partial[split] = acc; 2: partial[split] += acc
+// General-cell collocate kernels run one workgroup per CU: the row data of
+// the general path would otherwise spill registers in the inner loops
+// (measured faster despite the lower occupancy). The general integrate
+// kernels keep two (their x stage runs in two halves of the powers above
+// L = 8, which measured faster than one workgroup per CU without spills).
+template <int LM, bool GEN>
+__global__ __launch_bounds__(NROW, GEN ? 1 : 2) void k_collocate_tiles(
+    TileGrid tg, const EntryRec *__restrict__ recs, const int *__restrict__ tile_start,
+    const double *__restrict__ coef, int splits, double *__restrict__ out, int out_mode) {
+    using C = TileCfg<LM>;
+    constexpr int Q = C::Q;
+    __shared__ CollocSmem<LM, GEN> sm;
+    const int tile = blockIdx.x / splits, split = blockIdx.x % splits;
+    const int tix = tile % tg.ntx, tiy = (tile / tg.ntx) % tg.nty, tiz = tile / (tg.ntx * tg.nty);
+    const int x0 = tix * TX, y0 = tiy * TY, z0 = tiz * TZ;
+    const int t = threadIdx.x, xg = t & 3, yb = (t >> 2) & 3, z = t >> 4;
+    const int beg = tile_start[tile], end = tile_start[tile + 1];
+    const int per = (end - beg + splits - 1) / splits;
+    const int mybeg = min(end, beg + split * per), myend = min(end, mybeg + per);
+    if (out_mode != 1 && mybeg >= myend) return;
+    const int *rec_words = reinterpret_cast<const int *>(recs);
+
+    // prologue: records of chunks 0 and 1, then coefficients of chunk 0
+    for (int w = t; w < 2 * Q * RECW; w += NROW) {
+        const int item = mybeg + w / RECW;
+        if (item < myend)
+            reinterpret_cast<int *>(&sm.rec[w / (Q * RECW)][0])[w % (Q * RECW)] =
+                rec_words[(long long)item * RECW + w % RECW];
+    }
+    __syncthreads();
+    for (int w = t; w < Q * C::C3M; w += NROW) {
+        const int q = w / C::C3M, idx = w % C::C3M;
+        if (mybeg + q < myend && idx < ncoset(sm.rec[0][q].lp))
+            sm.coef[0][q][idx] = coef[sm.rec[0][q].coef_off + idx];
+    }
+    __syncthreads();
+
+    double acc[RB][XB];
+#pragma unroll
+    for (int r = 0; r < RB; r++)
+#pragma unroll
+        for (int j = 0; j < XB; j++) acc[r][j] = 0.0;
+
+    int k = 0;
+    for (int c = mybeg; c < myend; c += Q, k++) {
+        const int nq = min(Q, myend - c);
+        const int rs = k % 3, cs = k & 1;
+        // prefetch: records of chunk k+2, coefficients of chunk k+1
+        int pf_rec = 0;
+        if (t < Q * RECW && c + 2 * Q + t / RECW < myend)
+            pf_rec = rec_words[(long long)(c + 2 * Q) * RECW + t];
+        double pf_coef[C::NPF];
+#pragma unroll
+        for (int j = 0; j < C::NPF; j++) {
+            pf_coef[j] = 0.0;
+            const int w = t + j * NROW;
+            if (w < Q * C::C3M) {
+                const int q = w / C::C3M, idx = w % C::C3M;
+                const EntryRec &Rn = sm.rec[(k + 1) % 3][q];
+                if (c + Q + q < myend && idx < ncoset(Rn.lp)) pf_coef[j] = coef[Rn.coef_off + idx];
+            }
+        }
+        // table phase: wave q -> pair q
+        {
+            const int q = t / 64, lane = t % 64;
+            if (q < nq) {
+                const EntryRec &R = sm.rec[rs][q];
+                const int lp = R.lp;
+                if (lane == 0) sm.lp[q] = lp;
+#define GPW_P1_CALL(L)                                                                        \
+    tile_phase1<L, LM, true, GEN, Q>(R, q, lane, x0, y0, z0, tg, sm.coef[cs][q], sm.X[q],      \
+                                     sm.Y[q], nullptr, sm.cxy[q], sm.xi[q], sm.T[q],          \
+                                     sm.bmax[q], sm.bmin[q], sm.gr)
+                GPW_LP_DISPATCH(LM, GPW_P1_CALL)
+#undef GPW_P1_CALL
+            }
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int q = 0; q < nq; q++) {
+            int Tr[RB];
+            GenBlock gb;
+            bool full;
+            if (!classify_block<GEN>(sm, q, xg, yb, z, Tr, gb, full)) continue;
+            const int lp = sm.lp[q];
+#define GPW_COLLOC_CALL(L) colloc_block<L, LM, GEN>(sm, q, xg, yb, z, Tr, full, gb, acc)
+            GPW_LP_DISPATCH(LM, GPW_COLLOC_CALL)
+#undef GPW_COLLOC_CALL
+        }
+        // commit the prefetched data to the LDS rings
+        if (t < Q * RECW) reinterpret_cast<int *>(&sm.rec[(k + 2) % 3][0])[t] = pf_rec;
+#pragma unroll
+        for (int j = 0; j < C::NPF; j++) {
+            const int w = t + j * NROW;
+            if (w < Q * C::C3M) sm.coef[cs ^ 1][w / C::C3M][w % C::C3M] = pf_coef[j];
+        }
+        __syncthreads();
+    }
+    const int gz = z0 + z;
+    if (gz >= tg.nz) return;
+    const long long ng = (long long)tg.nx * tg.ny * tg.nz;
+    double *dst = out_mode == 0 ? out : out + split * ng;
+#pragma unroll
+    for (int r = 0; r < RB; r++) {
+        const int gy = y0 + yb * RB + r;
+        if (gy >= tg.ny) continue;
+        double *row = dst + ((long long)gz * tg.ny + gy) * tg.nx + x0 + xg * XB;
+#pragma unroll
+        for (int j = 0; j < XB; j++) {
+            if (x0 + xg * XB + j < tg.nx) {
+                if (out_mode == 1)
+                    row[j] = acc[r][j];
+                else
+                    row[j] += acc[r][j];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launcher (host side), instantiated once per cell kind in
+// gpw_tiles_colloc_ortho.hip / gpw_tiles_colloc_general.hip so the two
+// kernel families compile side by side.
+// class c handles L = lp + 1 in (4c, 4c + 4], the last class up to 17
+template <bool GEN>
+hipError_t launch_collocate_tiles_t(int cls, const TileGrid &tg, const EntryRec *recs,
+                                    const int *tile_start, const double *coef, int ntiles, int splits,
+                                    double *out, int out_mode, hipStream_t st) {
+    const dim3 grid(ntiles * splits), block(NROW);
+    switch (cls) {
+    case 0:
+        hipLaunchKernelGGL((k_collocate_tiles<4, GEN>), grid, block, 0, st, tg, recs, tile_start, coef,
+                           splits, out, out_mode);
+        break;
+    case 1:
+        hipLaunchKernelGGL((k_collocate_tiles<8, GEN>), grid, block, 0, st, tg, recs, tile_start, coef,
+                           splits, out, out_mode);
+        break;
+    case 2:
+        hipLaunchKernelGGL((k_collocate_tiles<12, GEN>), grid, block, 0, st, tg, recs, tile_start, coef,
+                           splits, out, out_mode);
+        break;
+    default:
+        hipLaunchKernelGGL((k_collocate_tiles<17, GEN>), grid, block, 0, st, tg, recs, tile_start, coef,
+                           splits, out, out_mode);
+        break;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace gpw
