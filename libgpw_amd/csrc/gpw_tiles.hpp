@@ -10,7 +10,7 @@ namespace gpw {
 
 template <int LM>
 struct TileCfg {
-    static constexpr int Q = LM <= 8 ? 4 : 2;           // pairs per chunk
+    static constexpr int Q = LM <= 4 ? 8 : LM <= 8 ? 4 : 2;  // pairs per chunk
     static constexpr int C2M = LM * (LM + 1) / 2;       // (lyp, lxp) terms
     static constexpr int C3M = LM * (LM + 1) * (LM + 2) / 6;
     static constexpr int GS = XB * LM + 4;              // doubles per x block (+32 B spreads banks)

@@ -188,10 +188,10 @@ __global__ __launch_bounds__(NROW, GEN ? 1 : 2) void k_collocate_tiles(
                 if (c + Q + q < myend && idx < ncoset(Rn.lp)) pf_coef[j] = coef[Rn.coef_off + idx];
             }
         }
-        // table phase: wave q -> pair q
+        // table phase: wave w -> pairs w, w + 4, ...
         {
-            const int q = t / 64, lane = t % 64;
-            if (q < nq) {
+            const int lane = t % 64;
+            for (int q = t / 64; q < nq; q += NROW / 64) {
                 const EntryRec &R = sm.rec[rs][q];
                 const int lp = R.lp;
                 if (lane == 0) sm.lp[q] = lp;

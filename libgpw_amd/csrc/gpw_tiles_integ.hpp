@@ -238,8 +238,8 @@ __device__ __forceinline__ void integrate_tile(IntegSmem<LM, GEN> &sm, int bid, 
         if (t < Q * RECW && c + Q + t / RECW < myend)
             pf_rec = rec_words[(long long)(c + Q) * RECW + t];
         {
-            const int q = t / 64, lane = t % 64;
-            if (q < nq) {
+            const int lane = t % 64;
+            for (int q = t / 64; q < nq; q += NROW / 64) {
                 const EntryRec &R = sm.rec[rs][q];
                 const int lp = R.lp + lp_add;
                 if (lane == 0) sm.lp[q] = lp;
