@@ -101,13 +101,13 @@ hipError_t launch_coef(const PairDev *pairs, const int *order, int pair0, int np
 size_t coef_smem_bytes(int lam, int lbm, bool lds_blocks);
 size_t coef_ws_doubles(int lam, int lbm);
 bool coef_needs_workspace(int lam, int lbm);
+// nhx: the most hab_cross_entries among the launched pairs (forces without tau)
 hipError_t launch_hab(const PairDev *pairs, const int *order, const int *segs, int seg0, int nseg,
-                      int tau, int with_forces, int la_max, int lb_max, int nab, const double *Mbuf,
+                      int tau, int with_forces, int la_max, int lb_max, int nhx, const double *Mbuf,
                       int mstride, const double *pab, double *hab, double *forces,
                       const LatticeCSR &lt, double *ws,
                       long long ws_stride, int ws_blocks, hipStream_t st);
-size_t hab_ws_doubles(int lamx, int lbmx, int lam, int lbm);
-bool hab_needs_workspace(int lamx, int lbmx, int lam, int lbm);
+size_t hab_ws_need(int la_max, int lb_max, int tau, int nhx);
 hipError_t launch_moments_small(int cls, const TileGrid &tg, const SubDev *subs, const int *list,
                                 int n, const int *prof, const double *grid, int lp_add, int mstride,
                                 double *Mbuf, hipStream_t st);

@@ -20,7 +20,7 @@ constexpr int NROW = TY * TZ;             // 256 threads per workgroup
 // a global workspace of HAB_WS_BLOCKS slices (one per resident workgroup);
 // TR_NT: their largest workgroup
 constexpr size_t COEF_LDS_MAX = 24 * 1024;
-constexpr size_t HAB_LDS_MAX = 64 * 1024;
+constexpr size_t HAB_LDS_MAX = 32 * 1024;
 constexpr int HAB_WS_BLOCKS = 4096;
 constexpr int TR_NT = 256;
 // the one-pair-per-wave transforms (tau = 0) take blocks up to this LDS size
@@ -52,6 +52,20 @@ __host__ __device__ constexpr int ncoset(int l) {
     return l < 0 ? 0 : ((l + 1) * (l + 2) * (l + 3)) / 6;
 }
 __host__ __device__ constexpr int ncoef2(int l) { return ((l + 1) * (l + 2)) / 2; }
+// sum of ncoset(k) over k <= l (0 for l < 0)
+__host__ __device__ constexpr int c4(int l) {
+    return l < 0 ? 0 : ((l + 1) * (l + 2) * (l + 3) * (l + 4)) / 24;
+}
+
+// heff entries k_hab keeps with forces and without tau: the block's own b
+// shells with the a shells la_min - 1 .. la_max + 1, then the b shells
+// lb_min - 1 and lb_max + 1 with the block's own a shells
+__host__ __device__ constexpr int hab_cross_entries(int la_min, int la_max, int lb_min, int lb_max) {
+    const int nax = ncoset(la_max + 1) - ncoset((la_min > 0 ? la_min - 1 : 0) - 1);
+    const int nbx = ncoset(lb_max + 1) - ncoset((lb_min > 0 ? lb_min - 1 : 0) - 1);
+    const int nam = ncoset(la_max) - ncoset(la_min - 1), nbm = ncoset(lb_max) - ncoset(lb_min - 1);
+    return nbm * nax + (nbx - nbm) * nam;
+}
 
 // CP2K Cartesian order (reference src/lib/grid_base_ref_c.cc:39-46)
 __host__ __device__ constexpr int coset(int lx, int ly, int lz) {

@@ -266,9 +266,17 @@ __global__ void k_sum_entries(const PairDev *__restrict__ pairs, int tau, int lp
 
 // heff = prefactor * alpha^T M over the extended block, then hab (rho copy /
 // tau adjoint) and forces (d phi/dA = 2 zeta phi_{a+1} - a phi_{a-1}).
-// Workgroups loop over pairs (pair0 + blockIdx.x, stride gridDim.x). heff and
-// eff live in LDS, or, when gws is given (blocks too large for LDS), in the
-// workgroup's slice gws + blockIdx.x * ws_stride of a global workspace.
+// Workgroups loop over pairs (pair0 + blockIdx.x, stride gridDim.x). heff,
+// eff and U live in LDS, or, when gws is given (blocks too large for LDS), in
+// the workgroup's slice gws + blockIdx.x * ws_stride of a global workspace.
+// Storage kept small so that most blocks fit in LDS:
+//   * U[az][bz] holds only its triangle ly + lx <= (lamx - az) + (lbmx - bz),
+//     the slices packed one after the other (offsets uoff);
+//   * with forces and without tau heff holds only the cross that the hab and
+//     force stages read (hab_cross_entries): the block's own b shells with
+//     every a shell of the extended block, then the b shells outside the
+//     block with the block's own a shells; otherwise the dense extended block;
+//   * without tau the forces weight heff with pab itself (no eff copy).
 // Work unit: a segment of pairs that share one hab block (segs: CSR over
 // positions of `order`, segments seg0 .. seg0 + nseg); one workgroup walks a
 // segment's pairs in order, so the += into the shared block is race-free and
@@ -284,8 +292,10 @@ __global__ __launch_bounds__(TR_NT) void k_hab(const PairDev *__restrict__ pairs
     extern __shared__ double sm_hab[];
     __shared__ int dec[NCOSET_LMAX];
     __shared__ int dec2[NC2MAX];
+    __shared__ int uoff[(LMAX + 1) * (LMAX + 1)];
     fill_decode(dec, threadIdx.x, blockDim.x);  // first read after the barrier below
     const int fx = with_forces ? 1 : 0;
+    const bool cross = with_forces && !tau;
     for (int sg = seg0 + blockIdx.x; sg < seg0 + nseg; sg += gridDim.x) {
     const int ip_end = segs ? segs[sg + 1] : sg + 1;
     for (int ip = segs ? segs[sg] : sg; ip < ip_end; ip++) {
@@ -302,15 +312,31 @@ __global__ __launch_bounds__(TR_NT) void k_hab(const PairDev *__restrict__ pairs
     const int nlax = ncoset(lamx), nlbx = ncoset(lbmx);
     const int n3x = ncoset(lpx);
     const int n2x = ncoef2(lpx);
+    const int nzz = (lamx + 1) * (lbmx + 1);
     double *M = sm_hab + nalpha;                    // ncoset(lpx)
     double *Mijk = M + n3x;                         // general cells: lattice moments
     double *red = Mijk + n3x;                       // 6 x (waves per workgroup)
-    // heff (nlbx x nlax), eff (ncoset(lbm) x ncoset(lam)), U ((lamx+1)(lbmx+1) x C2(lpx))
+    // only the shells lamx_lo..lamx, lbmx_lo..lbmx: the hab and force stages
+    // read nothing else
+    const int na_lo = ncoset(lamx_lo - 1), nb_lo = ncoset(lbmx_lo - 1);
+    const int nAx = nlax - na_lo, nBx = nlbx - nb_lo;
+    const int na_mid = ncoset(p.la_min - 1), nAmid = ncoset(p.la_max) - na_mid;
+    const int nb_mid = ncoset(p.lb_min - 1), nBmid = ncoset(p.lb_max) - nb_mid;
+    const int n1 = cross ? nBmid * nAx : nAx * nBx;
+    const int nout = cross ? n1 + (nBx - nBmid) * nAmid : n1;
+    // heff (nout cross entries, or nlbx x nlax), eff (tau only), U (packed slices)
     double *heff = gws ? gws + blockIdx.x * ws_stride : red + 6 * (TR_NT / 64);
-    double *eff = heff + nlax * nlbx;
-    double *U = eff + ncoset(lam) * ncoset(lbm);
+    double *eff = heff + (cross ? nout : nlax * nlbx);
+    double *U = eff + (tau ? ncoset(lam) * ncoset(lbm) : 0);
     fill_alpha(&pairs[pi], A, threadIdx.x, blockDim.x);
     fill_decode2(dec2, lpx, threadIdx.x, blockDim.x);
+    // offset of slice (az, bz): the triangles of the slices before it, in
+    // closed form (sums of ncoset over a range are differences of c4)
+    for (int zz = threadIdx.x; zz < nzz; zz += blockDim.x) {
+        const int az = zz / (lbmx + 1), bz = zz % (lbmx + 1), ma = lamx - az;
+        uoff[zz] = (c4(lpx) - c4(lpx - az)) - (c4(lamx - 1) - c4(lamx - 1 - az)) +
+                   (ncoset(ma + lbmx) - ncoset(ma + lbmx - bz));
+    }
     if (!lt.rp) {
         for (int c = threadIdx.x; c < n3x; c += blockDim.x) M[c] = Mbuf[(long long)pi * mstride + c];
     } else {
@@ -323,11 +349,11 @@ __global__ __launch_bounds__(TR_NT) void k_hab(const PairDev *__restrict__ pairs
     // adjoint of the factorised cab -> cxyz transform (see k_coef):
     //   U[az][bz][(ly,lx)] = sum_lz M[(lz,ly,lx)] alpha_z(bz,az,lz)
     //   heff[b][a] = prefactor sum_{ly,lx} U[az][bz][(ly,lx)] alpha_y(by,ay,ly) alpha_x(bx,ax,lx)
-    const int nzz = (lamx + 1) * (lbmx + 1);
     for (int w = threadIdx.x; w < nzz * n2x; w += blockDim.x) {
         const int zz = w / n2x, az = zz / (lbmx + 1), bz = zz % (lbmx + 1);
         const int dc = dec2[w % n2x], ly = dc & 255, lx = dc >> 8;
-        if (ly + lx > (lamx - az) + (lbmx - bz)) continue;
+        const int nl = (lamx - az) + (lbmx - bz);
+        if (ly + lx > nl) continue;
         // two accumulators: shorter dependency chains
         double acc = 0.0;
         const double *Az = &A.at(2, bz, az, 0);
@@ -339,20 +365,19 @@ __global__ __launch_bounds__(TR_NT) void k_hab(const PairDev *__restrict__ pairs
         }
         if (lz <= az + bz) acc += M[cidx(lpx, lz, ly, lx)] * Az[lz];
         acc += acc2;
-        U[w] = acc;
+        U[uoff[zz] + c2idx(nl, ly, lx)] = acc;
     }
     __syncthreads();
-    const int na_lo = ncoset(lamx_lo - 1), nb_lo = ncoset(lbmx_lo - 1);
-    // only the shells lamx_lo..lamx, lbmx_lo..lbmx: the hab and force stages
-    // read nothing else. Forces without tau read a cross of it: the block's
-    // own b shells with every a shell, and its own a shells with every b
-    // shell (not the corners a +- 1 with b +- 1), so only the cross is built.
-    const int nAx = nlax - na_lo, nBx = nlbx - nb_lo;
-    const bool cross = with_forces && !tau;
-    const int na_mid = ncoset(p.la_min - 1), nAmid = ncoset(p.la_max) - na_mid;
-    const int nb_mid = ncoset(p.lb_min - 1), nBmid = ncoset(p.lb_max) - nb_mid;
-    const int n1 = cross ? nBmid * nAx : nAx * nBx;
-    const int nout = cross ? n1 + (nBx - nBmid) * nAmid : n1;
+    // heff entry of (jco, ico): cross storage or dense
+    auto hix = [&](int jco, int ico) -> int {
+        if (!cross) return jco * nlax + ico;
+        if (jco >= nb_mid && jco < nb_mid + nBmid) return (jco - nb_mid) * nAx + (ico - na_lo);
+        const int r = jco < nb_mid ? jco - nb_lo : (jco - nb_mid - nBmid) + (nb_mid - nb_lo);
+        return n1 + r * nAmid + (ico - na_mid);
+    };
+    // Forces without tau read a cross of the extended block: the block's own
+    // b shells with every a shell, and its own a shells with every b shell
+    // (not the corners a +- 1 with b +- 1), so only the cross is built.
     for (int w = threadIdx.x; w < nout; w += blockDim.x) {
         int ico, jco;
         if (w < n1) {
@@ -366,11 +391,12 @@ __global__ __launch_bounds__(TR_NT) void k_hab(const PairDev *__restrict__ pairs
         const int da = dec[ico], db = dec[jco];
         const int ax = da & 15, ay = (da >> 4) & 15, az = da >> 8;
         const int bx = db & 15, by = (db >> 4) & 15, bz = db >> 8;
-        const double *Uz = U + (az * (lbmx + 1) + bz) * n2x;
+        const int nl = (lamx - az) + (lbmx - bz);
+        const double *Uz = U + uoff[az * (lbmx + 1) + bz];
         const double *Ax = &A.at(0, bx, ax, 0), *Ay = &A.at(1, by, ay, 0);
         double acc = 0.0;
         for (int ly = 0; ly <= ay + by; ly++) {
-            const double *Ur = Uz + c2idx(lpx, ly, 0);
+            const double *Ur = Uz + c2idx(nl, ly, 0);
             double sx = 0.0;
             double sx2 = 0.0;
             int lx = 0;
@@ -382,7 +408,7 @@ __global__ __launch_bounds__(TR_NT) void k_hab(const PairDev *__restrict__ pairs
             sx += sx2;
             acc += sx * Ay[ly];
         }
-        heff[jco * nlax + ico] = p.prefactor * acc;
+        heff[cross ? w : jco * nlax + ico] = p.prefactor * acc;
     }
     __syncthreads();
     // hab: loop over the original block entries
@@ -393,7 +419,7 @@ __global__ __launch_bounds__(TR_NT) void k_hab(const PairDev *__restrict__ pairs
         const int ico = na_lo0 + w % nA, jco = nb_lo0 + w / nA;
         double acc;
         if (!tau) {
-            acc = heff[jco * nlax + ico];
+            acc = heff[hix(jco, ico)];
         } else {
             const int da = dec[ico], db = dec[jco];
             const int a[3] = {da & 15, (da >> 4) & 15, da >> 8};
@@ -418,13 +444,13 @@ __global__ __launch_bounds__(TR_NT) void k_hab(const PairDev *__restrict__ pairs
     }
     if (!with_forces) continue;
     const int nla = ncoset(lam), nlb = ncoset(lbm);
-    build_eff(p, tau, pab, eff, nla, nlb, nla, threadIdx.x, blockDim.x);
+    if (tau) build_eff(p, tau, pab, eff, nla, nlb, nla, threadIdx.x, blockDim.x);
     double f[6] = {0, 0, 0, 0, 0, 0};
     const int la_lo = ncoset(max(p.la_min - tau, 0) - 1), lb_lo = ncoset(max(p.lb_min - tau, 0) - 1);
     for (int w = threadIdx.x; w < nla * nlb; w += blockDim.x) {
         const int ico = w % nla, jco = w / nla;
         if (ico < la_lo || jco < lb_lo) continue;
-        const double e = eff[jco * nla + ico];
+        const double e = tau ? eff[jco * nla + ico] : pab[p.block_off + (long long)jco * p.ld + ico];
         if (e == 0.0) continue;
         const int da = dec[ico], db = dec[jco];
         const int a[3] = {da & 15, (da >> 4) & 15, da >> 8};
@@ -434,15 +460,15 @@ __global__ __launch_bounds__(TR_NT) void k_hab(const PairDev *__restrict__ pairs
             int bp[3] = {b[0], b[1], b[2]}, bm[3] = {b[0], b[1], b[2]};
             ap[d]++;
             bp[d]++;
-            double ga = 2.0 * p.zeta * heff[jco * nlax + coset(ap[0], ap[1], ap[2])];
-            double gb = 2.0 * p.zetb * heff[coset(bp[0], bp[1], bp[2]) * nlax + ico];
+            double ga = 2.0 * p.zeta * heff[hix(jco, coset(ap[0], ap[1], ap[2]))];
+            double gb = 2.0 * p.zetb * heff[hix(coset(bp[0], bp[1], bp[2]), ico)];
             if (a[d] > 0) {
                 am[d]--;
-                ga -= a[d] * heff[jco * nlax + coset(am[0], am[1], am[2])];
+                ga -= a[d] * heff[hix(jco, coset(am[0], am[1], am[2]))];
             }
             if (b[d] > 0) {
                 bm[d]--;
-                gb -= b[d] * heff[coset(bm[0], bm[1], bm[2]) * nlax + ico];
+                gb -= b[d] * heff[hix(coset(bm[0], bm[1], bm[2]), ico)];
             }
             f[d] += e * ga;
             f[3 + d] += e * gb;
@@ -573,10 +599,39 @@ __global__ __launch_bounds__(NROW) void k_moments_small(TileGrid tg, const SubDe
                 const double *row = grid + ((long long)gz * tg.ny + gy) * tg.nx;
                 double e = exp(ex0), rho = exp(d0);
                 int gx = imod(s.base[0] + (ilo - s.lb[0]), s.period[0]);  // then stepped, wrapped
-                for (int i = ilo; i <= ihi; i++) {
+                const int P0 = s.period[0];
+                int i = ilo;
+                // four points per step: independent power chains, loads issued
+                // together; the sums keep the point order of the one-point loop
+                for (; i + 3 <= ihi; i += 4) {
+                    const int g1 = gx + 1 == P0 ? 0 : gx + 1;
+                    const int g2 = g1 + 1 == P0 ? 0 : g1 + 1;
+                    const int g3 = g2 + 1 == P0 ? 0 : g2 + 1;
+                    double p[4] = {row[gx], row[g1], row[g2], row[g3]};
+                    gx = g3 + 1 == P0 ? 0 : g3 + 1;
+                    double u[4];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        u[q] = gen ? (double)(i + q) - s.roff[0] : (i + q) * tg.dr[0] - s.roff[0];
+                        p[q] *= e;
+                        e *= rho;
+                        rho *= kap;
+                    }
+#pragma unroll
+                    for (int l = 0; l < LM; l++) {
+                        if (l < L) {
+#pragma unroll
+                            for (int q = 0; q < 4; q++) {
+                                tx[l] += p[q];
+                                p[q] *= u[q];
+                            }
+                        }
+                    }
+                }
+                for (; i <= ihi; i++) {
                     const double u = gen ? (double)i - s.roff[0] : i * tg.dr[0] - s.roff[0];
                     double p = row[gx] * e;
-                    gx = gx + 1 == s.period[0] ? 0 : gx + 1;
+                    gx = gx + 1 == P0 ? 0 : gx + 1;
 #pragma unroll
                     for (int l = 0; l < LM; l++) {
                         if (l < L) {
@@ -734,19 +789,36 @@ __global__ __launch_bounds__(NROW) void k_collocate_small(TileGrid tg, const Sub
         }
         double *out = box + ((long long)z * ny + y) * nx - s.lb[0];
         double e = ilo <= ihi ? exp(ex0) : 0.0, rho = ilo <= ihi ? exp(d0) : 0.0;
-        for (int i = s.lb[0]; i <= s.ub[0]; i++) {
-            double v = 0.0;
-            if (i >= ilo && i <= ihi) {
-                const double u = gen ? (double)i - s.roff[0] : i * tg.dr[0] - s.roff[0];
-                double p = cx[LM - 1];
+        for (int i = s.lb[0]; i < min(ilo, s.ub[0] + 1); i++) out[i] = 0.0;
+        int i = ilo;
+        // four points per step: independent Horner chains
+        for (; i + 3 <= ihi; i += 4) {
+            double u[4], p[4], ee[4];
 #pragma unroll
-                for (int l = LM - 2; l >= 0; l--) p = p * u + cx[l];
-                v = p * e;
+            for (int q = 0; q < 4; q++) {
+                u[q] = gen ? (double)(i + q) - s.roff[0] : (i + q) * tg.dr[0] - s.roff[0];
+                p[q] = cx[LM - 1];
+                ee[q] = e;
                 e *= rho;
                 rho *= kap;
             }
-            out[i] = v;
+#pragma unroll
+            for (int l = LM - 2; l >= 0; l--)
+#pragma unroll
+                for (int q = 0; q < 4; q++) p[q] = p[q] * u[q] + cx[l];
+#pragma unroll
+            for (int q = 0; q < 4; q++) out[i + q] = p[q] * ee[q];
         }
+        for (; i <= ihi; i++) {
+            const double u = gen ? (double)i - s.roff[0] : i * tg.dr[0] - s.roff[0];
+            double p = cx[LM - 1];
+#pragma unroll
+            for (int l = LM - 2; l >= 0; l--) p = p * u + cx[l];
+            out[i] = p * e;
+            e *= rho;
+            rho *= kap;
+        }
+        for (i = max(i, s.lb[0]); i <= s.ub[0]; i++) out[i] = 0.0;
     }
 }
 
@@ -1440,25 +1512,48 @@ hipError_t launch_coef(const PairDev *pairs, const int *order, int pair0, int np
     return hipGetLastError();
 }
 
-size_t hab_ws_doubles(int lamx, int lbmx, int lam, int lbm) {
-    return (size_t)ncoset(lamx) * ncoset(lbmx) + (size_t)ncoset(lam) * ncoset(lbm) +
-           (size_t)(lamx + 1) * (lbmx + 1) * ncoef2(lamx + lbmx);
+// k_hab per-pair blocks (heff, eff, packed U) in doubles, for an extended
+// block (lamx, lbmx) of a local block (lam, lbm); nhx >= 0: heff holds only
+// the cross of nhx entries (forces without tau), else the dense block
+size_t hab_ws_doubles(int lamx, int lbmx, int lam, int lbm, int tau, int nhx) {
+    size_t u = 0;
+    for (int az = 0; az <= lamx; az++)
+        for (int bz = 0; bz <= lbmx; bz++) u += (size_t)ncoef2((lamx - az) + (lbmx - bz));
+    const size_t heff = nhx >= 0 ? (size_t)nhx : (size_t)ncoset(lamx) * ncoset(lbmx);
+    const size_t eff = tau ? (size_t)ncoset(lam) * ncoset(lbm) : 0;
+    return heff + eff + u;
 }
 
-// LDS of k_hab: alpha, M, Mijk, the force reduction and (LDS mode) heff, eff
-size_t hab_smem_bytes(int lamx, int lbmx, int lam, int lbm, bool lds_blocks) {
+// LDS of k_hab: alpha, M, Mijk, the force reduction and (LDS mode) the blocks
+size_t hab_smem_bytes(int lamx, int lbmx, int lam, int lbm, int tau, int nhx, bool lds_blocks) {
     const int lpx = lamx + lbmx;
     size_t n = 3 * (size_t)(lbmx + 1) * (lamx + 1) * (lpx + 1) + 2 * (size_t)ncoset(lpx) + 6 * (TR_NT / 64);
-    if (lds_blocks) n += hab_ws_doubles(lamx, lbmx, lam, lbm);
+    if (lds_blocks) n += hab_ws_doubles(lamx, lbmx, lam, lbm, tau, nhx);
     return sizeof(double) * n;
 }
 
-bool hab_needs_workspace(int lamx, int lbmx, int lam, int lbm) {
-    return hab_smem_bytes(lamx, lbmx, lam, lbm, true) > HAB_LDS_MAX;
+bool hab_needs_workspace(int lamx, int lbmx, int lam, int lbm, int tau, int nhx) {
+    return hab_smem_bytes(lamx, lbmx, lam, lbm, tau, nhx, true) > HAB_LDS_MAX;
 }
 
+// workspace doubles per workgroup k_hab needs for blocks up to (la_max,
+// lb_max) with or without forces (nhx: the most cross entries, see
+// hab_cross_entries); 0 when every case fits in LDS
+size_t hab_ws_need(int la_max, int lb_max, int tau, int nhx) {
+    const int lam = la_max + tau, lbm = lb_max + tau;
+    size_t need = 0;
+    for (int fx = 0; fx <= 1; fx++) {
+        const int h = (fx && !tau) ? nhx : -1;
+        if (hab_needs_workspace(lam + fx, lbm + fx, lam, lbm, tau, h))
+            need = std::max(need, hab_ws_doubles(lam + fx, lbm + fx, lam, lbm, tau, h));
+    }
+    return need;
+}
+
+// nhx: the most heff cross entries (hab_cross_entries) among the launched
+// pairs, used with forces and without tau
 hipError_t launch_hab(const PairDev *pairs, const int *order, const int *segs, int seg0, int nseg,
-                      int tau, int with_forces, int la_max, int lb_max, int nab, const double *Mbuf,
+                      int tau, int with_forces, int la_max, int lb_max, int nhx, const double *Mbuf,
                       int mstride, const double *pab, double *hab, double *forces,
                       const LatticeCSR &lt, double *ws,
                       long long ws_stride, int ws_blocks, hipStream_t st) {
@@ -1466,12 +1561,12 @@ hipError_t launch_hab(const PairDev *pairs, const int *order, const int *segs, i
     const int npairs = nseg;
     const int fx = with_forces ? 1 : 0;
     const int lamx = la_max + tau + fx, lbmx = lb_max + tau + fx;
-    (void)nab;
-    const bool big = hab_needs_workspace(lamx, lbmx, la_max + tau, lb_max + tau);
-    if (big && (!ws || ws_stride < (long long)hab_ws_doubles(lamx, lbmx, la_max + tau, lb_max + tau) ||
-                ws_blocks < 1))
+    const int h = (with_forces && !tau) ? nhx : -1;
+    const int lam = la_max + tau, lbm = lb_max + tau;
+    const bool big = hab_needs_workspace(lamx, lbmx, lam, lbm, tau, h);
+    if (big && (!ws || ws_stride < (long long)hab_ws_doubles(lamx, lbmx, lam, lbm, tau, h) || ws_blocks < 1))
         return hipErrorInvalidValue;
-    const size_t smem = hab_smem_bytes(lamx, lbmx, la_max + tau, lb_max + tau, !big);
+    const size_t smem = hab_smem_bytes(lamx, lbmx, lam, lbm, tau, h, !big);
     const int nblk = big ? std::min(npairs, ws_blocks) : npairs;
     hipLaunchKernelGGL(k_hab, dim3(nblk), dim3(transform_threads(lamx + lbmx)), smem, st, pairs, order, segs, seg0, nseg, tau,
                        with_forces, Mbuf, mstride, pab, hab, forces, lt,

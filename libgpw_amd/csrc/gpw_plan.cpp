@@ -224,8 +224,10 @@ struct gpw_tasklist {
     // form one segment, walked by one workgroup of k_hab
     struct PairGroup {
         int start, count, la_max, lb_max, seg0, nseg, nab;
+        int nhx;  // most k_hab cross entries of a pair (hab_cross_entries)
     };
     int nab_max = 0;  // most block-shell entries of a pair (block_shell_entries)
+    int nhx_max = 0;  // most k_hab cross entries of a pair (hab_cross_entries)
     std::vector<PairGroup> groups;
     int *d_order = nullptr;
     int *d_segs = nullptr;
@@ -938,13 +940,14 @@ static gpw_status group_pairs(gpw_tasklist *tl, std::vector<int> &order, std::ve
     segs.clear();
     tl->groups.clear();
     tl->nab_max = 0;
+    tl->nhx_max = 0;
     for (auto &v : byc) {
         if (v.empty()) continue;
         if (!sorted)
             std::stable_sort(v.begin(), v.end(), [&](int a, int b) {
                 return tl->pairs_h[a].block_off < tl->pairs_h[b].block_off;
             });
-        gpw_tasklist::PairGroup g{(int)order.size(), (int)v.size(), 0, 0, (int)segs.size(), 0, 0};
+        gpw_tasklist::PairGroup g{(int)order.size(), (int)v.size(), 0, 0, (int)segs.size(), 0, 0, 0};
         long long last = -1;
         for (int i : v) {
             if (tl->pairs_h[i].block_off != last) {
@@ -957,9 +960,11 @@ static gpw_status group_pairs(gpw_tasklist *tl, std::vector<int> &order, std::ve
             g.la_max = std::max(g.la_max, pd.la_max);
             g.lb_max = std::max(g.lb_max, pd.lb_max);
             g.nab = std::max(g.nab, block_shell_entries(pd.la_min, pd.la_max, pd.lb_min, pd.lb_max));
+            g.nhx = std::max(g.nhx, hab_cross_entries(pd.la_min, pd.la_max, pd.lb_min, pd.lb_max));
         }
         tl->groups.push_back(g);
         tl->nab_max = std::max(tl->nab_max, g.nab);
+        tl->nhx_max = std::max(tl->nhx_max, g.nhx);
     }
     segs.push_back((int)order.size());
     return GPW_OK;
@@ -1154,8 +1159,7 @@ static gpw_status upload_tasklist(gpw_tasklist *tl, HostPlan &hp, bool device_it
         const int lam = tl->la_max + t, lbm = tl->lb_max + t;
         long long stride = 0;
         if (coef_needs_workspace(lam, lbm)) stride = (long long)coef_ws_doubles(lam, lbm);
-        if (hab_needs_workspace(lam + 1, lbm + 1, lam, lbm))
-            stride = std::max(stride, (long long)hab_ws_doubles(lam + 1, lbm + 1, lam, lbm));
+        stride = std::max(stride, (long long)hab_ws_need(tl->la_max, tl->lb_max, t, tl->nhx_max));
         if (stride > 0) {
             tl->hab_ws_stride = stride;
             tl->hab_ws_blocks = std::min(tl->npairs, HAB_WS_BLOCKS);
@@ -1721,7 +1725,7 @@ gpw_status gpw_tasklist_integrate(gpw_tasklist *tl, const double *grid, const do
         PhaseTimer ph(tl, st, PH_HAB);
         for (const auto &g : tl->groups)
             HIP_TRY(launch_hab(tl->d_pairs, tl->d_order, tl->d_segs, g.seg0, g.nseg, tl->compute_tau,
-                               wf, g.la_max, g.lb_max, g.nab, tl->d_Mbuf, tl->mstride, pab, hab, forces,
+                               wf, g.la_max, g.lb_max, g.nhx, tl->d_Mbuf, tl->mstride, pab, hab, forces,
                                tl->lat(), tl->d_hab_ws, tl->hab_ws_stride,
                                tl->hab_ws_blocks, st));
     }
@@ -1788,7 +1792,7 @@ gpw_status gpw_tasklist_integrate_plain(gpw_tasklist *tl, const double *grid, co
         }
         const PairDev &pd = tl->pairs_h[p];
         HIP_TRY(launch_hab(tl->d_pairs, nullptr, nullptr, p, 1, tl->compute_tau, wf, pd.la_max, pd.lb_max,
-                           block_shell_entries(pd.la_min, pd.la_max, pd.lb_min, pd.lb_max), tl->d_Mbuf,
+                           hab_cross_entries(pd.la_min, pd.la_max, pd.lb_min, pd.lb_max), tl->d_Mbuf,
                            tl->mstride, pab, hab, forces,
                            tl->lat(), tl->d_hab_ws, tl->hab_ws_stride,
                            tl->hab_ws_blocks, st));

@@ -190,8 +190,9 @@ gpw_status make_plan(const Task &t, int nspheres, const int *sphere_bounds, cons
     const int lam = t.p.la_max + t.tau, lbm = t.p.lb_max + t.tau;
     long long ws = 0;
     if (coef_needs_workspace(lam, lbm)) ws = (long long)coef_ws_doubles(lam, lbm);
-    if (hab_needs_workspace(lam + 1, lbm + 1, lam, lbm))
-        ws = std::max(ws, (long long)hab_ws_doubles(lam + 1, lbm + 1, lam, lbm));
+    ws = std::max(ws, (long long)hab_ws_need(t.p.la_max, t.p.lb_max, t.tau,
+                                              hab_cross_entries(t.p.la_min, t.p.la_max, t.p.lb_min,
+                                                                t.p.lb_max)));
     P.ws_doubles = ws;
     // device pair: in DEV mode pab / hab are the caller's maxco x maxco
     // arrays, read and written in place; otherwise a packed block
@@ -342,7 +343,7 @@ gpw_status run_integrate(const Plan &P, char *R, const double *grid, const doubl
                                      s.ub[1] - s.lb[1] + 1, 2 * wf, prof, src, P.mstride, Mbuf, st));
     }
     HIP_TRY(launch_hab(dpair, nullptr, nullptr, 0, 1, P.t.tau, wf, P.t.p.la_max, P.t.p.lb_max,
-                       block_shell_entries(P.t.p.la_min, P.t.p.la_max, P.t.p.lb_min, P.t.p.lb_max), Mbuf,
+                       hab_cross_entries(P.t.p.la_min, P.t.p.la_max, P.t.p.lb_min, P.t.p.lb_max), Mbuf,
                        P.mstride, pab, hab, forces, lattice_view(P, R), ws, P.ws_doubles, 1, st));
     return GPW_OK;
 }
