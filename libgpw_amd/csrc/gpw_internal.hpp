@@ -29,7 +29,7 @@ constexpr size_t WAVE_LDS_MAX = 64 * 1024;
 // points (general cells; SMALL_NX x SMALL_NYZ_ORTHO^2 on orthorhombic ones,
 // where the tile kernels are faster) are integrated by k_moments_small (one
 // workgroup per pair) instead of the tiles; limits chosen by measurement
-constexpr int SMALL_NX = 64, SMALL_NY = 48, SMALL_NZ = 48, SMALL_NYZ_ORTHO = 32, SMALL_LMAX = 12;
+constexpr int SMALL_NX = 64, SMALL_NY = 48, SMALL_NZ = 48, SMALL_NYZ_ORTHO = 32, SMALL_LMAX = 17;
 
 // collocate of small spheres: pairs whose cube is one sub-box of at most
 // CSMALL_N points per axis (general cells; CSMALL_N_ORTHO on orthorhombic

@@ -648,22 +648,32 @@ __global__ __launch_bounds__(NROW) void k_moments_small(TileGrid tg, const SubDe
                 if (l < L) sm.rows[t][l] = tx[l];
         }
         __syncthreads();
+        // y and z stages with four partial sums (shorter dependency chains)
         for (int w = t; w < np * n2; w += NROW) {
             const int zp = w / n2;
             int ly, lx;
             decode_c2(lp, w % n2, ly, lx);
-            double a = 0.0;
-            for (int y = 0; y < ny; y++) a += sm.Yt[y][ly] * sm.rows[zp * ny + y][lx];
-            sm.t2[zp][w % n2] = a;
+            const double *rw = &sm.rows[zp * ny][lx];
+            double a[4] = {0.0, 0.0, 0.0, 0.0};
+            int y = 0;
+            for (; y + 3 < ny; y += 4)
+#pragma unroll
+                for (int q = 0; q < 4; q++) a[q] += sm.Yt[y + q][ly] * rw[(y + q) * LM];
+            for (; y < ny; y++) a[0] += sm.Yt[y][ly] * rw[y * LM];
+            sm.t2[zp][w % n2] = (a[0] + a[1]) + (a[2] + a[3]);
         }
         __syncthreads();
         for (int c = t; c < n3; c += NROW) {
             int lz, ly, lx;
             decode_c3(lp, c, lz, ly, lx);
             const int c2 = c2idx(lp, ly, lx);
-            double a = sm.M[c];
-            for (int zp = 0; zp < np; zp++) a += sm.Zt[k0 + zp][lz] * sm.t2[zp][c2];
-            sm.M[c] = a;
+            double a[4] = {0.0, 0.0, 0.0, 0.0};
+            int zp = 0;
+            for (; zp + 3 < np; zp += 4)
+#pragma unroll
+                for (int q = 0; q < 4; q++) a[q] += sm.Zt[k0 + zp + q][lz] * sm.t2[zp + q][c2];
+            for (; zp < np; zp++) a[0] += sm.Zt[k0 + zp][lz] * sm.t2[zp][c2];
+            sm.M[c] += (a[0] + a[1]) + (a[2] + a[3]);
         }
         __syncthreads();
     }
@@ -1583,7 +1593,7 @@ hipError_t launch_sum_entries(const PairDev *pairs, int npairs, int tau, int lp_
     return hipGetLastError();
 }
 
-// class c: L = lp + lp_add + 1 in (4c, 4c + 4], c <= 2
+// class c: L = lp + lp_add + 1 in (4c, 4c + 4], the last class (c = 3) up to 17
 hipError_t launch_moments_small(int cls, const TileGrid &tg, const SubDev *subs, const int *list,
                                 int n, const int *prof, const double *grid, int lp_add, int mstride,
                                 double *Mbuf, hipStream_t st) {
@@ -1592,6 +1602,7 @@ hipError_t launch_moments_small(int cls, const TileGrid &tg, const SubDev *subs,
     case 0: hipLaunchKernelGGL(k_moments_small<4>, dim3(n), dim3(NROW), 0, st, tg, subs, list, prof, grid, lp_add, mstride, Mbuf); break;
     case 1: hipLaunchKernelGGL(k_moments_small<8>, dim3(n), dim3(NROW), 0, st, tg, subs, list, prof, grid, lp_add, mstride, Mbuf); break;
     case 2: hipLaunchKernelGGL(k_moments_small<12>, dim3(n), dim3(NROW), 0, st, tg, subs, list, prof, grid, lp_add, mstride, Mbuf); break;
+    case 3: hipLaunchKernelGGL(k_moments_small<17>, dim3(n), dim3(NROW), 0, st, tg, subs, list, prof, grid, lp_add, mstride, Mbuf); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1666,7 +1677,7 @@ hipError_t launch_plan_records(const SubDev *subs, const int *ent_sub, const lon
     return hipGetLastError();
 }
 
-// class c: L = lp + 1 in (4c, 4c + 4], c <= 2
+// class c: L = lp + 1 in (4c, 4c + 4], the last class (c = 3) up to 17
 hipError_t launch_collocate_small(int cls, const TileGrid &tg, const SubDev *subs, const int *list, int n,
                                   const double *coef, const int *prof, const long long *box_off, double *sval,
                                   hipStream_t st) {
@@ -1675,6 +1686,7 @@ hipError_t launch_collocate_small(int cls, const TileGrid &tg, const SubDev *sub
     case 0: hipLaunchKernelGGL(k_collocate_small<4>, dim3(n), dim3(NROW), 0, st, tg, subs, list, coef, prof, box_off, sval); break;
     case 1: hipLaunchKernelGGL(k_collocate_small<8>, dim3(n), dim3(NROW), 0, st, tg, subs, list, coef, prof, box_off, sval); break;
     case 2: hipLaunchKernelGGL(k_collocate_small<12>, dim3(n), dim3(NROW), 0, st, tg, subs, list, coef, prof, box_off, sval); break;
+    case 3: hipLaunchKernelGGL(k_collocate_small<17>, dim3(n), dim3(NROW), 0, st, tg, subs, list, coef, prof, box_off, sval); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -154,7 +154,7 @@ gpw_status gpw::profile_from_bounds(const int lb[3], const int ub[3], int nspher
 // each, so registers are sized for the class, not for the largest L)
 
 enum { PH_COEF, PH_CTILES, PH_CREDUCE, PH_ITILES, PH_ISUM, PH_HAB, NPH };
-constexpr int NSCLS = 3;
+constexpr int NSCLS = 4;
 // tile-list set of integrate without (wf = 0) / with (wf = 1) forces
 static inline int iset(int wf) { return wf ? 1 : 2; }
 constexpr int NEVENT = 2048;
@@ -735,7 +735,7 @@ static gpw_status plan_common(gpw_tasklist *tl, const std::vector<gpw_pair_desc>
                 sd.ub[2] - sd.lb[2] + 1 > std::min(nyz, SMALL_NZ))
                 continue;
             hp.small_mask[w * pairs.size() + p] = 1;
-            by_cls[(L - 1) / 4].push_back(si);
+            by_cls[std::min((L - 1) / 4, NSCLS - 1)].push_back(si);
         }
         for (int c = 0; c < NSCLS; c++) {
             tl->small_off[w][c] = (long long)hp.small.size();
@@ -765,7 +765,7 @@ static gpw_status plan_common(gpw_tasklist *tl, const std::vector<gpw_pair_desc>
             hp.small_mask[2 * pairs.size() + p] = 1;
             hp.box_off[si] = tot;
             tot += vol;
-            by_cls[(L - 1) / 4].push_back(si);
+            by_cls[std::min((L - 1) / 4, NSCLS - 1)].push_back(si);
         }
         for (int c = 0; c < NSCLS; c++) {
             tl->csmall_off[c] = (long long)hp.csmall.size();
@@ -1413,7 +1413,7 @@ gpw_status gpw_tasklist_stats_get(const gpw_tasklist *tl, gpw_tasklist_stats *o)
     o->flops_integrate[0] = tl->flops_integrate[0];
     o->flops_integrate[1] = tl->flops_integrate[1];
     for (int i = 0; i < 4; i++) o->plan_ms[i] = tl->plan_ms[i];
-    o->csmall_pairs = tl->csmall_cnt[0] + tl->csmall_cnt[1] + tl->csmall_cnt[2];
+    o->csmall_pairs = tl->csmall_cnt[0] + tl->csmall_cnt[1] + tl->csmall_cnt[2] + tl->csmall_cnt[3];
     o->csmall_points = tl->sval_total;
     long long ce = 0;
     for (int c = 0; c < NCLS; c++) ce += tl->class_entries[0][c];
