@@ -57,29 +57,56 @@ class MultiGrid:
             lv["tl"].integrate(v, hab, pab=pab, forces=None if forces is None else forces[k])
         return hab
 
-    def step(self, pab, grids, pots, hab, forces=None):
-        """One collocate + integrate (+ forces) of every level."""
+    def _level_streams(self):
+        if getattr(self, "_streams", None) is None:
+            self._streams = [torch.cuda.Stream() for _ in self.levels]
+        return self._streams
+
+    def step(self, pab, grids, pots, hab, forces=None, concurrent=True):
+        """One collocate + integrate (+ forces) of every level.
+
+        The levels share nothing they write (own grid, own task-list scratch,
+        disjoint hab blocks, own force buffer), so with ``concurrent`` each
+        level runs on a stream of its own, forked from and joined back to the
+        current stream; the coarse levels, too small to fill the GPU, then run
+        beside the fine one. Results are the same either way."""
         for g in grids:
             g.zero_()
         hab.zero_()
         if forces is not None:
             for f in forces:
                 f.zero_()
-        self.collocate(pab, grids)
-        self.integrate(pots, hab, pab=pab if forces is not None else None, forces=forces)
+        pab_f = pab if forces is not None else None
+        if not concurrent:
+            self.collocate(pab, grids)
+            self.integrate(pots, hab, pab=pab_f, forces=forces)
+            return
+        cur = torch.cuda.current_stream()
+        streams = self._level_streams()
+        for k, (lv, s) in enumerate(zip(self.levels, streams)):
+            s.wait_stream(cur)
+            with torch.cuda.stream(s):
+                lv["tl"].collocate(pab, grids[k])
+                lv["tl"].integrate(pots[k], hab, pab=pab_f,
+                                   forces=None if forces is None else forces[k])
+        for s in streams:
+            cur.wait_stream(s)
 
     def capture(self, pab, grids, pots, hab, forces=None):
         """HIP graph of ``step``; replaying it recomputes grids, hab and forces
-        from the current contents of pab and pots."""
+        from the current contents of pab and pots. The graph is recorded on
+        one stream, levels one after the other (a capture that forked to
+        other streams and joined them again ended in a crash of the capture's
+        end), so the eager concurrent ``step`` is the faster form."""
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(s):  # warm-up outside the graph
-            self.step(pab, grids, pots, hab, forces)
+            self.step(pab, grids, pots, hab, forces, concurrent=False)
         torch.cuda.current_stream().wait_stream(s)
         torch.cuda.synchronize()
         graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(graph):
-            self.step(pab, grids, pots, hab, forces)
+            self.step(pab, grids, pots, hab, forces, concurrent=False)
         return graph
 
     def forces_by_pair(self, forces) -> np.ndarray:
