@@ -87,6 +87,9 @@ hipError_t launch_reduce_partials(const double *partial, int splits, long long n
 hipError_t launch_integrate_tiles(int cls, const TileGrid &tg, const EntryRec *recs,
                                   const int *tile_start, const double *grid, int lp_add,
                                   int ntiles, int splits, double *partial, hipStream_t st);
+hipError_t launch_collocate_tiles_84(const TileGrid &tg, const EntryRec *recs8, const int *start8,
+                                     int splits8, const EntryRec *recs4, const int *start4, int splits4,
+                                     const double *coef, int ntiles, double *partial, hipStream_t st);
 hipError_t launch_integrate_tiles_84(const TileGrid &tg, const EntryRec *recs8, const int *start8,
                                      int splits8, const EntryRec *recs4, const int *start4,
                                      int splits4, const double *grid, int lp_add, int ntiles,

@@ -237,8 +237,10 @@ __global__ void k_fill(double *__restrict__ p, long long n, double v) {
 }
 
 // Mbuf[pair*mstride + c] = sum over the pair's entries of partial[ent_off[e] + c];
-// entries are numbered pair-major, so the pair's partials are one contiguous run
-__global__ void k_sum_entries(const PairDev *__restrict__ pairs, int tau, int lp_add,
+// entries are numbered pair-major, so the pair's partials are one contiguous run.
+// One workgroup of SUM_NT threads per pair.
+constexpr int SUM_NT = 64;
+__global__ __launch_bounds__(SUM_NT) void k_sum_entries(const PairDev *__restrict__ pairs, int tau, int lp_add,
                               const int *__restrict__ pe_start, const long long *__restrict__ ent_off,
                               const double *__restrict__ partial, int mstride,
                               const unsigned char *__restrict__ skip, double *__restrict__ Mbuf) {
@@ -250,7 +252,33 @@ __global__ void k_sum_entries(const PairDev *__restrict__ pairs, int tau, int lp
     const int b = pe_start[p], e = pe_start[p + 1];
     const double *src = b < e ? partial + ent_off[b] : partial;
     const int ne = e - b;
-    for (int c = threadIdx.x; c < n3; c += blockDim.x) {
+    const int t = threadIdx.x;
+    // short moment vectors: G = 64 / n3 groups of lanes take every G-th item,
+    // then the group sums are added in group order (fixed order: the result
+    // does not depend on scheduling)
+    const int G = n3 <= SUM_NT / 2 ? SUM_NT / n3 : 1;
+    if (G > 1) {
+        __shared__ double red[SUM_NT];
+        const int g = t / n3, c = t % n3;
+        if (g < G) {
+            double a0 = 0.0, a1 = 0.0;
+            int k = g;
+            for (; k + G < ne; k += 2 * G) {
+                a0 += src[(long long)k * n3 + c];
+                a1 += src[(long long)(k + G) * n3 + c];
+            }
+            if (k < ne) a0 += src[(long long)k * n3 + c];
+            red[t] = a0 + a1;
+        }
+        __syncthreads();
+        if (t < n3) {
+            double a = 0.0;
+            for (int q = 0; q < G; q++) a += red[q * n3 + t];
+            Mbuf[(long long)p * mstride + t] = a;
+        }
+        return;
+    }
+    for (int c = t; c < n3; c += blockDim.x) {
         double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
         int k = 0;
         for (; k + 4 <= ne; k += 4) {
@@ -1588,7 +1616,7 @@ hipError_t launch_sum_entries(const PairDev *pairs, int npairs, int tau, int lp_
                               const int *pe_start, const long long *ent_off,
                               const double *partial, int mstride, const unsigned char *skip,
                               double *Mbuf, hipStream_t st) {
-    hipLaunchKernelGGL(k_sum_entries, dim3(npairs), dim3(64), 0, st, pairs, tau, lp_add, pe_start,
+    hipLaunchKernelGGL(k_sum_entries, dim3(npairs), dim3(SUM_NT), 0, st, pairs, tau, lp_add, pe_start,
                        ent_off, partial, mstride, skip, Mbuf);
     return hipGetLastError();
 }

@@ -1434,12 +1434,31 @@ static int collocate_splits_max(const gpw_tasklist *tl) {
     return smax;
 }
 
+// Orthorhombic lists whose work is all in classes 0 and 1, both split: the
+// two classes run in one launch, each into partial grids of its own
+static bool collocate_fused(const gpw_tasklist *tl) {
+    return !tl->tg.general && tl->class_entries[0][0] && tl->class_entries[0][1] &&
+           !tl->class_entries[0][2] && !tl->class_entries[0][3] && tl->csplit[0] > 1 && tl->csplit[1] > 1;
+}
+
+// partial grids the collocate launches write (0: they add into the grid)
+static int collocate_partials(const gpw_tasklist *tl) {
+    if (collocate_fused(tl)) return tl->csplit[0] + tl->csplit[1];
+    const int smax = collocate_splits_max(tl);
+    return smax > 1 ? smax : 0;
+}
+
 // Collocate tile launches of every class. With smax == 1 the kernels add into
 // the grid; otherwise into smax partial grids (reduced afterwards): the class
 // with the most splits runs first and writes all smax partials (workgroups
-// without items write zeros), the others add into theirs.
-static hipError_t collocate_classes(gpw_tasklist *tl, double *grid, int smax, hipStream_t st) {
+// without items write zeros), the others add into theirs. Fused classes 0
+// and 1 (collocate_fused, when the partial grids are there) write theirs.
+static hipError_t collocate_classes(gpw_tasklist *tl, double *grid, int smax, bool fused, hipStream_t st) {
     const int nt1 = tl->ntiles + 1;
+    if (fused)
+        return launch_collocate_tiles_84(tl->tg, tl->d_recs + tl->list_off[0][1], tl->d_tile_start + (size_t)nt1,
+                                         tl->csplit[1], tl->d_recs + tl->list_off[0][0], tl->d_tile_start,
+                                         tl->csplit[0], tl->d_coef, tl->ntiles, tl->d_cpartial, st);
     int order[NCLS] = {0, 1, 2, 3};
     std::stable_sort(order, order + NCLS, [&](int a, int b) { return tl->csplit[a] > tl->csplit[b]; });
     bool first = true;
@@ -1612,18 +1631,18 @@ gpw_status gpw_tasklist_autotune(gpw_tasklist *tl, int max_splits, int reps, voi
     }
 #undef AT_TRY
     // keep only the partial grids the chosen collocate splits use
-    const int smax = collocate_splits_max(tl);
-    if (tl->cpart_cap != (smax > 1 ? smax : 0)) {
+    const int npart = collocate_partials(tl);
+    if (tl->cpart_cap != npart) {
         if (tl->d_cpartial) (void)hipFree(tl->d_cpartial);
         tl->d_cpartial = nullptr;
         tl->cpart_cap = 0;
-        if (smax > 1) {
-            hipError_t e = hipMalloc((void **)&tl->d_cpartial, sizeof(double) * ng * smax);
+        if (npart > 0) {
+            hipError_t e = hipMalloc((void **)&tl->d_cpartial, sizeof(double) * ng * npart);
             if (e != hipSuccess) {
                 cleanup();
                 return fail(GPW_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
             }
-            tl->cpart_cap = smax;
+            tl->cpart_cap = npart;
         }
     }
     cleanup();
@@ -1657,9 +1676,12 @@ gpw_status gpw_tasklist_collocate(gpw_tasklist *tl, const double *pab, double *g
         if (s) return s;
     }
     const int smax = collocate_splits_max(tl);
+    // fused classes when their partial grids are allocated (after autotune)
+    const bool fused = collocate_fused(tl) && tl->cpart_cap >= collocate_partials(tl);
+    const int nred = fused ? collocate_partials(tl) : (smax > 1 ? smax : 0);
     {
         PhaseTimer ph(tl, st, PH_CTILES);
-        HIP_TRY(collocate_classes(tl, grid, smax, st));
+        HIP_TRY(collocate_classes(tl, grid, smax, fused, st));
         // small spheres: each pair's box into the scratch buffer
         for (int c = 0; c < NSCLS; c++)
             HIP_TRY(launch_collocate_small(c, tl->tg, tl->d_subs, tl->d_csmall + tl->csmall_off[c],
@@ -1667,10 +1689,10 @@ gpw_status gpw_tasklist_collocate(gpw_tasklist *tl, const double *pab, double *g
                                            tl->d_sval, st));
     }
     const bool gather = tl->class_entries[3][0] > 0;
-    if (smax > 1 || gather) {
+    if (nred > 0 || gather) {
         PhaseTimer ph(tl, st, PH_CREDUCE);
         const long long ng = (long long)tl->tg.nx * tl->tg.ny * tl->tg.nz;
-        if (smax > 1) HIP_TRY(launch_reduce_partials(tl->d_cpartial, smax, ng, grid, st));
+        if (nred > 0) HIP_TRY(launch_reduce_partials(tl->d_cpartial, nred, ng, grid, st));
         // then the small boxes, tile by tile in list order
         if (gather)
             HIP_TRY(launch_gather_small(tl->tg, tl->d_recs + tl->list_off[3][0],

@@ -131,14 +131,17 @@ __device__ __forceinline__ void colloc_block(const CollocSmem<LM, GEN> &sm, int 
 // (measured faster despite the lower occupancy). The general integrate
 // kernels keep two (their x stage runs in two halves of the powers above
 // L = 8, which measured faster than one workgroup per CU without spills).
+// One workgroup's share of a tile: workgroup bid = tile * splits + split;
+// split buffers are out + (split_base + split) * grid points.
 template <int LM, bool GEN>
-__global__ __launch_bounds__(NROW, GEN ? 1 : 2) void k_collocate_tiles(
-    TileGrid tg, const EntryRec *__restrict__ recs, const int *__restrict__ tile_start,
-    const double *__restrict__ coef, int splits, double *__restrict__ out, int out_mode) {
+__device__ __forceinline__ void collocate_tile(CollocSmem<LM, GEN> &sm, int bid, const TileGrid &tg,
+                                               const EntryRec *__restrict__ recs,
+                                               const int *__restrict__ tile_start,
+                                               const double *__restrict__ coef, int splits,
+                                               double *__restrict__ out, int out_mode, int split_base) {
     using C = TileCfg<LM>;
     constexpr int Q = C::Q;
-    __shared__ CollocSmem<LM, GEN> sm;
-    const int tile = blockIdx.x / splits, split = blockIdx.x % splits;
+    const int tile = bid / splits, split = bid % splits;
     const int tix = tile % tg.ntx, tiy = (tile / tg.ntx) % tg.nty, tiz = tile / (tg.ntx * tg.nty);
     const int x0 = tix * TX, y0 = tiy * TY, z0 = tiz * TZ;
     const int t = threadIdx.x, xg = t & 3, yb = (t >> 2) & 3, z = t >> 4;
@@ -227,7 +230,7 @@ __global__ __launch_bounds__(NROW, GEN ? 1 : 2) void k_collocate_tiles(
     const int gz = z0 + z;
     if (gz >= tg.nz) return;
     const long long ng = (long long)tg.nx * tg.ny * tg.nz;
-    double *dst = out_mode == 0 ? out : out + split * ng;
+    double *dst = out_mode == 0 ? out : out + (split_base + split) * ng;
 #pragma unroll
     for (int r = 0; r < RB; r++) {
         const int gy = y0 + yb * RB + r;
@@ -243,6 +246,35 @@ __global__ __launch_bounds__(NROW, GEN ? 1 : 2) void k_collocate_tiles(
             }
         }
     }
+}
+
+template <int LM, bool GEN>
+__global__ __launch_bounds__(NROW, GEN ? 1 : 2) void k_collocate_tiles(
+    TileGrid tg, const EntryRec *__restrict__ recs, const int *__restrict__ tile_start,
+    const double *__restrict__ coef, int splits, double *__restrict__ out, int out_mode) {
+    __shared__ CollocSmem<LM, GEN> sm;
+    collocate_tile<LM, GEN>(sm, blockIdx.x, tg, recs, tile_start, coef, splits, out, out_mode, 0);
+}
+
+// Two classes in one launch (workgroups of class A first, then class B),
+// each into split buffers of its own (A: 0 .. splitsA - 1, B: splitsA ..
+// splitsA + splitsB - 1, written, not added), so the tail of one class
+// overlaps the other; LDS is the union of both.
+template <int LMA, int LMB, bool GEN>
+__global__ __launch_bounds__(NROW, 2) void k_collocate_tiles2(
+    TileGrid tg, const EntryRec *__restrict__ recsA, const int *__restrict__ startA, int splitsA,
+    const EntryRec *__restrict__ recsB, const int *__restrict__ startB, int splitsB,
+    const double *__restrict__ coef, int ntiles, double *__restrict__ partial) {
+    __shared__ union {
+        CollocSmem<LMA, GEN> a;
+        CollocSmem<LMB, GEN> b;
+    } sm;
+    const int na = ntiles * splitsA;
+    if ((int)blockIdx.x < na)
+        collocate_tile<LMA, GEN>(sm.a, blockIdx.x, tg, recsA, startA, coef, splitsA, partial, 1, 0);
+    else
+        collocate_tile<LMB, GEN>(sm.b, blockIdx.x - na, tg, recsB, startB, coef, splitsB, partial, 1,
+                                 splitsA);
 }
 
 // ---------------------------------------------------------------------------
