@@ -8,9 +8,19 @@
 
 namespace gpw {
 
+// Collocate chunk configuration: the general-cell kernels run one workgroup
+// per CU and have LDS to spare, so their classes L <= 8 and L <= 12 take
+// eight and four pairs per chunk (more work between barriers; with four the
+// table phase keeps every wave busy)
+template <int LM, bool GEN>
+struct CollocCfg : TileCfg<LM> {
+    static constexpr int Q = (GEN && LM == 12) ? 4 : (GEN && LM == 8) ? 8 : TileCfg<LM>::Q;
+    static constexpr int NPF = (Q * TileCfg<LM>::C3M + NROW - 1) / NROW;
+};
+
 template <int LM, bool GEN>
 struct CollocSmem {
-    using C = TileCfg<LM>;
+    using C = CollocCfg<LM, GEN>;
     GenRows<C::Q, GEN> gr;
     EntryRec rec[3][C::Q];
     double coef[2][C::Q][C::C3M];
@@ -139,8 +149,9 @@ __device__ __forceinline__ void collocate_tile(CollocSmem<LM, GEN> &sm, int bid,
                                                const int *__restrict__ tile_start,
                                                const double *__restrict__ coef, int splits,
                                                double *__restrict__ out, int out_mode, int split_base) {
-    using C = TileCfg<LM>;
+    using C = CollocCfg<LM, GEN>;
     constexpr int Q = C::Q;
+    static_assert(Q * RECW <= NROW, "one record word per thread in the prefetch");
     const int tile = bid / splits, split = bid % splits;
     const int tix = tile % tg.ntx, tiy = (tile / tg.ntx) % tg.nty, tiz = tile / (tg.ntx * tg.nty);
     const int x0 = tix * TX, y0 = tiy * TY, z0 = tiz * TZ;

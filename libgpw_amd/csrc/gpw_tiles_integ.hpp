@@ -199,6 +199,7 @@ __device__ __forceinline__ void integrate_tile(IntegSmem<LM, GEN> &sm, int bid, 
                                                int splits, double *__restrict__ partial) {
     using C = TileCfg<LM>;
     constexpr int Q = C::Q;
+    static_assert(Q * RECW <= NROW, "one record word per thread in the prefetch");
     const int tile = bid / splits, split = bid % splits;
     const int tix = tile % tg.ntx, tiy = (tile / tg.ntx) % tg.nty, tiz = tile / (tg.ntx * tg.nty);
     const int x0 = tix * TX, y0 = tiy * TY, z0 = tiz * TZ;
