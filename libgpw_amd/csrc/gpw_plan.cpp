@@ -1587,6 +1587,24 @@ gpw_status gpw_tasklist_autotune(gpw_tasklist *tl, int max_splits, int reps, voi
             std::copy(pick, pick + NCLS, bestc);
         }
     }
+    // classes 0 and 1 fused (collocate_fused): each into partial grids of its
+    // own, so the reduction sums splits0 + splits1 of them (time taken as
+    // proportional to the grids read plus the one written, from the largest
+    // measured reduction)
+    if (!tl->tg.general && tl->class_entries[0][0] && tl->class_entries[0][1] &&
+        !tl->class_entries[0][2] && !tl->class_entries[0][3] && nc > 1) {
+        const double tred1 = tr[nc - 1] / (cand[nc - 1] + 1);
+        for (int ia = 1; ia < nc; ia++)
+            for (int ib = 1; ib < nc; ib++) {
+                const double tot = tc[ia] + tc[(size_t)nc + ib] + tred1 * (cand[ia] + cand[ib] + 1);
+                if (tot < best) {
+                    best = tot;
+                    bestc[0] = cand[ia];
+                    bestc[1] = cand[ib];
+                    bestc[2] = bestc[3] = std::max(cand[ia], cand[ib]);
+                }
+            }
+    }
     // cost of the splits in use before tuning (collocate + integrate)
     double before = 0.0, after = best;
     {
