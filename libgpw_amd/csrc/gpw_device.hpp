@@ -32,11 +32,21 @@ __device__ __forceinline__ int row_threshold(const int *__restrict__ prof, int p
 // coefficient transform: pab block -> coef_xyz (cab -> cxyz)
 // (grid_base_ref_c.cc:65-315 as the direct triple product)
 
+// alpha_d(lb, la, k) for k <= la + lb (the only non-zero ones), packed: per
+// axis d the rows (lb, la), lb outer, each of length la + lb + 1
+__host__ __device__ constexpr int alpha_doubles(int lam, int lbm) {
+    return 3 * ((lam + 1) * (lbm + 1) * (lam + lbm + 2)) / 2;
+}
 struct AlphaView {
-    double *a;  // [3][lbm+1][lam+1][lpm+1]
-    int lam, lbm, lpm;
+    double *a;
+    int lam, lbm;
+    __device__ AlphaView(double *a_, int lam_, int lbm_) : a(a_), lam(lam_), lbm(lbm_) {}
     __device__ double &at(int d, int lb, int la, int k) const {
-        return a[((d * (lbm + 1) + lb) * (lam + 1) + la) * (lpm + 1) + k];
+        // rows before (lb, la): sum over lb' < lb, la' <= lam of (la' + lb' + 1),
+        // then over la' < la of (la' + lb + 1)
+        const int off = lb * ((lam + 1) * lam / 2) + (lam + 1) * (lb * (lb + 1) / 2) +
+                        la * (la - 1) / 2 + la * (lb + 1);
+        return a[d * (alpha_doubles(lam, lbm) / 3) + off + k];
     }
 };
 
@@ -65,7 +75,7 @@ __device__ inline void fill_alpha(const PairDev *__restrict__ pp, AlphaView A, i
         const int la = w % (A.lam + 1);
         const int lb = (w / (A.lam + 1)) % (A.lbm + 1);
         const int d = w / ((A.lam + 1) * (A.lbm + 1));
-        for (int k = 0; k <= A.lpm; k++) A.at(d, lb, la, k) = 0.0;
+        for (int k = 0; k <= la + lb; k++) A.at(d, lb, la, k) = 0.0;
         const double pa = pp->rp[d] - pp->ra[d];
         const double pb = pp->rp[d] - pp->rb[d];
         double powa = 1.0;

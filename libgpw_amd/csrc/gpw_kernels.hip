@@ -52,8 +52,8 @@ __global__ __launch_bounds__(TR_NT) void k_coef(const PairDev *__restrict__ pair
     const int lam = p.la_max + tau, lbm = p.lb_max + tau;
     const int lam_lo = max(p.la_min - tau, 0), lbm_lo = max(p.lb_min - tau, 0);
     const int lp = lam + lbm;
-    AlphaView A{sm_coef, lam, lbm, lp};
-    const int nalpha = 3 * (lbm + 1) * (lam + 1) * (lp + 1);
+    AlphaView A(sm_coef, lam, lbm);
+    const int nalpha = alpha_doubles(lam, lbm);
     const int nla = ncoset(lam), nlb = ncoset(lbm);
     const int n2 = ncoef2(lp), n3 = ncoset(lp);
     double *eff = gws ? gws + blockIdx.x * ws_stride : sm_coef + nalpha;
@@ -124,7 +124,7 @@ __device__ __forceinline__ void shell_range(int lmin, int lmax, int &lo, int &n)
 // at most nab (a, b) entries: alpha, the block, the slice S1[by][ay][lx] and
 // (general cells) the xyz coefficients before the lattice transform
 __host__ __device__ constexpr int coefw_doubles(int la, int lb, int nab) {
-    return 3 * (lb + 1) * (la + 1) * (la + lb + 1) + nab + (la + 1) * (lb + 1) * (la + lb + 1) + ncoset(la + lb);
+    return alpha_doubles(la, lb) + nab + (la + 1) * (lb + 1) * (la + lb + 1) + ncoset(la + lb);
 }
 
 // cab -> cxyz of one pair per wave: for each slice (az, bz)
@@ -150,8 +150,8 @@ __global__ __launch_bounds__(TW) void k_coef_wave(const PairDev *__restrict__ pa
         int na_lo, nA, nb_lo, nB;
         shell_range(p.la_min, la, na_lo, nA);
         shell_range(p.lb_min, lb, nb_lo, nB);
-        AlphaView A{smw, la, lb, lp};
-        double *P = smw + 3 * (lb + 1) * (la + 1) * (lp + 1);  // P[jb][ia] = pab[nb_lo + jb][na_lo + ia]
+        AlphaView A(smw, la, lb);
+        double *P = smw + alpha_doubles(la, lb);  // P[jb][ia] = pab[nb_lo + jb][na_lo + ia]
         double *S1 = P + nA * nB;
         double *cx = S1 + (la + 1) * (lb + 1) * (lp + 1);
         fill_alpha(&pairs[pi], A, lane, TW);
@@ -309,7 +309,9 @@ __global__ __launch_bounds__(SUM_NT) void k_sum_entries(const PairDev *__restric
 // positions of `order`, segments seg0 .. seg0 + nseg); one workgroup walks a
 // segment's pairs in order, so the += into the shared block is race-free and
 // deterministic. segs NULL: one pair per segment, positions seg0 .. seg0 + nseg.
-__global__ __launch_bounds__(TR_NT) void k_hab(const PairDev *__restrict__ pairs,
+// Registers capped for eight waves per SIMD (more pairs in flight; measured
+// faster than the uncapped 75 VGPRs).
+__global__ __launch_bounds__(TR_NT, 8) void k_hab(const PairDev *__restrict__ pairs,
                                             const int *__restrict__ order,
                                             const int *__restrict__ segs, int seg0, int nseg,
                                             int tau, int with_forces, const double *__restrict__ Mbuf,
@@ -335,8 +337,8 @@ __global__ __launch_bounds__(TR_NT) void k_hab(const PairDev *__restrict__ pairs
     const int lamx_lo = max(max(p.la_min - tau, 0) - fx, 0);
     const int lbmx_lo = max(max(p.lb_min - tau, 0) - fx, 0);
     const int lpx = lamx + lbmx;
-    AlphaView A{sm_hab, lamx, lbmx, lpx};
-    const int nalpha = 3 * (lbmx + 1) * (lamx + 1) * (lpx + 1);
+    AlphaView A(sm_hab, lamx, lbmx);
+    const int nalpha = alpha_doubles(lamx, lbmx);
     const int nlax = ncoset(lamx), nlbx = ncoset(lbmx);
     const int n3x = ncoset(lpx);
     const int n2x = ncoef2(lpx);
@@ -1509,8 +1511,7 @@ size_t coef_ws_doubles(int lam, int lbm) {
 }
 
 size_t coef_smem_bytes(int lam, int lbm, bool lds_blocks) {
-    const int lp = lam + lbm;
-    size_t n = 3 * (size_t)(lbm + 1) * (lam + 1) * (lp + 1);
+    size_t n = (size_t)alpha_doubles(lam, lbm);
     if (lds_blocks) n += coef_ws_doubles(lam, lbm);
     return sizeof(double) * n;
 }
@@ -1565,7 +1566,7 @@ size_t hab_ws_doubles(int lamx, int lbmx, int lam, int lbm, int tau, int nhx) {
 // LDS of k_hab: alpha, M, Mijk, the force reduction and (LDS mode) the blocks
 size_t hab_smem_bytes(int lamx, int lbmx, int lam, int lbm, int tau, int nhx, bool lds_blocks) {
     const int lpx = lamx + lbmx;
-    size_t n = 3 * (size_t)(lbmx + 1) * (lamx + 1) * (lpx + 1) + 2 * (size_t)ncoset(lpx) + 6 * (TR_NT / 64);
+    size_t n = (size_t)alpha_doubles(lamx, lbmx) + 2 * (size_t)ncoset(lpx) + 6 * (TR_NT / 64);
     if (lds_blocks) n += hab_ws_doubles(lamx, lbmx, lam, lbm, tau, nhx);
     return sizeof(double) * n;
 }
