@@ -204,13 +204,139 @@ __device__ inline void build_eff(const PairDev &p, int tau, const double *__rest
 
 // General cells: out = T in (dir 0) or T^T in (dir 1) for the monomials of
 // total degree <= lp, from the CSR rows of (lp, dir).
-__device__ inline void lattice_transform(const LatticeCSR &lt, int lp, int dir, const double *in,
+__device__ inline void lattice_transform(const LatticeTab &lt, int lp, int dir, const double *in,
                                   double *out, int tid, int nthr) {
-    const int *rp = lt.rp + lt.rp_off[2 * lp + dir];
-    for (int r = tid; r < ncoset(lp); r += nthr) {
+    const LatBlk b = lt.blk[2 * lp + dir];
+    const int *col = lt.col + b.base;
+    const double *val = lt.val + b.base;
+    for (int r = tid; r < b.n; r += nthr) {
         double a = 0.0;
-        for (int k = rp[r]; k < rp[r + 1]; k++) a += lt.val[k] * in[lt.col[k]];
+        // in row order: the fill adds exact zeros at the end
+#pragma unroll 4
+        for (int k = 0; k < b.width; k++) a += val[k * b.n + r] * in[col[k * b.n + r]];
         out[r] = a;
+    }
+}
+
+// Powers pw[m][n] = h[m]^n (n <= LPMAX) of the six steps of a stepped cell,
+// in LDS.
+struct StepTabs {
+    double pw[6][LPMAX + 1];
+};
+constexpr int STEP_TABS_DOUBLES = sizeof(StepTabs) / sizeof(double);
+__device__ inline void fill_step_tabs(const LatticeTab &lt, StepTabs &st, int tid, int nthr) {
+    for (int m = tid; m < 6; m += nthr) {
+        double v = 1.0;
+        for (int n = 0; n <= LPMAX; n++) {
+            st.pw[m][n] = v;
+            v *= lt.h[m];
+        }
+    }
+}
+
+// One pass of lattice_stepped (OP 0, 1, 2: A, B, C; 3, 4, 5: A', B', C'),
+// src -> dst. Output (s2, s1, s0) sums src along one line of the index
+// triangle, e = 0 .. the slot that bounds it; the source index moves along
+// the line by the closed-form steps of cidx (cidx(lp, z, y, x) =
+// ncoset(lp) - ncoset(m) + y (m + 1) - y (y - 1) / 2 + x, m = lp - z).
+template <int OP>
+__device__ inline void stepped_pass(const StepTabs &st, int lp, const double *src, double *dst,
+                                    int tid, int nthr) {
+    constexpr int SLOT = (OP == 0 || OP == 1) ? 0 : (OP == 2 || OP == 3) ? 1 : 2;
+    constexpr int SM = (OP == 0 || OP == 3) ? 1 : (OP == 1 || OP == 4) ? 2 : 4;  // the pass's step
+    const double step = st.pw[SM][1];
+    const int n = ncoset(lp);
+    for (int w = tid; w < n; w += nthr) {
+        int s0, s1, s2;
+        if (SLOT == 0)
+            decode_c3(lp, w, s0, s1, s2);
+        else if (SLOT == 1)
+            decode_c3(lp, w, s1, s0, s2);
+        else
+            decode_c3(lp, w, s2, s1, s0);
+        const int ne = step == 0.0 ? 0 : SLOT == 0 ? s0 : SLOT == 1 ? s1 : s2;
+        const int out = cidx(lp, s2, s1, s0);
+        int at = out, z = s2, y = s1;  // source index and its z-like, y-like slots
+        double acc = 0.0, pe = 1.0;    // pe = step^e
+        // binomial of the term, C(s, e) (A, B, C) or C(s + e, e) (A', B',
+        // C'), by the exact integer recurrence in float (values < 2^24; the
+        // rounding of the approximate reciprocal is far below 1/2)
+        const int sb = OP == 2 || OP == 5 ? s1 : s0;
+        float bin = 1.0f;
+        for (int e = 0; e <= ne; e++) {
+            const int m = lp - z;
+            const double c = (double)bin;
+            int d;
+            if (OP == 0) {  // (s2, s1 + e, s0 - e)
+                d = m - y;
+                y++;
+            } else if (OP == 1) {  // (s2 + e, s1, s0 - e)
+                d = (m + 1) * (m + 2) / 2 - y - 1;
+                z++;
+            } else if (OP == 2) {  // (s2 + e, s1 - e, s0)
+                d = (m + 1) * m / 2;
+                z++;
+                y--;
+            } else if (OP == 3) {  // (s2, s1 - e, s0 + e)
+                d = y - m - 1;
+                y--;
+            } else if (OP == 4) {  // (s2 - e, s1, s0 + e)
+                d = y + 1 - (m + 2) * (m + 3) / 2;
+                z--;
+            } else {  // (s2 - e, s1 + e, s0)
+                d = -((m + 2) * (m + 1) / 2);
+                z--;
+                y++;
+            }
+            acc += c * pe * src[at];
+            pe *= step;
+            at += d;
+            const int num = OP <= 2 ? sb - e : sb + e + 1;
+            bin = rintf(bin * (float)num * __builtin_amdgcn_rcpf((float)(e + 1)));
+        }
+        if (OP == 1 || OP == 5) acc *= st.pw[3][s1];
+        if (OP == 2) acc *= st.pw[5][s2];
+        if (OP == 3) acc *= st.pw[0][s0];
+        dst[out] = acc;
+    }
+}
+
+// Lattice transform of a stepped cell (LatticeTab::stepped) in three passes
+// of one Cartesian substitution each: with u = a0 i + b0 j, x = u + c0 k,
+// y = b1 j + c1 k, z = c2 k and triples (z-like, y-like, x-like) powers
+// packed by cidx,
+// dir 1, moments of i^p j^q k^r -> of x^lx y^ly z^lz:
+//   A: m1(r, q, p)     = sum_{e <= p} C(p, e) a0^(p-e) b0^e in(r, q + e, p - e)
+//   B: m2(r, q, l)     = sum_{e <= l} C(l, e) c0^e m1(r + e, q, l - e)
+//   C: out(n, m, l)    = c2^n sum_{e <= m} C(m, e) b1^(m-e) c1^e m2(n + e, m - e, l)
+// dir 0, coefficients of x^lx y^ly z^lz -> of i^p j^q k^r (the transposes,
+// in reverse order):
+//   C': g(t, q, l)     = b1^q sum_{e <= t} C(q + e, e) c1^e c2^(t-e) in(t - e, q + e, l)
+//   B': h(s, q, m)     = sum_{e <= s} C(m + e, e) c0^e g(s - e, q, m + e)
+//   A': out(r, n, m)   = a0^m sum_{e <= n} C(m + e, e) b0^e h(r, n - e, m + e)
+// (the multinomial expansion of grid_base_ref_c.cc:647-698, the tables' T,
+// factorised one substitution at a time; terms of a zero step are skipped).
+// The powers that vary along a sum travel with the data: the input comes
+// scaled, dir 1 by a0^p (its x-like power), dir 0 by c2^lz (its z-like
+// power), and pass B scales m2 by b1^q, so that each term is a binomial,
+// one running power of the pass's step and one LDS read.
+// Buffers: a -> b -> a -> out, a barrier between passes; a is overwritten,
+// out may be b. Each pass visits its outputs ordered by the slot that
+// bounds its sum, so the lanes of a wave run loops of nearly equal length.
+__device__ inline void lattice_stepped(const StepTabs &st, int lp, int dir, double *a, double *b,
+                                       double *out, int tid, int nthr) {
+    if (dir == 1) {
+        stepped_pass<0>(st, lp, a, b, tid, nthr);
+        __syncthreads();
+        stepped_pass<1>(st, lp, b, a, tid, nthr);
+        __syncthreads();
+        stepped_pass<2>(st, lp, a, out, tid, nthr);
+    } else {
+        stepped_pass<5>(st, lp, a, b, tid, nthr);
+        __syncthreads();
+        stepped_pass<4>(st, lp, b, a, tid, nthr);
+        __syncthreads();
+        stepped_pass<3>(st, lp, a, out, tid, nthr);
     }
 }
 

@@ -42,11 +42,12 @@ struct PairGeom {
     double R2 = 0.0; // general cells: radius^2
 };
 
-// lattice transforms of a general cell as CSR (see lattice_csr)
+// lattice transforms of a general cell in the padded row layout of
+// LatticeTab (see lattice_tables)
 struct LatticeHost {
-    std::vector<int> rp, col;
+    std::vector<int> col;
     std::vector<double> val;
-    std::vector<long long> rpoff;
+    std::vector<LatBlk> blk;
 };
 
 void profile_from_radius(double radius, const double dr[3], Profile &P, int lb[3], int ub[3]);
@@ -63,8 +64,10 @@ gpw_status make_subs(const gpw_grid_desc &g, const PairDev &pd, int pair_index, 
 void multinomial_matrix(const double dh[3][3], int lp, double *T);
 // lattice transforms of a general cell: all degrees lp <= lpm, or only the
 // listed ones
-gpw_status lattice_csr(const double dh[3][3], int lpm, LatticeHost &out);
-gpw_status lattice_csr_some(const double dh[3][3], const int *lps, int n, LatticeHost &out);
+// triangular steps (LatticeTab::stepped): h = {a0, b0, c0, b1, c1, c2}
+bool stepped_cell(const double dh[3][3], double h[6]);
+gpw_status lattice_tables(const double dh[3][3], int lpm, LatticeHost &out);
+gpw_status lattice_tables_some(const double dh[3][3], const int *lps, int n, LatticeHost &out);
 
 // tile-kernel launchers per cell kind (GEN: general cell), defined in
 // gpw_tiles_colloc.hpp / gpw_tiles_integ.hpp and instantiated in the
@@ -99,7 +102,7 @@ hipError_t launch_integrate_tiles_84(const TileGrid &tg, const EntryRec *recs8, 
 int block_shell_entries(int la_min, int la_max, int lb_min, int lb_max);
 hipError_t launch_coef(const PairDev *pairs, const int *order, int pair0, int npairs, int tau,
                        int lam_max, int lbm_max, int nab, const double *pab, double *coef,
-                       const LatticeCSR &lt, double *ws,
+                       const LatticeTab &lt, double *ws,
                        long long ws_stride, int ws_blocks, hipStream_t st);
 size_t coef_smem_bytes(int lam, int lbm, bool lds_blocks);
 size_t coef_ws_doubles(int lam, int lbm);
@@ -108,7 +111,7 @@ bool coef_needs_workspace(int lam, int lbm);
 hipError_t launch_hab(const PairDev *pairs, const int *order, const int *segs, int seg0, int nseg,
                       int tau, int with_forces, int la_max, int lb_max, int nhx, const double *Mbuf,
                       int mstride, const double *pab, double *hab, double *forces,
-                      const LatticeCSR &lt, double *ws,
+                      const LatticeTab &lt, double *ws,
                       long long ws_stride, int ws_blocks, hipStream_t st);
 size_t hab_ws_need(int la_max, int lb_max, int tau, int nhx);
 hipError_t launch_moments_small(int cls, const TileGrid &tg, const SubDev *subs, const int *list,

@@ -85,16 +85,30 @@ __host__ __device__ constexpr int c2idx(int lp, int lyp, int lxp) {
     return lyp * (lp + 1) - (lyp * (lyp - 1)) / 2 + lxp;
 }
 
-// Lattice transforms of a general cell in CSR form, for every lp <= LPMAX and
-// both directions (dir 0: coef_ijk = T coef_xyz, dir 1: M_xyz = T^T M_ijk).
-// Rows of (lp, dir) start at rp + rp_off[2 * lp + dir]; their entries are
-// absolute indices into col / val. T is block-diagonal by degree and, for
-// cells with axis-aligned steps, sparse within the blocks.
-struct LatticeCSR {
-    const int *rp = nullptr;
+// Lattice transforms of a general cell for every lp <= LPMAX and both
+// directions (dir 0: coef_ijk = T coef_xyz, dir 1: M_xyz = T^T M_ijk). T is
+// block-diagonal by degree and, for cells with axis-aligned steps, sparse
+// within the blocks. Each (lp, dir) is a block of n = ncoset(lp) rows padded
+// to a common width: entry k of row r is col / val[base + k * n + r], the
+// non-zeros of a row in ascending column order, then (column 0, value 0)
+// fill. Consecutive rows are consecutive in memory, so the lanes of a wave
+// that own consecutive rows load each entry together, and a row's entries
+// are independent loads.
+struct LatBlk {
+    long long base = 0;
+    int width = 0, n = 0;
+};
+// Cells whose steps are triangular, dh[0] = (a0, 0, 0), dh[1] = (b0, b1, 0),
+// dh[2] = (c0, c1, c2) (a along x, b in the xy plane: the usual cell
+// orientation), need no tables: the transform runs in two passes from
+// h = {a0, b0, c0, b1, c1, c2} (lattice_stepped).
+struct LatticeTab {
     const int *col = nullptr;
     const double *val = nullptr;
-    const long long *rp_off = nullptr;
+    const LatBlk *blk = nullptr;  // [2 * lp + dir]; null: stepped or orthorhombic cell
+    int stepped = 0;
+    double h[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    __host__ __device__ bool general() const { return blk != nullptr || stepped != 0; }
 };
 
 // Per-pair parameters on the device (one per primitive pair).

@@ -32,8 +32,9 @@ namespace gpw {
 //   coef[(lz,ly,lx)]    = prefactor sum_{az,bz} Ty[az][bz][(ly,lx)] alpha_z(bz,az,lz)
 // with (ax, bx) over the shells of the (tau-shifted) block; Ty holds only
 // ly + lx <= (lam - az) + (lbm - bz). Dynamic LDS: alpha, eff, Ty (+ cxyz).
-// lt.rp set (general cells): coef_ijk = T coef_xyz, T the multinomial expansion
-// of grid_base_ref_c.cc:647-698 for each lp.
+// General cells (lt.general()): coef_ijk = T coef_xyz, T the multinomial
+// expansion of grid_base_ref_c.cc:647-698 for each lp (tables, or the two
+// passes of lattice_stepped).
 // Workgroups loop over pairs (pair0 + blockIdx.x, stride gridDim.x); eff, Ty
 // and cxyz live in LDS after alpha, or, when gws is given (blocks too large
 // for LDS), in the workgroup's slice gws + blockIdx.x * ws_stride.
@@ -41,10 +42,14 @@ __global__ __launch_bounds__(TR_NT) void k_coef(const PairDev *__restrict__ pair
                                              const int *__restrict__ order, int pair0,
                                              int npairs, int tau, const double *__restrict__ pab,
                                              double *__restrict__ coef,
-                                             LatticeCSR lt,
+                                             LatticeTab lt,
                                              double *__restrict__ gws, long long ws_stride) {
-    extern __shared__ double sm_coef[];
+    extern __shared__ double sm_dyn[];
     __shared__ int dec2[NC2MAX];
+    // stepped cells: the transform's tables first in the dynamic LDS
+    StepTabs &stt = *reinterpret_cast<StepTabs *>(sm_dyn);
+    double *sm_coef = sm_dyn + (lt.stepped ? STEP_TABS_DOUBLES : 0);
+    if (lt.stepped) fill_step_tabs(lt, stt, threadIdx.x, blockDim.x);
     for (int ip = pair0 + blockIdx.x; ip < pair0 + npairs; ip += gridDim.x) {
     __syncthreads();  // LDS / workspace reuse across pairs
     const int pi = order ? order[ip] : ip;
@@ -84,7 +89,7 @@ __global__ __launch_bounds__(TR_NT) void k_coef(const PairDev *__restrict__ pair
         Ty[w] = acc;
     }
     __syncthreads();
-    double *dst = lt.rp ? cxyz : coef + p.coef_off;
+    double *dst = lt.general() ? cxyz : coef + p.coef_off;
     for (int c = threadIdx.x; c < n3; c += blockDim.x) {
         int lz, ly, lx;
         decode_c3(lp, c, lz, ly, lx);
@@ -95,11 +100,16 @@ __global__ __launch_bounds__(TR_NT) void k_coef(const PairDev *__restrict__ pair
                 if (ly + lx > (lam - az) + (lbm - bz)) break;
                 acc += Ty[(az * (lbm + 1) + bz) * n2 + c2] * A.at(2, bz, az, lz);
             }
-        dst[c] = p.prefactor * acc;
+        // stepped cells: scaled by c2^lz for the transform
+        dst[c] = p.prefactor * acc * (lt.stepped ? stt.pw[5][lz] : 1.0);
     }
-    if (!lt.rp) continue;
+    if (!lt.general()) continue;
     __syncthreads();
-    lattice_transform(lt, lp, 0, cxyz, coef + p.coef_off, threadIdx.x, blockDim.x);
+    // Ty (no longer read) is the stepped transform's scratch
+    if (lt.stepped)
+        lattice_stepped(stt, lp, 0, cxyz, Ty, coef + p.coef_off, threadIdx.x, blockDim.x);
+    else
+        lattice_transform(lt, lp, 0, cxyz, coef + p.coef_off, threadIdx.x, blockDim.x);
     }
 }
 
@@ -126,6 +136,18 @@ __device__ __forceinline__ void shell_range(int lmin, int lmax, int &lo, int &n)
 __host__ __device__ constexpr int coefw_doubles(int la, int lb, int nab) {
     return alpha_doubles(la, lb) + nab + (la + 1) * (lb + 1) * (la + lb + 1) + ncoset(la + lb);
 }
+// the part before the xyz coefficients (alpha, the smallest block of single
+// shells, the slice) holds ncoset(la + lb) doubles for all la, lb <= LMAX:
+// the scratch of the stepped lattice transform
+constexpr bool coefw_scratch_fits() {
+    for (int la = 0; la <= LMAX; la++)
+        for (int lb = 0; lb <= LMAX; lb++)
+            if (alpha_doubles(la, lb) + (la + 1) * (la + 2) / 2 * ((lb + 1) * (lb + 2) / 2) +
+                    (la + 1) * (lb + 1) * (la + lb + 1) < ncoset(la + lb))
+                return false;
+    return true;
+}
+static_assert(coefw_scratch_fits(), "k_coef_wave scratch of the stepped lattice transform");
 
 // cab -> cxyz of one pair per wave: for each slice (az, bz)
 //   S1[by][ay][lx] = sum_{ax, bx} pab[b][a] alpha_x(bx, ax, lx)
@@ -138,10 +160,14 @@ template <int LPM>
 __global__ __launch_bounds__(TW) void k_coef_wave(const PairDev *__restrict__ pairs,
                                                   const int *__restrict__ order, int pair0, int npairs,
                                                   const double *__restrict__ pab, double *__restrict__ coef,
-                                                  LatticeCSR lt) {
+                                                  LatticeTab lt) {
     constexpr int NCOL = (ncoef2(LPM - 1) + TW - 1) / TW;
-    extern __shared__ double smw[];
+    extern __shared__ double sm_dyn[];
+    // stepped cells: the transform's tables first in the dynamic LDS
+    StepTabs &stt = *reinterpret_cast<StepTabs *>(sm_dyn);
+    double *smw = sm_dyn + (lt.stepped ? STEP_TABS_DOUBLES : 0);
     const int lane = threadIdx.x;
+    if (lt.stepped) fill_step_tabs(lt, stt, lane, TW);
     for (int ip = pair0 + blockIdx.x; ip < pair0 + npairs; ip += gridDim.x) {
         __syncthreads();  // LDS reuse across pairs
         const int pi = order ? order[ip] : ip;
@@ -202,17 +228,25 @@ __global__ __launch_bounds__(TW) void k_coef_wave(const PairDev *__restrict__ pa
                 __syncthreads();
             }
         }
-        double *dst = lt.rp ? cx : coef + p.coef_off;
+        double *dst = lt.general() ? cx : coef + p.coef_off;
 #pragma unroll
         for (int s = 0; s < NCOL; s++) {
             if (lane + TW * s >= n2) continue;
 #pragma unroll
             for (int lz = 0; lz < LPM; lz++)
-                if (lz <= lp - ly[s] - lx[s]) dst[cidx(lp, lz, ly[s], lx[s])] = p.prefactor * c[s][lz];
+                if (lz <= lp - ly[s] - lx[s])
+                    dst[cidx(lp, lz, ly[s], lx[s])] =
+                        p.prefactor * c[s][lz] * (lt.stepped ? stt.pw[5][lz] : 1.0);  // see k_coef
         }
-        if (lt.rp) {
+        if (lt.general()) {
             __syncthreads();
-            lattice_transform(lt, lp, 0, cx, coef + p.coef_off, lane, TW);
+            // stepped: the alpha table, block and slice (no longer read)
+            // are the scratch; they hold ncoset(lp) doubles for every
+            // la, lb <= LMAX (coefw_scratch_fits)
+            if (lt.stepped)
+                lattice_stepped(stt, lp, 0, cx, smw, coef + p.coef_off, lane, TW);
+            else
+                lattice_transform(lt, lp, 0, cx, coef + p.coef_off, lane, TW);
         }
     }
 }
@@ -310,20 +344,27 @@ __global__ __launch_bounds__(SUM_NT) void k_sum_entries(const PairDev *__restric
 // segment's pairs in order, so the += into the shared block is race-free and
 // deterministic. segs NULL: one pair per segment, positions seg0 .. seg0 + nseg.
 // Registers capped for eight waves per SIMD (more pairs in flight; measured
-// faster than the uncapped 75 VGPRs).
+// faster than the uncapped 75 VGPRs). LK: the cell's lattice transform (0
+// none: orthorhombic; 1 tables; 2 stepped), one instance each so that the
+// orthorhombic kernel carries no transform code.
+template <int LK>
 __global__ __launch_bounds__(TR_NT, 8) void k_hab(const PairDev *__restrict__ pairs,
                                             const int *__restrict__ order,
                                             const int *__restrict__ segs, int seg0, int nseg,
                                             int tau, int with_forces, const double *__restrict__ Mbuf,
                                             int mstride, const double *__restrict__ pab,
                                             double *__restrict__ hab, double *__restrict__ forces,
-                                            LatticeCSR lt,
+                                            LatticeTab lt,
                                             double *__restrict__ gws, long long ws_stride) {
-    extern __shared__ double sm_hab[];
+    extern __shared__ double sm_dyn[];
     __shared__ int dec[NCOSET_LMAX];
     __shared__ int dec2[NC2MAX];
     __shared__ int uoff[(LMAX + 1) * (LMAX + 1)];
+    // stepped cells: the transform's tables first in the dynamic LDS
+    StepTabs &stt = *reinterpret_cast<StepTabs *>(sm_dyn);
+    double *sm_hab = sm_dyn + ((LK == 2) ? STEP_TABS_DOUBLES : 0);
     fill_decode(dec, threadIdx.x, blockDim.x);  // first read after the barrier below
+    if ((LK == 2)) fill_step_tabs(lt, stt, threadIdx.x, blockDim.x);
     const int fx = with_forces ? 1 : 0;
     const bool cross = with_forces && !tau;
     for (int sg = seg0 + blockIdx.x; sg < seg0 + nseg; sg += gridDim.x) {
@@ -367,13 +408,26 @@ __global__ __launch_bounds__(TR_NT, 8) void k_hab(const PairDev *__restrict__ pa
         uoff[zz] = (c4(lpx) - c4(lpx - az)) - (c4(lamx - 1) - c4(lamx - 1 - az)) +
                    (ncoset(ma + lbmx) - ncoset(ma + lbmx - bz));
     }
-    if (!lt.rp) {
+    if (!(LK != 0)) {
         for (int c = threadIdx.x; c < n3x; c += blockDim.x) M[c] = Mbuf[(long long)pi * mstride + c];
     } else {
-        // M_xyz = T^T M_ijk (adjoint of the lattice transform)
-        for (int c = threadIdx.x; c < n3x; c += blockDim.x) Mijk[c] = Mbuf[(long long)pi * mstride + c];
+        // M_xyz = T^T M_ijk (adjoint of the lattice transform); stepped
+        // cells: scaled by a0^i for the transform
+        for (int c = threadIdx.x; c < n3x; c += blockDim.x) {
+            double v = Mbuf[(long long)pi * mstride + c];
+            if ((LK == 2)) {
+                int iz, iy, ix;
+                decode_c3(lpx, c, iz, iy, ix);
+                v *= stt.pw[0][ix];
+            }
+            Mijk[c] = v;
+        }
         __syncthreads();
-        lattice_transform(lt, lpx, 1, Mijk, M, threadIdx.x, blockDim.x);
+        if ((LK == 2)) {
+            lattice_stepped(stt, lpx, 1, Mijk, M, M, threadIdx.x, blockDim.x);
+        } else {
+            lattice_transform(lt, lpx, 1, Mijk, M, threadIdx.x, blockDim.x);
+        }
     }
     __syncthreads();
     // adjoint of the factorised cab -> cxyz transform (see k_coef):
@@ -1526,12 +1580,13 @@ int block_shell_entries(int la_min, int la_max, int lb_min, int lb_max) {
 // la_max / lb_max are at most la_max / lb_max (LDS sizing)
 hipError_t launch_coef(const PairDev *pairs, const int *order, int pair0, int npairs, int tau,
                        int lam_max, int lbm_max, int nab, const double *pab, double *coef,
-                       const LatticeCSR &lt, double *ws,
+                       const LatticeTab &lt, double *ws,
                        long long ws_stride, int ws_blocks, hipStream_t st) {
     if (npairs <= 0) return hipSuccess;
-    if (!tau && sizeof(double) * coefw_doubles(lam_max, lbm_max, nab) <= WAVE_LDS_MAX) {
+    const size_t xs = lt.stepped ? sizeof(StepTabs) : 0;  // stepped transform's tables
+    if (!tau && sizeof(double) * coefw_doubles(lam_max, lbm_max, nab) + xs <= WAVE_LDS_MAX) {
         // one pair per wave
-        const size_t smem = sizeof(double) * coefw_doubles(lam_max, lbm_max, nab);
+        const size_t smem = sizeof(double) * coefw_doubles(lam_max, lbm_max, nab) + xs;
         switch (class_of(lam_max + lbm_max)) {
         case 0: hipLaunchKernelGGL(k_coef_wave<4>, dim3(npairs), dim3(TW), smem, st, pairs, order, pair0, npairs, pab, coef, lt); break;
         case 1: hipLaunchKernelGGL(k_coef_wave<8>, dim3(npairs), dim3(TW), smem, st, pairs, order, pair0, npairs, pab, coef, lt); break;
@@ -1544,7 +1599,7 @@ hipError_t launch_coef(const PairDev *pairs, const int *order, int pair0, int np
     const bool big = coef_needs_workspace(lam, lbm);
     if (big && (!ws || ws_stride < (long long)coef_ws_doubles(lam, lbm) || ws_blocks < 1))
         return hipErrorInvalidValue;
-    const size_t smem = coef_smem_bytes(lam, lbm, !big);
+    const size_t smem = coef_smem_bytes(lam, lbm, !big) + xs;
     const int nblk = big ? std::min(npairs, ws_blocks) : npairs;
     hipLaunchKernelGGL(k_coef, dim3(nblk), dim3(transform_threads(lam + lbm)), smem, st, pairs, order, pair0, npairs, tau, pab,
                        coef, lt, big ? ws : nullptr, big ? ws_stride : 0LL);
@@ -1594,7 +1649,7 @@ size_t hab_ws_need(int la_max, int lb_max, int tau, int nhx) {
 hipError_t launch_hab(const PairDev *pairs, const int *order, const int *segs, int seg0, int nseg,
                       int tau, int with_forces, int la_max, int lb_max, int nhx, const double *Mbuf,
                       int mstride, const double *pab, double *hab, double *forces,
-                      const LatticeCSR &lt, double *ws,
+                      const LatticeTab &lt, double *ws,
                       long long ws_stride, int ws_blocks, hipStream_t st) {
     if (nseg <= 0) return hipSuccess;
     const int npairs = nseg;
@@ -1605,9 +1660,11 @@ hipError_t launch_hab(const PairDev *pairs, const int *order, const int *segs, i
     const bool big = hab_needs_workspace(lamx, lbmx, lam, lbm, tau, h);
     if (big && (!ws || ws_stride < (long long)hab_ws_doubles(lamx, lbmx, lam, lbm, tau, h) || ws_blocks < 1))
         return hipErrorInvalidValue;
-    const size_t smem = hab_smem_bytes(lamx, lbmx, lam, lbm, tau, h, !big);
+    const size_t smem = hab_smem_bytes(lamx, lbmx, lam, lbm, tau, h, !big) +
+                        (lt.stepped ? sizeof(StepTabs) : 0);  // stepped transform's tables
     const int nblk = big ? std::min(npairs, ws_blocks) : npairs;
-    hipLaunchKernelGGL(k_hab, dim3(nblk), dim3(transform_threads(lamx + lbmx)), smem, st, pairs, order, segs, seg0, nseg, tau,
+    auto kern = lt.stepped ? k_hab<2> : lt.blk ? k_hab<1> : k_hab<0>;
+    hipLaunchKernelGGL(kern, dim3(nblk), dim3(transform_threads(lamx + lbmx)), smem, st, pairs, order, segs, seg0, nseg, tau,
                        with_forces, Mbuf, mstride, pab, hab, forces, lt,
                        big ? ws : nullptr, big ? ws_stride : 0LL);
     return hipGetLastError();

@@ -212,10 +212,17 @@ struct gpw_tasklist {
     double *d_Mbuf = nullptr;
     int mstride = 0;
     // general cells: lattice transforms (CSR) per lp and direction
-    int *d_mt_rp = nullptr, *d_mt_col = nullptr;
+    int *d_mt_col = nullptr;
     double *d_mt_val = nullptr;
-    long long *d_mt_rpoff = nullptr;
-    LatticeCSR lat() const { return LatticeCSR{d_mt_rp, d_mt_col, d_mt_val, d_mt_rpoff}; }
+    LatBlk *d_mt_blk = nullptr;
+    int lat_stepped = 0;
+    double lat_h[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    LatticeTab lat() const {
+        LatticeTab t{d_mt_col, d_mt_val, d_mt_blk};
+        t.stepped = lat_stepped;
+        for (int i = 0; i < 6; i++) t.h[i] = lat_h[i];
+        return t;
+    }
     long long coef_total = 0;        // doubles of d_coef
     // pairs grouped by block size for the per-pair transform kernels (one
     // launch per group, LDS sized for the group): d_order[start .. start+count)
@@ -248,8 +255,8 @@ struct gpw_tasklist {
                          (void **)&d_small, (void **)&d_small_mask, (void **)&d_csmall, (void **)&d_box_off,
                          (void **)&d_sval, (void **)&d_recs, (void **)&d_pe_start,
                          (void **)&d_ent_off0, (void **)&d_ent_off2, (void **)&d_coef, (void **)&d_partial,
-                         (void **)&d_Mbuf, (void **)&d_mt_rp, (void **)&d_mt_col, (void **)&d_mt_val,
-                         (void **)&d_mt_rpoff, (void **)&d_hab_ws, (void **)&d_order, (void **)&d_segs};
+                         (void **)&d_Mbuf, (void **)&d_mt_col, (void **)&d_mt_val,
+                         (void **)&d_mt_blk, (void **)&d_hab_ws, (void **)&d_order, (void **)&d_segs};
         for (void **p : ptrs) {
             if (*p) (void)hipFree(*p);
             *p = nullptr;
@@ -562,14 +569,13 @@ void gpw::multinomial_matrix(const double dh[3][3], int lp, double *T) {
             }
 }
 
-// Lattice transforms T(lp) of a general cell for lp = 0 .. lpm as CSR, in
-// both directions (dir 0: T, dir 1: T^T), exact zeros dropped; row pointers
-// of (lp, dir) start at rpoff[2 * lp + dir].
-static gpw_status lattice_csr_of(const double dh[3][3], const std::vector<int> &lps, LatticeHost &out) {
-    out.rp.clear();
+// Lattice transforms T(lp) of a general cell, in both directions (dir 0: T,
+// dir 1: T^T), exact zeros dropped, each (lp, dir) a block of rows padded to
+// its longest row and stored entry-major (LatticeTab).
+static gpw_status lattice_tables_of(const double dh[3][3], const std::vector<int> &lps, LatticeHost &out) {
     out.col.clear();
     out.val.clear();
-    out.rpoff.assign(2 * (LPMAX + 1), 0);
+    out.blk.assign(2 * (LPMAX + 1), LatBlk{});
     std::vector<double> T;
     for (int lp : lps) {
         const int n = ncoset(lp);
@@ -577,37 +583,59 @@ static gpw_status lattice_csr_of(const double dh[3][3], const std::vector<int> &
         multinomial_matrix(dh, lp, T.data());
         const double *Tl = T.data();
         for (int dir = 0; dir < 2; dir++) {
-            out.rpoff[2 * lp + dir] = (long long)out.rp.size();
+            auto at = [&](int r, int c) { return dir == 0 ? Tl[(size_t)r * n + c] : Tl[(size_t)c * n + r]; };
+            int width = 0;
             for (int r = 0; r < n; r++) {
-                out.rp.push_back((int)out.val.size());
+                int nz = 0;
+                for (int c = 0; c < n; c++) nz += at(r, c) != 0.0;
+                width = std::max(width, nz);
+            }
+            LatBlk &b = out.blk[2 * lp + dir];
+            b.base = (long long)out.val.size();
+            b.width = width;
+            b.n = n;
+            out.col.resize(out.col.size() + (size_t)width * n, 0);
+            out.val.resize(out.val.size() + (size_t)width * n, 0.0);
+            for (int r = 0; r < n; r++) {
+                int k = 0;
                 for (int c = 0; c < n; c++) {
-                    const double v = dir == 0 ? Tl[(size_t)r * n + c] : Tl[(size_t)c * n + r];
-                    if (v != 0.0) {
-                        out.col.push_back(c);
-                        out.val.push_back(v);
-                    }
+                    const double v = at(r, c);
+                    if (v == 0.0) continue;
+                    out.col[b.base + (size_t)k * n + r] = c;
+                    out.val[b.base + (size_t)k * n + r] = v;
+                    k++;
                 }
             }
-            out.rp.push_back((int)out.val.size());
         }
     }
     if (out.val.size() > 0x7fffffffULL) return fail(GPW_ERR_ARG, "lattice transform too large");
     return GPW_OK;
 }
 
-gpw_status gpw::lattice_csr(const double dh[3][3], int lpm, LatticeHost &out) {
+bool gpw::stepped_cell(const double dh[3][3], double h[6]) {
+    if (dh[0][1] != 0.0 || dh[0][2] != 0.0 || dh[1][2] != 0.0) return false;
+    h[0] = dh[0][0];
+    h[1] = dh[1][0];
+    h[2] = dh[2][0];
+    h[3] = dh[1][1];
+    h[4] = dh[2][1];
+    h[5] = dh[2][2];
+    return true;
+}
+
+gpw_status gpw::lattice_tables(const double dh[3][3], int lpm, LatticeHost &out) {
     std::vector<int> lps;
     for (int lp = 0; lp <= lpm; lp++) lps.push_back(lp);
-    return lattice_csr_of(dh, lps, out);
+    return lattice_tables_of(dh, lps, out);
 }
 
 // only the degrees lps[0 .. n) (rows of other degrees are absent)
-gpw_status gpw::lattice_csr_some(const double dh[3][3], const int *lps, int n, LatticeHost &out) {
+gpw_status gpw::lattice_tables_some(const double dh[3][3], const int *lps, int n, LatticeHost &out) {
     std::vector<int> v;
     for (int i = 0; i < n; i++)
         if (lps[i] >= 0 && lps[i] <= LPMAX && std::find(v.begin(), v.end(), lps[i]) == v.end())
             v.push_back(lps[i]);
-    return lattice_csr_of(dh, v, out);
+    return lattice_tables_of(dh, v, out);
 }
 
 // Host part of a task list: profiles, pairs, sub-boxes, tile work lists.
@@ -1166,13 +1194,14 @@ static gpw_status upload_tasklist(gpw_tasklist *tl, HostPlan &hp, bool device_it
             HIP_TRY(hipMalloc((void **)&tl->d_hab_ws, sizeof(double) * stride * tl->hab_ws_blocks));
         }
     }
-    if (tl->tg.general) {
+    if (tl->tg.general && stepped_cell(tl->grid.dh, tl->lat_h)) {
+        tl->lat_stepped = 1;
+    } else if (tl->tg.general) {
         LatticeHost lh;
-        if ((st = lattice_csr(tl->grid.dh, std::min(tl->lp_max + 2, LPMAX), lh))) return st;
-        if ((st = upload(&tl->d_mt_rp, lh.rp))) return st;
+        if ((st = lattice_tables(tl->grid.dh, std::min(tl->lp_max + 2, LPMAX), lh))) return st;
         if ((st = upload(&tl->d_mt_col, lh.col))) return st;
         if ((st = upload(&tl->d_mt_val, lh.val))) return st;
-        if ((st = upload(&tl->d_mt_rpoff, lh.rpoff))) return st;
+        if ((st = upload(&tl->d_mt_blk, lh.blk))) return st;
     }
     return GPW_OK;
 }

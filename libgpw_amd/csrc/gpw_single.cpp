@@ -117,13 +117,15 @@ struct Plan {
     long long nbox = 0;           // doubles of all boxes
     long long ngrid_pts = 0;
     TileGrid tg{};
-    LatticeHost lat;
+    LatticeHost lat;  // tables (general cells that are not stepped)
+    int lat_stepped = 0;
+    double lat_h[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     int general = 0, lp = 0, mstride = 0;
     long long ws_doubles = 0;
     Mode mode = DEV;
     // byte offsets in the device region; [0, head) is uploaded from the host
-    size_t o_pair = 0, o_subs = 0, o_bsubs = 0, o_prof = 0, o_lrp = 0, o_lcol = 0, o_lval = 0,
-           o_lrpoff = 0, o_pab = 0, head = 0;
+    size_t o_pair = 0, o_subs = 0, o_bsubs = 0, o_prof = 0, o_lcol = 0, o_lval = 0, o_lblk = 0,
+           o_pab = 0, head = 0;
     size_t o_grid = 0, o_coef = 0, o_mbuf = 0, o_ws = 0, o_hab = 0, o_forces = 0, total = 0;
 };
 
@@ -180,11 +182,13 @@ gpw_status make_plan(const Task &t, int nspheres, const int *sphere_bounds, cons
         P.nbox += v;
     }
     P.mode = !host_arrays ? DEV : P.nbox < P.ngrid_pts ? HOST_WINDOW : HOST_FULL;
-    if (P.general) {
+    if (P.general && stepped_cell(g.dh, P.lat_h)) {
+        P.lat_stepped = 1;
+    } else if (P.general) {
         // lattice transforms of the two degrees the kernels read: lp (k_coef)
         // and lp + 2 (k_hab with the force extension)
         const int lps[2] = {P.lp, std::min(P.lp + 2, LPMAX)};
-        if ((st = lattice_csr_some(g.dh, lps, 2, P.lat))) return st;
+        if ((st = lattice_tables_some(g.dh, lps, 2, P.lat))) return st;
     }
     P.mstride = ncoset(P.lp + 2);
     const int lam = t.p.la_max + t.tau, lbm = t.p.lb_max + t.tau;
@@ -215,10 +219,9 @@ gpw_status make_plan(const Task &t, int nspheres, const int *sphere_bounds, cons
     P.o_subs = put(P.subs.size() * sizeof(SubDev));
     P.o_bsubs = put(P.mode == HOST_WINDOW ? P.bsubs.size() * sizeof(SubDev) : 0);
     P.o_prof = put(P.prof.T.size() * sizeof(int));
-    P.o_lrp = put(P.lat.rp.size() * sizeof(int));
     P.o_lcol = put(P.lat.col.size() * sizeof(int));
     P.o_lval = put(P.lat.val.size() * sizeof(double));
-    P.o_lrpoff = put(P.lat.rpoff.size() * sizeof(long long));
+    P.o_lblk = put(P.lat.blk.size() * sizeof(LatBlk));
     P.o_pab = put(P.mode == DEV ? 0 : (size_t)na * nb * sizeof(double));
     P.head = o;
     const long long gpts = P.mode == HOST_WINDOW ? P.nbox : P.mode == HOST_FULL ? P.ngrid_pts : 0;
@@ -241,13 +244,12 @@ void stage_head(const Plan &P, const double *pab_host, std::vector<char> &stage)
     if (P.mode == HOST_WINDOW)
         std::memcpy(stage.data() + P.o_bsubs, P.bsubs.data(), P.bsubs.size() * sizeof(SubDev));
     if (!P.prof.T.empty()) std::memcpy(stage.data() + P.o_prof, P.prof.T.data(), P.prof.T.size() * sizeof(int));
-    if (P.general) {
-        std::memcpy(stage.data() + P.o_lrp, P.lat.rp.data(), P.lat.rp.size() * sizeof(int));
+    if (P.general && !P.lat_stepped) {
         if (!P.lat.col.empty()) {
             std::memcpy(stage.data() + P.o_lcol, P.lat.col.data(), P.lat.col.size() * sizeof(int));
             std::memcpy(stage.data() + P.o_lval, P.lat.val.data(), P.lat.val.size() * sizeof(double));
         }
-        std::memcpy(stage.data() + P.o_lrpoff, P.lat.rpoff.data(), P.lat.rpoff.size() * sizeof(long long));
+        std::memcpy(stage.data() + P.o_lblk, P.lat.blk.data(), P.lat.blk.size() * sizeof(LatBlk));
     }
     if (P.mode != DEV) {
         // packed block ncoset(lb_max) x ncoset(la_max) of the caller's pab
@@ -285,11 +287,18 @@ void for_box_rows(const Plan &P, F &&f) {
     }
 }
 
-LatticeCSR lattice_view(const Plan &P, char *R) {
-    if (!P.general) return LatticeCSR{};
-    return LatticeCSR{reinterpret_cast<int *>(R + P.o_lrp), reinterpret_cast<int *>(R + P.o_lcol),
-                      reinterpret_cast<double *>(R + P.o_lval),
-                      reinterpret_cast<long long *>(R + P.o_lrpoff)};
+LatticeTab lattice_view(const Plan &P, char *R) {
+    LatticeTab t{};
+    if (!P.general) return t;
+    if (P.lat_stepped) {
+        t.stepped = 1;
+        for (int i = 0; i < 6; i++) t.h[i] = P.lat_h[i];
+        return t;
+    }
+    t.col = reinterpret_cast<int *>(R + P.o_lcol);
+    t.val = reinterpret_cast<double *>(R + P.o_lval);
+    t.blk = reinterpret_cast<LatBlk *>(R + P.o_lblk);
+    return t;
 }
 
 // the TileGrid a sub-box's launch indexes: the grid, or its compact box
