@@ -41,13 +41,16 @@ struct AlphaView {
     double *a;
     int lam, lbm;
     __device__ AlphaView(double *a_, int lam_, int lbm_) : a(a_), lam(lam_), lbm(lbm_) {}
-    __device__ double &at(int d, int lb, int la, int k) const {
-        // rows before (lb, la): sum over lb' < lb, la' <= lam of (la' + lb' + 1),
-        // then over la' < la of (la' + lb + 1)
-        const int off = lb * ((lam + 1) * lam / 2) + (lam + 1) * (lb * (lb + 1) / 2) +
-                        la * (la - 1) / 2 + la * (lb + 1);
-        return a[d * (alpha_doubles(lam, lbm) / 3) + off + k];
+    // offset of row (lb, la) within a table: sum over lb' < lb, la' <= lam of
+    // (la' + lb' + 1), then over la' < la of (la' + lb + 1)
+    __device__ int rowoff(int lb, int la) const {
+        return lb * ((lam + 1) * lam / 2) + (lam + 1) * (lb * (lb + 1) / 2) + la * (la - 1) / 2 +
+               la * (lb + 1);
     }
+    // rowoff(lb + 1, la) - rowoff(lb, la)
+    __device__ int rowstep(int lb, int la) const { return (lam + 1) * lam / 2 + (lam + 1) * (lb + 1) + la; }
+    __device__ double *table(int d) const { return a + d * (alpha_doubles(lam, lbm) / 3); }
+    __device__ double &at(int d, int lb, int la, int k) const { return table(d)[rowoff(lb, la) + k]; }
 };
 
 // binomial coefficients C(n, k), n < 20 (Pascal's triangle at compile time)
@@ -75,20 +78,26 @@ __device__ inline void fill_alpha(const PairDev *__restrict__ pp, AlphaView A, i
         const int la = w % (A.lam + 1);
         const int lb = (w / (A.lam + 1)) % (A.lbm + 1);
         const int d = w / ((A.lam + 1) * (A.lbm + 1));
-        for (int k = 0; k <= la + lb; k++) A.at(d, lb, la, k) = 0.0;
+        double *row = A.table(d) + A.rowoff(lb, la);
+        for (int k = 0; k <= la + lb; k++) row[k] = 0.0;
         const double pa = pp->rp[d] - pp->ra[d];
         const double pb = pp->rp[d] - pp->rb[d];
         double powa = 1.0;
+        // binomials by the exact integer recurrence C(n, i + 1) = C(n, i)
+        // (n - i) / (i + 1) in float (values < 2^24; the rounding of the
+        // approximate reciprocal is far below 1/2)
+        float ca = 1.0f;
         for (int i = 0; i <= la; i++) {
             // C(la, la - i) = C(la, i): the term (x - xp)^(la - i) (xp - xa)^i
-            const double ca = (double)kBinom.c[la][i];
             double powb = 1.0;
+            float cb = 1.0f;
             for (int j = 0; j <= lb; j++) {
-                const double cb = (double)kBinom.c[lb][j];
-                A.at(d, lb, la, (la - i) + (lb - j)) += ca * cb * powa * powb;
+                row[(la - i) + (lb - j)] += (double)ca * (double)cb * powa * powb;
                 powb *= pb;
+                cb = rintf(cb * (float)(lb - j) * __builtin_amdgcn_rcpf((float)(j + 1)));
             }
             powa *= pa;
+            ca = rintf(ca * (float)(la - i) * __builtin_amdgcn_rcpf((float)(i + 1)));
         }
     }
 }

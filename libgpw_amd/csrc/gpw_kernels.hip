@@ -194,6 +194,7 @@ __global__ __launch_bounds__(TW) void k_coef_wave(const PairDev *__restrict__ pa
             for (int k = 0; k < LPM; k++) c[s][k] = 0.0;
         }
         __syncthreads();
+        const double *A0 = A.table(0), *A1 = A.table(1);
         for (int az = 0; az <= la; az++) {
             for (int bz = 0; bz <= lb; bz++) {
                 const int ma = la - az, mb = lb - bz, X = ma + mb + 1;
@@ -208,22 +209,42 @@ __global__ __launch_bounds__(TW) void k_coef_wave(const PairDev *__restrict__ pa
                     double s = 0.0;
                     for (int ax = ax_lo; ax <= ax_hi; ax++) {
                         const double *col = P + coset(ax, ay, az) - na_lo;
-                        for (int bx = max(bx_lo, l - ax); bx <= bx_hi; bx++)
-                            s += col[(coset(bx, by, bz) - nb_lo) * nA] * A.at(0, bx, ax, l);
+                        const int bx0 = max(bx_lo, l - ax);
+                        if (bx0 > bx_hi) continue;
+                        // along bx: the block row and the alpha_x row step
+                        // by closed-form differences (coset(bx + 1, by, bz) -
+                        // coset(bx, by, bz) = (lb + 1)(lb + 2)/2, lb = bx + by + bz)
+                        int lbx = bx0 + by + bz;
+                        int jb = (coset(bx0, by, bz) - nb_lo) * nA;
+                        int ao = A.rowoff(bx0, ax) + l;
+                        for (int bx = bx0; bx <= bx_hi; bx++) {
+                            s += col[jb] * A0[ao];
+                            jb += ((lbx + 1) * (lbx + 2) / 2) * nA;
+                            lbx++;
+                            ao += A.rowstep(bx, ax);
+                        }
                     }
                     S1[w] = s;
                 }
                 __syncthreads();
+                const double *Az = A.table(2) + A.rowoff(bz, az);
 #pragma unroll
                 for (int s = 0; s < NCOL; s++) {
                     if (lane + TW * s >= n2 || ly[s] + lx[s] > ma + mb) continue;
                     double t = 0.0;
-                    for (int ay = 0; ay <= ma; ay++)
-                        for (int by = max(0, ly[s] - ay); by <= mb; by++)
-                            t += A.at(1, by, ay, ly[s]) * S1[(by * (ma + 1) + ay) * X + lx[s]];
+                    for (int ay = 0; ay <= ma; ay++) {
+                        const int by0 = max(0, ly[s] - ay);
+                        int ao = A.rowoff(by0, ay) + ly[s];
+                        int si = (by0 * (ma + 1) + ay) * X + lx[s];
+                        for (int by = by0; by <= mb; by++) {
+                            t += A1[ao] * S1[si];
+                            ao += A.rowstep(by, ay);
+                            si += (ma + 1) * X;
+                        }
+                    }
 #pragma unroll
                     for (int lz = 0; lz < LPM; lz++)
-                        if (lz <= az + bz) c[s][lz] += t * A.at(2, bz, az, lz);
+                        if (lz <= az + bz) c[s][lz] += t * Az[lz];
                 }
                 __syncthreads();
             }
@@ -433,21 +454,34 @@ __global__ __launch_bounds__(TR_NT, 8) void k_hab(const PairDev *__restrict__ pa
     // adjoint of the factorised cab -> cxyz transform (see k_coef):
     //   U[az][bz][(ly,lx)] = sum_lz M[(lz,ly,lx)] alpha_z(bz,az,lz)
     //   heff[b][a] = prefactor sum_{ly,lx} U[az][bz][(ly,lx)] alpha_y(by,ay,ly) alpha_x(bx,ax,lx)
+    const float rn2 = __builtin_amdgcn_rcpf((float)n2x), rnb = __builtin_amdgcn_rcpf((float)(lbmx + 1));
     for (int w = threadIdx.x; w < nzz * n2x; w += blockDim.x) {
-        const int zz = w / n2x, az = zz / (lbmx + 1), bz = zz % (lbmx + 1);
-        const int dc = dec2[w % n2x], ly = dc & 255, lx = dc >> 8;
+        // quotients by float reciprocals, corrected to the exact ones
+        int zz = (int)((float)w * rn2);
+        zz -= zz * n2x > w;
+        zz += (zz + 1) * n2x <= w;
+        int az = (int)((float)zz * rnb);
+        az -= az * (lbmx + 1) > zz;
+        az += (az + 1) * (lbmx + 1) <= zz;
+        const int bz = zz - az * (lbmx + 1);
+        const int dc = dec2[w - zz * n2x], ly = dc & 255, lx = dc >> 8;
         const int nl = (lamx - az) + (lbmx - bz);
         if (ly + lx > nl) continue;
-        // two accumulators: shorter dependency chains
+        // two accumulators: shorter dependency chains; the moment index
+        // steps by cidx(lpx, lz + 1, ly, lx) - cidx(lpx, lz, ly, lx) =
+        // (m + 1)(m + 2)/2 - ly, m = lpx - lz
         double acc = 0.0;
         const double *Az = &A.at(2, bz, az, 0);
         double acc2 = 0.0;
-        int lz = 0;
+        int lz = 0, at = cidx(lpx, 0, ly, lx);
         for (; lz + 1 <= az + bz; lz += 2) {
-            acc += M[cidx(lpx, lz, ly, lx)] * Az[lz];
-            acc2 += M[cidx(lpx, lz + 1, ly, lx)] * Az[lz + 1];
+            const int m = lpx - lz;
+            const int at1 = at + (m + 1) * (m + 2) / 2 - ly;
+            acc += M[at] * Az[lz];
+            acc2 += M[at1] * Az[lz + 1];
+            at = at1 + m * (m + 1) / 2 - ly;
         }
-        if (lz <= az + bz) acc += M[cidx(lpx, lz, ly, lx)] * Az[lz];
+        if (lz <= az + bz) acc += M[at] * Az[lz];
         acc += acc2;
         U[uoff[zz] + c2idx(nl, ly, lx)] = acc;
     }
