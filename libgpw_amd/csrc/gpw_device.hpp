@@ -102,6 +102,15 @@ __device__ inline void fill_alpha(const PairDev *__restrict__ pp, AlphaView A, i
     }
 }
 
+// q = w / d, r = w % d for 0 <= w < 2^16, 1 <= d < 2^12, with rd = an
+// approximate 1 / d: the float quotient is off by at most one, then corrected
+__device__ __forceinline__ void fdivmod(int w, int d, float rd, int &q, int &r) {
+    q = (int)((float)w * rd);
+    q -= q * d > w;
+    q += (q + 1) * d <= w;
+    r = w - q * d;
+}
+
 __device__ __forceinline__ void decode_coset(int l, int c, int &lx, int &ly, int &lz) {
     // inverse of coset() within shell l: c = ((l-lx)(l-lx+1))/2 + lz
     int m = 0;

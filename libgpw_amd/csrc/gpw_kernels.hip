@@ -496,14 +496,20 @@ __global__ __launch_bounds__(TR_NT, 8) void k_hab(const PairDev *__restrict__ pa
     // Forces without tau read a cross of the extended block: the block's own
     // b shells with every a shell, and its own a shells with every b shell
     // (not the corners a +- 1 with b +- 1), so only the cross is built.
+    const double *A0 = A.table(0), *A1 = A.table(1);
+    const float rAx = __builtin_amdgcn_rcpf((float)nAx), rAmid = __builtin_amdgcn_rcpf((float)nAmid);
     for (int w = threadIdx.x; w < nout; w += blockDim.x) {
         int ico, jco;
         if (w < n1) {
-            ico = na_lo + w % nAx;
-            jco = (cross ? nb_mid : nb_lo) + w / nAx;
+            int q, r;
+            fdivmod(w, nAx, rAx, q, r);
+            ico = na_lo + r;
+            jco = (cross ? nb_mid : nb_lo) + q;
         } else {
-            const int w2 = w - n1, r = w2 / nAmid, nlow = nb_mid - nb_lo;
-            ico = na_mid + w2 % nAmid;
+            const int nlow = nb_mid - nb_lo;
+            int r, rem;
+            fdivmod(w - n1, nAmid, rAmid, r, rem);
+            ico = na_mid + rem;
             jco = r < nlow ? nb_lo + r : nb_mid + nBmid + (r - nlow);
         }
         const int da = dec[ico], db = dec[jco];
@@ -511,10 +517,10 @@ __global__ __launch_bounds__(TR_NT, 8) void k_hab(const PairDev *__restrict__ pa
         const int bx = db & 15, by = (db >> 4) & 15, bz = db >> 8;
         const int nl = (lamx - az) + (lbmx - bz);
         const double *Uz = U + uoff[az * (lbmx + 1) + bz];
-        const double *Ax = &A.at(0, bx, ax, 0), *Ay = &A.at(1, by, ay, 0);
+        const double *Ax = A0 + A.rowoff(bx, ax), *Ay = A1 + A.rowoff(by, ay);
         double acc = 0.0;
-        for (int ly = 0; ly <= ay + by; ly++) {
-            const double *Ur = Uz + c2idx(nl, ly, 0);
+        const double *Ur = Uz;  // row ly of the slice's (ly, lx) triangle over nl
+        for (int ly = 0; ly <= ay + by; ly++, Ur += nl + 2 - ly) {
             double sx = 0.0;
             double sx2 = 0.0;
             int lx = 0;
@@ -533,8 +539,11 @@ __global__ __launch_bounds__(TR_NT, 8) void k_hab(const PairDev *__restrict__ pa
     const int na0 = ncoset(p.la_max), na_lo0 = ncoset(p.la_min - 1);
     const int nb0 = ncoset(p.lb_max), nb_lo0 = ncoset(p.lb_min - 1);
     const int nA = na0 - na_lo0, nB = nb0 - nb_lo0;
+    const float rA = __builtin_amdgcn_rcpf((float)nA);
     for (int w = threadIdx.x; w < nA * nB; w += blockDim.x) {
-        const int ico = na_lo0 + w % nA, jco = nb_lo0 + w / nA;
+        int q, r;
+        fdivmod(w, nA, rA, q, r);
+        const int ico = na_lo0 + r, jco = nb_lo0 + q;
         double acc;
         if (!tau) {
             acc = heff[hix(jco, ico)];
@@ -565,8 +574,10 @@ __global__ __launch_bounds__(TR_NT, 8) void k_hab(const PairDev *__restrict__ pa
     if (tau) build_eff(p, tau, pab, eff, nla, nlb, nla, threadIdx.x, blockDim.x);
     double f[6] = {0, 0, 0, 0, 0, 0};
     const int la_lo = ncoset(max(p.la_min - tau, 0) - 1), lb_lo = ncoset(max(p.lb_min - tau, 0) - 1);
+    const float rla = __builtin_amdgcn_rcpf((float)nla);
     for (int w = threadIdx.x; w < nla * nlb; w += blockDim.x) {
-        const int ico = w % nla, jco = w / nla;
+        int jco, ico;
+        fdivmod(w, nla, rla, jco, ico);
         if (ico < la_lo || jco < lb_lo) continue;
         const double e = tau ? eff[jco * nla + ico] : pab[p.block_off + (long long)jco * p.ld + ico];
         if (e == 0.0) continue;
