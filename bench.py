@@ -392,7 +392,8 @@ def run_multigrid(args, torch, G, Wk, dev, cfg):
         integ += pm["integrate"]
         level_ms.append({"grid": list(lv["npts"]), "pairs": int(len(lv["index"])),
                          "sphere_points": st["cube_points"], "collocate_splits": st["csplits"],
-                         "integrate_splits": st["isplits"][1], "phase_ms": pm})
+                         "integrate_splits": st["isplits"][1], "phase_ms": pm,
+                         "plan_ms": {k: round(v, 2) for k, v in st["plan_ms"].items()}})
     # two forms of the same step: one HIP graph (levels one after the other on
     # one stream) and eager launches with each level on a stream of its own;
     # the faster one is the line's value, both are reported

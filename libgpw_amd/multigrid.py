@@ -13,6 +13,8 @@ clear memory with fill kernels, so they are legal inside capture.
 """
 from __future__ import annotations
 
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 import torch
 
@@ -25,14 +27,27 @@ class MultiGrid:
         (every pair, global block offsets) and ``levels`` (npts, dh, index)."""
         self.levels = []
         p = cfg["pairs"]
+        jobs = []
         for lv in cfg["levels"]:
             idx = np.asarray(lv["index"])
             if idx.size == 0:
                 continue
             sub = {k: np.asarray(v)[idx] for k, v in p.items()}
-            g = grid_desc(lv["npts"], lv["dh"])
-            self.levels.append({"npts": tuple(lv["npts"]), "index": idx, "grid_desc": g,
-                                "tl": TaskList(g, sub, compute_tau=compute_tau, splits=splits)})
+            jobs.append((lv, idx, sub, grid_desc(lv["npts"], lv["dh"])))
+        # The levels are planned concurrently, one thread each: the planner
+        # runs in the library with the GIL released, so the levels' host
+        # geometry, uploads and device work lists overlap. Each thread plans
+        # on the caller's current device.
+        device = torch.cuda.current_device()
+
+        def plan(job):
+            torch.cuda.set_device(device)
+            return TaskList(job[3], job[2], compute_tau=compute_tau, splits=splits)
+
+        with ThreadPoolExecutor(max_workers=max(1, len(jobs))) as ex:
+            tls = list(ex.map(plan, jobs))
+        for (lv, idx, _, g), tl in zip(jobs, tls):
+            self.levels.append({"npts": tuple(lv["npts"]), "index": idx, "grid_desc": g, "tl": tl})
         self.npairs = int(sum(len(lv["index"]) for lv in self.levels))
 
     def autotune(self, max_splits: int = 0, reps: int = 3):
