@@ -6,7 +6,9 @@ Default workload (--config 2): 20,000 primitive pairs, la, lb uniform in
 cubic cell, 128^3 periodic grid, coefficients uniform[-1, 1], cube radii from
 eps = 1e-10 (synthetic, seeded). One step = zero the grid, collocate every
 pair onto it, zero the blocks, integrate a fixed uniform[-1, 1] potential back
-onto every pair's block; all inputs resident in HBM. Prints one JSON line.
+onto every pair's block; all inputs resident in HBM. The step is timed as
+eager launches and as replays of one HIP graph of it; the faster form is
+the value (both are reported). Prints one JSON line.
 
 Also measured in the same run: the plain one-launch-sequence-per-pair GPU
 form on the same task list, the CPU oracle (OpenMP) on a bounded sample of
@@ -248,17 +250,36 @@ def run_single(args, torch, G, Wk, dev, cfg):
         tl.collocate(pab, grid)
         tl.integrate(V, hab, pab=pab if forces is not None else None, forces=forces)
 
-    for _ in range(args.warmup):
+    def timed(run):
+        for _ in range(args.warmup):
+            run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            run()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    # two forms of the same step, each timed over the same K steps: eager
+    # launches, and replays of one HIP graph of the step (the fills and the
+    # task-list calls allocate nothing, so the step captures as it is); the
+    # faster one is the line's value, both are reported
+    wall_eager = timed(step)
+    graph = torch.cuda.CUDAGraph()
+    torch.cuda.synchronize()
+    with torch.cuda.graph(graph):
         step()
     torch.cuda.synchronize()
-    # per-phase HIP events recorded by the library on the launch stream
+    wall_graph = timed(graph.replay)
+    wall = min(wall_eager, wall_graph)
+    form = "eager launches" if wall_eager <= wall_graph else "HIP graph replay"
+    # per-phase HIP events recorded by the library on the launch stream, in a
+    # separate pass of K eager steps (the events are not in the timed steps)
     tl.set_timing(True)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
+    for _ in range(args.steps):
         step()
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
     ph = tl.timings()
     tl.set_timing(False)
     phase_ms = {k: v[0] / max(v[1], 1) for k, v in ph.items()}
@@ -280,6 +301,9 @@ def run_single(args, torch, G, Wk, dev, cfg):
                       "integrate_splits": st["isplits"][1 if with_forces else 0],
                       "tile": st["tile"], "sub_boxes": st["nsub"],
                       "work_items": st["nentries"], "sphere_points": pts, "plan_s": round(plan_s, 4),
+                      "step_form": form,
+                      "ms_per_step_eager": round(wall_eager * 1e3 / args.steps, 4),
+                      "ms_per_step_graph": round(wall_graph * 1e3 / args.steps, 4),
                       "library_load_s": round(load_s, 4),
                       "plan_ms": {k: round(v, 2) for k, v in st["plan_ms"].items()},
                       "phase_ms": {k: round(v, 4) for k, v in phase_ms.items()}},
