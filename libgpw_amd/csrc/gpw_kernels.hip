@@ -574,11 +574,14 @@ __global__ __launch_bounds__(TR_NT, 8) void k_hab(const PairDev *__restrict__ pa
     if (tau) build_eff(p, tau, pab, eff, nla, nlb, nla, threadIdx.x, blockDim.x);
     double f[6] = {0, 0, 0, 0, 0, 0};
     const int la_lo = ncoset(max(p.la_min - tau, 0) - 1), lb_lo = ncoset(max(p.lb_min - tau, 0) - 1);
-    const float rla = __builtin_amdgcn_rcpf((float)nla);
-    for (int w = threadIdx.x; w < nla * nlb; w += blockDim.x) {
+    // the entries of shells la_lo.., lb_lo.. only
+    const int nAf = nla - la_lo, nBf = nlb - lb_lo;
+    const float rAf = __builtin_amdgcn_rcpf((float)nAf);
+    for (int w = threadIdx.x; w < nAf * nBf; w += blockDim.x) {
         int jco, ico;
-        fdivmod(w, nla, rla, jco, ico);
-        if (ico < la_lo || jco < lb_lo) continue;
+        fdivmod(w, nAf, rAf, jco, ico);
+        ico += la_lo;
+        jco += lb_lo;
         const double e = tau ? eff[jco * nla + ico] : pab[p.block_off + (long long)jco * p.ld + ico];
         if (e == 0.0) continue;
         const int da = dec[ico], db = dec[jco];
