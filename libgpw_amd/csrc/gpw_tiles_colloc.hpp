@@ -35,12 +35,19 @@ struct CollocSmem {
 
 // register block of one thread for one pair: rows y = 4*yb + r of plane z,
 // points x = 8*xg + j
-template <int L, int LM, bool GEN>
-__device__ __forceinline__ void colloc_block(const CollocSmem<LM, GEN> &sm, int q, int xg, int yb,
+template <int L, int LM, bool GEN, bool SLOW>
+__device__ __forceinline__ bool colloc_block(const CollocSmem<LM, GEN> &sm, int q, int xg, int yb,
                                              int z, const int (&Tr)[RB], bool full,
                                              const GenBlock &gb, double (&acc)[RB][XB]) {
     constexpr int lp = L - 1;
     (void)full;
+    // general cells: the Gaussian's row factors before the y stage, so that
+    // their exponentials are not live beside cx; a block the recursion
+    // cannot take is left to the per-point pass (SLOW)
+    double e[RB], rho[RB];
+    if constexpr (GEN && !SLOW) {
+        if (!gen_row_start(sm.gr, q, z, yb, gb, e, rho)) return false;
+    }
     double cx[RB][L];
     const double *cxy = sm.cxy[q][z];
 #pragma unroll
@@ -64,8 +71,7 @@ __device__ __forceinline__ void colloc_block(const CollocSmem<LM, GEN> &sm, int 
     }
     const double *Xg = sm.X[q] + xg * TileCfg<LM>::GS;
     if constexpr (GEN) {
-        double e[RB], rho[RB];
-        if (gen_row_start(sm.gr, q, z, yb, gb, e, rho)) {
+        if constexpr (!SLOW) {
 #pragma unroll
             for (int j = 0; j < XB; j++) {
                 double xv[L];
@@ -132,6 +138,7 @@ __device__ __forceinline__ void colloc_block(const CollocSmem<LM, GEN> &sm, int 
             }
         }
     }
+    return true;
 }
 
 
@@ -218,6 +225,7 @@ __device__ __forceinline__ void collocate_tile(CollocSmem<LM, GEN> &sm, int bid,
             }
         }
         __syncthreads();
+        unsigned slow = 0;  // general cells: pairs whose block takes the per-point pass
 #pragma unroll 1
         for (int q = 0; q < nq; q++) {
             int Tr[RB];
@@ -225,9 +233,27 @@ __device__ __forceinline__ void collocate_tile(CollocSmem<LM, GEN> &sm, int bid,
             bool full;
             if (!classify_block<GEN>(sm, q, xg, yb, z, Tr, gb, full)) continue;
             const int lp = sm.lp[q];
-#define GPW_COLLOC_CALL(L) colloc_block<L, LM, GEN>(sm, q, xg, yb, z, Tr, full, gb, acc)
+            bool done = true;
+#define GPW_COLLOC_CALL(L) done = colloc_block<L, LM, GEN, false>(sm, q, xg, yb, z, Tr, full, gb, acc)
             GPW_LP_DISPATCH(LM, GPW_COLLOC_CALL)
 #undef GPW_COLLOC_CALL
+            if (!done) slow |= 1u << q;
+        }
+        if constexpr (GEN) {
+            // the rare blocks with per-point exponentials, in a region of
+            // their own (its registers do not add to the pass above)
+#pragma unroll 1
+            for (int q = 0; q < nq; q++) {
+                if (!(slow >> q & 1u)) continue;
+                int Tr[RB];
+                GenBlock gb;
+                bool full;
+                classify_block<GEN>(sm, q, xg, yb, z, Tr, gb, full);
+                const int lp = sm.lp[q];
+#define GPW_COLLOC_CALL(L) colloc_block<L, LM, GEN, true>(sm, q, xg, yb, z, Tr, full, gb, acc)
+                GPW_LP_DISPATCH(LM, GPW_COLLOC_CALL)
+#undef GPW_COLLOC_CALL
+            }
         }
         // commit the prefetched data to the LDS rings
         if (t < Q * RECW) reinterpret_cast<int *>(&sm.rec[(k + 2) % 3][0])[t] = pf_rec;
