@@ -61,7 +61,8 @@ __device__ __forceinline__ double row_sum16(double v) { return stride4_sum(quad_
 template <int L0, int L1, int LM, bool GEN>
 __device__ __forceinline__ void integ_x_part(IntegSmem<LM, GEN> &sm, int q, int xg, int yb, int z,
                                              const int (&Tr)[RB], int mode, const GenBlock &gb,
-                                             const double (&v)[RB][XB], double *t1) {
+                                             const double (&v)[RB][XB], double *t1, bool fast,
+                                             const double (&e0)[RB], const double (&rho0)[RB]) {
     constexpr int NL = L1 - L0;
     double tx[RB][NL];
 #pragma unroll
@@ -73,7 +74,12 @@ __device__ __forceinline__ void integ_x_part(IntegSmem<LM, GEN> &sm, int q, int 
         const bool wfull = __all(mode == 2);  // wave-uniform among the active lanes
         if constexpr (GEN) {
             double e[RB], rho[RB];
-            if (gen_row_start(sm.gr, q, z, yb, gb, e, rho)) {
+#pragma unroll
+            for (int r = 0; r < RB; r++) {
+                e[r] = e0[r];
+                rho[r] = rho0[r];
+            }
+            if (fast) {
 #pragma unroll
                 for (int j = 0; j < XB; j++) {
                     double xv[NL];
@@ -170,8 +176,14 @@ __device__ __forceinline__ void integ_block(IntegSmem<LM, GEN> &sm, int q, int x
     // the accumulator registers: no spills at two workgroups per CU)
     constexpr int LA = (GEN && L > 8) ? (L + 1) / 2 : L;
     double *t1 = sm.t1[wave] + zl * ZS;
-    integ_x_part<0, LA>(sm, q, xg, yb, z, Tr, mode, gb, v, t1);
-    if constexpr (LA < L) integ_x_part<LA, L>(sm, q, xg, yb, z, Tr, mode, gb, v, t1);
+    // general cells: the Gaussian's row factors once for both halves
+    double e0[RB], rho0[RB];
+    bool fast = false;
+    if constexpr (GEN) {
+        if (mode != 0) fast = gen_row_start(sm.gr, q, z, yb, gb, e0, rho0);
+    }
+    integ_x_part<0, LA>(sm, q, xg, yb, z, Tr, mode, gb, v, t1, fast, e0, rho0);
+    if constexpr (LA < L) integ_x_part<LA, L>(sm, q, xg, yb, z, Tr, mode, gb, v, t1, fast, e0, rho0);
     __builtin_amdgcn_wave_barrier();
     const double *Yq = &sm.Y[q][0][0];
     constexpr int YS = TileCfg<LM>::YS;
