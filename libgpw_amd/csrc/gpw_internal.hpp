@@ -100,7 +100,7 @@ struct LatBlk {
 };
 // Cells whose steps are triangular, dh[0] = (a0, 0, 0), dh[1] = (b0, b1, 0),
 // dh[2] = (c0, c1, c2) (a along x, b in the xy plane: the usual cell
-// orientation), need no tables: the transform runs in two passes from
+// orientation), need no tables: the transform runs in three passes from
 // h = {a0, b0, c0, b1, c1, c2} (lattice_stepped).
 struct LatticeTab {
     const int *col = nullptr;

@@ -1,16 +1,16 @@
 """GPU tests of the two forms of the general-cell lattice transform.
 
 Cells whose steps are triangular (a along x, b in the xy plane) run the
-transform in two passes from the six steps; every other general cell reads
-the expanded tables. Each form is checked against the oracle with all three
-off-diagonal steps non-zero:
+transform in three substitution passes from the six steps; every other
+general cell reads the expanded tables. Checked against the oracle:
 
-* a triclinic cell with alpha, beta, gamma all off 90 degrees (two-pass form,
-  b0, c0 and c1 non-zero), and
-* the same cell rotated about a skew axis (no zero step: tables),
-
-through task lists (tile kernels with and without splits, and the plain
-kernels; rho and tau; forces; la_min < la_max) and the single-task drop-in.
+* a triclinic cell with alpha, beta, gamma all off 90 degrees (the passes,
+  b0, c0 and c1 non-zero) and the same cell rotated about a skew axis (no
+  zero step: the tables), through task lists (tile kernels with and without
+  splits, and the plain kernels; rho and tau; forces; la_min < la_max) and
+  the single-task drop-in;
+* task lists on cells with only c0 or only c1 non-zero (the passes whose
+  step is zero skip their sums).
 Tolerance 1e-10 relative (max|x - ref| / max|ref|).
 """
 import sys
@@ -40,8 +40,19 @@ def _rotation(axis, angle):
 
 
 def cell_steps(form):
-    """dh (rows: steps) of the test cell in the given form"""
+    """dh (rows: steps) of the test cell in the given form: "stepped" (a
+    triclinic cell, a along x, b in the xy plane), "rotated" (the same cell
+    turned about a skew axis: no zero step), "c0" / "c1" (an orthorhombic cell
+    with only dh[2][0], or only dh[2][1], non-zero: the passes with zero
+    steps b0 and c1, or b0 and c0)"""
     from libgpw_amd import workloads as Wk
+    if form in ("c0", "c1"):
+        dh = np.diag(np.array([9.0, 9.9, 10.8]) / np.asarray(NPTS, dtype=np.float64))
+        if form == "c0":
+            dh[2, 0] = 0.3 * dh[0, 0]
+        else:
+            dh[2, 1] = -0.25 * dh[1, 1]
+        return dh
     hmat = Wk.triclinic_cell(9.0, 9.9, 10.8, 75.0, 80.0, 70.0)
     if form == "rotated":
         hmat = hmat @ _rotation([1.0, 2.0, 0.5], 0.6).T
@@ -55,10 +66,10 @@ def cell_steps(form):
 
 
 @pytest.mark.parametrize("tau", [False, True])
-@pytest.mark.parametrize("form", ["stepped", "rotated"])
+@pytest.mark.parametrize("form", ["stepped", "rotated", "c0", "c1"])
 def test_tasklist_cell_forms(gpw, oracle, torch_cuda, form, tau):
     torch = torch_cuda
-    rng = np.random.default_rng(515 + 2 * tau + (form == "rotated"))
+    rng = np.random.default_rng(515 + 2 * tau + 10 * ["stepped", "rotated", "c0", "c1"].index(form))
     dh = cell_steps(form)
     cell_m = dh * np.asarray(NPTS)[:, None]
     pairs, nblock = _ranged_pairs(rng, 150, cell_m)
