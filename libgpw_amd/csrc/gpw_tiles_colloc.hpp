@@ -9,12 +9,13 @@
 namespace gpw {
 
 // Collocate chunk configuration: the general-cell kernels run one workgroup
-// per CU and have LDS to spare, so their classes L <= 8 and L <= 12 take
-// eight and four pairs per chunk (more work between barriers; with four the
-// table phase keeps every wave busy)
+// per CU, so their class L <= 12 takes four pairs per chunk (with four the
+// table phase keeps every wave busy). The class L <= 8 keeps four too: eight
+// shorten its own time a little but double its LDS, and less of the other
+// work of the step then fits beside it on a CU.
 template <int LM, bool GEN>
 struct CollocCfg : TileCfg<LM> {
-    static constexpr int Q = (GEN && LM == 12) ? 4 : (GEN && LM == 8) ? 8 : TileCfg<LM>::Q;
+    static constexpr int Q = (GEN && LM == 12) ? 4 : TileCfg<LM>::Q;
     static constexpr int NPF = (Q * TileCfg<LM>::C3M + NROW - 1) / NROW;
 };
 
