@@ -20,7 +20,7 @@ constexpr int NROW = TY * TZ;             // 256 threads per workgroup
 // a global workspace of HAB_WS_BLOCKS slices (one per resident workgroup);
 // TR_NT: their largest workgroup
 constexpr size_t COEF_LDS_MAX = 24 * 1024;
-constexpr size_t HAB_LDS_MAX = 32 * 1024;
+constexpr size_t HAB_LDS_MAX = 28 * 1024;
 constexpr int HAB_WS_BLOCKS = 4096;
 constexpr int TR_NT = 256;
 // the one-pair-per-wave transforms (tau = 0) take blocks up to this LDS size
