@@ -220,16 +220,17 @@ __device__ __forceinline__ void gen_row_coef(const GenRows<Q, true> &gr, int q, 
 // e(u+1) = e(u) rho(u), rho(u+1) = rho(u) kap, started at u0 (two exp per
 // row instead of one per point). Returns false when the recursion cannot be
 // used: the block is not contiguous, or the start value of a row with points
-// inside underflows (later factors would be lost).
+// inside underflows (later factors would be lost). Register r holds row
+// yb * RB + (r ^ rx).
 template <int Q>
 __device__ __forceinline__ bool gen_row_start(const GenRows<Q, true> &gr, int q, int z, int yb,
                                               const GenBlock &gb, double (&e)[RB],
-                                              double (&rho)[RB]) {
+                                              double (&rho)[RB], int rx = 0) {
     bool fast = gb.contig;
 #pragma unroll
     for (int r = 0; r < RB; r++) {
         double qb, qc;
-        gen_row_coef(gr, q, z, yb * RB + r, qb, qc);
+        gen_row_coef(gr, q, z, yb * RB + (r ^ rx), qb, qc);
         e[r] = exp((gb.qa * gb.u0 + qb) * gb.u0 + qc);
         rho[r] = exp(gb.qa * (2.0 * gb.u0 + 1.0) + qb);
         fast = fast && (gb.bits[r] == 0 || e[r] >= 1e-290);
@@ -279,15 +280,17 @@ __device__ __forceinline__ bool gen_row_start(const GenRows<Q, true> &gr, int q,
 
 // classification of a thread's register block for pair q: returns false when
 // no point of the block is inside; sets full when every point is inside.
+// Register r of Tr and gb.bits describes row yb * RB + (r ^ rx).
 template <bool GEN, class Smem>
 __device__ __forceinline__ bool classify_block(const Smem &sm, int q, int xg, int yb, int z,
-                                               int (&Tr)[RB], GenBlock &gb, bool &full) {
+                                               int (&Tr)[RB], GenBlock &gb, bool &full,
+                                               int rx = 0) {
     const int bmn = sm.bmin[q][xg], bmx = sm.bmax[q][xg];
     if constexpr (!GEN) {
         int tmin = BIG, tmax = -1;
 #pragma unroll
         for (int r = 0; r < RB; r++) {
-            Tr[r] = sm.T[q][z * TY + yb * RB + r];
+            Tr[r] = sm.T[q][z * TY + yb * RB + (r ^ rx)];
             tmin = min(tmin, Tr[r]);
             tmax = max(tmax, Tr[r]);
         }
@@ -301,7 +304,7 @@ __device__ __forceinline__ bool classify_block(const Smem &sm, int q, int xg, in
         gb.contig = c0 != BIG;
 #pragma unroll
         for (int r = 0; r < RB; r++) {
-            const int row = z * TY + yb * RB + r;
+            const int row = z * TY + yb * RB + (r ^ rx);
             const int lo = sm.gr.lo[q][row], hi = sm.gr.hi[q][row];
             any = any || (lo <= hi && lo <= bmx && hi >= bmn);
             full = full && lo <= bmn && bmx <= hi;

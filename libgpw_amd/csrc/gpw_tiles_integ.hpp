@@ -9,21 +9,12 @@
 namespace gpw {
 
 // DPP move of a double (two 32-bit halves); CTRL is the dpp_ctrl code
+// (0xB1: quad_perm [1,0,3,2], lane ^ 1; 0x4E: quad_perm [2,3,0,1], lane ^ 2)
 template <int CTRL>
 __device__ __forceinline__ double dpp_f64(double a) {
     const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(a), CTRL, 0xF, 0xF, false);
     const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(a), CTRL, 0xF, 0xF, false);
     return __hiloint2double(hi, lo);
-}
-// sum over the 4 lanes of a quad (quad_perm [1,0,3,2] then [2,3,0,1])
-__device__ __forceinline__ double quad_sum(double v) {
-    v += dpp_f64<0xB1>(v);
-    return v + dpp_f64<0x4E>(v);
-}
-// sum over lanes l, l+4, l+8, l+12 of a DPP row (row_ror:4 then row_ror:8)
-__device__ __forceinline__ double stride4_sum(double v) {
-    v += dpp_f64<0x124>(v);
-    return v + dpp_f64<0x128>(v);
 }
 
 // Per-wave buffer of the integrate transposition: the x-stage results of the
@@ -53,11 +44,11 @@ struct IntegSmem {
     int lp[C::Q];
 };
 
-// sum of a double over the 16 lanes of a DPP row; every lane gets the sum
-__device__ __forceinline__ double row_sum16(double v) { return stride4_sum(quad_sum(v)); }
-
 // x stage of integ_block for the powers L0 <= l < L1 of the x table, summed
-// over the lane quad and stored to the wave's transposition rows t1
+// over the lane quad and stored to the wave's transposition rows t1. Register
+// row r of lane xg holds grid row yb * RB + (r ^ xg), so the quad sum is a
+// reduce-scatter with the same register operands on every lane: three DPP
+// moves and adds per power leave row xg's sum in register row 0.
 template <int L0, int L1, int LM, bool GEN>
 __device__ __forceinline__ void integ_x_part(IntegSmem<LM, GEN> &sm, int q, int xg, int yb, int z,
                                              const int (&Tr)[RB], int mode, const GenBlock &gb,
@@ -112,9 +103,9 @@ __device__ __forceinline__ void integ_x_part(IntegSmem<LM, GEN> &sm, int q, int 
                         double vj = 0.0;
 #pragma unroll
                         for (int jj = 0; jj < XB; jj++) vj = jj == j ? v[r][jj] : vj;
-                        const int row = z * TY + yb * RB + r;
+                        const int row = z * TY + yb * RB + (r ^ xg);
                         double qb, qc;
-                        gen_row_coef(sm.gr, q, z, yb * RB + r, qb, qc);
+                        gen_row_coef(sm.gr, q, z, yb * RB + (r ^ xg), qb, qc);
                         const bool in = code >= sm.gr.lo[q][row] && code <= sm.gr.hi[q][row];
                         const double w = in ? vj * exp((gb.qa * u + qb) * u + qc) : 0.0;
 #pragma unroll
@@ -139,15 +130,16 @@ __device__ __forceinline__ void integ_x_part(IntegSmem<LM, GEN> &sm, int q, int 
             }
         }
     }
-    // rows of the quad summed over its four x blocks; lane xg stores row r = xg
+    // rows of the quad summed over its four x blocks: lane xg ^ 1 holds row xg
+    // in register row 1 and row xg ^ 2 in register row 3, lane xg ^ 2 the
+    // pair sum of row xg in register row 2; lane xg stores row xg
+    static_assert(RB == 4, "reduce-scatter over a lane quad");
 #pragma unroll
-    for (int r = 0; r < RB; r++) {
-#pragma unroll
-        for (int l = 0; l < NL; l++) tx[r][l] = quad_sum(tx[r][l]);
-        if (xg == r) {
-#pragma unroll
-            for (int l = 0; l < NL; l++) t1[(yb * RB + r) * RowT1<LM>::RS + L0 + l] = tx[r][l];
-        }
+    for (int l = 0; l < NL; l++) {
+        tx[0][l] += dpp_f64<0xB1>(tx[1][l]);
+        tx[2][l] += dpp_f64<0xB1>(tx[3][l]);
+        tx[0][l] += dpp_f64<0x4E>(tx[2][l]);
+        t1[(yb * RB + xg) * RowT1<LM>::RS + L0 + l] = tx[0][l];
     }
 }
 
@@ -180,7 +172,7 @@ __device__ __forceinline__ void integ_block(IntegSmem<LM, GEN> &sm, int q, int x
     double e0[RB], rho0[RB];
     bool fast = false;
     if constexpr (GEN) {
-        if (mode != 0) fast = gen_row_start(sm.gr, q, z, yb, gb, e0, rho0);
+        if (mode != 0) fast = gen_row_start(sm.gr, q, z, yb, gb, e0, rho0, xg);
     }
     integ_x_part<0, LA>(sm, q, xg, yb, z, Tr, mode, gb, v, t1, fast, e0, rho0);
     if constexpr (LA < L) integ_x_part<LA, L>(sm, q, xg, yb, z, Tr, mode, gb, v, t1, fast, e0, rho0);
@@ -232,8 +224,8 @@ __device__ __forceinline__ void integrate_tile(IntegSmem<LM, GEN> &sm, int bid, 
     {
         const int gz = z0 + z;
 #pragma unroll
-        for (int r = 0; r < RB; r++) {
-            const int gy = y0 + yb * RB + r;
+        for (int r = 0; r < RB; r++) {  // register row r: grid row r ^ xg
+            const int gy = y0 + yb * RB + (r ^ xg);
             const bool rowok = gy < tg.ny && gz < tg.nz;
             const long long row = ((long long)gz * tg.ny + gy) * tg.nx + x0 + xg * XB;
 #pragma unroll
@@ -270,7 +262,8 @@ __device__ __forceinline__ void integrate_tile(IntegSmem<LM, GEN> &sm, int bid, 
             int Tr[RB];
             GenBlock gb;
             bool full = false;
-            const int mode = classify_block<GEN>(sm, q, xg, yb, z, Tr, gb, full) ? (full ? 2 : 1) : 0;
+            const int mode =
+                classify_block<GEN>(sm, q, xg, yb, z, Tr, gb, full, xg) ? (full ? 2 : 1) : 0;
             const int lp = sm.lp[q];
 #define GPW_INTEG_CALL(L) integ_block<L, LM, GEN>(sm, q, xg, yb, z, Tr, mode, gb, v, t)
             GPW_LP_DISPATCH(LM, GPW_INTEG_CALL)
