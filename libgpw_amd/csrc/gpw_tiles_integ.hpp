@@ -147,7 +147,8 @@ __device__ __forceinline__ void integ_x_part(IntegSmem<LM, GEN> &sm, int q, int 
 // q: x stage in registers, sum of the four x blocks of each row over the lane
 // quad (DPP), then a wave-local transposition through LDS for the y stage:
 // lane (zl, c) of the wave sums Y[y][lyp] * tx[zl][y][lxp] over the 16 rows
-// of its plane for the (lyp, lxp) terms c, c + 16, ...  A wave holds four
+// of its plane for the (lyp, lxp) terms c, c + 16, ... (from L = 6 on, for
+// the 2 x 2 blocks of terms c, c + 16, ...).  A wave holds four
 // whole z-planes, so the transposition needs no workgroup barrier (LDS
 // operations of one wave complete in order).
 template <int L, int LM, bool GEN>
@@ -179,6 +180,39 @@ __device__ __forceinline__ void integ_block(IntegSmem<LM, GEN> &sm, int q, int x
     __builtin_amdgcn_wave_barrier();
     const double *Yq = &sm.Y[q][0][0];
     constexpr int YS = TileCfg<LM>::YS;
+    constexpr int RS = RowT1<LM>::RS;
+    if constexpr (L >= 6) {
+        // 2 x 2 blocks (lyp0 .. lyp0 + 1, lxp0 .. lxp0 + 1), lyp0 and lxp0
+        // even, lyp0 + lxp0 <= lp: two paired LDS reads per four FMAs. Terms
+        // of a block past the triangle are computed from in-bounds LDS words
+        // (the next row's first entries) and not stored.
+        constexpr int NB = (lp / 2 + 1) * (lp / 2 + 2) / 2;
+        for (int b = cq; b < NB; b += 16) {
+            int by = 0, bx = b;
+            while (bx > (lp - 2 * by) / 2) {
+                bx -= (lp - 2 * by) / 2 + 1;
+                by++;
+            }
+            const int ly = 2 * by, lx = 2 * bx;
+            double a00 = 0.0, a01 = 0.0, a10 = 0.0, a11 = 0.0;
+#pragma unroll 4
+            for (int y = 0; y < TY; y++) {
+                const double y0 = Yq[y * YS + ly], y1 = Yq[y * YS + ly + 1];
+                const double x0 = t1[y * RS + lx], x1 = t1[y * RS + lx + 1];
+                a00 += y0 * x0;
+                a01 += y0 * x1;
+                a10 += y1 * x0;
+                a11 += y1 * x1;
+            }
+            const int c = c2idx(lp, ly, lx), cn = c + lp - ly + 1;
+            uzp[c] = a00;
+            if (lx + 1 <= lp - ly) uzp[c + 1] = a01;
+            if (lx + 1 <= lp - ly - 1) uzp[cn + 1] = a11;
+            if (lx <= lp - ly - 1) uzp[cn] = a10;
+        }
+        __builtin_amdgcn_wave_barrier();
+        return;
+    }
     for (int c = cq; c < N2; c += 16) {
         int lyp = 0, lxp = c;
         while (lxp > lp - lyp) {
