@@ -196,6 +196,16 @@ __device__ __forceinline__ void tile_phase1(const EntryRec &R, int q, int lane, 
     }
 }
 
+// DPP move of a double (two 32-bit halves) within a lane quad; CTRL is the
+// dpp_ctrl code (0xB1: quad_perm [1,0,3,2], lane ^ 1; 0x4E: [2,3,0,1],
+// lane ^ 2; 0x1B: [3,2,1,0], lane ^ 3). Every lane of the quad must be active.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double a) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(a), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(a), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
 // Phase-3 data of one thread's register block on the general path.
 // contig: the valid points j of the block have the lattice indices c0 + j;
 // then bit j of bits[r] says whether point j of row r is valid and inside

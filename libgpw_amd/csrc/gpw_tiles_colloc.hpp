@@ -34,42 +34,75 @@ struct CollocSmem {
     int lp[C::Q];
 };
 
-// register block of one thread for one pair: rows y = 4*yb + r of plane z,
-// points x = 8*xg + j
+// Register row r of a thread holds grid row 4*yb + (r ^ rx): general cells
+// permute the rows by rx = xg, so that the four lanes of a quad share the y
+// stage (each computes one row; orthorhombic cells, whose kernels are short
+// of registers, compute all four rows per lane and keep rx = 0).
+template <bool GEN>
+__device__ __forceinline__ int colloc_rx(int xg) { return GEN ? xg : 0; }
+
+// register block of one thread for one pair: rows y = 4*yb + (r ^ rx) of
+// plane z in register row r, points x = 8*xg + j. General cells: entered by
+// every lane of the wave (the quad exchanges the y stage's rows); lanes whose
+// block has no point inside (active false) stop after the y stage.
 template <int L, int LM, bool GEN, bool SLOW>
 __device__ __forceinline__ bool colloc_block(const CollocSmem<LM, GEN> &sm, int q, int xg, int yb,
                                              int z, const int (&Tr)[RB], bool full,
-                                             const GenBlock &gb, double (&acc)[RB][XB]) {
+                                             const GenBlock &gb, double (&acc)[RB][XB],
+                                             bool active) {
     constexpr int lp = L - 1;
     (void)full;
     // general cells: the Gaussian's row factors before the y stage, so that
     // their exponentials are not live beside cx; a block the recursion
     // cannot take is left to the per-point pass (SLOW)
     double e[RB], rho[RB];
-    if constexpr (GEN && !SLOW) {
-        if (!gen_row_start(sm.gr, q, z, yb, gb, e, rho)) return false;
-    }
+    bool fast = true;
+    const int rx = colloc_rx<GEN>(xg);
+    if constexpr (GEN && !SLOW) fast = gen_row_start(sm.gr, q, z, yb, gb, e, rho, rx);
     double cx[RB][L];
     const double *cxy = sm.cxy[q][z];
+    if constexpr (GEN && L >= 3) {
+        // y stage of the lane's own row 4*yb + xg; lane xg ^ s holds row
+        // 4*yb + (s ^ xg) in its register row 0, so three DPP moves per
+        // power fill the other register rows (a quarter of the FMAs of each
+        // lane computing all four rows)
 #pragma unroll
-    for (int r = 0; r < RB; r++)
+        for (int l = 0; l < L; l++) cx[0][l] = 0.0;
 #pragma unroll
-        for (int l = 0; l < L; l++) cx[r][l] = 0.0;
-    // y stage, rows innermost: each cxy value is loaded once and used by its
-    // RB FMAs at once (a row-outer order keeps every cxy value live across
-    // the rows and spills registers)
+        for (int lyp = 0; lyp <= lp; lyp++) {
+            const double y = sm.Y[q][yb * RB + xg][lyp];
 #pragma unroll
-    for (int lyp = 0; lyp <= lp; lyp++) {
-        double y[RB];
+            for (int lxp = 0; lxp <= lp - lyp; lxp++) cx[0][lxp] += cxy[c2idx(lp, lyp, lxp)] * y;
+        }
+        static_assert(RB == 4, "row exchange within a lane quad");
 #pragma unroll
-        for (int r = 0; r < RB; r++) y[r] = sm.Y[q][yb * RB + r][lyp];
+        for (int l = 0; l < L; l++) {
+            cx[1][l] = dpp_f64<0xB1>(cx[0][l]);
+            cx[2][l] = dpp_f64<0x4E>(cx[0][l]);
+            cx[3][l] = dpp_f64<0x1B>(cx[0][l]);
+        }
+    } else {
 #pragma unroll
-        for (int lxp = 0; lxp <= lp - lyp; lxp++) {
-            const double c = cxy[c2idx(lp, lyp, lxp)];
+        for (int r = 0; r < RB; r++)
 #pragma unroll
-            for (int r = 0; r < RB; r++) cx[r][lxp] += c * y[r];
+            for (int l = 0; l < L; l++) cx[r][l] = 0.0;
+        // y stage, rows innermost: each cxy value is loaded once and used by
+        // its RB FMAs at once
+#pragma unroll
+        for (int lyp = 0; lyp <= lp; lyp++) {
+            double y[RB];
+#pragma unroll
+            for (int r = 0; r < RB; r++) y[r] = sm.Y[q][yb * RB + (r ^ rx)][lyp];
+#pragma unroll
+            for (int lxp = 0; lxp <= lp - lyp; lxp++) {
+                const double c = cxy[c2idx(lp, lyp, lxp)];
+#pragma unroll
+                for (int r = 0; r < RB; r++) cx[r][lxp] += c * y[r];
+            }
         }
     }
+    if (!active) return true;
+    if (!fast) return false;
     const double *Xg = sm.X[q] + xg * TileCfg<LM>::GS;
     if constexpr (GEN) {
         if constexpr (!SLOW) {
@@ -108,9 +141,9 @@ __device__ __forceinline__ bool colloc_block(const CollocSmem<LM, GEN> &sm, int 
                     double v = 0.0;
 #pragma unroll
                     for (int l = 0; l < L; l++) v += cx[r][l] * xv[l];
-                    const int row = z * TY + yb * RB + r;
+                    const int row = z * TY + yb * RB + (r ^ rx);
                     double qb, qc;
-                    gen_row_coef(sm.gr, q, z, yb * RB + r, qb, qc);
+                    gen_row_coef(sm.gr, q, z, yb * RB + (r ^ rx), qb, qc);
                     const bool in = code >= sm.gr.lo[q][row] && code <= sm.gr.hi[q][row];
                     d[r] = in ? v * exp((gb.qa * u + qb) * u + qc) : 0.0;
                 }
@@ -232,10 +265,14 @@ __device__ __forceinline__ void collocate_tile(CollocSmem<LM, GEN> &sm, int bid,
             int Tr[RB];
             GenBlock gb;
             bool full;
-            if (!classify_block<GEN>(sm, q, xg, yb, z, Tr, gb, full)) continue;
+            const bool active =
+                classify_block<GEN>(sm, q, xg, yb, z, Tr, gb, full, colloc_rx<GEN>(xg));
+            // general cells: wave-uniform, so the quads stay whole
+            if (GEN ? !__any(active) : !active) continue;
             const int lp = sm.lp[q];
             bool done = true;
-#define GPW_COLLOC_CALL(L) done = colloc_block<L, LM, GEN, false>(sm, q, xg, yb, z, Tr, full, gb, acc)
+#define GPW_COLLOC_CALL(L)                                                                     \
+    done = colloc_block<L, LM, GEN, false>(sm, q, xg, yb, z, Tr, full, gb, acc, active)
             GPW_LP_DISPATCH(LM, GPW_COLLOC_CALL)
 #undef GPW_COLLOC_CALL
             if (!done) slow |= 1u << q;
@@ -245,13 +282,14 @@ __device__ __forceinline__ void collocate_tile(CollocSmem<LM, GEN> &sm, int bid,
             // their own (its registers do not add to the pass above)
 #pragma unroll 1
             for (int q = 0; q < nq; q++) {
-                if (!(slow >> q & 1u)) continue;
+                const bool mine = slow >> q & 1u;
+                if (!__any(mine)) continue;
                 int Tr[RB];
                 GenBlock gb;
                 bool full;
-                classify_block<GEN>(sm, q, xg, yb, z, Tr, gb, full);
+                classify_block<GEN>(sm, q, xg, yb, z, Tr, gb, full, colloc_rx<GEN>(xg));
                 const int lp = sm.lp[q];
-#define GPW_COLLOC_CALL(L) colloc_block<L, LM, GEN, true>(sm, q, xg, yb, z, Tr, full, gb, acc)
+#define GPW_COLLOC_CALL(L) colloc_block<L, LM, GEN, true>(sm, q, xg, yb, z, Tr, full, gb, acc, mine)
                 GPW_LP_DISPATCH(LM, GPW_COLLOC_CALL)
 #undef GPW_COLLOC_CALL
             }
@@ -270,8 +308,8 @@ __device__ __forceinline__ void collocate_tile(CollocSmem<LM, GEN> &sm, int bid,
     const long long ng = (long long)tg.nx * tg.ny * tg.nz;
     double *dst = out_mode == 0 ? out : out + (split_base + split) * ng;
 #pragma unroll
-    for (int r = 0; r < RB; r++) {
-        const int gy = y0 + yb * RB + r;
+    for (int r = 0; r < RB; r++) {  // register row r: grid row r ^ rx
+        const int gy = y0 + yb * RB + (r ^ colloc_rx<GEN>(xg));
         if (gy >= tg.ny) continue;
         double *row = dst + ((long long)gz * tg.ny + gy) * tg.nx + x0 + xg * XB;
 #pragma unroll

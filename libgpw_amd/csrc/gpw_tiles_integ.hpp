@@ -8,15 +8,6 @@
 
 namespace gpw {
 
-// DPP move of a double (two 32-bit halves); CTRL is the dpp_ctrl code
-// (0xB1: quad_perm [1,0,3,2], lane ^ 1; 0x4E: quad_perm [2,3,0,1], lane ^ 2)
-template <int CTRL>
-__device__ __forceinline__ double dpp_f64(double a) {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(a), CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(a), CTRL, 0xF, 0xF, false);
-    return __hiloint2double(hi, lo);
-}
-
 // Per-wave buffer of the integrate transposition: the x-stage results of the
 // wave's 64 rows (4 z-planes x 16 y), row (zl, y) at zl * ZS + y * RS. A store
 // writes rows 4 * yb + r (yb, zl = 0..3) of one r at once; RS = 1 (mod 4) and
