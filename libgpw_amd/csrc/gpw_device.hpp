@@ -15,6 +15,10 @@ __device__ __forceinline__ int imod(int a, int m) {
     int r = a % m;
     return r < 0 ? r + m : r;
 }
+// a mod m for 0 <= a < 2m: the grid index of a sub-box offset, base + (i - lb),
+// where the base lies in [0, m) and a sub-box spans at most one period
+// (non-periodic axes: m = BIG and a below the grid size)
+__device__ __forceinline__ int wrap_once(int a, int m) { return a >= m ? a - m : a; }
 
 // mirror index of an offset: the reference pairs offsets ig <= 0 with 1 - ig
 // (grid_base_ref_c.cc:418-474); ii' = -ig for both members of the pair.

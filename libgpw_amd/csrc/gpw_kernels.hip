@@ -726,11 +726,11 @@ __global__ __launch_bounds__(NROW) void k_moments_small(TileGrid tg, const SubDe
                 d0 = -s.zetp * (2.0 * rx * tg.dr[0] + tg.dr[0] * tg.dr[0]);
             }
             if (ilo <= ihi) {
-                const int gz = imod(s.base[2] + (k - s.lb[2]), s.period[2]);
-                const int gy = imod(s.base[1] + (j - s.lb[1]), s.period[1]);
+                const int gz = wrap_once(s.base[2] + (k - s.lb[2]), s.period[2]);
+                const int gy = wrap_once(s.base[1] + (j - s.lb[1]), s.period[1]);
                 const double *row = grid + ((long long)gz * tg.ny + gy) * tg.nx;
                 double e = exp(ex0), rho = exp(d0);
-                int gx = imod(s.base[0] + (ilo - s.lb[0]), s.period[0]);  // then stepped, wrapped
+                int gx = wrap_once(s.base[0] + (ilo - s.lb[0]), s.period[0]);  // then stepped, wrapped
                 const int P0 = s.period[0];
                 int i = ilo;
                 // four points per step: independent power chains, loads issued
