@@ -259,6 +259,14 @@ __device__ __forceinline__ void collocate_tile(CollocSmem<LM, GEN> &sm, int bid,
             }
         }
         __syncthreads();
+        // commit the prefetched data to the LDS rings (slots this chunk does
+        // not read), so that its registers are free in the point phase
+        if (t < Q * RECW) reinterpret_cast<int *>(&sm.rec[(k + 2) % 3][0])[t] = pf_rec;
+#pragma unroll
+        for (int j = 0; j < C::NPF; j++) {
+            const int w = t + j * NROW;
+            if (w < Q * C::C3M) sm.coef[cs ^ 1][w / C::C3M][w % C::C3M] = pf_coef[j];
+        }
         unsigned slow = 0;  // general cells: pairs whose block takes the per-point pass
 #pragma unroll 1
         for (int q = 0; q < nq; q++) {
@@ -293,13 +301,6 @@ __device__ __forceinline__ void collocate_tile(CollocSmem<LM, GEN> &sm, int bid,
                 GPW_LP_DISPATCH(LM, GPW_COLLOC_CALL)
 #undef GPW_COLLOC_CALL
             }
-        }
-        // commit the prefetched data to the LDS rings
-        if (t < Q * RECW) reinterpret_cast<int *>(&sm.rec[(k + 2) % 3][0])[t] = pf_rec;
-#pragma unroll
-        for (int j = 0; j < C::NPF; j++) {
-            const int w = t + j * NROW;
-            if (w < Q * C::C3M) sm.coef[cs ^ 1][w / C::C3M][w % C::C3M] = pf_coef[j];
         }
         __syncthreads();
     }

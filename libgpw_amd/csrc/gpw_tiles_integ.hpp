@@ -282,6 +282,8 @@ __device__ __forceinline__ void integrate_tile(IntegSmem<LM, GEN> &sm, int bid, 
             }
         }
         __syncthreads();
+        // commit the prefetched records (a ring slot this chunk does not read)
+        if (t < Q * RECW) reinterpret_cast<int *>(&sm.rec[rs ^ 1][0])[t] = pf_rec;
 #pragma unroll 1
         for (int q = 0; q < nq; q++) {
             int Tr[RB];
@@ -329,7 +331,6 @@ __device__ __forceinline__ void integrate_tile(IntegSmem<LM, GEN> &sm, int bid, 
                 partial[sm.rec[rs][q].ent_off + c3] = (a0 + a1) + (a2 + a3);
             }
         }
-        if (t < Q * RECW) reinterpret_cast<int *>(&sm.rec[rs ^ 1][0])[t] = pf_rec;
         __syncthreads();
     }
 }
