@@ -24,10 +24,14 @@ struct CollocSmem {
     using C = CollocCfg<LM, GEN>;
     GenRows<C::Q, GEN> gr;
     EntryRec rec[3][C::Q];
+    // coefficients: orthorhombic cells at cidx(lp, lzp, lyp, lxp); general
+    // cells in the layout of the class's largest degree, cidx(LM - 1, ...),
+    // zero above the pair's lp
     double coef[2][C::Q][C::C3M];
     double X[C::Q][C::XS];
     double Y[C::Q][TY][C::YS];
-    double cxy[C::Q][TZ][C::C2M];
+    double cxy[C::Q][TZ][C::C2M];  // at c2idx(lp, ...); general cells c2idx(LM - 1, ...)
+    int dec[GEN ? C::C3M : 1];     // general cells: (lzp, lyp, lxp) of each coef entry
     int xi[C::Q][TX];
     int T[C::Q][NROW];
     int bmax[C::Q][NXG], bmin[C::Q][NXG];
@@ -72,7 +76,7 @@ __device__ __forceinline__ bool colloc_block(const CollocSmem<LM, GEN> &sm, int 
         for (int lyp = 0; lyp <= lp; lyp++) {
             const double y = sm.Y[q][yb * RB + xg][lyp];
 #pragma unroll
-            for (int lxp = 0; lxp <= lp - lyp; lxp++) cx[0][lxp] += cxy[c2idx(lp, lyp, lxp)] * y;
+            for (int lxp = 0; lxp <= lp - lyp; lxp++) cx[0][lxp] += cxy[c2idx(LM - 1, lyp, lxp)] * y;
         }
         static_assert(RB == 4, "row exchange within a lane quad");
 #pragma unroll
@@ -95,7 +99,7 @@ __device__ __forceinline__ bool colloc_block(const CollocSmem<LM, GEN> &sm, int 
             for (int r = 0; r < RB; r++) y[r] = sm.Y[q][yb * RB + (r ^ rx)][lyp];
 #pragma unroll
             for (int lxp = 0; lxp <= lp - lyp; lxp++) {
-                const double c = cxy[c2idx(lp, lyp, lxp)];
+                const double c = cxy[GEN ? c2idx(LM - 1, lyp, lxp) : c2idx(lp, lyp, lxp)];
 #pragma unroll
                 for (int r = 0; r < RB; r++) cx[r][lxp] += c * y[r];
             }
@@ -176,6 +180,79 @@ __device__ __forceinline__ bool colloc_block(const CollocSmem<LM, GEN> &sm, int 
 }
 
 
+// coefficient at entry (lzp, lyp, lxp) = d of the LM - 1 layout for the
+// pair of record R (zero above its lp); coefficients are stored per pair in
+// the cidx(lp, ...) order
+template <int LM>
+__device__ __forceinline__ double coef_entry(int d, const EntryRec &R, const double *__restrict__ coef) {
+    const int lz = d & 31, ly = (d >> 5) & 31, lx = d >> 10;
+    return lz + ly + lx <= R.lp ? coef[R.coef_off + cidx(R.lp, lz, ly, lx)] : 0.0;
+}
+
+// first (lyp, lxp) term (c2idx(LM - 1, ...) order) of part P of the z stage:
+// four parts of about equal numbers of lzp terms
+template <int LM>
+__host__ __device__ constexpr int zpart_begin(int P) {
+    int total = 0;
+    for (int ly = 0; ly < LM; ly++)
+        for (int lx = 0; lx < LM - ly; lx++) total += LM - ly - lx;
+    int acc = 0, c = 0;
+    for (int ly = 0; ly < LM; ly++)
+        for (int lx = 0; lx < LM - ly; lx++) {
+            if (4 * acc >= P * total) return c;
+            acc += LM - ly - lx;
+            c++;
+        }
+    return c;
+}
+
+// z stage, part P: cxy[q][z][(lyp, lxp)] = sum_lzp coef[q][(lzp, lyp, lxp)]
+// Z_q(z)^lzp for the terms of part P, all pairs of the chunk at once (lane
+// -> pair q, z-plane), over the LM - 1 layout: the entries above a pair's lp
+// are zero and add exact zeros, so each sum equals the one over its own lp
+// in the same order
+template <int LM, int P, bool GEN>
+__device__ __forceinline__ void colloc_zpart(CollocSmem<LM, GEN> &sm, int rs, int cs, int nq,
+                                             int lane, int z0, const TileGrid &tg) {
+    constexpr int lpm = LM - 1;
+    constexpr int BEG = zpart_begin<LM>(P), END = zpart_begin<LM>(P + 1);
+    using C = CollocCfg<LM, GEN>;
+    for (int sb = 0; sb < C::Q * TZ; sb += 64) {
+        const int slot = sb + lane, q = slot / TZ, pos = slot % TZ;
+        if (q >= nq) continue;
+        const EntryRec &R = sm.rec[rs][q];
+        // the z table entry of the table phase
+        const int g = z0 + pos;
+        int i = BIG;
+        bool ok = g < tg.nz;
+        if (ok) {
+            i = R.lb[2] + imod(g - R.base[2], tg.period[2]);
+            ok = i <= R.ub[2];
+        }
+        const double r = GEN ? (double)i - R.roff[2] : i * tg.dr[2] - R.roff[2];
+        double v = ok ? (GEN ? 1.0 : exp(-R.zetp * r * r)) : 0.0;
+        double tv[LM];
+#pragma unroll
+        for (int l = 0; l < LM; l++) {
+            tv[l] = v;
+            v *= r;
+        }
+        const double *cf = sm.coef[cs][q];
+        double *out = sm.cxy[q][pos];
+#pragma unroll
+        for (int lyp = 0; lyp <= lpm; lyp++)
+#pragma unroll
+            for (int lxp = 0; lxp <= lpm - lyp; lxp++) {
+                const int c2 = c2idx(lpm, lyp, lxp);
+                if (c2 < BEG || c2 >= END) continue;
+                double a = 0.0;
+#pragma unroll
+                for (int lzp = 0; lzp <= lpm - lyp - lxp; lzp++) a += cf[cidx(lpm, lzp, lyp, lxp)] * tv[lzp];
+                out[c2] = ok ? a : 0.0;
+            }
+    }
+}
+
 // out_mode 0: grid += acc (splits == 1); 1: partial[split] = acc; 2: partial[split] += acc
 // General-cell collocate kernels run one workgroup per CU: the row data of
 // the general path would otherwise spill registers in the inner loops
@@ -203,7 +280,15 @@ __device__ __forceinline__ void collocate_tile(CollocSmem<LM, GEN> &sm, int bid,
     if (out_mode != 1 && mybeg >= myend) return;
     const int *rec_words = reinterpret_cast<const int *>(recs);
 
-    // prologue: records of chunks 0 and 1, then coefficients of chunk 0
+    // prologue: records of chunks 0 and 1, the coefficient layout's decode
+    // table, then coefficients of chunk 0
+    if constexpr (GEN) {
+        for (int c = t; c < C::C3M; c += NROW) {
+            int lz, ly, lx;
+            decode_c3(LM - 1, c, lz, ly, lx);
+            sm.dec[c] = lz | ly << 5 | lx << 10;
+        }
+    }
     for (int w = t; w < 2 * Q * RECW; w += NROW) {
         const int item = mybeg + w / RECW;
         if (item < myend)
@@ -213,7 +298,9 @@ __device__ __forceinline__ void collocate_tile(CollocSmem<LM, GEN> &sm, int bid,
     __syncthreads();
     for (int w = t; w < Q * C::C3M; w += NROW) {
         const int q = w / C::C3M, idx = w % C::C3M;
-        if (mybeg + q < myend && idx < ncoset(sm.rec[0][q].lp))
+        if constexpr (GEN)
+            sm.coef[0][q][idx] = mybeg + q < myend ? coef_entry<LM>(sm.dec[idx], sm.rec[0][q], coef) : 0.0;
+        else if (mybeg + q < myend && idx < ncoset(sm.rec[0][q].lp))
             sm.coef[0][q][idx] = coef[sm.rec[0][q].coef_off + idx];
     }
     __syncthreads();
@@ -240,7 +327,11 @@ __device__ __forceinline__ void collocate_tile(CollocSmem<LM, GEN> &sm, int bid,
             if (w < Q * C::C3M) {
                 const int q = w / C::C3M, idx = w % C::C3M;
                 const EntryRec &Rn = sm.rec[(k + 1) % 3][q];
-                if (c + Q + q < myend && idx < ncoset(Rn.lp)) pf_coef[j] = coef[Rn.coef_off + idx];
+                if constexpr (GEN) {
+                    if (c + Q + q < myend) pf_coef[j] = coef_entry<LM>(sm.dec[idx], Rn, coef);
+                } else if (c + Q + q < myend && idx < ncoset(Rn.lp)) {
+                    pf_coef[j] = coef[Rn.coef_off + idx];
+                }
             }
         }
         // table phase: wave w -> pairs w, w + 4, ...
@@ -250,12 +341,24 @@ __device__ __forceinline__ void collocate_tile(CollocSmem<LM, GEN> &sm, int bid,
                 const EntryRec &R = sm.rec[rs][q];
                 const int lp = R.lp;
                 if (lane == 0) sm.lp[q] = lp;
+            // orthorhombic cells: the z lanes of the table phase also run the
+            // z stage of their pair
 #define GPW_P1_CALL(L)                                                                        \
-    tile_phase1<L, LM, true, GEN, Q>(R, q, lane, x0, y0, z0, tg, sm.coef[cs][q], sm.X[q],      \
-                                     sm.Y[q], nullptr, sm.cxy[q], sm.xi[q], sm.T[q],          \
-                                     sm.bmax[q], sm.bmin[q], sm.gr)
+    tile_phase1<L, LM, !GEN, GEN, Q>(R, q, lane, x0, y0, z0, tg, GEN ? nullptr : sm.coef[cs][q], \
+                                     sm.X[q], sm.Y[q], nullptr, GEN ? nullptr : sm.cxy[q],      \
+                                     sm.xi[q], sm.T[q], sm.bmax[q], sm.bmin[q], sm.gr)
                 GPW_LP_DISPATCH(LM, GPW_P1_CALL)
 #undef GPW_P1_CALL
+            }
+            if constexpr (GEN) {
+                // general cells: z stage of every pair of the chunk, wave w
+                // takes part w of the (lyp, lxp) terms, lane -> (pair, z-plane)
+                switch (t / 64) {
+                case 0: colloc_zpart<LM, 0, GEN>(sm, rs, cs, nq, lane, z0, tg); break;
+                case 1: colloc_zpart<LM, 1, GEN>(sm, rs, cs, nq, lane, z0, tg); break;
+                case 2: colloc_zpart<LM, 2, GEN>(sm, rs, cs, nq, lane, z0, tg); break;
+                default: colloc_zpart<LM, 3, GEN>(sm, rs, cs, nq, lane, z0, tg); break;
+                }
             }
         }
         __syncthreads();
